@@ -1,0 +1,166 @@
+"""ctypes binding of the C ABI in include/diffsph.h (libdiffsph_amd.so).
+
+The library is loaded from the package directory.  There is no fallback:
+if the HIP library is missing every op raises, so a GPU run can never
+silently fall back to another implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdiffsph_amd.so")
+
+c_f32p = C.c_void_p  # device pointers are passed as void*
+
+
+class Domain(C.Structure):
+    _fields_ = [("min", C.c_float * 2), ("max", C.c_float * 2), ("periodic", C.c_int32 * 2)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("ncell", C.c_int32 * 2), ("hcell", C.c_float), ("pad_", C.c_int32)]
+
+
+class Pairs(C.Structure):
+    _fields_ = [
+        ("nq", C.c_int64),
+        ("nr", C.c_int64),
+        ("offsets", C.c_void_p),
+        ("nbr", C.c_void_p),
+        ("qpos", C.c_void_p),
+        ("rpos", C.c_void_p),
+        ("qh", C.c_void_p),
+        ("rh", C.c_void_p),
+        ("h_uniform", C.c_float),
+        ("mode", C.c_int32),
+        ("domain", Domain),
+    ]
+
+
+class DfsphParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float),
+        ("rho0", C.c_float),
+        ("omega", C.c_float),
+        ("error_threshold", C.c_float),
+        ("min_iters", C.c_int32),
+        ("max_iters", C.c_int32),
+        ("clamp_pressure", C.c_int32),
+        ("source_density", C.c_int32),
+        ("check_every", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("final_error", C.c_double),
+        ("initial_error", C.c_double),
+    ]
+
+
+class ViscParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_float),
+        ("cs", C.c_float),
+        ("rho0", C.c_float),
+        ("eps", C.c_float),
+        ("pi_switch", C.c_int32),
+        ("scheme", C.c_int32),
+    ]
+
+
+V = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+F32 = C.c_float
+SZ = C.c_size_t
+P = C.POINTER
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "dsph_status_string": (C.c_char_p, [C.c_int]),
+    "dsph_abi_version": (C.c_int, []),
+    "dsph_grid_dims": (C.c_int, [P(Domain), F32, P(Grid)]),
+    "dsph_grid_build_workspace": (SZ, [I64, P(Grid)]),
+    "dsph_grid_build": (C.c_int, [P(Domain), P(Grid), V, V, I64, V, V, V, V, V, V, SZ, V]),
+    "dsph_neighbor_count": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, V, V]),
+    "dsph_neighbor_fill": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, V, V, I64, V]),
+    "dsph_wrap_positions": (C.c_int, [P(Domain), V, I64, V, V]),
+    "dsph_scan_workspace": (SZ, [I64]),
+    "dsph_scan_exclusive": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
+    "dsph_gather_f32": (C.c_int, [V, V, I64, I32, V, V]),
+    "dsph_scatter_f32": (C.c_int, [V, V, I64, I32, V, V]),
+    "dsph_pairs_export": (C.c_int, [P(Pairs), V, V, V, V, V, V, V, V, V, V, V, V, SZ, V]),
+    "dsph_pairs_export_workspace": (SZ, [I64]),
+    "dsph_density": (C.c_int, [P(Pairs), V, V, V, V, V]),
+    "dsph_sph_operation": (C.c_int, [P(Pairs), I32, I32, V, V, V, V, V, V, I32, V, V]),
+    "dsph_dfsph_prepare": (C.c_int, [P(Pairs), V, V, V, V, V, F32, F32, P(ViscParams), V, V, V, V, V]),
+    "dsph_dfsph_workspace": (SZ, [I64]),
+    "dsph_dfsph_solve": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V, V, P(SolveStats), V, SZ, V]),
+    "dsph_pressure_accel": (C.c_int, [P(Pairs), V, V, V, V, V]),
+    "dsph_dfsph_iterate": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, I32, P(C.c_double), V, SZ, V]),
+    "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),
+    "dsph_integrate_semi_implicit": (C.c_int, [V, V, V, V, V, V, V, I64, F32, V, V]),
+    "dsph_op_sph_density": (C.c_int, [V, V, V, I32, V, V, V, V]),
+    "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, V, V, I32, V, V, V, V]),
+    "dsph_op_scatter_add": (C.c_int, [V, I32, I32, V, V, V, V]),
+}
+
+_lib = None
+
+
+class DsphError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load (once) and return the HIP library; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DsphError(
+                f"HIP library {LIB_PATH} is not built; run `python -m diffsph_amd.build` "
+                "(the diffsph_amd ops have no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().dsph_status_string(status).decode()
+        raise DsphError(f"{what}: {msg} (status {status})")
+
+
+def ptr(t) -> int | None:
+    """Raw device pointer of a tensor (None for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def make_domain(min_extent, max_extent, periodic) -> Domain:
+    d = Domain()
+    mn = [float(v) for v in (min_extent if hasattr(min_extent, "__len__") else [min_extent] * 2)]
+    mx = [float(v) for v in (max_extent if hasattr(max_extent, "__len__") else [max_extent] * 2)]
+    pr = periodic if hasattr(periodic, "__len__") else [periodic] * 2
+    for k in range(2):
+        d.min[k] = mn[k]
+        d.max[k] = mx[k]
+        d.periodic[k] = 1 if bool(pr[k]) else 0
+    return d

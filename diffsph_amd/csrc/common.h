@@ -1,0 +1,173 @@
+// Shared device helpers of the particle hot path (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <cmath>
+
+#include "../../include/diffsph.h"
+
+namespace dsph {
+
+constexpr int kBlock = 256;  // 4 wave64s per workgroup
+
+// Device form of a domain: L = fl32(max-min) and half = fl32(L/2) are what
+// the reference's float32 tensor arithmetic produces
+// (v2/modules/neighborhood.py:430, v2/math.py:4-6).
+struct Dom {
+  float min[2];
+  float L[2];
+  float half[2];
+  int periodic[2];
+};
+
+inline Dom make_dom(const dsph_domain& d) {
+  Dom o;
+  for (int k = 0; k < 2; ++k) {
+    o.min[k] = d.min[k];
+    o.L[k] = d.max[k] - d.min[k];
+    o.half[k] = (float)(((double)d.max[k] - (double)d.min[k]) / 2.0);
+    o.periodic[k] = d.periodic[k] != 0;
+  }
+  return o;
+}
+
+// torch.remainder for float32: r = fmod(a,b); r += b when r and b differ in
+// sign and r != 0.
+__device__ __forceinline__ float py_remainder(float a, float b) {
+  float r = fmodf(a, b);
+  if (r != 0.0f && ((r < 0.0f) != (b < 0.0f))) r = r + b;
+  return r;
+}
+
+// getPeriodicPosition (v2/modules/neighborhood.py:430).
+__device__ __forceinline__ float wrap_coord(float x, const Dom& d, int k) {
+  if (!d.periodic[k]) return x;
+  return py_remainder(x - d.min[k], d.L[k]) + d.min[k];
+}
+
+// v2/math.py:4-6 `mod`, applied per axis to a pair offset.
+__device__ __forceinline__ float mod_offset(float x, const Dom& d, int k) {
+  if (!d.periodic[k]) return x;
+  float ax = fabsf(x);
+  if (ax > d.half[k]) {
+    float s = x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+    return s * (py_remainder(ax + d.min[k], d.L[k]) + d.min[k]);
+  }
+  return x;
+}
+
+// |d|^2 with both products rounded before the add (no FMA contraction), so
+// the neighbor predicate is reproducible bit for bit on the CPU.
+__device__ __forceinline__ float dist2_exact(float dx, float dy) {
+  return __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
+}
+
+// Wendland C2 in 2-D (kernelFunctions/Wendland2.py): C_2 = 7/pi.
+constexpr float kC2 = (float)(7.0 / 3.14159265358979323846);
+
+__device__ __forceinline__ float clamp01(float t) { return fminf(fmaxf(t, 0.0f), 1.0f); }
+
+// W(q,h) = (1-q)^4 (1+4q) C / h^2
+__device__ __forceinline__ float wendland_w(float q, float h) {
+  float t = clamp01(1.0f - q);
+  float t2 = t * t;
+  return (t2 * t2 * (1.0f + 4.0f * q)) * kC2 / (h * h);
+}
+// dk/dq C / h^3 with dk/dq = -20 q (1-q)^3; grad W = xhat * this.
+__device__ __forceinline__ float wendland_dw(float q, float h) {
+  float t = clamp01(1.0f - q);
+  return (-20.0f * q * (t * t * t)) * kC2 / (h * h * h);
+}
+
+__device__ __forceinline__ float pick_h(int mode, float hi, float hj) {
+  return mode == DSPH_MODE_SCATTER ? hj : (mode == DSPH_MODE_GATHER ? hi : 0.5f * (hi + hj));
+}
+
+// Geometry of one pair (query i, reference j) evaluated on the fly, with the
+// reference's conventions: d = x_i - x_j (min-image), r = |d|,
+// xhat = d/(r+1e-7), q = r/h_ij (v2/modules/neighborhood.py:445-451).
+struct Geom {
+  float dx, dy;   // x_i - x_j
+  float r, q, h;
+  float ex, ey;   // xhat
+  float w;        // W_ij
+  float gx, gy;   // grad_i W_ij
+};
+
+__device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const Dom& d) {
+  Geom g;
+  g.dx = mod_offset(pi.x - pj.x, d, 0);
+  g.dy = mod_offset(pi.y - pj.y, d, 1);
+  g.r = __fsqrt_rn(dist2_exact(g.dx, g.dy));
+  g.h = h;
+  g.q = __fdiv_rn(g.r, h);
+  float inv = 1.0f / (g.r + 1e-7f);
+  g.ex = g.dx * inv;
+  g.ey = g.dy * inv;
+  g.w = wendland_w(g.q, h);
+  float dw = wendland_dw(g.q, h);
+  g.gx = g.ex * dw;
+  g.gy = g.ey * dw;
+  return g;
+}
+
+struct PairsDev {
+  int nq, nr;
+  const int32_t* off;
+  const int32_t* nbr;
+  const float2* qpos;
+  const float2* rpos;
+  const float* qh;
+  const float* rh;
+  float h0;
+  int mode;
+  Dom dom;
+};
+
+inline PairsDev make_pairs(const dsph_pairs& p) {
+  PairsDev o;
+  o.nq = (int)p.nq;
+  o.nr = (int)p.nr;
+  o.off = p.offsets;
+  o.nbr = p.nbr;
+  o.qpos = reinterpret_cast<const float2*>(p.qpos);
+  o.rpos = reinterpret_cast<const float2*>(p.rpos);
+  o.qh = p.qh;
+  o.rh = p.rh;
+  o.h0 = p.h_uniform;
+  o.mode = p.mode;
+  o.dom = make_dom(p.domain);
+  return o;
+}
+
+__device__ __forceinline__ float h_of(const float* arr, int idx, float h0) {
+  return arr ? arr[idx] : h0;
+}
+
+inline int grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (int)(g < 1 ? 1 : g);
+}
+
+inline int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DSPH_OK : DSPH_ERR_LAUNCH;
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace dsph
+
+#define DSPH_CHECK_HIP(expr)                          \
+  do {                                                \
+    hipError_t _e = (expr);                           \
+    if (_e != hipSuccess) return DSPH_ERR_LAUNCH;     \
+  } while (0)
+
+#define DSPH_LAUNCH_CHECK()                           \
+  do {                                                \
+    int _s = ::dsph::launch_status();                 \
+    if (_s != DSPH_OK) return _s;                     \
+  } while (0)

@@ -1,0 +1,611 @@
+// DFSPH relaxed-Jacobi pressure solve (v2/modules/dfsph.py) as gather
+// kernels with device-side loop control.
+//
+// Per iteration two pair sweeps:
+//   A: a_i = -(1/rho_i) sum_j V_j (p_i + p_j) gradW_ij
+//   B: K_i = -dtf sum_j V_j (a_j - a_i) . gradW_ij ; r_i = K_i - s_i ;
+//      p_i += omega (s_i - K_i)/alpha_i (clamped at 0) -- in place, a thread
+//      only touches its own p_i in B.
+// B also reduces mean(clamp(r_i/rho0, min=-thr)) in fp64: every block
+// writes a partial, the last block to finish (ticket counter) sums the
+// partials in a fixed order, advances the iteration counter and sets `done`
+// with the reference's loop condition.  Every kernel of the loop returns at
+// once when `done` is set, so the host may queue several iterations ahead
+// and synchronise only once per chunk.
+#include "common.h"
+
+namespace dsph {
+
+struct SolveCtrl {
+  int iter;
+  int done;
+  unsigned int ticket;
+  int min_iters;
+  int max_iters;
+  int pad;
+  double err;
+  double err0;
+  double thr;
+};
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Deterministic block sum (fixed shuffle tree, then fixed wave order).
+__device__ __forceinline__ double block_sum_d(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  v = wave_sum_d(v);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+  }
+  __syncthreads();
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Coefficients for both solves + divergence of v + viscosity, one sweep.
+// ---------------------------------------------------------------------------
+struct PrepArgs {
+  const float2* vel;
+  const float* mass;   // unused by alpha (reference uses areas*rho0) but kept for V
+  const float* areas;
+  const float* vol;    // m/rho
+  const float* rho;
+  float dt, dt2, rho0;
+  // viscosity
+  int visc_on;
+  int visc_scheme;
+  float visc_alpha, visc_cs, visc_rho0, visc_eps;
+  int pi_switch;
+  float visc_factor;   // 2(d+2) nu rho0 for the viscid scheme
+};
+
+__global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
+                                                          float* __restrict__ alpha_den, float* __restrict__ div_src,
+                                                          float2* __restrict__ visc_acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float2 vi = A.vel[i];
+  float k1x = 0.f, k1y = 0.f, k2 = 0.f, dv = 0.f, vsx = 0.f, vsy = 0.f;
+  const int b = P.off[i], e = P.off[i + 1];
+  for (int k = b; k < e; ++k) {
+    const int j = P.nbr[k];
+    const float hj = h_of(P.rh, j, P.h0);
+    const float h = pick_h(P.mode, hi, hj);
+    Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const float Vj = A.vol[j];
+    const float2 vj = A.vel[j];
+    // alpha terms (dfsph.py:13-20)
+    k1x += Vj * g.gx;
+    k1y += Vj * g.gy;
+    const float c2 = (Vj * Vj) / (A.areas[j] * A.rho0);
+    k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
+    // divergence 'difference' of v (sphOps.py:92-101)
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
+    if (A.visc_on) {
+      // delta-SPH viscosity (viscosity.py:141-166 / 205-218)
+      const float vx = vi.x - vj.x, vy = vi.y - vj.y;
+      const float rij = g.q * g.h;
+      const float vr = vx * g.ex + vy * g.ey;
+      float pi = vr / (rij + A.visc_eps * g.h * g.h);
+      if (A.pi_switch) pi = vr < 0.0f ? pi : 0.0f;
+      const float f = Vj * pi;
+      vsx += f * g.gx;
+      vsy += f * g.gy;
+    }
+  }
+  const float Vi = A.vol[i];
+  const float mass_i = A.areas[i] * A.rho0;
+  const float k1sq = k1x * k1x + k1y * k1y;
+  {
+    const float fac = -A.dt2 * Vi;
+    alpha_den[i] = fac / mass_i * k1sq + fac * k2;
+  }
+  {
+    const float fac = -A.dt * Vi;
+    alpha_div[i] = fac / mass_i * k1sq + fac * k2;
+  }
+  div_src[i] = dv;
+  if (A.visc_on && visc_acc) {
+    float s;
+    if (A.visc_scheme == 0)
+      s = A.visc_alpha * hi * A.visc_cs * A.visc_rho0 / A.rho[i];
+    else
+      s = A.visc_factor / A.rho[i];
+    visc_acc[i] = make_float2(s * vsx, s * vsy);
+  }
+}
+
+// Density source: s_i = 1 - areas_i/V_i - dt * div(v*) with v* = v + dt adv
+// (dfsph.py:57, 62-63).
+__global__ void __launch_bounds__(kBlock) k_dfsph_density_source(PairsDev P, const float2* __restrict__ vel,
+                                                                 const float2* __restrict__ adv,
+                                                                 const float* __restrict__ vol,
+                                                                 const float* __restrict__ areas, float dt,
+                                                                 float* __restrict__ src) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float2 a_i = adv ? adv[i] : make_float2(0.f, 0.f);
+  const float2 vi = make_float2(vel[i].x + dt * a_i.x, vel[i].y + dt * a_i.y);
+  float dv = 0.f;
+  const int b = P.off[i], e = P.off[i + 1];
+  for (int k = b; k < e; ++k) {
+    const int j = P.nbr[k];
+    const float hj = h_of(P.rh, j, P.h0);
+    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+    const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
+    const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
+    const float Vj = vol[j];
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
+  }
+  src[i] = (1.0f - areas[i] / vol[i]) - dt * dv;
+}
+
+// Divergence source with advection: s_i = div(v + dt adv) (dfsph.py:65).
+__global__ void __launch_bounds__(kBlock) k_dfsph_divergence_source(PairsDev P, const float2* __restrict__ vel,
+                                                                    const float2* __restrict__ adv,
+                                                                    const float* __restrict__ vol, float dt,
+                                                                    float* __restrict__ src) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float2 vi = make_float2(vel[i].x + dt * adv[i].x, vel[i].y + dt * adv[i].y);
+  float dv = 0.f;
+  const int b = P.off[i], e = P.off[i + 1];
+  for (int k = b; k < e; ++k) {
+    const int j = P.nbr[k];
+    const float hj = h_of(P.rh, j, P.h0);
+    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+    const float2 vj = make_float2(vel[j].x + dt * adv[j].x, vel[j].y + dt * adv[j].y);
+    const float Vj = vol[j];
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
+  }
+  src[i] = dv;
+}
+
+// Sweep A (dfsph.py:35-37).  `ctrl` NULL -> unconditional.
+__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ vol,
+                                                          const float* __restrict__ rho, const float* __restrict__ p,
+                                                          float2* __restrict__ acc, const SolveCtrl* ctrl) {
+  if (ctrl && ctrl->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float pi = p[i];
+  float ax = 0.f, ay = 0.f;
+  const int b = P.off[i], e = P.off[i + 1];
+  for (int k = b; k < e; ++k) {
+    const int j = P.nbr[k];
+    const float hj = h_of(P.rh, j, P.h0);
+    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+    const float Vj = vol[j];
+    const float s = pi + p[j];
+    ax += s * (Vj * g.gx);
+    ay += s * (Vj * g.gy);
+  }
+  const float r = rho[i];
+  acc[i] = make_float2(-ax / r, -ay / r);
+}
+
+struct SweepBArgs {
+  const float* vol;
+  const float* src;
+  const float* alpha;
+  float* p;
+  float* residual;
+  const float2* acc;
+  double* partials;
+  SolveCtrl* ctrl;
+  float dtf;
+  float omega;
+  float inv_rho0_dummy;
+  float rho0;
+  float neg_thr;
+  int clamp;
+  int n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs B) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ int last;
+  if (B.ctrl->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double err = 0.0;
+  if (i < P.nq) {
+    const float2 xi = P.qpos[i];
+    const float hi = h_of(P.qh, i, P.h0);
+    const float2 ai = B.acc[i];
+    float sum = 0.f;
+    const int b = P.off[i], e = P.off[i + 1];
+    for (int k = b; k < e; ++k) {
+      const int j = P.nbr[k];
+      const float hj = h_of(P.rh, j, P.h0);
+      Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+      const float Vj = B.vol[j];
+      const float2 aj = B.acc[j];
+      sum += (aj.x - ai.x) * (Vj * g.gx) + (aj.y - ai.y) * (Vj * g.gy);
+    }
+    const float K = -B.dtf * sum;
+    const float s = B.src[i];
+    const float res = K - s;
+    float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
+    if (B.clamp) pn = fmaxf(pn, 0.0f);
+    B.p[i] = pn;
+    if (B.residual) B.residual[i] = res;
+    err = (double)fmaxf(res / B.rho0, B.neg_thr);
+  }
+  double bs = block_sum_d(err, sh);
+  if (threadIdx.x == 0) {
+    B.partials[blockIdx.x] = bs;
+    __threadfence();
+    unsigned int t = atomicAdd(&B.ctrl->ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // Last block: fixed-order reduction of all partials.
+  double v = 0.0;
+  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) v += __builtin_nontemporal_load(&B.partials[k]);
+  double tot = block_sum_d(v, sh);
+  if (threadIdx.x == 0) {
+    SolveCtrl* c = B.ctrl;
+    double mean = tot / (double)B.n;
+    c->iter += 1;
+    c->err = mean;
+    if (c->iter == 1) c->err0 = mean;
+    c->ticket = 0;
+    bool go = c->iter < c->max_iters && (c->iter < c->min_iters || mean > c->thr);
+    c->done = go ? 0 : 1;
+  }
+}
+
+__global__ void k_ctrl_init(SolveCtrl* c, int min_iters, int max_iters, double thr) {
+  c->iter = 0;
+  c->ticket = 0;
+  c->min_iters = min_iters;
+  c->max_iters = max_iters;
+  c->err = 0.0;
+  c->err0 = 0.0;
+  c->thr = thr;
+  bool go = 0 < max_iters && (0 < min_iters || 0.0 > thr);
+  c->done = go ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Density error (fp64): mean and max |rho/rho0 - 1|.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_density_error(const float* __restrict__ rho, int n, float rho0,
+                                                          double* __restrict__ partial_sum,
+                                                          double* __restrict__ partial_max) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0, m = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double e = fabs((double)rho[i] / (double)rho0 - 1.0);
+    s += e;
+    m = fmax(m, e);
+  }
+  double bs = block_sum_d(s, sh);
+  // max: shuffle tree
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  __shared__ double shm[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mm = fmax(mm, shm[w]);
+    partial_sum[blockIdx.x] = bs;
+    partial_max[blockIdx.x] = mm;
+  }
+}
+
+__global__ void k_density_error_final(const double* ps, const double* pm, int nb, int n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0, m = 0.0;
+  for (int k = 0; k < nb; ++k) {
+    s += ps[k];
+    m = fmax(m, pm[k]);
+  }
+  out[0] = n > 0 ? s / (double)n : 0.0;
+  out[1] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Semi-implicit Euler (integration.py:111-138).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_integrate_sie(float2* __restrict__ pos, float2* __restrict__ vel,
+                                                          const float2* __restrict__ a0, const float2* __restrict__ a1,
+                                                          const float2* __restrict__ a2, const float2* __restrict__ a3,
+                                                          float gx, float gy, int has_g, int n, float dt,
+                                                          float2* __restrict__ dudt_out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // dudt = a_p + g + visc + a_div, summed left to right (dfsph.py:209)
+  float2 d = a0 ? a0[i] : make_float2(0.f, 0.f);
+  if (has_g) {
+    d.x = d.x + gx;
+    d.y = d.y + gy;
+  }
+  if (a1) {
+    d.x = d.x + a1[i].x;
+    d.y = d.y + a1[i].y;
+  }
+  if (a2) {
+    d.x = d.x + a2[i].x;
+    d.y = d.y + a2[i].y;
+  }
+  if (a3) {
+    d.x = d.x + a3[i].x;
+    d.y = d.y + a3[i].y;
+  }
+  float2 v = vel[i];
+  v.x = v.x + d.x * dt;
+  v.y = v.y + d.y * dt;
+  float2 x = pos[i];
+  x.x = x.x + v.x * dt;
+  x.y = x.y + v.y * dt;
+  vel[i] = v;
+  pos[i] = x;
+  if (dudt_out) dudt_out[i] = d;
+}
+
+int check_pairs(const dsph_pairs* p);
+
+inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace dsph
+
+using namespace dsph;
+
+extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, const float* mass, const float* areas,
+                                  const float* vol, const float* rho, float dt, float rho0,
+                                  const dsph_visc_params* visc, float* alpha_div, float* alpha_den, float* div_src,
+                                  float* visc_acc, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vel || !areas || !vol || !rho || !alpha_div || !alpha_den || !div_src) return DSPH_ERR_ARG;
+  if (visc && !visc_acc) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  PrepArgs A;
+  A.vel = reinterpret_cast<const float2*>(vel);
+  A.mass = mass;
+  A.areas = areas;
+  A.vol = vol;
+  A.rho = rho;
+  A.dt = dt;
+  A.dt2 = (float)((double)dt * (double)dt);
+  A.rho0 = rho0;
+  A.visc_on = visc != nullptr;
+  A.visc_scheme = visc ? visc->scheme : 0;
+  A.visc_alpha = visc ? visc->alpha : 0.f;
+  A.visc_cs = visc ? visc->cs : 0.f;
+  A.visc_rho0 = visc ? visc->rho0 : 0.f;
+  A.visc_eps = visc ? visc->eps : 0.f;
+  A.pi_switch = visc ? visc->pi_switch : 0;
+  A.visc_factor = visc ? (float)(2.0 * (2 + 2) * (double)visc->alpha * (double)visc->rho0) : 0.f;
+  k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, A, alpha_div, alpha_den, div_src,
+                                                                    reinterpret_cast<float2*>(visc_acc));
+  return launch_status();
+}
+
+extern "C" size_t dsph_dfsph_workspace(int64_t n) {
+  size_t nb = (size_t)grid_for(n > 0 ? n : 1);
+  return al256(sizeof(SolveCtrl)) + al256(nb * sizeof(double)) + al256(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
+}
+
+extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* vel,
+                                const float* adv, const float* rho, const float* vol, const float* areas,
+                                const float* alpha, const float* src0, float* p, float* accel, float* residual,
+                                dsph_solve_stats* stats, void* ws, size_t ws_bytes, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm || pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  const int64_t n = pairs->nq;
+  if (n == 0) {
+    if (stats) {
+      stats->iterations = 0;
+      stats->converged = 1;
+      stats->final_error = 0.0;
+      stats->initial_error = 0.0;
+    }
+    return DSPH_OK;
+  }
+  if (!vel || !rho || !vol || !areas || !alpha || !p || !accel) return DSPH_ERR_ARG;
+  if (!prm->source_density && !src0 && !adv) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_dfsph_workspace(n)) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  PairsDev P = make_pairs(*pairs);
+  const int nb = grid_for(n);
+  char* w = static_cast<char*>(ws);
+  SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
+  w += al256(sizeof(SolveCtrl));
+  double* partials = reinterpret_cast<double*>(w);
+  w += al256((size_t)nb * sizeof(double));
+  float* src = reinterpret_cast<float*>(w);
+
+  const float dt = prm->dt;
+  const float dtf = prm->source_density ? (float)((double)dt * (double)dt) : dt;
+  // Source term.
+  if (prm->source_density) {
+    k_dfsph_density_source<<<nb, kBlock, 0, st>>>(P, reinterpret_cast<const float2*>(vel),
+                                                  reinterpret_cast<const float2*>(adv), vol, areas, dt, src);
+  } else if (adv) {
+    k_dfsph_divergence_source<<<nb, kBlock, 0, st>>>(P, reinterpret_cast<const float2*>(vel),
+                                                     reinterpret_cast<const float2*>(adv), vol, dt, src);
+  } else {
+    DSPH_CHECK_HIP(hipMemcpyAsync(src, src0, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  }
+  DSPH_LAUNCH_CHECK();
+  k_ctrl_init<<<1, 1, 0, st>>>(ctrl, prm->min_iters, prm->max_iters, (double)prm->error_threshold);
+  DSPH_LAUNCH_CHECK();
+
+  SweepBArgs B;
+  B.vol = vol;
+  B.src = src;
+  B.alpha = alpha;
+  B.p = p;
+  B.residual = residual;
+  B.acc = reinterpret_cast<const float2*>(accel);
+  B.partials = partials;
+  B.ctrl = ctrl;
+  B.dtf = dtf;
+  B.omega = prm->omega;
+  B.inv_rho0_dummy = 0.f;
+  B.rho0 = prm->rho0;
+  B.neg_thr = -prm->error_threshold;
+  B.clamp = prm->clamp_pressure;
+  B.n = (int)n;
+
+  const int chunk = prm->check_every > 0 ? prm->check_every : 8;
+  SolveCtrl host;
+  int launched = 0;
+  bool done = prm->max_iters <= 0;
+  while (!done && launched < prm->max_iters) {
+    int k = chunk;
+    if (launched + k > prm->max_iters) k = prm->max_iters - launched;
+    for (int t = 0; t < k; ++t) {
+      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+      k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+    }
+    DSPH_LAUNCH_CHECK();
+    launched += k;
+    DSPH_CHECK_HIP(hipMemcpyAsync(&host, ctrl, sizeof(SolveCtrl), hipMemcpyDeviceToHost, st));
+    DSPH_CHECK_HIP(hipStreamSynchronize(st));
+    done = host.done != 0;
+  }
+  if (prm->max_iters <= 0) {
+    DSPH_CHECK_HIP(hipMemcpyAsync(&host, ctrl, sizeof(SolveCtrl), hipMemcpyDeviceToHost, st));
+    DSPH_CHECK_HIP(hipStreamSynchronize(st));
+  }
+  // Final pressure acceleration (dfsph.py:105).
+  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
+  DSPH_LAUNCH_CHECK();
+  if (stats) {
+    stats->iterations = host.iter;
+    stats->final_error = host.err;
+    stats->initial_error = host.err0;
+    stats->converged = host.iter > 0 ? (host.err <= (double)prm->error_threshold ? 1 : 0) : 1;
+  }
+  return DSPH_OK;
+}
+
+extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p,
+                                   float* accel, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel),
+                                                                    nullptr);
+  return launch_status();
+}
+
+extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* rho,
+                                  const float* vol, const float* alpha, const float* src, float* p, float* accel,
+                                  float* residual, int32_t iterations, double* final_error, void* ws, size_t ws_bytes,
+                                  void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm || pairs->nq != pairs->nr || iterations < 0) return DSPH_ERR_ARG;
+  const int64_t n = pairs->nq;
+  if (n == 0 || iterations == 0) return DSPH_OK;
+  if (!rho || !vol || !alpha || !src || !p || !accel) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_dfsph_workspace(n)) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  PairsDev P = make_pairs(*pairs);
+  const int nb = grid_for(n);
+  char* w = static_cast<char*>(ws);
+  SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
+  w += al256(sizeof(SolveCtrl));
+  double* partials = reinterpret_cast<double*>(w);
+  // Fixed count: the loop condition can never stop it early.
+  k_ctrl_init<<<1, 1, 0, st>>>(ctrl, iterations, iterations, 0.0);
+  SweepBArgs B;
+  B.vol = vol;
+  B.src = src;
+  B.alpha = alpha;
+  B.p = p;
+  B.residual = residual;
+  B.acc = reinterpret_cast<const float2*>(accel);
+  B.partials = partials;
+  B.ctrl = ctrl;
+  B.dtf = prm->source_density ? (float)((double)prm->dt * (double)prm->dt) : prm->dt;
+  B.omega = prm->omega;
+  B.inv_rho0_dummy = 0.f;
+  B.rho0 = prm->rho0;
+  B.neg_thr = -prm->error_threshold;
+  B.clamp = prm->clamp_pressure;
+  B.n = (int)n;
+  for (int t = 0; t < iterations; ++t) {
+    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+    k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+  }
+  DSPH_LAUNCH_CHECK();
+  if (final_error) {
+    SolveCtrl host;
+    DSPH_CHECK_HIP(hipMemcpyAsync(&host, ctrl, sizeof(SolveCtrl), hipMemcpyDeviceToHost, st));
+    DSPH_CHECK_HIP(hipStreamSynchronize(st));
+    *final_error = host.err;
+  }
+  return DSPH_OK;
+}
+
+extern "C" int dsph_density_error(const float* rho, int64_t n, float rho0, double* out2, void* ws, size_t ws_bytes,
+                                  void* stream) {
+  if (!out2 || n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (n > 0 && !rho) return DSPH_ERR_ARG;
+  const int nb = 1024;
+  size_t need = al256(sizeof(double) * nb) * 2 + al256(sizeof(double) * 2);
+  if (!ws || ws_bytes < need) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  char* w = static_cast<char*>(ws);
+  double* ps = reinterpret_cast<double*>(w);
+  double* pm = reinterpret_cast<double*>(w + al256(sizeof(double) * nb));
+  double* out = reinterpret_cast<double*>(w + 2 * al256(sizeof(double) * nb));
+  k_density_error<<<nb, kBlock, 0, st>>>(rho, (int)n, rho0, ps, pm);
+  k_density_error_final<<<1, 1, 0, st>>>(ps, pm, nb, (int)n, out);
+  DSPH_LAUNCH_CHECK();
+  DSPH_CHECK_HIP(hipMemcpyAsync(out2, out, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+  DSPH_CHECK_HIP(hipStreamSynchronize(st));
+  return DSPH_OK;
+}
+
+extern "C" int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0, const float* a1, const float* a2,
+                                            const float* a3, const float* gravity2, int64_t n, float dt,
+                                            float* dudt_out, void* stream) {
+  if (n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  if (!pos || !vel) return DSPH_ERR_ARG;
+  float gx = 0.f, gy = 0.f;
+  int has_g = 0;
+  if (gravity2) {
+    gx = gravity2[0];
+    gy = gravity2[1];
+    has_g = 1;
+  }
+  k_integrate_sie<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(
+      reinterpret_cast<float2*>(pos), reinterpret_cast<float2*>(vel), reinterpret_cast<const float2*>(a0),
+      reinterpret_cast<const float2*>(a1), reinterpret_cast<const float2*>(a2), reinterpret_cast<const float2*>(a3),
+      gx, gy, has_g, (int)n, dt, reinterpret_cast<float2*>(dudt_out));
+  return launch_status();
+}

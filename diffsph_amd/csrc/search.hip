@@ -1,0 +1,343 @@
+// Cell-list neighbor search on a dense grid (gfx950).
+//
+// Build: key = cell of the wrapped position -> per-cell histogram (atomics
+// give a provisional slot) -> exclusive scan -> scatter -> per-cell sort by
+// original index (makes the order deterministic) -> gather positions and
+// supports into cell order.
+// Query: one thread per query particle walks the 3x3 cell block; the three
+// cells of a row are one contiguous slice of the sorted arrays unless the
+// row wraps around a periodic x boundary.
+#include "common.h"
+
+namespace dsph {
+
+int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
+size_t scan_ws(int64_t n);
+
+struct GridDev {
+  int nc[2];
+  float hcell;
+};
+
+__device__ __forceinline__ int cell_coord(float x, const Dom& d, int k, const GridDev& g) {
+  float f = floorf(__fdiv_rn(x - d.min[k], g.hcell));
+  int c = (int)f;
+  if (f < 0.0f) c = 0;
+  if (c > g.nc[k] - 1) c = g.nc[k] - 1;
+  return c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_keys(const float2* __restrict__ pos, int n, Dom d, GridDev g,
+                                                      int32_t* __restrict__ keys, int32_t* __restrict__ slot,
+                                                      int32_t* __restrict__ cell_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float2 p = pos[i];
+  float wx = wrap_coord(p.x, d, 0), wy = wrap_coord(p.y, d, 1);
+  int cx = cell_coord(wx, d, 0, g), cy = cell_coord(wy, d, 1, g);
+  int key = cy * g.nc[0] + cx;
+  keys[i] = key;
+  slot[i] = atomicAdd(&cell_count[key], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cell_scatter(int n, const int32_t* __restrict__ keys,
+                                                         const int32_t* __restrict__ slot,
+                                                         const int32_t* __restrict__ cell_start,
+                                                         int32_t* __restrict__ sort_index) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sort_index[cell_start[keys[i]] + slot[i]] = i;
+}
+
+// Insertion sort of each cell's slice (cells hold ~16 particles).
+__global__ void __launch_bounds__(kBlock) k_cell_sort(int ncell, const int32_t* __restrict__ cell_start,
+                                                      int32_t* __restrict__ sort_index) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = cell_start[c], e = cell_start[c + 1];
+  for (int a = b + 1; a < e; ++a) {
+    int v = sort_index[a];
+    int k = a - 1;
+    while (k >= b && sort_index[k] > v) {
+      sort_index[k + 1] = sort_index[k];
+      --k;
+    }
+    sort_index[k + 1] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sorted_gather(int n, Dom d, const int32_t* __restrict__ sort_index,
+                                                          const float2* __restrict__ pos, const float* __restrict__ h,
+                                                          float2* __restrict__ spos, float* __restrict__ sh,
+                                                          int32_t* __restrict__ rank) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = sort_index[p];
+  float2 x = pos[i];
+  spos[p] = make_float2(wrap_coord(x.x, d, 0), wrap_coord(x.y, d, 1));
+  if (sh) sh[p] = h[i];
+  if (rank) rank[i] = p;
+}
+
+__global__ void __launch_bounds__(kBlock) k_wrap(const float2* __restrict__ pos, int n, Dom d,
+                                                 float2* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float2 p = pos[i];
+  out[i] = make_float2(wrap_coord(p.x, d, 0), wrap_coord(p.y, d, 1));
+}
+
+struct SearchArgs {
+  Dom d;
+  GridDev g;
+  const int32_t* cs;
+  const float2* rpos;
+  const float* rh;
+  const float2* qpos;
+  const float* qh;
+  float h0;
+  int mode;
+  int wrapped;
+  int nq;
+};
+
+__device__ __forceinline__ bool keep_pair(float2 qi, float hi, float2 rj, float hj, const SearchArgs& a) {
+  float dx = mod_offset(qi.x - rj.x, a.d, 0);
+  float dy = mod_offset(qi.y - rj.y, a.d, 1);
+  float r = __fsqrt_rn(dist2_exact(dx, dy));
+  float h = pick_h(a.mode, hi, hj);
+  return __fdiv_rn(r, h) <= 1.0f;
+}
+
+// Visit the candidate slices of query i in offset order (dy=-1,0,1; dx=-1,0,1).
+template <class F>
+__device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, F&& f) {
+  const int ncx = a.g.nc[0], ncy = a.g.nc[1];
+  int cx = cell_coord(qi.x, a.d, 0, a.g), cy = cell_coord(qi.y, a.d, 1, a.g);
+  for (int oy = -1; oy <= 1; ++oy) {
+    int yy = cy + oy;
+    if (a.d.periodic[1]) {
+      yy = yy < 0 ? yy + ncy : (yy >= ncy ? yy - ncy : yy);
+    } else if (yy < 0 || yy >= ncy) {
+      continue;
+    }
+    const int row = yy * ncx;
+    bool wraps = a.d.periodic[0] && (cx == 0 || cx == ncx - 1);
+    if (!wraps) {
+      int x0 = cx - 1 < 0 ? 0 : cx - 1;
+      int x1 = cx + 1 > ncx - 1 ? ncx - 1 : cx + 1;
+      f(a.cs[row + x0], a.cs[row + x1 + 1]);
+    } else {
+      for (int ox = -1; ox <= 1; ++ox) {
+        int xx = cx + ox;
+        xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
+        f(a.cs[row + xx], a.cs[row + xx + 1]);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nq) return;
+  float2 qi = a.qpos[i];
+  if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
+  float hi = h_of(a.qh, i, a.h0);
+  int c = 0;
+  for_candidates(a, qi, [&](int b, int e) {
+    for (int j = b; j < e; ++j) c += keep_pair(qi, hi, a.rpos[j], h_of(a.rh, j, a.h0), a) ? 1 : 0;
+  });
+  counts[i] = c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
+                                                          int32_t* __restrict__ nbr, int64_t cap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nq) return;
+  float2 qi = a.qpos[i];
+  if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
+  float hi = h_of(a.qh, i, a.h0);
+  int64_t o = off[i];
+  const int64_t end = off[i + 1];
+  for_candidates(a, qi, [&](int b, int e) {
+    for (int j = b; j < e; ++j) {
+      if (keep_pair(qi, hi, a.rpos[j], h_of(a.rh, j, a.h0), a)) {
+        if (o < end && o < cap) nbr[o] = j;
+        ++o;
+      }
+    }
+  });
+}
+
+inline GridDev make_grid(const dsph_grid& g) {
+  GridDev o;
+  o.nc[0] = g.ncell[0];
+  o.nc[1] = g.ncell[1];
+  o.hcell = g.hcell;
+  return o;
+}
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+static int check_grid(const dsph_grid* g) {
+  if (!g || g->ncell[0] < 1 || g->ncell[1] < 1 || !(g->hcell > 0.0f)) return DSPH_ERR_ARG;
+  if ((int64_t)g->ncell[0] * g->ncell[1] >= (int64_t)INT32_MAX) return DSPH_ERR_DIM;
+  return DSPH_OK;
+}
+
+static int make_search(SearchArgs& a, const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                       const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos,
+                       const float* qh, int64_t nq, int32_t wrapped, float h0, int32_t mode) {
+  if (!domain || check_grid(grid) != DSPH_OK) return DSPH_ERR_ARG;
+  if (nq < 0 || nref < 0 || nq >= INT32_MAX || nref >= INT32_MAX) return DSPH_ERR_ARG;
+  if (mode < 0 || mode > 2) return DSPH_ERR_ARG;
+  if (nq > 0 && (!qpos || !cell_start)) return DSPH_ERR_ARG;
+  if (nref > 0 && !sorted_pos) return DSPH_ERR_ARG;
+  // Supports: a NULL array means the uniform h0, which must then be > 0.
+  if (((mode != DSPH_MODE_GATHER && !sorted_h) || (mode != DSPH_MODE_SCATTER && !qh)) && !(h0 > 0.0f))
+    return DSPH_ERR_ARG;
+  a.d = make_dom(*domain);
+  a.g = make_grid(*grid);
+  a.cs = cell_start;
+  a.rpos = reinterpret_cast<const float2*>(sorted_pos);
+  a.rh = sorted_h;
+  a.qpos = reinterpret_cast<const float2*>(qpos);
+  a.qh = qh;
+  a.h0 = h0;
+  a.mode = mode;
+  a.wrapped = wrapped != 0;
+  a.nq = (int)nq;
+  return DSPH_OK;
+}
+
+}  // namespace dsph
+
+using namespace dsph;
+
+extern "C" int dsph_grid_dims(const dsph_domain* domain, float hsearch, dsph_grid* grid) {
+  if (!domain || !grid || !(hsearch > 0.0f)) return DSPH_ERR_ARG;
+  // Cells are kept 0.1% wider than the search radius so that float32
+  // rounding of cell coordinates can never separate two particles closer
+  // than the radius by two cells.
+  const double hs = (double)hsearch * 1.001;
+  double hc = 0.0;
+  int nc[2];
+  for (int k = 0; k < 2; ++k) {
+    double L = (double)domain->max[k] - (double)domain->min[k];
+    if (!(L > 0.0)) return DSPH_ERR_ARG;
+    double n = std::floor(L / hs);
+    if (n < 1.0) n = 1.0;
+    double side = L / n;
+    if (side > hc) hc = side;
+  }
+  // One square side for both axes: the larger one (reference compute_h).
+  float hcell = (float)hc;
+  if ((double)hcell < hs) hcell = nextafterf(hcell, INFINITY);
+  for (int k = 0; k < 2; ++k) {
+    double L = (double)domain->max[k] - (double)domain->min[k];
+    double n;
+    if (domain->periodic[k]) {
+      // A periodic axis is tiled by floor(L/hcell) cells, each >= hcell wide
+      // (the last cell absorbs the remainder), so the wrap keeps adjacency.
+      // (+1e-4: hcell may have been rounded up by one float ulp; the last
+      // cell is then at least (1-1e-4) hcell > the search radius wide.)
+      n = std::floor(L / (double)hcell + 1e-4);
+      if (n < 3.0) return DSPH_ERR_DIM;
+    } else {
+      n = std::ceil(L / (double)hcell);
+      if (n < 1.0) n = 1.0;
+    }
+    if (n > 1e8) return DSPH_ERR_DIM;
+    nc[k] = (int)n;
+  }
+  grid->ncell[0] = nc[0];
+  grid->ncell[1] = nc[1];
+  grid->hcell = hcell;
+  grid->pad_ = 0;
+  return DSPH_OK;
+}
+
+extern "C" size_t dsph_grid_build_workspace(int64_t n, const dsph_grid* grid) {
+  if (!grid) return 0;
+  int64_t ncell = (int64_t)grid->ncell[0] * grid->ncell[1];
+  return align256((size_t)n * 4) * 2 + align256((size_t)ncell * 4) + scan_ws(ncell);
+}
+
+extern "C" int dsph_grid_build(const dsph_domain* domain, const dsph_grid* grid, const float* pos, const float* h,
+                               int64_t n, int32_t* cell_start, int32_t* sort_index, int32_t* rank,
+                               float* sorted_pos, float* sorted_h, void* ws, size_t ws_bytes, void* stream) {
+  if (!domain || check_grid(grid) != DSPH_OK || n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (!cell_start || (n > 0 && (!pos || !sort_index || !sorted_pos))) return DSPH_ERR_ARG;
+  if (sorted_h && !h) return DSPH_ERR_ARG;
+  // A periodic axis is only valid when the cells tile it (L/hcell >= nc).
+  for (int k = 0; k < 2; ++k) {
+    if (domain->periodic[k] && grid->ncell[k] < 3) return DSPH_ERR_DIM;
+  }
+  const int64_t ncell = (int64_t)grid->ncell[0] * grid->ncell[1];
+  if (!ws || ws_bytes < dsph_grid_build_workspace(n, grid)) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  char* w = static_cast<char*>(ws);
+  int32_t* keys = reinterpret_cast<int32_t*>(w);
+  w += align256((size_t)n * 4);
+  int32_t* slot = reinterpret_cast<int32_t*>(w);
+  w += align256((size_t)n * 4);
+  int32_t* count = reinterpret_cast<int32_t*>(w);
+  w += align256((size_t)ncell * 4);
+  void* scan_w = w;
+  size_t scan_b = scan_ws(ncell);
+
+  Dom d = make_dom(*domain);
+  GridDev g = make_grid(*grid);
+  DSPH_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)ncell * 4, st));
+  if (n > 0) {
+    k_cell_keys<<<grid_for(n), kBlock, 0, st>>>(reinterpret_cast<const float2*>(pos), (int)n, d, g, keys, slot, count);
+    DSPH_LAUNCH_CHECK();
+  }
+  int s = scan_exclusive(count, ncell, cell_start, scan_w, scan_b, st);
+  if (s != DSPH_OK) return s;
+  if (n > 0) {
+    k_cell_scatter<<<grid_for(n), kBlock, 0, st>>>((int)n, keys, slot, cell_start, sort_index);
+    k_cell_sort<<<grid_for(ncell), kBlock, 0, st>>>((int)ncell, cell_start, sort_index);
+    k_sorted_gather<<<grid_for(n), kBlock, 0, st>>>((int)n, d, sort_index, reinterpret_cast<const float2*>(pos), h,
+                                                    reinterpret_cast<float2*>(sorted_pos), sorted_h, rank);
+    DSPH_LAUNCH_CHECK();
+  }
+  return DSPH_OK;
+}
+
+extern "C" int dsph_neighbor_count(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                                   const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos,
+                                   const float* qh, int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                   int32_t* counts, void* stream) {
+  SearchArgs a;
+  int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
+                      h_uniform, mode);
+  if (s != DSPH_OK) return s;
+  if (nq > 0 && !counts) return DSPH_ERR_ARG;
+  if (nq == 0) return DSPH_OK;
+  k_neighbor_count<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, counts);
+  return launch_status();
+}
+
+extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                                  const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos,
+                                  const float* qh, int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                  const int32_t* offsets, int32_t* nbr, int64_t capacity, void* stream) {
+  SearchArgs a;
+  int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
+                      h_uniform, mode);
+  if (s != DSPH_OK) return s;
+  if (nq == 0) return DSPH_OK;
+  if (!offsets || (capacity > 0 && !nbr) || capacity < 0) return DSPH_ERR_ARG;
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, nbr, capacity);
+  return launch_status();
+}
+
+extern "C" int dsph_wrap_positions(const dsph_domain* domain, const float* pos, int64_t n, float* out, void* stream) {
+  if (!domain || n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  if (!pos || !out) return DSPH_ERR_ARG;
+  k_wrap<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(reinterpret_cast<const float2*>(pos), (int)n,
+                                                        make_dom(*domain), reinterpret_cast<float2*>(out));
+  return launch_status();
+}

@@ -1,0 +1,223 @@
+"""DFSPH on the GPU: the reference's scheme over the HIP pair kernels.
+
+Follows src/diffSPH/v2/simulationSchemes/dfsph.py:148-227 and
+src/diffSPH/v2/modules/dfsph.py:52-109 with semi-implicit Euler
+(src/diffSPH/v2/modules/integration.py:111-138):
+
+1. neighbor search (cell sort + CSR list, every step),
+2. summation density, V = m/rho,
+3. one sweep: alpha for both solves, div(v) and delta-SPH viscosity,
+4. divergence solve (omega 0.3, threshold 1e-2/rho0, warm start 0.75 p_prev),
+5. density solve (omega 0.3, threshold 5e-4/rho0, warm start 0.75 p_prev,
+   advection = gravity + viscosity + divergence acceleration),
+6. dudt = a_p + g + visc + a_div; v += dt dudt; x += dt v.
+
+The particle state is kept in cell order: every step re-sorts it (the
+permutation is near the identity after the first step) and carries a UID
+so callers can map back to the input order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr
+from .neighborhood import WORKSPACE, CellGrid, NeighborList, search
+
+
+@dataclass
+class DFSPHConfig:
+    dt: float
+    rho0: float = 1.0
+    support: float = 0.0            # h (uniform)
+    domain_min: tuple = (-1.0, -1.0)
+    domain_max: tuple = (1.0, 1.0)
+    periodic: tuple = (True, True)
+    omega_div: float = 0.3
+    omega_den: float = 0.3
+    threshold_div: float = 1e-2     # divided by rho0 as the reference does
+    threshold_den: float = 5e-4
+    min_iters: int = 2
+    max_iters: int = 256
+    clamp_pressure: bool = True
+    warm_start: float = 0.75        # p0 = 0.75 p_prev (modules/dfsph.py:70-75)
+    viscosity: str | None = "deltaSPH_inviscid"
+    visc_alpha: float = 0.01
+    visc_nu: float = 0.01
+    cs: float = 10.0
+    visc_eps: float = 1e-6
+    pi_switch: bool = False
+    gravity: tuple | None = None
+    check_every: int = 0
+
+
+@dataclass
+class StepStats:
+    div_iters: int = 0
+    den_iters: int = 0
+    div_error: float = 0.0
+    den_error: float = 0.0
+    den_error0: float = 0.0
+    num_pairs: int = 0
+
+
+class DFSPHSolver:
+    """Owns the particle state (cell-ordered, on the GPU) and steps it."""
+
+    def __init__(self, positions, velocities, masses, areas, cfg: DFSPHConfig, uid=None):
+        dev = positions.device
+        if not positions.is_cuda:
+            raise RuntimeError("DFSPHSolver needs GPU tensors")
+        self.cfg = cfg
+        self.n = positions.shape[0]
+        self.domain = _lib.make_domain(cfg.domain_min, cfg.domain_max, cfg.periodic)
+        f = dict(dtype=torch.float32, device=dev)
+        self.pos = positions.detach().to(**f).contiguous().clone()
+        self.vel = velocities.detach().to(**f).contiguous().clone()
+        self.mass = masses.detach().to(**f).contiguous().clone()
+        self.areas = areas.detach().to(**f).contiguous().clone()
+        self.uid = (uid if uid is not None else torch.arange(self.n, device=dev, dtype=torch.int32)).to(
+            torch.int32).contiguous().clone()
+        self.p_div = None
+        self.p_den = None
+        n = self.n
+        self.rho = torch.empty(n, **f)
+        self.vol = torch.empty(n, **f)
+        self.alpha_div = torch.empty(n, **f)
+        self.alpha_den = torch.empty(n, **f)
+        self.div_src = torch.empty(n, **f)
+        self.visc = torch.zeros(n, 2, **f)
+        self.a_div = torch.empty(n, 2, **f)
+        self.a_den = torch.empty(n, 2, **f)
+        self.adv = torch.empty(n, 2, **f)
+        self.dudt = torch.empty(n, 2, **f)
+        self.residual = torch.empty(n, **f)
+        self.grid = None
+        self.nl = None
+        self.step_count = 0
+        self.hooks = {}  # optional: 'den_init' -> callable(solver) returning an initial pressure
+
+    # -- helpers ---------------------------------------------------------
+    def _permute(self, t, idx):
+        if t is None:
+            return None
+        w = 1 if t.dim() == 1 else t.shape[1]
+        out = torch.empty_like(t)
+        if t.dtype == torch.float32:
+            check(lib().dsph_gather_f32(ptr(t), ptr(idx), self.n, w, ptr(out), _lib.stream_handle(t.device)),
+                  "gather")
+        else:
+            out = t[idx.long()]
+        return out
+
+    def neighbor_step(self):
+        cfg = self.cfg
+        grid = CellGrid.build(self.pos, None, self.domain, cfg.support, h_uniform=cfg.support)
+        si = grid.sort_index
+        self.pos = self._permute(self.pos, si)
+        self.vel = self._permute(self.vel, si)
+        self.mass = self._permute(self.mass, si)
+        self.areas = self._permute(self.areas, si)
+        self.uid = self._permute(self.uid, si)
+        self.p_div = self._permute(self.p_div, si)
+        self.p_den = self._permute(self.p_den, si)
+        self.grid = grid
+        self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True)
+        return self.nl
+
+    def _visc_params(self):
+        cfg = self.cfg
+        if cfg.viscosity is None:
+            return None
+        vp = _lib.ViscParams()
+        vp.alpha = cfg.visc_alpha if cfg.viscosity == "deltaSPH_inviscid" else cfg.visc_nu
+        vp.cs = cfg.cs
+        vp.rho0 = cfg.rho0
+        vp.eps = cfg.visc_eps
+        vp.pi_switch = 1 if cfg.pi_switch else 0
+        vp.scheme = 0 if cfg.viscosity == "deltaSPH_inviscid" else 1
+        return vp
+
+    def _solve(self, pairs, density: bool, p: torch.Tensor, accel: torch.Tensor, adv=None):
+        cfg = self.cfg
+        prm = _lib.DfsphParams()
+        prm.dt = cfg.dt
+        prm.rho0 = cfg.rho0
+        prm.omega = cfg.omega_den if density else cfg.omega_div
+        prm.error_threshold = (cfg.threshold_den if density else cfg.threshold_div) / cfg.rho0
+        prm.min_iters = cfg.min_iters
+        prm.max_iters = cfg.max_iters
+        prm.clamp_pressure = 1 if cfg.clamp_pressure else 0
+        prm.source_density = 1 if density else 0
+        prm.check_every = cfg.check_every
+        stats = _lib.SolveStats()
+        L = lib()
+        ws = WORKSPACE.get(L.dsph_dfsph_workspace(self.n), p.device)
+        alpha = self.alpha_den if density else self.alpha_div
+        check(L.dsph_dfsph_solve(C.byref(pairs), C.byref(prm), ptr(self.vel), ptr(adv), ptr(self.rho),
+                                 ptr(self.vol), ptr(self.areas), ptr(alpha), ptr(self.div_src), ptr(p),
+                                 ptr(accel), ptr(self.residual), C.byref(stats), ptr(ws), ws.numel(),
+                                 _lib.stream_handle(p.device)), "dfsph_solve")
+        return stats
+
+    # -- one step --------------------------------------------------------
+    def step(self) -> StepStats:
+        cfg = self.cfg
+        L = lib()
+        st = _lib.stream_handle(self.pos.device)
+        nl = self.neighbor_step()
+        pairs = nl.pairs()
+        out = StepStats(num_pairs=nl.num_pairs)
+        check(L.dsph_density(C.byref(pairs), ptr(self.mass), ptr(self.mass), ptr(self.rho), ptr(self.vol), st),
+              "density")
+        vp = self._visc_params()
+        check(L.dsph_dfsph_prepare(C.byref(pairs), ptr(self.vel), ptr(self.mass), ptr(self.areas), ptr(self.vol),
+                                   ptr(self.rho), C.c_float(cfg.dt), C.c_float(cfg.rho0),
+                                   C.byref(vp) if vp is not None else None, ptr(self.alpha_div),
+                                   ptr(self.alpha_den), ptr(self.div_src), ptr(self.visc) if vp else None, st),
+              "dfsph_prepare")
+        if vp is None:
+            self.visc.zero_()
+        # divergence solve (advection = 0: v* = v)
+        p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
+        s = self._solve(pairs, False, p_div, self.a_div, None)
+        out.div_iters, out.div_error = s.iterations, s.final_error
+        self.p_div = p_div
+        # advection for the density solve: g + visc + a_div (dfsph.py:199)
+        adv = self.adv
+        torch.add(self.visc, self.a_div, out=adv)
+        if cfg.gravity is not None:
+            g = torch.tensor(cfg.gravity, dtype=torch.float32, device=adv.device)
+            adv.copy_((g + self.visc) + self.a_div)
+        if "den_init" in self.hooks:
+            p_den = self.hooks["den_init"](self)
+        else:
+            p_den = (cfg.warm_start * self.p_den) if self.p_den is not None else torch.zeros_like(self.rho)
+        s = self._solve(pairs, True, p_den, self.a_den, adv)
+        out.den_iters, out.den_error, out.den_error0 = s.iterations, s.final_error, s.initial_error
+        self.p_den = p_den
+        g = None
+        if cfg.gravity is not None:
+            g = torch.tensor(cfg.gravity, dtype=torch.float32, device=self.pos.device)
+        check(L.dsph_integrate_semi_implicit(ptr(self.pos), ptr(self.vel), ptr(self.a_den), ptr(self.visc),
+                                             ptr(self.a_div), None, ptr(g), self.n, C.c_float(cfg.dt),
+                                             ptr(self.dudt), st), "integrate")
+        self.step_count += 1
+        return out
+
+    def density_error(self):
+        """(mean, max) |rho/rho0 - 1| of the current summation density."""
+        out = (C.c_double * 2)()
+        ws = WORKSPACE.get(1 << 16, self.rho.device)
+        check(lib().dsph_density_error(ptr(self.rho), self.n, C.c_float(self.cfg.rho0), out, ptr(ws), ws.numel(),
+                                       _lib.stream_handle(self.rho.device)), "density_error")
+        return float(out[0]), float(out[1])
+
+    def state_by_uid(self):
+        """Positions, velocities and densities in input (UID) order."""
+        order = torch.argsort(self.uid.long())
+        return self.pos[order], self.vel[order], self.rho[order]

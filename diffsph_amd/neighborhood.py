@@ -1,0 +1,258 @@
+"""Cell-list neighbor search on the GPU (host side).
+
+Mirrors the reference's v2 neighbor search interface
+(src/diffSPH/v2/modules/neighborhood.py:352-593) on top of the HIP search in
+``csrc/search.hip``:
+
+* :class:`CellGrid` sorts a particle set into cell order (the reference's
+  ``buildDataStructure`` / ``buildCompactHashMap``; a dense periodic grid
+  replaces the hash map),
+* :func:`search` builds the CSR neighbor list of a query set against a grid,
+* :func:`neighborSearch` returns the reference's ``(datastructure,
+  neighborDict)`` pair with the same dict keys (``indices``, ``distances``,
+  ``vectors``, ``supports``, ``kernels``, ``gradients``, ``numNeighbors``,
+  ``neighborOffsets``...), materialising per-pair arrays only there.
+
+Pairs are kept when |mod(x_i - x_j)| / h_j <= 1 ('scatter' mode), evaluated
+in float32 with separately rounded products and IEEE sqrt/div.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+MODES = {"scatter": 0, "gather": 1, "symmetric": 2}
+
+
+class Workspace:
+    """Grow-only scratch buffer per device (the C ABI never allocates)."""
+
+    def __init__(self):
+        self._buf = {}
+
+    def get(self, nbytes: int, device) -> torch.Tensor:
+        key = str(device)
+        nbytes = max(int(nbytes), 256)
+        b = self._buf.get(key)
+        if b is None or b.numel() < nbytes:
+            b = torch.empty(nbytes + nbytes // 4, dtype=torch.uint8, device=device)
+            self._buf[key] = b
+        return b
+
+
+WORKSPACE = Workspace()
+
+
+def _domain_from_config(config):
+    d = config["domain"]
+    return _lib.make_domain(_as_list(d["minExtent"]), _as_list(d["maxExtent"]),
+                            _as_list(d.get("periodicity", d.get("periodic", False))))
+
+
+def _as_list(v):
+    if isinstance(v, torch.Tensor):
+        return [float(x) for x in v.detach().cpu().reshape(-1).tolist()] if v.dtype != torch.bool else [
+            bool(x) for x in v.detach().cpu().reshape(-1).tolist()]
+    if isinstance(v, (list, tuple)):
+        return list(v)
+    return [v, v]
+
+
+def _check_f32(t, name, shape_tail=None):
+    if not isinstance(t, torch.Tensor):
+        raise RuntimeError(f"{name} is not defined")
+    if t.dtype != torch.float32:
+        raise RuntimeError(f"{name} must be float32")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} is not contiguous")
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} is not on the GPU")
+    if shape_tail is not None and tuple(t.shape[1:]) != tuple(shape_tail):
+        raise RuntimeError(f"{name} is not of the correct dimension")
+
+
+@dataclass
+class CellGrid:
+    """A particle set sorted into a dense cell grid."""
+    domain: _lib.Domain
+    grid: _lib.Grid
+    n: int
+    cell_start: torch.Tensor   # int32 [ncell+1]
+    sort_index: torch.Tensor   # int32 [n]   sorted slot -> original index
+    rank: torch.Tensor         # int32 [n]   original index -> sorted slot
+    sorted_pos: torch.Tensor   # f32 [n,2]   wrapped
+    sorted_h: torch.Tensor | None
+    h_uniform: float
+
+    @staticmethod
+    def build(positions: torch.Tensor, supports: torch.Tensor | None, domain: _lib.Domain,
+              hsearch: float, h_uniform: float = 0.0) -> "CellGrid":
+        _check_f32(positions, "positions", (2,))
+        n = positions.shape[0]
+        dev = positions.device
+        grid = _lib.Grid()
+        check(lib().dsph_grid_dims(C.byref(domain), C.c_float(hsearch), C.byref(grid)), "grid_dims")
+        ncell = grid.ncell[0] * grid.ncell[1]
+        cell_start = torch.empty(ncell + 1, dtype=torch.int32, device=dev)
+        sort_index = torch.empty(n, dtype=torch.int32, device=dev)
+        rank = torch.empty(n, dtype=torch.int32, device=dev)
+        sorted_pos = torch.empty(n, 2, dtype=torch.float32, device=dev)
+        sorted_h = None
+        if supports is not None:
+            _check_f32(supports, "supports")
+            sorted_h = torch.empty(n, dtype=torch.float32, device=dev)
+        wsb = lib().dsph_grid_build_workspace(n, C.byref(grid))
+        ws = WORKSPACE.get(wsb, dev)
+        check(lib().dsph_grid_build(C.byref(domain), C.byref(grid), ptr(positions), ptr(supports), n,
+                                    ptr(cell_start), ptr(sort_index), ptr(rank), ptr(sorted_pos),
+                                    ptr(sorted_h), ptr(ws), ws.numel(), _lib.stream_handle(dev)),
+              "grid_build")
+        return CellGrid(domain, grid, n, cell_start, sort_index, rank, sorted_pos, sorted_h,
+                        float(h_uniform))
+
+
+@dataclass
+class NeighborList:
+    """CSR neighbor list; rows are query particles, entries sorted reference slots."""
+    counts: torch.Tensor    # int32 [nq]
+    offsets: torch.Tensor   # int32 [nq+1]
+    nbr: torch.Tensor       # int32 [P]
+    num_pairs: int
+    qpos: torch.Tensor      # wrapped query positions [nq,2] (row order)
+    qh: torch.Tensor | None
+    grid: CellGrid
+    mode: int
+
+    def pairs(self) -> _lib.Pairs:
+        p = _lib.Pairs()
+        p.nq = self.counts.shape[0]
+        p.nr = self.grid.n
+        p.offsets = ptr(self.offsets)
+        p.nbr = ptr(self.nbr)
+        p.qpos = ptr(self.qpos)
+        p.rpos = ptr(self.grid.sorted_pos)
+        p.qh = ptr(self.qh)
+        p.rh = ptr(self.grid.sorted_h)
+        p.h_uniform = self.grid.h_uniform
+        p.mode = self.mode
+        p.domain = self.grid.domain
+        return p
+
+
+def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
+           qpos_wrapped: bool = False, capacity: int | None = None) -> NeighborList:
+    """Neighbor list of the query set `qpos` against the sorted set `grid`.
+
+    Counts, scans (one host read of the pair total, as the reference's
+    `.item()` does) and fills.  With ``qpos_wrapped`` the queries are the
+    grid's own wrapped sorted positions."""
+    _check_f32(qpos, "queryPositions", (2,))
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    nq = qpos.shape[0]
+    dev = qpos.device
+    st = _lib.stream_handle(dev)
+    counts = torch.empty(nq, dtype=torch.int32, device=dev)
+    L = lib()
+    args = (C.byref(grid.domain), C.byref(grid.grid), ptr(grid.cell_start), ptr(grid.sorted_pos),
+            ptr(grid.sorted_h), grid.n, ptr(qpos), ptr(qh), nq, 1 if qpos_wrapped else 0,
+            C.c_float(grid.h_uniform), m)
+    check(L.dsph_neighbor_count(*args, ptr(counts), st), "neighbor_count")
+    offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+    ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
+    total = C.c_int64(0)
+    check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), C.byref(total), ptr(ws), ws.numel(), st), "scan")
+    npairs = int(total.value)
+    cap = max(npairs, 1) if capacity is None else capacity
+    nbr = torch.empty(cap, dtype=torch.int32, device=dev)
+    check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(nbr), cap, st), "neighbor_fill")
+    if qpos_wrapped:
+        wq = qpos
+    else:
+        wq = wrap_positions(qpos, grid.domain)
+    return NeighborList(counts, offsets, nbr[:npairs], npairs, wq, qh, grid, m)
+
+
+def wrap_positions(pos: torch.Tensor, domain: _lib.Domain) -> torch.Tensor:
+    """getPeriodicPosition (v2/modules/neighborhood.py:407-430) in float32."""
+    _check_f32(pos, "positions", (2,))
+    out = torch.empty_like(pos)
+    check(lib().dsph_wrap_positions(C.byref(domain), ptr(pos), pos.shape[0], ptr(out),
+                                    _lib.stream_handle(pos.device)), "wrap_positions")
+    return out
+
+
+def export_pairs(nl: NeighborList, q_sort_index=None, q_rank=None, r_sort_index=None,
+                 want=("i", "j", "distances", "vectors", "supports", "kernels", "gradients")):
+    """Per-pair arrays of the reference's neighborDict in original indices."""
+    nq = nl.counts.shape[0]
+    dev = nl.offsets.device
+    P = nl.num_pairs
+    out_off = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+    res = {}
+
+    def mk(name, shape, dtype):
+        if name in want:
+            res[name] = torch.empty(shape, dtype=dtype, device=dev)
+        return res.get(name)
+
+    oi = mk("i", (P,), torch.int64)
+    oj = mk("j", (P,), torch.int64)
+    od = mk("distances", (P,), torch.float32)
+    ov = mk("vectors", (P, 2), torch.float32)
+    oh = mk("supports", (P,), torch.float32)
+    ow = mk("kernels", (P,), torch.float32)
+    og = mk("gradients", (P, 2), torch.float32)
+    L = lib()
+    ws = WORKSPACE.get(L.dsph_pairs_export_workspace(nq), dev)
+    pr = nl.pairs()
+    check(L.dsph_pairs_export(C.byref(pr), ptr(q_sort_index), ptr(q_rank), ptr(r_sort_index), ptr(out_off),
+                              ptr(oi), ptr(oj), ptr(od), ptr(ov), ptr(oh), ptr(ow), ptr(og), ptr(ws), ws.numel(),
+                              _lib.stream_handle(dev)), "pairs_export")
+    res["offsets"] = out_off
+    return res
+
+
+def neighborSearch(queryState, referenceState, config, computeKernels=True, priorState=None,
+                   neighborDatastructure=None, verbose=False):
+    """Drop-in for v2/modules/neighborhood.py:491-593 (scatter mode, kernel
+    values from the Wendland C2 kernel).  Returns ``(datastructure,
+    neighborDict)``; the datastructure is a :class:`CellGrid` of the
+    reference set.  The CSR pair set is recomputed on every call (the
+    reference's Verlet list only saves candidate generation, the filtered
+    result is the same)."""
+    domain = _domain_from_config(config)
+    rpos = referenceState["positions"]
+    rh = referenceState["supports"].contiguous()
+    qpos = queryState["positions"].contiguous()
+    qh = queryState["supports"].contiguous()
+    hmax = float(rh.max().item())
+    grid = CellGrid.build(rpos.contiguous(), rh, domain, hmax)
+    same = queryState is referenceState
+    if same:
+        nl = search(grid, grid.sorted_pos, grid.sorted_h, "scatter", qpos_wrapped=True)
+        ex = export_pairs(nl, grid.sort_index, grid.rank, grid.sort_index)
+    else:
+        nl = search(grid, qpos, qh, "scatter")
+        ex = export_pairs(nl, None, None, grid.sort_index)
+    counts = (ex["offsets"][1:] - ex["offsets"][:-1]).to(torch.int32)
+    nd = {
+        "indices": (ex["i"], ex["j"]),
+        "fullIndices": (ex["i"], ex["j"]),
+        "distances": ex["distances"],
+        "vectors": ex["vectors"],
+        "supports": ex["supports"],
+        "initialPositions": (qpos, rpos),
+        "numNeighbors": counts,
+        "neighborOffsets": ex["offsets"][:-1].contiguous(),
+        "fullNumNeighbors": counts,
+        "fullNeighborOffsets": ex["offsets"][:-1].contiguous(),
+    }
+    if computeKernels:
+        nd["kernels"] = ex["kernels"]
+        nd["gradients"] = ex["gradients"]
+    return grid, nd

@@ -1,0 +1,333 @@
+/*
+ * diffsph.h -- C ABI of the MI355X (gfx950) particle hot path.
+ *
+ * Every entry point takes plain device pointers, sizes and scalars, launches
+ * HIP kernels on the given stream (a hipStream_t passed as void*, NULL = the
+ * null stream) and returns a dsph_status.  No entry point allocates device
+ * memory: scratch comes from a caller-provided workspace whose size is given
+ * by the matching *_workspace() query.  Entry points are stateless and
+ * thread-safe; calls on one stream are ordered, calls on different streams
+ * that share output buffers race.
+ *
+ * Indexing: particle counts are int64 in the ABI, indices on the device are
+ * int32 (so a single set is limited to 2^31-1 particles and 2^31-1 pairs).
+ * Positions and velocities are [n,2] row-major float32 (x0,y0,x1,y1,...),
+ * scalars are [n] float32.
+ *
+ * Each declaration cites the reference interface it replaces (paths are
+ * relative to the diffSPH repository root).
+ */
+#ifndef DIFFSPH_AMD_H
+#define DIFFSPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dsph_status {
+  DSPH_OK = 0,
+  DSPH_ERR_ARG = 1,        /* invalid argument (null pointer, bad size, bad mode) */
+  DSPH_ERR_DIM = 2,        /* unsupported dimension / grid shape */
+  DSPH_ERR_LAUNCH = 3,     /* a HIP call or kernel launch failed */
+  DSPH_ERR_NOT_CONVERGED = 4,
+  DSPH_ERR_CAPACITY = 5,   /* output buffer or workspace too small */
+  DSPH_ERR_DEVICE = 6      /* no usable HIP device */
+} dsph_status;
+
+/* Support-radius mode of a neighbor search, as the reference's supportMode
+ * (neighborhood.h, countNeighborsForParticle): pair kept when
+ * SCATTER: |x_ij| / h_j <= 1, GATHER: |x_ij| / h_i <= 1,
+ * SYMMETRIC: |x_ij| / ((h_i+h_j)/2) <= 1. */
+enum { DSPH_MODE_SCATTER = 0, DSPH_MODE_GATHER = 1, DSPH_MODE_SYMMETRIC = 2 };
+
+/* Axis-aligned 2-D domain.  Periodic axes wrap positions with
+ * torch.remainder semantics (v2/modules/neighborhood.py:407-430) and pair
+ * offsets with v2/math.py:4-6 `mod`. */
+typedef struct dsph_domain {
+  float min[2];
+  float max[2];
+  int32_t periodic[2];
+} dsph_domain;
+
+/* Dense cell grid used by the neighbor search (replaces the compact hash
+ * map of neighborDebug.ipynb cell 4 / torchCompactRadius.buildDataStructure,
+ * v2/modules/neighborhood.py:403).  Cells are squares of side `hcell`
+ * (>= the largest search radius); cell (cx,cy) has linear key cy*ncell[0]+cx. */
+typedef struct dsph_grid {
+  int32_t ncell[2];
+  float hcell;
+  int32_t pad_;
+} dsph_grid;
+
+/* A CSR neighbor list plus what is needed to evaluate every pair on the fly.
+ * Row i (a query particle) holds reference indices nbr[offsets[i] ..
+ * offsets[i+1]).  Positions are the *wrapped* positions the search used. */
+typedef struct dsph_pairs {
+  int64_t nq;               /* query particles (rows)          */
+  int64_t nr;               /* reference particles             */
+  const int32_t* offsets;   /* [nq+1]                          */
+  const int32_t* nbr;       /* [offsets[nq]]                   */
+  const float* qpos;        /* [nq,2] wrapped query positions  */
+  const float* rpos;        /* [nr,2] wrapped reference pos.   */
+  const float* qh;          /* [nq] or NULL -> h_uniform       */
+  const float* rh;          /* [nr] or NULL -> h_uniform       */
+  float h_uniform;
+  int32_t mode;             /* DSPH_MODE_*                     */
+  dsph_domain domain;
+} dsph_pairs;
+
+const char* dsph_status_string(int status);
+int dsph_abi_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Neighbor search                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Host-only: grid for a domain and a largest search radius `hsearch`.
+ * Periodic axes get floor(L/hsearch) cells of side L/ncell (reference
+ * compute_h, neighborhoodDebug.ipynb cell 2: one side for both axes, the
+ * larger of the two); a periodic axis needs >= 3 cells. */
+int dsph_grid_dims(const dsph_domain* domain, float hsearch, dsph_grid* grid);
+
+size_t dsph_grid_build_workspace(int64_t n, const dsph_grid* grid);
+
+/* Sort a particle set into cell order (replaces sortReferenceParticles +
+ * buildCompactHashMap, neighborhoodDebug.ipynb cells 3-4).  Outputs:
+ *   cell_start[ncell+1]  first sorted index of every cell,
+ *   sort_index[n]        original index of sorted slot p (stable: ascending
+ *                        original index inside a cell),
+ *   rank[n]              sorted slot of original particle i (may be NULL),
+ *   sorted_pos[n,2]      wrapped positions in sorted order,
+ *   sorted_h[n]          supports in sorted order (NULL if h is NULL). */
+int dsph_grid_build(const dsph_domain* domain, const dsph_grid* grid,
+                    const float* pos, const float* h, int64_t n,
+                    int32_t* cell_start, int32_t* sort_index, int32_t* rank,
+                    float* sorted_pos, float* sorted_h,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* Count, for each query particle, the reference particles within its support
+ * (replaces countNeighbors, neighborhood.cpp:65-227, and the counting pass
+ * of computeNeighborhood, v2/cppSrc/computeNeighborhood.h:115-146).
+ * `qpos_wrapped` != 0 when qpos already holds wrapped positions.
+ * The predicate is evaluated in float32 as
+ *   d = mod(wrap(x_i) - wrap(x_j)); r = sqrt_rn(d.x*d.x + d.y*d.y) (no FMA);
+ *   keep iff r /_rn h_mode <= 1
+ * (v2/modules/neighborhood.py:445-453). */
+int dsph_neighbor_count(const dsph_domain* domain, const dsph_grid* grid,
+                        const int32_t* cell_start, const float* sorted_pos,
+                        const float* sorted_h, int64_t nref,
+                        const float* qpos, const float* qh, int64_t nq,
+                        int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                        int32_t* counts, void* stream);
+
+/* Fill the CSR list: row i receives the sorted reference indices of its
+ * neighbors, cells visited dy-major (-1,0,+1) then dx, particles in sorted
+ * order (replaces buildNeighborList, neighborhood.cpp:253-435).
+ * offsets[nq+1] is the exclusive scan of the counts; entries past
+ * `capacity` are not written and DSPH_ERR_CAPACITY is returned. */
+int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
+                       const int32_t* cell_start, const float* sorted_pos,
+                       const float* sorted_h, int64_t nref,
+                       const float* qpos, const float* qh, int64_t nq,
+                       int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                       const int32_t* offsets, int32_t* nbr, int64_t capacity,
+                       void* stream);
+
+/* out = getPeriodicPosition(pos) (v2/modules/neighborhood.py:407-430):
+ * periodic axes x' = remainder(x - min, max - min) + min in float32. */
+int dsph_wrap_positions(const dsph_domain* domain, const float* pos, int64_t n,
+                        float* out, void* stream);
+
+size_t dsph_scan_workspace(int64_t n);
+/* out[0..n] = exclusive prefix sum of in[0..n) (out[n] = total).  If
+ * total_host != NULL the total is also copied to host memory (this
+ * synchronises the stream, as the reference's `.item()` at
+ * v2/cppSrc/computeNeighborhood.cpp:58 does). */
+int dsph_scan_exclusive(const int32_t* in, int64_t n, int32_t* out,
+                        int64_t* total_host, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
+/* Gather helpers for moving per-particle data between original and sorted
+ * order: dst[p] = src[index[p]] for `width` floats per particle. */
+int dsph_gather_f32(const float* src, const int32_t* index, int64_t n,
+                    int32_t width, float* dst, void* stream);
+/* dst[index[p]] = src[p] */
+int dsph_scatter_f32(const float* src, const int32_t* index, int64_t n,
+                     int32_t width, float* dst, void* stream);
+
+/* Materialise the per-pair arrays of the reference's neighborDict
+ * (v2/modules/neighborhood.py:433-460, 560-591) for a CSR list whose rows
+ * are sorted query slots: i,j are ORIGINAL indices (int64), rows emitted in
+ * original query order (row of original query q = qrank[q]); within a row
+ * the order is the CSR order.  distances = r/h, vectors = d/(r+1e-7),
+ * supports = h, kernels = W, gradients = grad W (Wendland C2, 2-D).
+ * out_offsets[nq+1] are the CSR offsets in original order.  Any output
+ * pointer may be NULL. */
+int dsph_pairs_export(const dsph_pairs* pairs, const int32_t* q_sort_index,
+                      const int32_t* q_rank, const int32_t* r_sort_index,
+                      int32_t* out_offsets, int64_t* out_i, int64_t* out_j,
+                      float* out_distances, float* out_vectors,
+                      float* out_supports, float* out_kernels,
+                      float* out_gradients, void* workspace,
+                      size_t workspace_bytes, void* stream);
+size_t dsph_pairs_export_workspace(int64_t nq);
+
+/* ---------------------------------------------------------------------- */
+/* SPH pair operators over a dsph_pairs list (kernel values on the fly)     */
+/* ---------------------------------------------------------------------- */
+
+/* rho_i = sum_j m_j W_ij (v2/sphOps.py:19-29, v2/modules/density.py:6-15).
+ * Also writes vol_i = m_i/rho_i when vol != NULL (the reference's
+ * 'actualArea', v2/modules/dfsph.py:59). `qmass` may be NULL when vol is. */
+int dsph_density(const dsph_pairs* pairs, const float* rmass,
+                 const float* qmass, float* rho, float* vol, void* stream);
+
+/* Generic operator of v2/sphOps.py:253-282 over on-the-fly kernels:
+ * op = 0 interpolate  sum_j m_j/rho_j q_j W_ij
+ *      1 gradient     (mode: 0 naive, 1 difference, 2 summation, 3 symmetric)
+ *      2 divergence   (mode as above; q is [n,2])
+ * `width` = components of q (1 or 2).  out is [nq, width] (interpolate),
+ * [nq, width, 2] (gradient), [nq] (divergence). */
+enum { DSPH_OP_INTERPOLATE = 0, DSPH_OP_GRADIENT = 1, DSPH_OP_DIVERGENCE = 2 };
+enum { DSPH_GRAD_NAIVE = 0, DSPH_GRAD_DIFFERENCE = 1, DSPH_GRAD_SUMMATION = 2, DSPH_GRAD_SYMMETRIC = 3 };
+int dsph_sph_operation(const dsph_pairs* pairs, int32_t op, int32_t grad_mode,
+                       const float* qmass, const float* rmass,
+                       const float* qrho, const float* rrho,
+                       const float* qq, const float* rq, int32_t width,
+                       float* out, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* DFSPH (v2/modules/dfsph.py, v2/simulationSchemes/dfsph.py:148-227)       */
+/* ---------------------------------------------------------------------- */
+
+typedef struct dsph_dfsph_params {
+  float dt;
+  float rho0;
+  float omega;            /* relaxation factor */
+  float error_threshold;  /* on mean(clamp(r/rho0, min=-thr)) */
+  int32_t min_iters;
+  int32_t max_iters;
+  int32_t clamp_pressure; /* p = max(p, 0) after every update */
+  int32_t source_density; /* 1: density source, 0: divergence source */
+  int32_t check_every;    /* iterations launched between host checks (0 = 8) */
+  int32_t pad_;
+} dsph_dfsph_params;
+
+typedef struct dsph_solve_stats {
+  int32_t iterations;
+  int32_t converged;       /* 1 when the loop ended below the threshold */
+  double final_error;      /* reference error metric of the last iteration */
+  double initial_error;    /* error metric of the first iteration */
+} dsph_solve_stats;
+
+/* Per-particle solver coefficients after summation density: for both
+ * solves in one pair sweep (v2/modules/dfsph.py:7-33, 59-65):
+ *   alpha_div = -dt   * V_i (|sum V_j gradW|^2 / m_i + sum V_j^2/m_j |gradW|^2)
+ *   alpha_den = -dt^2 * (same bracket)
+ *   div_i     = sum V_j (v_j - v_i) . gradW_ij    (divergence source)
+ * and the delta-SPH inviscid viscosity acceleration
+ * (v2/modules/viscosity.py:123-170) when visc != NULL.  areas are the rest
+ * volumes, masses m = areas*rho0 in the alpha term as the reference. */
+typedef struct dsph_visc_params {
+  float alpha;   /* diffusion.alpha */
+  float cs;      /* fluid.cs        */
+  float rho0;
+  float eps;     /* diffusion.eps   */
+  int32_t pi_switch;
+  int32_t scheme; /* 0 deltaSPH_inviscid, 1 deltaSPH_viscid (alpha = nu) */
+} dsph_visc_params;
+
+int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel,
+                       const float* mass, const float* areas, const float* vol,
+                       const float* rho, float dt, float rho0,
+                       const dsph_visc_params* visc,
+                       float* alpha_div, float* alpha_den, float* div_src,
+                       float* visc_acc, void* stream);
+
+size_t dsph_dfsph_workspace(int64_t n);
+
+/* Relaxed-Jacobi pressure solve (dfsphSolve, v2/modules/dfsph.py:52-109).
+ * Inputs: vel, adv (advection acceleration; v* = vel + dt*adv), vol=m/rho,
+ * areas, alpha (for this solve), src0 (for the divergence solve: the
+ * divergence of vel; ignored for the density solve, whose source is
+ * computed here as 1 - areas/vol - dt*div(v*)).  p: initial pressure on
+ * entry (warm start), solution on exit.  accel: pressure acceleration of the
+ * final p.  Convergence is decided on the device (fp64 mean residual);
+ * the host is synchronised once every `check_every` iterations. */
+int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
+                     const float* vel, const float* adv, const float* rho,
+                     const float* vol, const float* areas, const float* alpha,
+                     const float* src0, float* p, float* accel,
+                     float* residual, dsph_solve_stats* stats,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Pressure acceleration a_i = -(1/rho_i) sum_j V_j (p_i + p_j) gradW_ij
+ * (computePressureAcceleration, v2/modules/dfsph.py:35-37); one pair sweep. */
+int dsph_pressure_accel(const dsph_pairs* pairs, const float* vol,
+                        const float* rho, const float* p, float* accel,
+                        void* stream);
+
+/* Exactly `iterations` Jacobi iterations (sweep A + sweep B each, no
+ * convergence test, no final acceleration) from the given p and source
+ * (updatePressure, v2/modules/dfsph.py:39-50).  final_error (host, may be
+ * NULL -> no synchronisation) receives the error metric of the last one. */
+int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
+                       const float* rho, const float* vol, const float* alpha,
+                       const float* src, float* p, float* accel,
+                       float* residual, int32_t iterations,
+                       double* final_error, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* Error of a density field against rest density: out[0] = mean |rho/rho0-1|,
+ * out[1] = max |rho/rho0-1| (fp64 reduction), copied to host. */
+int dsph_density_error(const float* rho, int64_t n, float rho0, double* out2,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Semi-implicit Euler (v2/modules/integration.py:111-138):
+ * v += dt*dudt; x += dt*v.  dudt = sum of up to 4 acceleration fields
+ * (NULL entries skipped), as dfsph.py:209 a_p + g + visc + a_div. */
+int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
+                                 const float* a1, const float* a2,
+                                 const float* a3, const float* gravity2,
+                                 int64_t n, float dt, float* dudt_out,
+                                 void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Drop-in ops with the reference's per-pair inputs                         */
+/* ---------------------------------------------------------------------- */
+
+/* sphDensity (partiBench/cppOp.cpp:5-106): out_i = sum_k m_j[j_k] W_k over
+ * CSR row i given by (numNeighbors, neighborOffset). */
+int dsph_op_sph_density(const float* masses_j, const int64_t* indices_j,
+                        const float* kernels, int32_t num_particles,
+                        const int32_t* num_neighbors,
+                        const int32_t* neighbor_offset, float* out,
+                        void* stream);
+
+/* sphInterpolation (partiBench/cppOp.cpp:110-222,
+ * sphOperations/cppOp.cpp:112-132): out_i = sum_k m_j/rho_j q_j W_k for a
+ * quantity of `width` components.  (The reference's CUDA branch computes
+ * the density instead -- partiBench/cppOp.cpp:160-169; this follows the
+ * documented formula of cppOp.h:90-125.) */
+int dsph_op_sph_interpolation(const float* masses_j, const float* densities_j,
+                              const float* quantities_j, int32_t width,
+                              const int64_t* indices_j, const float* kernels,
+                              int32_t num_particles,
+                              const int32_t* num_neighbors,
+                              const int32_t* neighbor_offset, float* out,
+                              void* stream);
+
+/* scatterAdd (sphOperations/cppOp.cpp:364-423): out_i[c] = sum over the CSR
+ * segment of row i of input[k, c]. */
+int dsph_op_scatter_add(const float* input, int32_t width,
+                        int32_t num_particles, const int32_t* num_neighbors,
+                        const int32_t* neighbor_offset, float* out,
+                        void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFSPH_AMD_H */

@@ -1,0 +1,675 @@
+// CPU oracle of the diffSPH particle hot path.
+//
+// TEST INFRASTRUCTURE ONLY: this library is the parity checker and the timed
+// CPU baseline.  Only tests/, the smoke check and the CPU-baseline leg of the
+// benchmark load it; the product path (diffsph_amd) never does.
+//
+// It restates the reference algorithms in plain C++17 + OpenMP, one OpenMP
+// loop over particles walking a CSR neighbor list, as the reference's own
+// CPU twins do (partiBench/cppOp.cpp:57-69, partiBench/viscosityKernel.cpp:
+// 74-77, neighborhood.cpp:204-223).  Each function names the reference lines
+// it follows.  Arithmetic is templated on the scalar type so the same code
+// gives the float32 oracle and the float64 error reference.
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).  FP
+// contraction is off so that the float32 neighbor predicate is evaluated
+// with separately rounded products, the contract the GPU search follows.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+
+struct orc_domain {
+  float min[2];
+  float max[2];
+  int32_t periodic[2];
+};
+
+}  // extern "C"
+
+namespace {
+
+// torch.remainder for floats (the reference's `%` and torch.remainder).
+template <class T>
+T py_remainder(T a, T b) {
+  T r = std::fmod(a, b);
+  if (r != T(0) && ((r < T(0)) != (b < T(0)))) r = r + b;
+  return r;
+}
+
+template <class T>
+struct DomT {
+  T min[2], L[2], half[2];
+  int periodic[2];
+  explicit DomT(const orc_domain& d) {
+    for (int k = 0; k < 2; ++k) {
+      min[k] = (T)d.min[k];
+      // float32 tensor arithmetic: max - min in the working type.
+      L[k] = (T)d.max[k] - (T)d.min[k];
+      half[k] = (T)(((double)d.max[k] - (double)d.min[k]) / 2.0);
+      periodic[k] = d.periodic[k] != 0;
+    }
+  }
+};
+
+// getPeriodicPosition, v2/modules/neighborhood.py:407-430.
+template <class T>
+T wrap(T x, const DomT<T>& d, int k) {
+  if (!d.periodic[k]) return x;
+  return py_remainder<T>(x - d.min[k], d.L[k]) + d.min[k];
+}
+
+// mod, v2/math.py:4-6.
+template <class T>
+T modoff(T x, const DomT<T>& d, int k) {
+  if (!d.periodic[k]) return x;
+  T ax = std::fabs(x);
+  if (ax > d.half[k]) {
+    T s = x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0));
+    return s * (py_remainder<T>(ax + d.min[k], d.L[k]) + d.min[k]);
+  }
+  return x;
+}
+
+template <class T>
+T pick_h(int mode, T hi, T hj) {
+  return mode == 0 ? hj : (mode == 1 ? hi : T(0.5) * (hi + hj));
+}
+
+// Wendland C2, 2-D (kernelFunctions/Wendland2.py:5-36); cpow clamps to [0,1]
+// (kernels.py:69-70).
+template <class T>
+T cpow(T q, int p) {
+  T t = std::min(std::max(q, T(0)), T(1));
+  T r = T(1);
+  for (int k = 0; k < p; ++k) r *= t;
+  return r;
+}
+template <class T>
+T C2() { return (T)(7.0 / 3.14159265358979323846); }
+template <class T>
+T kernelW(T q, T h) { return cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q) * C2<T>() / (h * h); }
+template <class T>
+T kernelDW(T q, T h) { return (T(-20) * q * cpow<T>(T(1) - q, 3)) * C2<T>() / (h * h * h); }
+
+template <class T>
+struct G {
+  T dx, dy, r, q, h, ex, ey, w, gx, gy;
+};
+
+// Pair geometry with the reference's conventions
+// (v2/modules/neighborhood.py:445-451, 587-591): d = x_i - x_j,
+// xhat = d / (r + 1e-7), q = r / h.
+template <class T>
+G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
+  G<T> g;
+  g.dx = modoff<T>(xi[0] - xj[0], d, 0);
+  g.dy = modoff<T>(xi[1] - xj[1], d, 1);
+  T a = g.dx * g.dx;
+  T b = g.dy * g.dy;
+  g.r = std::sqrt(a + b);
+  g.h = h;
+  g.q = g.r / h;
+  g.ex = g.dx / (g.r + (T)1e-7f);
+  g.ey = g.dy / (g.r + (T)1e-7f);
+  g.w = kernelW<T>(g.q, h);
+  T dw = kernelDW<T>(g.q, h);
+  g.gx = g.ex * dw;
+  g.gy = g.ey * dw;
+  return g;
+}
+
+// The neighbor predicate in float32: r = sqrt(dx*dx + dy*dy) with separately
+// rounded products (this file is compiled with -ffp-contract=off), q = r/h,
+// keep iff q <= 1 (v2/modules/neighborhood.py:445-453).
+bool keep_f32(const float* qi, float hi, const float* rj, float hj, const DomT<float>& d, int mode) {
+  float dx = modoff<float>(qi[0] - rj[0], d, 0);
+  float dy = modoff<float>(qi[1] - rj[1], d, 1);
+  float a = dx * dx;
+  float b = dy * dy;
+  float r = std::sqrt(a + b);
+  float h = pick_h<float>(mode, hi, hj);
+  return r / h <= 1.0f;
+}
+
+int cell_of(float x, const DomT<float>& d, int k, int nc, float hcell) {
+  float f = std::floor((x - d.min[k]) / hcell);
+  int c = (int)f;
+  if (f < 0.0f) c = 0;
+  if (c > nc - 1) c = nc - 1;
+  return c;
+}
+
+int nthreads_override = 0;
+
+void set_threads() {
+#ifdef _OPENMP
+  if (nthreads_override > 0) omp_set_num_threads(nthreads_override);
+#endif
+}
+
+template <class T>
+const T* row2(const T* a, int64_t i) { return a + 2 * i; }
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n) { nthreads_override = n; }
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return nthreads_override > 0 ? nthreads_override : omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// Grid of the cell-list search: one square side for both axes, periodic
+// axes tiled by floor(L/hcell) cells (the reference's compute_h,
+// neighborhoodDebug.ipynb cell 2), cells 0.1% wider than the search radius.
+int orc_grid_dims(const orc_domain* dom, float hsearch, int32_t* ncx, int32_t* ncy, float* hcell_out) {
+  const double hs = (double)hsearch * 1.001;
+  double hc = 0.0;
+  for (int k = 0; k < 2; ++k) {
+    double L = (double)dom->max[k] - (double)dom->min[k];
+    if (!(L > 0.0)) return 1;
+    double n = std::floor(L / hs);
+    if (n < 1.0) n = 1.0;
+    hc = std::max(hc, L / n);
+  }
+  float hcell = (float)hc;
+  if ((double)hcell < hs) hcell = std::nextafter(hcell, INFINITY);
+  int nc[2];
+  for (int k = 0; k < 2; ++k) {
+    double L = (double)dom->max[k] - (double)dom->min[k];
+    double n = dom->periodic[k] ? std::floor(L / (double)hcell + 1e-4) : std::ceil(L / (double)hcell);
+    if (dom->periodic[k] && n < 3.0) return 2;
+    if (n < 1.0) n = 1.0;
+    nc[k] = (int)n;
+  }
+  *ncx = nc[0];
+  *ncy = nc[1];
+  *hcell_out = hcell;
+  return 0;
+}
+
+// Counting sort by cell key, stable in the original index
+// (sortReferenceParticles + buildCompactHashMap, neighborhoodDebug.ipynb
+// cells 3-4, with a dense table instead of a hash map).
+int orc_grid_build(const orc_domain* dom, int32_t ncx, int32_t ncy, float hcell, const float* pos, int64_t n,
+                   int32_t* cell_start, int32_t* sort_index, float* sorted_pos) {
+  DomT<float> d(*dom);
+  const int64_t ncell = (int64_t)ncx * ncy;
+  std::vector<int32_t> key(n);
+  std::vector<int32_t> cnt(ncell + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    float wx = wrap<float>(pos[2 * i], d, 0), wy = wrap<float>(pos[2 * i + 1], d, 1);
+    int cx = cell_of(wx, d, 0, ncx, hcell), cy = cell_of(wy, d, 1, ncy, hcell);
+    key[i] = cy * ncx + cx;
+    cnt[key[i]]++;
+  }
+  cell_start[0] = 0;
+  for (int64_t c = 0; c < ncell; ++c) cell_start[c + 1] = cell_start[c] + cnt[c];
+  std::vector<int32_t> cur(cell_start, cell_start + ncell);
+  for (int64_t i = 0; i < n; ++i) sort_index[cur[key[i]]++] = (int32_t)i;
+  for (int64_t p = 0; p < n; ++p) {
+    int32_t i = sort_index[p];
+    sorted_pos[2 * p] = wrap<float>(pos[2 * i], d, 0);
+    sorted_pos[2 * p + 1] = wrap<float>(pos[2 * i + 1], d, 1);
+  }
+  return 0;
+}
+
+// Cell-list neighbor search in the canonical order (rows = queries, cells
+// dy-major then dx over offsets -1..1, particles in sorted order).  Pass
+// nbr == NULL to count only.  Queries are wrapped here (qpos raw).
+int orc_neighbors_cell(const orc_domain* dom, int32_t ncx, int32_t ncy, float hcell, const int32_t* cell_start,
+                       const float* spos, const float* sh, const float* qpos, const float* qh, int64_t nq,
+                       float h0, int32_t mode, int32_t* counts, const int32_t* offsets, int32_t* nbr) {
+  DomT<float> d(*dom);
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; ++i) {
+    float qi[2] = {wrap<float>(qpos[2 * i], d, 0), wrap<float>(qpos[2 * i + 1], d, 1)};
+    float hi = qh ? qh[i] : h0;
+    int cx = cell_of(qi[0], d, 0, ncx, hcell), cy = cell_of(qi[1], d, 1, ncy, hcell);
+    int c = 0;
+    int64_t o = offsets ? offsets[i] : 0;
+    for (int oy = -1; oy <= 1; ++oy) {
+      int yy = cy + oy;
+      if (d.periodic[1]) yy = (yy + ncy) % ncy;
+      else if (yy < 0 || yy >= ncy) continue;
+      for (int ox = -1; ox <= 1; ++ox) {
+        int xx = cx + ox;
+        if (d.periodic[0]) xx = (xx + ncx) % ncx;
+        else if (xx < 0 || xx >= ncx) continue;
+        int cell = yy * ncx + xx;
+        for (int j = cell_start[cell]; j < cell_start[cell + 1]; ++j) {
+          float hj = sh ? sh[j] : h0;
+          if (keep_f32(qi, hi, spos + 2 * (int64_t)j, hj, d, mode)) {
+            if (nbr) nbr[o++] = j;
+            ++c;
+          }
+        }
+      }
+    }
+    if (counts) counts[i] = c;
+  }
+  return 0;
+}
+
+// Brute force O(nq*nr) (v2/neighborhoodFallback/naive.py:9-35 with the
+// filter predicate of v2/modules/neighborhood.py:453).  Rows in query order,
+// j ascending.  nbr == NULL counts only.
+int orc_neighbors_brute(const orc_domain* dom, const float* qpos, const float* qh, int64_t nq, const float* rpos,
+                        const float* rh, int64_t nr, float h0, int32_t mode, int32_t* counts, const int32_t* offsets,
+                        int32_t* nbr) {
+  DomT<float> d(*dom);
+  set_threads();
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < nq; ++i) {
+    float qi[2] = {wrap<float>(qpos[2 * i], d, 0), wrap<float>(qpos[2 * i + 1], d, 1)};
+    float hi = qh ? qh[i] : h0;
+    int c = 0;
+    int64_t o = offsets ? offsets[i] : 0;
+    for (int64_t j = 0; j < nr; ++j) {
+      float rj[2] = {wrap<float>(rpos[2 * j], d, 0), wrap<float>(rpos[2 * j + 1], d, 1)};
+      float hj = rh ? rh[j] : h0;
+      if (keep_f32(qi, hi, rj, hj, d, mode)) {
+        if (nbr) nbr[o++] = (int32_t)j;
+        ++c;
+      }
+    }
+    if (counts) counts[i] = c;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pair operators over a CSR list; positions are the wrapped positions.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <class T>
+struct Csr {
+  int64_t nq;
+  const int32_t* off;
+  const int32_t* nbr;
+  const T* qpos;
+  const T* rpos;
+  const T* qh;
+  const T* rh;
+  T h0;
+  int mode;
+};
+
+template <class T>
+G<T> pg(const Csr<T>& c, int64_t i, int32_t j, const DomT<T>& d) {
+  T hi = c.qh ? c.qh[i] : c.h0;
+  T hj = c.rh ? c.rh[j] : c.h0;
+  return geom<T>(c.qpos + 2 * i, c.rpos + 2 * (int64_t)j, pick_h<T>(c.mode, hi, hj), d);
+}
+
+// sphDensityInterpolation, v2/sphOps.py:19-29.
+template <class T>
+void density(const Csr<T>& c, const DomT<T>& d, const T* rmass, const T* qmass, T* rho, T* vol) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T acc = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      acc += rmass[j] * g.w;
+    }
+    rho[i] = acc;
+    if (vol) vol[i] = qmass[i] / acc;
+  }
+}
+
+// v2/sphOps.py:6-149 (interpolate / gradient / divergence).
+template <class T>
+void sph_operation(const Csr<T>& c, const DomT<T>& d, int op, int gm, const T* qm, const T* rm, const T* qr,
+                   const T* rr, const T* qq, const T* rq, int W, T* out) {
+  (void)qm;
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T acc[4] = {0, 0, 0, 0};
+    T qi[2] = {0, 0};
+    for (int cc = 0; cc < W; ++cc) qi[cc] = qq ? qq[i * W + cc] : T(0);
+    T rho_i = qr ? qr[i] : T(1);
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T qj[2] = {0, 0};
+      for (int cc = 0; cc < W; ++cc) qj[cc] = rq[(int64_t)j * W + cc];
+      if (op == 0) {
+        T kk = rm[j] / rr[j] * g.w;
+        for (int cc = 0; cc < W; ++cc) acc[cc] += qj[cc] * kk;
+      } else if (gm == 3) {
+        T kx = rm[j] * g.gx, ky = rm[j] * g.gy;
+        T irj = T(1) / (rr[j] * rr[j]), iri = T(1) / (rho_i * rho_i);
+        if (op == 1) {
+          for (int cc = 0; cc < W; ++cc) {
+            T a = qi[cc] * iri + qj[cc] * irj;
+            acc[2 * cc] += a * kx;
+            acc[2 * cc + 1] += a * ky;
+          }
+        } else {
+          acc[0] += (qi[0] * iri + qj[0] * irj) * kx + (qi[1] * iri + qj[1] * irj) * ky;
+        }
+      } else {
+        T V = rm[j] / rr[j];
+        T kx = V * g.gx, ky = V * g.gy;
+        if (op == 1) {
+          for (int cc = 0; cc < W; ++cc) {
+            T a = gm == 1 ? qi[cc] - qj[cc] : (gm == 2 ? qi[cc] + qj[cc] : qj[cc]);
+            acc[2 * cc] += a * kx;
+            acc[2 * cc + 1] += a * ky;
+          }
+        } else {
+          T ax = gm == 1 ? qj[0] - qi[0] : (gm == 2 ? qj[0] + qi[0] : qj[0]);
+          T ay = gm == 1 ? qj[1] - qi[1] : (gm == 2 ? qj[1] + qi[1] : qj[1]);
+          acc[0] += ax * kx + ay * ky;
+        }
+      }
+    }
+    if (op == 0) {
+      for (int cc = 0; cc < W; ++cc) out[i * W + cc] = acc[cc];
+    } else if (op == 1) {
+      for (int cc = 0; cc < 2 * W; ++cc) out[i * 2 * W + cc] = gm == 3 ? rho_i * acc[cc] : acc[cc];
+    } else {
+      out[i] = gm == 3 ? rho_i * acc[0] : acc[0];
+    }
+  }
+}
+
+struct ViscP {
+  int on, scheme, pi_switch;
+  double alpha, cs, rho0, eps;
+};
+
+// computeAlpha (v2/modules/dfsph.py:7-27) for both solves, divergence of v
+// (computeSourceTerm, :30-33) and the delta-SPH viscosity
+// (v2/modules/viscosity.py:123-170 inviscid, :200-218 viscid).
+template <class T>
+void dfsph_prepare(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* areas, const T* vol, const T* rho, T dt,
+                   T rho0, const ViscP& vp, T* alpha_div, T* alpha_den, T* div_src, T* visc) {
+  const T dt2 = (T)((double)dt * (double)dt);
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T k1x = 0, k1y = 0, k2 = 0, dv = 0, vsx = 0, vsy = 0;
+    const T hi = c.qh ? c.qh[i] : c.h0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T Vj = vol[j];
+      k1x += Vj * g.gx;
+      k1y += Vj * g.gy;
+      T c2 = (Vj * Vj) / (areas[j] * rho0);
+      k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
+      T kx = Vj * g.gx, ky = Vj * g.gy;
+      dv += (vel[2 * j] - vel[2 * i]) * kx + (vel[2 * j + 1] - vel[2 * i + 1]) * ky;
+      if (vp.on) {
+        T vx = vel[2 * i] - vel[2 * j], vy = vel[2 * i + 1] - vel[2 * j + 1];
+        T rij = g.q * g.h;
+        T vr = vx * g.ex + vy * g.ey;
+        T pi = vr / (rij + (T)vp.eps * g.h * g.h);
+        if (vp.pi_switch) pi = vr < T(0) ? pi : T(0);
+        vsx += (Vj * pi) * g.gx;
+        vsy += (Vj * pi) * g.gy;
+      }
+    }
+    T Vi = vol[i];
+    T mass_i = areas[i] * rho0;
+    T k1sq = k1x * k1x + k1y * k1y;
+    T fac = -dt2 * Vi;
+    alpha_den[i] = fac / mass_i * k1sq + fac * k2;
+    fac = -dt * Vi;
+    alpha_div[i] = fac / mass_i * k1sq + fac * k2;
+    div_src[i] = dv;
+    if (vp.on && visc) {
+      T s;
+      if (vp.scheme == 0)
+        s = (T)vp.alpha * hi * (T)vp.cs * (T)vp.rho0 / rho[i];
+      else
+        s = (T)(2.0 * 4.0 * vp.alpha * vp.rho0) / rho[i];
+      visc[2 * i] = s * vsx;
+      visc[2 * i + 1] = s * vsy;
+    }
+  }
+}
+
+// computePressureAcceleration (v2/modules/dfsph.py:35-37).
+template <class T>
+void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, const T* p, T* acc) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T ax = 0, ay = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T s = p[i] + p[j];
+      ax += s * (vol[j] * g.gx);
+      ay += s * (vol[j] * g.gy);
+    }
+    acc[2 * i] = -ax / rho[i];
+    acc[2 * i + 1] = -ay / rho[i];
+  }
+}
+
+// updatePressure (v2/modules/dfsph.py:39-50); returns sum of
+// clamp(r/rho0, min=-thr) (accumulated in double).
+template <class T>
+double sweep_b(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* acc, const T* src, const T* alpha, T dtf,
+               T omega, T rho0, T thr, int clampp, T* p, T* resid) {
+  double tot = 0.0;
+  set_threads();
+#pragma omp parallel for schedule(static) reduction(+ : tot)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T sum = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      sum += (acc[2 * j] - acc[2 * i]) * (vol[j] * g.gx) + (acc[2 * j + 1] - acc[2 * i + 1]) * (vol[j] * g.gy);
+    }
+    T K = -dtf * sum;
+    T s = src[i];
+    T res = K - s;
+    T pn = p[i] + omega * (s - K) / alpha[i];
+    if (clampp) pn = std::max(pn, T(0));
+    p[i] = pn;
+    if (resid) resid[i] = res;
+    tot += (double)std::max(res / rho0, -thr);
+  }
+  return tot;
+}
+
+// Source terms (v2/modules/dfsph.py:57-65).
+template <class T>
+void source(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* adv, const T* vol, const T* areas, T dt,
+            int density_src, T* src) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T ax = adv ? adv[2 * i] : T(0), ay = adv ? adv[2 * i + 1] : T(0);
+    T vix = vel[2 * i] + dt * ax, viy = vel[2 * i + 1] + dt * ay;
+    T dv = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T bx = adv ? adv[2 * j] : T(0), by = adv ? adv[2 * j + 1] : T(0);
+      T vjx = vel[2 * j] + dt * bx, vjy = vel[2 * j + 1] + dt * by;
+      T kx = vol[j] * g.gx, ky = vol[j] * g.gy;
+      dv += (vjx - vix) * kx + (vjy - viy) * ky;
+    }
+    src[i] = density_src ? (T(1) - areas[i] / vol[i]) - dt * dv : dv;
+  }
+}
+
+template <class T>
+Csr<T> make_csr(int64_t nq, const int32_t* off, const int32_t* nbr, const T* qpos, const T* rpos, const T* qh,
+                const T* rh, T h0, int mode) {
+  Csr<T> c;
+  c.nq = nq;
+  c.off = off;
+  c.nbr = nbr;
+  c.qpos = qpos;
+  c.rpos = rpos;
+  c.qh = qh;
+  c.rh = rh;
+  c.h0 = h0;
+  c.mode = mode;
+  return c;
+}
+
+template <class T>
+int dfsph_solve(const Csr<T>& c, const DomT<T>& d, T dt, T rho0, T omega, T thr, int min_iters, int max_iters,
+                int clampp, int density_src, const T* vel, const T* adv, const T* rho, const T* vol, const T* areas,
+                const T* alpha, const T* src0, T* p, T* acc, T* resid, int32_t* iters_out, double* err_out,
+                double* err0_out) {
+  const int64_t n = c.nq;
+  std::vector<T> src(n);
+  if (density_src || adv)
+    source<T>(c, d, vel, adv, vol, areas, dt, density_src, src.data());
+  else
+    std::copy(src0, src0 + n, src.begin());
+  const T dtf = density_src ? (T)((double)dt * (double)dt) : dt;
+  // while i < maxIters and (i < minIters or error > errorThreshold)
+  // (v2/modules/dfsph.py:88-103)
+  int it = 0;
+  double err = 0.0, err0 = 0.0;
+  while (it < max_iters && (it < min_iters || err > (double)thr)) {
+    sweep_a<T>(c, d, vol, rho, p, acc);
+    double tot = sweep_b<T>(c, d, vol, acc, src.data(), alpha, dtf, omega, rho0, thr, clampp, p, resid);
+    err = tot / (double)n;
+    if (it == 0) err0 = err;
+    ++it;
+  }
+  sweep_a<T>(c, d, vol, rho, p, acc);
+  *iters_out = it;
+  *err_out = err;
+  *err0_out = err0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+#define ORC_CSR(T) make_csr<T>(nq, off, nbr, qpos, rpos, qh, rh, h0, mode)
+
+int orc_density_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
+                    const float* rpos, const float* qh, const float* rh, float h0, int32_t mode, const float* rmass,
+                    const float* qmass, float* rho, float* vol) {
+  density<float>(ORC_CSR(float), DomT<float>(*dom), rmass, qmass, rho, vol);
+  return 0;
+}
+int orc_density_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const double* qpos,
+                    const double* rpos, const double* qh, const double* rh, double h0, int32_t mode,
+                    const double* rmass, const double* qmass, double* rho, double* vol) {
+  density<double>(ORC_CSR(double), DomT<double>(*dom), rmass, qmass, rho, vol);
+  return 0;
+}
+
+int orc_sph_operation_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                          const float* qpos, const float* rpos, const float* qh, const float* rh, float h0,
+                          int32_t mode, int32_t op, int32_t gm, const float* qm, const float* rm, const float* qr,
+                          const float* rr, const float* qq, const float* rq, int32_t width, float* out) {
+  sph_operation<float>(ORC_CSR(float), DomT<float>(*dom), op, gm, qm, rm, qr, rr, qq, rq, width, out);
+  return 0;
+}
+
+int orc_dfsph_prepare_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                          const float* qpos, const float* qh, float h0, int32_t mode, const float* vel,
+                          const float* areas, const float* vol, const float* rho, float dt, float rho0,
+                          int32_t visc_on, int32_t visc_scheme, float v_alpha, float v_cs, float v_rho0, float v_eps,
+                          int32_t pi_switch, float* alpha_div, float* alpha_den, float* div_src, float* visc) {
+  ViscP vp{visc_on, visc_scheme, pi_switch, v_alpha, v_cs, v_rho0, v_eps};
+  const float* rpos = qpos;
+  const float* rh = qh;
+  dfsph_prepare<float>(ORC_CSR(float), DomT<float>(*dom), vel, areas, vol, rho, dt, rho0, vp, alpha_div, alpha_den,
+                       div_src, visc);
+  return 0;
+}
+int orc_dfsph_prepare_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                          const double* qpos, const double* qh, double h0, int32_t mode, const double* vel,
+                          const double* areas, const double* vol, const double* rho, double dt, double rho0,
+                          int32_t visc_on, int32_t visc_scheme, double v_alpha, double v_cs, double v_rho0,
+                          double v_eps, int32_t pi_switch, double* alpha_div, double* alpha_den, double* div_src,
+                          double* visc) {
+  ViscP vp{visc_on, visc_scheme, pi_switch, v_alpha, v_cs, v_rho0, v_eps};
+  const double* rpos = qpos;
+  const double* rh = qh;
+  dfsph_prepare<double>(ORC_CSR(double), DomT<double>(*dom), vel, areas, vol, rho, dt, rho0, vp, alpha_div,
+                        alpha_den, div_src, visc);
+  return 0;
+}
+
+int orc_dfsph_solve_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
+                        const float* qh, float h0, int32_t mode, float dt, float rho0, float omega, float thr,
+                        int32_t min_iters, int32_t max_iters, int32_t clampp, int32_t density_src, const float* vel,
+                        const float* adv, const float* rho, const float* vol, const float* areas, const float* alpha,
+                        const float* src0, float* p, float* acc, float* resid, int32_t* iters, double* err,
+                        double* err0) {
+  const float* rpos = qpos;
+  const float* rh = qh;
+  return dfsph_solve<float>(ORC_CSR(float), DomT<float>(*dom), dt, rho0, omega, thr, min_iters, max_iters, clampp,
+                            density_src, vel, adv, rho, vol, areas, alpha, src0, p, acc, resid, iters, err, err0);
+}
+int orc_dfsph_solve_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                        const double* qpos, const double* qh, double h0, int32_t mode, double dt, double rho0,
+                        double omega, double thr, int32_t min_iters, int32_t max_iters, int32_t clampp,
+                        int32_t density_src, const double* vel, const double* adv, const double* rho,
+                        const double* vol, const double* areas, const double* alpha, const double* src0, double* p,
+                        double* acc, double* resid, int32_t* iters, double* err, double* err0) {
+  const double* rpos = qpos;
+  const double* rh = qh;
+  return dfsph_solve<double>(ORC_CSR(double), DomT<double>(*dom), dt, rho0, omega, thr, min_iters, max_iters,
+                             clampp, density_src, vel, adv, rho, vol, areas, alpha, src0, p, acc, resid, iters, err,
+                             err0);
+}
+
+// Wendland C2 values for a list of (q, h): W and dW/dr (kernelFunctions/
+// Wendland2.py:30-36), in float64.
+int orc_wendland(const double* q, const double* h, int64_t n, double* w, double* dw) {
+  for (int64_t i = 0; i < n; ++i) {
+    w[i] = kernelW<double>(q[i], h[i]);
+    dw[i] = kernelDW<double>(q[i], h[i]);
+  }
+  return 0;
+}
+
+// Semi-implicit Euler, v2/modules/integration.py:111-138; dudt summed as
+// v2/simulationSchemes/dfsph.py:209 (a_p + g + visc + a_div).
+int orc_integrate_sie_f32(float* pos, float* vel, const float* a0, const float* a1, const float* a2, const float* a3,
+                          const float* g, int64_t n, float dt) {
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 2; ++k) {
+      float d = a0 ? a0[2 * i + k] : 0.0f;
+      if (g) d = d + g[k];
+      if (a1) d = d + a1[2 * i + k];
+      if (a2) d = d + a2[2 * i + k];
+      if (a3) d = d + a3[2 * i + k];
+      float v = vel[2 * i + k] + d * dt;
+      vel[2 * i + k] = v;
+      pos[2 * i + k] = pos[2 * i + k] + v * dt;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+"""numpy front end of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py -- never by the diffsph_amd product path.
+
+All functions take and return numpy arrays on the host.  `build()` compiles
+the shared library with oracle/Makefile when it is missing or stale.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+SRC = os.path.join(HERE, "oracle.cpp")
+
+
+class Domain(C.Structure):
+    _fields_ = [("min", C.c_float * 2), ("max", C.c_float * 2), ("periodic", C.c_int32 * 2)]
+
+
+def make_domain(mn, mx, periodic) -> Domain:
+    d = Domain()
+    for k in range(2):
+        d.min[k] = float(mn[k])
+        d.max[k] = float(mx[k])
+        d.periodic[k] = 1 if periodic[k] else 0
+    return d
+
+
+def build(force: bool = False) -> str:
+    stale = not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC)
+    if force or stale:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle inputs must be contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(C.c_int(n))
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
+@dataclass
+class Grid:
+    ncx: int
+    ncy: int
+    hcell: float
+    cell_start: np.ndarray
+    sort_index: np.ndarray
+    sorted_pos: np.ndarray
+
+
+def grid_dims(dom: Domain, hsearch: float):
+    ncx, ncy, hc = C.c_int32(), C.c_int32(), C.c_float()
+    rc = lib().orc_grid_dims(C.byref(dom), C.c_float(hsearch), C.byref(ncx), C.byref(ncy), C.byref(hc))
+    if rc != 0:
+        raise ValueError("grid_dims failed (periodic axis needs >= 3 cells)")
+    return ncx.value, ncy.value, hc.value
+
+
+def grid_build(dom: Domain, pos: np.ndarray, hsearch: float) -> Grid:
+    pos = f32(pos)
+    n = pos.shape[0]
+    ncx, ncy, hc = grid_dims(dom, hsearch)
+    cs = np.zeros(ncx * ncy + 1, np.int32)
+    si = np.zeros(n, np.int32)
+    sp = np.zeros((n, 2), np.float32)
+    lib().orc_grid_build(C.byref(dom), C.c_int32(ncx), C.c_int32(ncy), C.c_float(hc), _p(pos), C.c_int64(n),
+                         _p(cs), _p(si), _p(sp))
+    return Grid(ncx, ncy, hc, cs, si, sp)
+
+
+def neighbors_cell(dom: Domain, g: Grid, qpos, qh=None, h0=0.0, mode=0, sorted_h=None):
+    """CSR (counts, offsets[nq+1], nbr) in canonical order; nbr are sorted slots."""
+    qpos = f32(qpos)
+    qh = f32(qh)
+    sorted_h = f32(sorted_h)
+    nq = qpos.shape[0]
+    counts = np.zeros(nq, np.int32)
+    L = lib()
+    args = (C.byref(dom), C.c_int32(g.ncx), C.c_int32(g.ncy), C.c_float(g.hcell), _p(g.cell_start),
+            _p(g.sorted_pos), _p(sorted_h), _p(qpos), _p(qh), C.c_int64(nq), C.c_float(h0), C.c_int32(mode))
+    L.orc_neighbors_cell(*args, _p(counts), None, None)
+    off = np.zeros(nq + 1, np.int32)
+    np.cumsum(counts, out=off[1:])
+    nbr = np.zeros(max(int(off[-1]), 1), np.int32)
+    L.orc_neighbors_cell(*args, None, _p(off), _p(nbr))
+    return counts, off, nbr[: int(off[-1])]
+
+
+def neighbors_brute(dom: Domain, qpos, rpos, qh=None, rh=None, h0=0.0, mode=0):
+    """Brute-force CSR (rows in query order, j ascending, original indices)."""
+    qpos, rpos, qh, rh = f32(qpos), f32(rpos), f32(qh), f32(rh)
+    nq, nr = qpos.shape[0], rpos.shape[0]
+    counts = np.zeros(nq, np.int32)
+    L = lib()
+    args = (C.byref(dom), _p(qpos), _p(qh), C.c_int64(nq), _p(rpos), _p(rh), C.c_int64(nr), C.c_float(h0),
+            C.c_int32(mode))
+    L.orc_neighbors_brute(*args, _p(counts), None, None)
+    off = np.zeros(nq + 1, np.int32)
+    np.cumsum(counts, out=off[1:])
+    nbr = np.zeros(max(int(off[-1]), 1), np.int32)
+    L.orc_neighbors_brute(*args, None, _p(off), _p(nbr))
+    return counts, off, nbr[: int(off[-1])]
+
+
+def density(dom, off, nbr, qpos, rpos, rmass, qmass=None, h0=0.0, qh=None, rh=None, mode=0, dtype=np.float32):
+    nq = qpos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    qpos, rpos, rmass, qmass, qh, rh = cv(qpos), cv(rpos), cv(rmass), cv(qmass), cv(qh), cv(rh)
+    rho = np.zeros(nq, dtype)
+    vol = np.zeros(nq, dtype) if qmass is not None else None
+    fn = lib().orc_density_f32 if dtype == np.float32 else lib().orc_density_f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    fn(C.byref(dom), C.c_int64(nq), _p(off), _p(nbr), _p(qpos), _p(rpos), _p(qh), _p(rh), hc(h0), C.c_int32(mode),
+       _p(rmass), _p(qmass), _p(rho), _p(vol))
+    return rho, vol
+
+
+def sph_operation(dom, off, nbr, qpos, rpos, op, gm, rm, rr, rq, width, qm=None, qr=None, qq=None, h0=0.0,
+                  qh=None, rh=None, mode=0):
+    nq = qpos.shape[0]
+    qpos, rpos = f32(qpos), f32(rpos)
+    arrs = [f32(a) for a in (qm, rm, qr, rr, qq, rq, qh, rh)]
+    qm, rm, qr, rr, qq, rq, qh, rh = arrs
+    if op == 0:
+        out = np.zeros((nq, width), np.float32)
+    elif op == 1:
+        out = np.zeros((nq, width, 2), np.float32)
+    else:
+        out = np.zeros(nq, np.float32)
+    lib().orc_sph_operation_f32(C.byref(dom), C.c_int64(nq), _p(off), _p(nbr), _p(qpos), _p(rpos), _p(qh), _p(rh),
+                                C.c_float(h0), C.c_int32(mode), C.c_int32(op), C.c_int32(gm), _p(qm), _p(rm),
+                                _p(qr), _p(rr), _p(qq), _p(rq), C.c_int32(width), _p(out))
+    return out
+
+
+@dataclass
+class Visc:
+    scheme: int = 0      # 0 deltaSPH_inviscid, 1 deltaSPH_viscid
+    alpha: float = 0.01  # alpha, or nu for the viscid scheme
+    cs: float = 10.0
+    rho0: float = 1.0
+    eps: float = 1e-6
+    pi_switch: bool = False
+
+
+def dfsph_prepare(dom, off, nbr, pos, vel, areas, vol, rho, dt, rho0, visc: Visc | None, h0, dtype=np.float32):
+    n = pos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    pos, vel, areas, vol, rho = cv(pos), cv(vel), cv(areas), cv(vol), cv(rho)
+    ad = np.zeros(n, dtype)
+    an = np.zeros(n, dtype)
+    ds = np.zeros(n, dtype)
+    vs = np.zeros((n, 2), dtype)
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    fn = lib().orc_dfsph_prepare_f32 if dtype == np.float32 else lib().orc_dfsph_prepare_f64
+    v = visc or Visc()
+    fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(pos), None, hc(h0), C.c_int32(0), _p(vel), _p(areas),
+       _p(vol), _p(rho), hc(dt), hc(rho0), C.c_int32(1 if visc else 0), C.c_int32(v.scheme), hc(v.alpha), hc(v.cs),
+       hc(v.rho0), hc(v.eps), C.c_int32(1 if v.pi_switch else 0), _p(ad), _p(an), _p(ds), _p(vs))
+    return ad, an, ds, vs
+
+
+def dfsph_solve(dom, off, nbr, pos, h0, dt, rho0, omega, thr, min_iters, max_iters, clamp, density_src, vel, adv,
+                rho, vol, areas, alpha, src0, p0, dtype=np.float32):
+    n = pos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    pos, vel, adv, rho, vol, areas, alpha, src0 = (cv(a) for a in (pos, vel, adv, rho, vol, areas, alpha, src0))
+    p = cv(p0).copy()
+    acc = np.zeros((n, 2), dtype)
+    res = np.zeros(n, dtype)
+    it = C.c_int32()
+    err, err0 = C.c_double(), C.c_double()
+    fn = lib().orc_dfsph_solve_f32 if dtype == np.float32 else lib().orc_dfsph_solve_f64
+    fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(pos), None, hc(h0), C.c_int32(0), hc(dt), hc(rho0),
+       hc(omega), hc(thr), C.c_int32(min_iters), C.c_int32(max_iters), C.c_int32(1 if clamp else 0),
+       C.c_int32(1 if density_src else 0), _p(vel), _p(adv), _p(rho), _p(vol), _p(areas), _p(alpha), _p(src0),
+       _p(p), _p(acc), _p(res), C.byref(it), C.byref(err), C.byref(err0))
+    return p, acc, res, it.value, err.value, err0.value
+
+
+def integrate_sie(pos, vel, a0, a1, a2, a3, g, dt):
+    pos, vel = f32(pos).copy(), f32(vel).copy()
+    a0, a1, a2, a3 = f32(a0), f32(a1), f32(a2), f32(a3)
+    g = f32(g) if g is not None else None
+    lib().orc_integrate_sie_f32(_p(pos), _p(vel), _p(a0), _p(a1), _p(a2), _p(a3), _p(g), C.c_int64(pos.shape[0]),
+                                C.c_float(dt))
+    return pos, vel
+
+
+# ---------------------------------------------------------------------------
+# Composed DFSPH step (v2/simulationSchemes/dfsph.py:148-227 with the
+# semi-implicit Euler of v2/modules/integration.py:111-138).
+# ---------------------------------------------------------------------------
+@dataclass
+class DfsphState:
+    pos: np.ndarray       # [n,2] (unwrapped, as the reference keeps them)
+    vel: np.ndarray
+    mass: np.ndarray
+    areas: np.ndarray
+    uid: np.ndarray
+    p_div: np.ndarray | None = None
+    p_den: np.ndarray | None = None
+
+
+def dfsph_step(st: DfsphState, dom: Domain, h: float, dt: float, rho0: float = 1.0, omega=0.3, thr_div=1e-2,
+               thr_den=5e-4, min_iters=2, max_iters=256, visc: Visc | None = Visc(), warm=0.75, gravity=None,
+               reorder=True, dtype=np.float32):
+    """One DFSPH step in float32 (or float64 for the error reference).
+
+    With ``reorder`` the state is sorted into cell order first, exactly as
+    the GPU solver keeps it, so both walk every neighbor list in the same
+    order.  Returns (state, info dict)."""
+    g = grid_build(dom, st.pos, h)
+    if reorder:
+        si = g.sort_index
+        st = DfsphState(st.pos[si], st.vel[si], st.mass[si], st.areas[si], st.uid[si],
+                        None if st.p_div is None else st.p_div[si], None if st.p_den is None else st.p_den[si])
+        g = grid_build(dom, st.pos, h)
+    cnt, off, nbr_s = neighbors_cell(dom, g, st.pos, h0=h)
+    # neighbors_cell returns sorted slots of g; map to row indices of st
+    nbr = g.sort_index[nbr_s].astype(np.int32)
+    wpos = np.empty_like(g.sorted_pos)
+    wpos[g.sort_index] = g.sorted_pos
+    cv = f32 if dtype == np.float32 else f64
+    wpos = cv(wpos)
+    rho, vol = density(dom, off, nbr, wpos, wpos, st.mass, st.mass, h0=h, dtype=dtype)
+    ad, an, ds, vs = dfsph_prepare(dom, off, nbr, wpos, st.vel, st.areas, vol, rho, dt, rho0, visc, h, dtype)
+    if visc is None:
+        vs = np.zeros_like(vs)
+    n = st.pos.shape[0]
+    p0 = (dtype(warm) * cv(st.p_div)) if st.p_div is not None else np.zeros(n, dtype)
+    p_div, a_div, _, it_div, e_div, _ = dfsph_solve(dom, off, nbr, wpos, h, dt, rho0, omega, thr_div / rho0,
+                                                    min_iters, max_iters, True, False, st.vel, None, rho, vol,
+                                                    st.areas, ad, ds, p0, dtype)
+    gv = np.zeros(2, dtype) if gravity is None else np.asarray(gravity, dtype)
+    adv = (gv[None, :] + vs) + a_div
+    p0 = (dtype(warm) * cv(st.p_den)) if st.p_den is not None else np.zeros(n, dtype)
+    p_den, a_den, res, it_den, e_den, e_den0 = dfsph_solve(dom, off, nbr, wpos, h, dt, rho0, omega, thr_den / rho0,
+                                                           min_iters, max_iters, True, True, st.vel, adv, rho, vol,
+                                                           st.areas, an, ds, p0, dtype)
+    if dtype == np.float32:
+        pos, vel = integrate_sie(st.pos, st.vel, a_den, vs, a_div, None, gravity, dt)
+    else:
+        dudt = ((a_den + gv[None, :]) + vs) + a_div
+        vel = cv(st.vel) + dudt * dt
+        pos = cv(st.pos) + vel * dt
+    new = DfsphState(pos, vel, st.mass, st.areas, st.uid, p_div, p_den)
+    info = dict(rho=rho, vol=vol, alpha_div=ad, alpha_den=an, div_src=ds, visc=vs, a_div=a_div, a_den=a_den,
+                p_div=p_div, p_den=p_den, it_div=it_div, it_den=it_den, err_div=e_div, err_den=e_den,
+                err_den0=e_den0, num_pairs=int(off[-1]), uid=st.uid)
+    return new, info
+
+
+def wendland(q, h):
+    """(W, dW/dr) of the Wendland C2 kernel in float64."""
+    q = f64(np.atleast_1d(q))
+    h = f64(np.broadcast_to(np.atleast_1d(h), q.shape))
+    w = np.zeros_like(q)
+    dw = np.zeros_like(q)
+    lib().orc_wendland(_p(q), _p(h), C.c_int64(q.size), _p(w), _p(dw))
+    return w, dw

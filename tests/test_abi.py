@@ -1,0 +1,65 @@
+"""The C-ABI library loads and exports every entry point include/diffsph.h
+declares (no compute: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "diffsph.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(dsph_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "dsph_neighbor_count" in names and "dsph_dfsph_solve" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from diffsph_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libdiffsph_amd.so is not built (run python -m diffsph_amd.build)")
+    h = C.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(h, n)]
+    assert not missing, missing
+    # every declared function has a ctypes signature in the binding
+    assert not [n for n in declared_functions() if n not in _lib.SIGNATURES]
+
+
+def test_host_only_entry_points_run_without_gpu():
+    from diffsph_amd import _lib
+    L = _lib.lib()
+    assert L.dsph_abi_version() >= 1
+    assert L.dsph_status_string(5) == b"output buffer or workspace too small"
+    dom = _lib.make_domain((-1, -1), (1, 1), (True, True))
+    g = _lib.Grid()
+    assert L.dsph_grid_dims(C.byref(dom), C.c_float(2.0 / 1024 * 3.989), C.byref(g)) == 0
+    assert (g.ncell[0], g.ncell[1]) == (256, 256) and g.hcell == pytest.approx(2.0 / 256)
+    # invalid arguments are rejected with a status, not a fault
+    assert L.dsph_grid_dims(C.byref(dom), C.c_float(0.9), C.byref(g)) == 2
+    assert L.dsph_grid_dims(C.byref(dom), C.c_float(-1.0), C.byref(g)) == 1
+    assert L.dsph_neighbor_count(None, None, None, None, None, 0, None, None, 0, 0, 0.1, 0, None, None) == 1
+    assert L.dsph_scan_workspace(10_000_000) >= 4 * (10_000_000 // 2048)
+
+
+def test_grid_dims_match_oracle(orc):
+    from diffsph_amd import _lib
+    L = _lib.lib()
+    for (mn, mx, per, h) in [((-1, -1), (1, 1), (True, True), 0.0078),
+                             ((0, 0), (4, 2), (False, False), 0.0156),
+                             ((-1, -1), (1, 1), (True, True), 2.0 / 12 * 3.989),
+                             ((0, 0), (3.7, 1.3), (True, True), 0.05)]:
+        dom = _lib.make_domain(mn, mx, per)
+        g = _lib.Grid()
+        assert L.dsph_grid_dims(C.byref(dom), C.c_float(h), C.byref(g)) == 0
+        ncx, ncy, hc = orc.grid_dims(orc.make_domain(mn, mx, per), h)
+        assert (g.ncell[0], g.ncell[1], g.hcell) == (ncx, ncy, hc)
+        assert g.hcell >= h

@@ -1,0 +1,111 @@
+"""GPU parity of the DFSPH step (v2/simulationSchemes/dfsph.py) vs the oracle.
+
+Both keep the state in cell order, so every neighbor list is walked in the
+same order; per-step fields must agree within 1e-5 relative (float32), and
+the iteration counts of both solves must be identical.  The float64 oracle
+gives the error of the float32 GPU fields, which is reported.
+"""
+import numpy as np
+import pytest
+import torch
+
+from diffsph_amd import sampling
+from diffsph_amd.dfsph import DFSPHConfig, DFSPHSolver
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _rel(gpu, ref):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return np.abs(gpu - ref).max() / max(np.abs(ref).max(), 1e-30)
+
+
+def _setup(nside, seed, jit=0.1):
+    d = sampling.dfsph_random_flow(nside, seed=seed, jitter_width=jit)
+    dt = 3.0 * d["dx"] / 10.0  # acoustic CFL 1.5 with c_s = 10 |u|max
+    return d, dt
+
+
+def _solver(d, dt, **kw):
+    dev = torch.device("cuda:0")
+    cfg = DFSPHConfig(dt=dt, support=d["h"], **kw)
+    T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+    return DFSPHSolver(T(d["pos"]), T(d["vel"]), T(d["mass"]), T(d["area"]), cfg)
+
+
+@pytest.mark.parametrize("nside,seed,steps", [(64, 0, 3), (128, 1, 2)])
+def test_dfsph_steps_match_oracle(orc, nside, seed, steps):
+    d, dt = _setup(nside, seed)
+    print(f"case nside={nside} seed={seed} steps={steps}", flush=True)
+    s = _solver(d, dt)
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = d["pos"].shape[0]
+    ost = orc.DfsphState(d["pos"].copy(), d["vel"].copy(), d["mass"].copy(), d["area"].copy(),
+                         np.arange(n, dtype=np.int32))
+    for k in range(steps):
+        stats = s.step()
+        ost, info = orc.dfsph_step(ost, dom, d["h"], dt)
+        # same cell order on both sides
+        np.testing.assert_array_equal(s.uid.cpu().numpy(), info["uid"])
+        assert stats.num_pairs == info["num_pairs"]
+        assert stats.div_iters == info["it_div"], (stats.div_iters, info["it_div"])
+        assert stats.den_iters == info["it_den"], (stats.den_iters, info["it_den"])
+        for name, gpu, ref in [("rho", s.rho, info["rho"]), ("alpha_den", s.alpha_den, info["alpha_den"]),
+                               ("visc", s.visc, info["visc"]), ("p_div", s.p_div, info["p_div"]),
+                               ("p_den", s.p_den, info["p_den"]), ("a_den", s.a_den, info["a_den"]),
+                               ("vel", s.vel, ost.vel), ("pos", s.pos, ost.pos)]:
+            e = _rel(gpu.cpu().numpy(), ref)
+            print(f"step {k} {name}: rel err vs oracle {e:.2e}", flush=True)
+            assert e <= RTOL, (k, name, e)
+        assert abs(stats.den_error - info["err_den"]) <= 1e-6 * max(1e-12, abs(info["err_den"])) + 1e-9
+
+
+def test_dfsph_float64_error_reported(orc):
+    """Error of the float32 GPU step against a float64 run of the same ops."""
+    d, dt = _setup(64, 3)
+    s = _solver(d, dt)
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = d["pos"].shape[0]
+    ost = orc.DfsphState(d["pos"].copy(), d["vel"].copy(), d["mass"].copy(), d["area"].copy(),
+                         np.arange(n, dtype=np.int32))
+    s.step()
+    _, info = orc.dfsph_step(ost, dom, d["h"], dt, dtype=np.float64)
+    errs = {k: _rel(v.cpu().numpy(), info[r]) for k, v, r in [("density", s.rho, "rho"), ("pressure", s.p_den, "p_den"),
+                                                           ("accel", s.a_den, "a_den")]}
+    print("float32 GPU vs float64 oracle:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert errs["density"] < 1e-5
+    # pressure from an iterative solve: iteration counts may differ by one
+    assert errs["pressure"] < 5e-2
+
+
+def test_dfsph_1m_step_properties():
+    """The 1M-particle workload: one step converges both solves and keeps
+    the density error small."""
+    d, dt = _setup(1024, 0)
+    s = _solver(d, dt)
+    for k in range(2):
+        st = s.step()
+        print(f"1M step {k}: div it {st.div_iters} err {st.div_error:.3e}; den it {st.den_iters} "
+              f"err {st.den_error:.3e}; pairs {st.num_pairs}", flush=True)
+        assert st.den_iters < 256 and st.den_error <= 5e-4
+        assert st.div_iters < 256 and st.div_error <= 1e-2
+    mean_e, max_e = s.density_error()
+    print(f"density error mean {mean_e:.3e} max {max_e:.3e}", flush=True)
+    assert mean_e < 1e-2
+    # UID permutation is a bijection
+    assert torch.equal(torch.sort(s.uid.long()).values, torch.arange(s.n, device=s.uid.device))
+
+
+def test_dfsph_solver_limits_and_nan_guard():
+    """maxIters caps the loop (reference dfsph.py:88); minIters=0 with error 0
+    runs no iteration, as the reference's while-condition does."""
+    d, dt = _setup(48, 5)
+    s = _solver(d, dt, max_iters=2, threshold_den=0.0, threshold_div=0.0)
+    st = s.step()
+    assert st.den_iters == 2 and st.div_iters == 2
+    s2 = _solver(d, dt, min_iters=0)
+    st2 = s2.step()
+    assert st2.den_iters == 0 and st2.div_iters == 0
