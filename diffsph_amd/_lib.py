@@ -108,9 +108,9 @@ SIGNATURES = {
     "dsph_dfsph_iterate": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, I32, P(C.c_double), V, SZ, V]),
     "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),
     "dsph_integrate_semi_implicit": (C.c_int, [V, V, V, V, V, V, V, I64, F32, V, V]),
-    "dsph_op_sph_density": (C.c_int, [V, V, V, I32, V, V, V, V]),
-    "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, V, V, I32, V, V, V, V]),
-    "dsph_op_scatter_add": (C.c_int, [V, I32, I32, V, V, V, V]),
+    "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),
+    "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, I64, V, V, I64, I32, V, V, V, V]),
+    "dsph_op_scatter_add": (C.c_int, [V, I32, I64, I32, V, V, V, V]),
 }
 
 _lib = None

@@ -58,6 +58,11 @@ __device__ __forceinline__ float mod_offset(float x, const Dom& d, int k) {
   return x;
 }
 
+// Correctly rounded sqrt.  (HIP's __fsqrt_rn is the native approximate sqrt
+// unless OCML_BASIC_ROUNDED_OPERATIONS is defined; sqrtf is IEEE under
+// -fhip-fp32-correctly-rounded-divide-sqrt.)
+__device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
+
 // |d|^2 with both products rounded before the add (no FMA contraction), so
 // the neighbor predicate is reproducible bit for bit on the CPU.
 __device__ __forceinline__ float dist2_exact(float dx, float dy) {
@@ -69,16 +74,21 @@ constexpr float kC2 = (float)(7.0 / 3.14159265358979323846);
 
 __device__ __forceinline__ float clamp01(float t) { return fminf(fmaxf(t, 0.0f), 1.0f); }
 
-// W(q,h) = (1-q)^4 (1+4q) C / h^2
+// All float expressions below are evaluated exactly as written: the library
+// is compiled with -ffp-contract=off and IEEE division/sqrt, so every value
+// is reproduced bit for bit by a CPU implementation of the same expressions.
+
+// W(q,h) = (1-q)^4 (1+4q) C / h^2, cpow(1-q,4) = ((t*t)*t)*t
 __device__ __forceinline__ float wendland_w(float q, float h) {
   float t = clamp01(1.0f - q);
-  float t2 = t * t;
-  return (t2 * t2 * (1.0f + 4.0f * q)) * kC2 / (h * h);
+  float t4 = ((t * t) * t) * t;
+  return ((t4 * (1.0f + 4.0f * q)) * kC2) / (h * h);
 }
 // dk/dq C / h^3 with dk/dq = -20 q (1-q)^3; grad W = xhat * this.
 __device__ __forceinline__ float wendland_dw(float q, float h) {
   float t = clamp01(1.0f - q);
-  return (-20.0f * q * (t * t * t)) * kC2 / (h * h * h);
+  float t3 = (t * t) * t;
+  return (((-20.0f * q) * t3) * kC2) / ((h * h) * h);
 }
 
 __device__ __forceinline__ float pick_h(int mode, float hi, float hj) {
@@ -100,12 +110,12 @@ __device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const D
   Geom g;
   g.dx = mod_offset(pi.x - pj.x, d, 0);
   g.dy = mod_offset(pi.y - pj.y, d, 1);
-  g.r = __fsqrt_rn(dist2_exact(g.dx, g.dy));
+  g.r = sqrt_rn(dist2_exact(g.dx, g.dy));
   g.h = h;
   g.q = __fdiv_rn(g.r, h);
-  float inv = 1.0f / (g.r + 1e-7f);
-  g.ex = g.dx * inv;
-  g.ey = g.dy * inv;
+  const float den = g.r + 1e-7f;
+  g.ex = __fdiv_rn(g.dx, den);
+  g.ey = __fdiv_rn(g.dy, den);
   g.w = wendland_w(g.q, h);
   float dw = wendland_dw(g.q, h);
   g.gx = g.ex * dw;

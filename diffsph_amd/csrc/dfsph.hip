@@ -244,10 +244,11 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     const float s = B.src[i];
     const float res = K - s;
     float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
-    if (B.clamp) pn = fmaxf(pn, 0.0f);
+    if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0) / std::max order
     B.p[i] = pn;
     if (B.residual) B.residual[i] = res;
-    err = (double)fmaxf(res / B.rho0, B.neg_thr);
+    const float rr = res / B.rho0;
+    err = (double)(rr < B.neg_thr ? B.neg_thr : rr);
   }
   double bs = block_sum_d(err, sh);
   if (threadIdx.x == 0) {

@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
     const float2 xj = P.rpos[j];
     float dx = mod_offset(xi.x - xj.x, P.dom, 0), dy = mod_offset(xi.y - xj.y, P.dom, 1);
     float h = pick_h(P.mode, hi, hj);
-    float q = __fdiv_rn(__fsqrt_rn(dist2_exact(dx, dy)), h);
+    float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
     acc += rmass[j] * wendland_w(q, h);
   }
   rho[i] = acc;
@@ -196,44 +196,62 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const float* __restrict__ sr
 // Drop-in ops over the reference's per-pair arrays (CSR given by
 // numNeighbors / neighborOffset).
 // ---------------------------------------------------------------------------
+// The caller's CSR may be inconsistent: each segment is clipped to
+// [0, num_pairs) and neighbor indices outside [0, num_ref) are skipped, so a
+// bad argument can never make the kernel read out of bounds.
+__device__ __forceinline__ void seg_bounds(const int32_t* off, const int32_t* nn, int i, int64_t np, int64_t& b,
+                                           int64_t& e) {
+  b = off[i];
+  e = b + (int64_t)nn[i];
+  if (b < 0) b = 0;
+  if (e > np) e = np;
+}
+
 __global__ void __launch_bounds__(kBlock) k_op_density(const float* __restrict__ mj, const int64_t* __restrict__ jj,
-                                                       const float* __restrict__ W, int n,
+                                                       const float* __restrict__ W, int n, int64_t nref, int64_t np,
                                                        const int32_t* __restrict__ nn, const int32_t* __restrict__ off,
                                                        float* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float acc = 0.0f;
-  const int b = off[i], e = b + nn[i];
-  for (int k = b; k < e; ++k) acc += mj[jj[k]] * W[k];
+  int64_t b, e;
+  seg_bounds(off, nn, i, np, b, e);
+  for (int64_t k = b; k < e; ++k) {
+    const int64_t j = jj[k];
+    if (j >= 0 && j < nref) acc += mj[j] * W[k];
+  }
   out[i] = acc;
 }
 
 __global__ void __launch_bounds__(kBlock) k_op_interp(const float* __restrict__ mj, const float* __restrict__ rj,
                                                       const float* __restrict__ qj, int w,
                                                       const int64_t* __restrict__ jj, const float* __restrict__ W,
-                                                      int n, const int32_t* __restrict__ nn,
+                                                      int n, int64_t nref, int64_t np, const int32_t* __restrict__ nn,
                                                       const int32_t* __restrict__ off, float* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const int b = off[i], e = b + nn[i];
-  for (int k = b; k < e; ++k) {
+  int64_t b, e;
+  seg_bounds(off, nn, i, np, b, e);
+  for (int64_t k = b; k < e; ++k) {
     const int64_t j = jj[k];
+    if (j < 0 || j >= nref) continue;
     const float f = mj[j] / rj[j] * W[k];
     for (int c = 0; c < w; ++c) acc[c] += qj[j * w + c] * f;
   }
   for (int c = 0; c < w; ++c) out[(int64_t)i * w + c] = acc[c];
 }
 
-__global__ void __launch_bounds__(kBlock) k_op_scatter_add(const float* __restrict__ in, int w, int n,
+__global__ void __launch_bounds__(kBlock) k_op_scatter_add(const float* __restrict__ in, int w, int n, int64_t np,
                                                            const int32_t* __restrict__ nn,
                                                            const int32_t* __restrict__ off, float* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  const int b = off[i], e = b + nn[i];
-  for (int k = b; k < e; ++k)
-    for (int c = 0; c < w; ++c) acc[c] += in[(int64_t)k * w + c];
+  int64_t b, e;
+  seg_bounds(off, nn, i, np, b, e);
+  for (int64_t k = b; k < e; ++k)
+    for (int c = 0; c < w; ++c) acc[c] += in[k * w + c];
   for (int c = 0; c < w; ++c) out[(int64_t)i * w + c] = acc[c];
 }
 
@@ -346,39 +364,43 @@ extern "C" int dsph_scatter_f32(const float* src, const int32_t* index, int64_t 
   return launch_status();
 }
 
-extern "C" int dsph_op_sph_density(const float* masses_j, const int64_t* indices_j, const float* kernels,
-                                   int32_t num_particles, const int32_t* num_neighbors,
-                                   const int32_t* neighbor_offset, float* out, void* stream) {
-  if (num_particles < 0) return DSPH_ERR_ARG;
+extern "C" int dsph_op_sph_density(const float* masses_j, int64_t num_ref, const int64_t* indices_j,
+                                   const float* kernels, int64_t num_pairs, int32_t num_particles,
+                                   const int32_t* num_neighbors, const int32_t* neighbor_offset, float* out,
+                                   void* stream) {
+  if (num_particles < 0 || num_ref < 0 || num_pairs < 0) return DSPH_ERR_ARG;
   if (num_particles == 0) return DSPH_OK;
-  if (!masses_j || !indices_j || !kernels || !num_neighbors || !neighbor_offset || !out) return DSPH_ERR_ARG;
-  k_op_density<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(masses_j, indices_j, kernels, num_particles,
-                                                                         num_neighbors, neighbor_offset, out);
+  if (!num_neighbors || !neighbor_offset || !out) return DSPH_ERR_ARG;
+  if ((num_pairs > 0 && (!indices_j || !kernels)) || (num_ref > 0 && !masses_j)) return DSPH_ERR_ARG;
+  k_op_density<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(
+      masses_j, indices_j, kernels, num_particles, num_ref, num_pairs, num_neighbors, neighbor_offset, out);
   return launch_status();
 }
 
 extern "C" int dsph_op_sph_interpolation(const float* masses_j, const float* densities_j, const float* quantities_j,
-                                         int32_t width, const int64_t* indices_j, const float* kernels,
-                                         int32_t num_particles, const int32_t* num_neighbors,
-                                         const int32_t* neighbor_offset, float* out, void* stream) {
-  if (num_particles < 0 || width < 1 || width > 4) return DSPH_ERR_ARG;
+                                         int32_t width, int64_t num_ref, const int64_t* indices_j,
+                                         const float* kernels, int64_t num_pairs, int32_t num_particles,
+                                         const int32_t* num_neighbors, const int32_t* neighbor_offset, float* out,
+                                         void* stream) {
+  if (num_particles < 0 || width < 1 || width > 4 || num_ref < 0 || num_pairs < 0) return DSPH_ERR_ARG;
   if (num_particles == 0) return DSPH_OK;
-  if (!masses_j || !densities_j || !quantities_j || !indices_j || !kernels || !num_neighbors || !neighbor_offset ||
-      !out)
+  if (!num_neighbors || !neighbor_offset || !out) return DSPH_ERR_ARG;
+  if ((num_pairs > 0 && (!indices_j || !kernels)) || (num_ref > 0 && (!masses_j || !densities_j || !quantities_j)))
     return DSPH_ERR_ARG;
-  k_op_interp<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(
-      masses_j, densities_j, quantities_j, width, indices_j, kernels, num_particles, num_neighbors, neighbor_offset,
-      out);
+  k_op_interp<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(masses_j, densities_j, quantities_j, width,
+                                                                        indices_j, kernels, num_particles, num_ref,
+                                                                        num_pairs, num_neighbors, neighbor_offset,
+                                                                        out);
   return launch_status();
 }
 
-extern "C" int dsph_op_scatter_add(const float* input, int32_t width, int32_t num_particles,
+extern "C" int dsph_op_scatter_add(const float* input, int32_t width, int64_t num_pairs, int32_t num_particles,
                                    const int32_t* num_neighbors, const int32_t* neighbor_offset, float* out,
                                    void* stream) {
-  if (num_particles < 0 || width < 1 || width > 4) return DSPH_ERR_ARG;
+  if (num_particles < 0 || width < 1 || width > 4 || num_pairs < 0) return DSPH_ERR_ARG;
   if (num_particles == 0) return DSPH_OK;
-  if (!input || !num_neighbors || !neighbor_offset || !out) return DSPH_ERR_ARG;
-  k_op_scatter_add<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(input, width, num_particles,
+  if ((num_pairs > 0 && !input) || !num_neighbors || !neighbor_offset || !out) return DSPH_ERR_ARG;
+  k_op_scatter_add<<<grid_for(num_particles), kBlock, 0, as_stream(stream)>>>(input, width, num_particles, num_pairs,
                                                                              num_neighbors, neighbor_offset, out);
   return launch_status();
 }

@@ -299,30 +299,37 @@ int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
 /* Drop-in ops with the reference's per-pair inputs                         */
 /* ---------------------------------------------------------------------- */
 
+/* The three drop-ins below take the reference's per-pair arrays.  The CSR
+ * given by (numNeighbors, neighborOffset) is clipped to [0, num_pairs) and
+ * neighbor indices outside [0, num_ref) are skipped, so inconsistent input
+ * cannot make a kernel read out of bounds (the reference trusts them). */
+
 /* sphDensity (partiBench/cppOp.cpp:5-106): out_i = sum_k m_j[j_k] W_k over
- * CSR row i given by (numNeighbors, neighborOffset). */
-int dsph_op_sph_density(const float* masses_j, const int64_t* indices_j,
-                        const float* kernels, int32_t num_particles,
+ * CSR row i.  masses_j has num_ref entries, indices_j / kernels num_pairs. */
+int dsph_op_sph_density(const float* masses_j, int64_t num_ref,
+                        const int64_t* indices_j, const float* kernels,
+                        int64_t num_pairs, int32_t num_particles,
                         const int32_t* num_neighbors,
                         const int32_t* neighbor_offset, float* out,
                         void* stream);
 
 /* sphInterpolation (partiBench/cppOp.cpp:110-222,
  * sphOperations/cppOp.cpp:112-132): out_i = sum_k m_j/rho_j q_j W_k for a
- * quantity of `width` components.  (The reference's CUDA branch computes
- * the density instead -- partiBench/cppOp.cpp:160-169; this follows the
- * documented formula of cppOp.h:90-125.) */
+ * quantity of `width` (1..4) components.  (The reference's CUDA branch
+ * computes the density instead -- partiBench/cppOp.cpp:160-169; this follows
+ * the documented formula of partiBench/cppOp.h:90-125.) */
 int dsph_op_sph_interpolation(const float* masses_j, const float* densities_j,
                               const float* quantities_j, int32_t width,
-                              const int64_t* indices_j, const float* kernels,
+                              int64_t num_ref, const int64_t* indices_j,
+                              const float* kernels, int64_t num_pairs,
                               int32_t num_particles,
                               const int32_t* num_neighbors,
                               const int32_t* neighbor_offset, float* out,
                               void* stream);
 
 /* scatterAdd (sphOperations/cppOp.cpp:364-423): out_i[c] = sum over the CSR
- * segment of row i of input[k, c]. */
-int dsph_op_scatter_add(const float* input, int32_t width,
+ * segment of row i of input[k, c] (input has num_pairs rows). */
+int dsph_op_scatter_add(const float* input, int32_t width, int64_t num_pairs,
                         int32_t num_particles, const int32_t* num_neighbors,
                         const int32_t* neighbor_offset, float* out,
                         void* stream);

@@ -94,10 +94,12 @@ T cpow(T q, int p) {
 }
 template <class T>
 T C2() { return (T)(7.0 / 3.14159265358979323846); }
+// k(q) C / h^2 and dk/dq C / h^3, evaluated left to right as written in
+// Wendland2.py:30-36 (cpow(x,p) = x*x*...*x).
 template <class T>
-T kernelW(T q, T h) { return cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q) * C2<T>() / (h * h); }
+T kernelW(T q, T h) { return ((cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q)) * C2<T>()) / (h * h); }
 template <class T>
-T kernelDW(T q, T h) { return (T(-20) * q * cpow<T>(T(1) - q, 3)) * C2<T>() / (h * h * h); }
+T kernelDW(T q, T h) { return (((T(-20) * q) * cpow<T>(T(1) - q, 3)) * C2<T>()) / ((h * h) * h); }
 
 template <class T>
 struct G {

@@ -1,0 +1,65 @@
+"""Per-pair geometry and kernel values are bit-exact against a numpy float32
+evaluation of the reference formulas (numpy float32 arithmetic is IEEE
+correctly rounded, no contraction):
+  d = mod(x_i - x_j), r = sqrt(d.x^2 + d.y^2), q = r/h, xhat = d/(r+1e-7),
+  W = (((1-q)^4 (1+4q)) C)/h^2, gradW = xhat (((-20 q)(1-q)^3) C)/h^3
+(v2/modules/neighborhood.py:445-451, kernelFunctions/Wendland2.py:30-36)."""
+import numpy as np
+import pytest
+import torch
+
+from diffsph_amd import _lib, sampling
+from diffsph_amd.neighborhood import CellGrid, export_pairs, search
+
+pytestmark = pytest.mark.gpu
+
+f = np.float32
+
+
+def _remainder(a, b):
+    r = np.fmod(a, b)
+    m = (r != 0) & ((r < 0) != (b < 0))
+    r = r.copy()
+    r[m] = r[m] + b
+    return r
+
+
+def _mod(x, mn, L, half):
+    ax = np.abs(x)
+    s = np.sign(x).astype(f)
+    y = (s * (_remainder((ax + f(mn)).astype(f), f(L)) + f(mn))).astype(f)
+    return np.where(ax > f(half), y, x).astype(f)
+
+
+@pytest.mark.parametrize("nside,jit", [(64, 0.1), (80, 0.7)])
+def test_pair_geometry_bitexact(nside, jit):
+    d = sampling.dfsph_random_flow(nside, seed=nside, jitter_width=jit)
+    dev = torch.device("cuda:0")
+    dom = _lib.make_domain((-1, -1), (1, 1), (True, True))
+    g = CellGrid.build(torch.from_numpy(d["pos"]).to(dev), None, dom, d["h"], h_uniform=d["h"])
+    nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True)
+    ex = export_pairs(nl)
+    off = nl.offsets.cpu().numpy()
+    nbr = nl.nbr.cpu().numpy()
+    sp = g.sorted_pos.cpu().numpy()
+    i = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    j = nbr
+    dx = _mod((sp[i, 0] - sp[j, 0]).astype(f), -1.0, 2.0, 1.0)
+    dy = _mod((sp[i, 1] - sp[j, 1]).astype(f), -1.0, 2.0, 1.0)
+    r = np.sqrt(((dx * dx).astype(f) + (dy * dy).astype(f)).astype(f)).astype(f)
+    h = f(d["h"])
+    q = (r / h).astype(f)
+    np.testing.assert_array_equal(ex["distances"].cpu().numpy(), q)
+    den = (r + f(1e-7)).astype(f)
+    exx = (dx / den).astype(f)
+    eyy = (dy / den).astype(f)
+    np.testing.assert_array_equal(ex["vectors"].cpu().numpy(), np.stack([exx, eyy], 1))
+    t = np.clip((f(1) - q).astype(f), f(0), f(1)).astype(f)
+    t3 = ((t * t).astype(f) * t).astype(f)
+    t4 = (t3 * t).astype(f)
+    C2 = f(7.0 / np.pi)
+    W = ((((t4 * (f(1) + (f(4) * q).astype(f))).astype(f)) * C2).astype(f) / (h * h)).astype(f)
+    np.testing.assert_array_equal(ex["kernels"].cpu().numpy(), W)
+    dw = (((((f(-20) * q).astype(f) * t3).astype(f)) * C2).astype(f) / ((h * h) * h)).astype(f)
+    G = np.stack([(exx * dw).astype(f), (eyy * dw).astype(f)], 1)
+    np.testing.assert_array_equal(ex["gradients"].cpu().numpy(), G)

@@ -196,29 +196,47 @@ def test_dropin_ops_match_torch_reference():
     rho = rng.uniform(0.9, 1.1, n).astype(np.float32)
     W = rng.uniform(0, 2, P).astype(np.float32)
     q = rng.standard_normal((n, 3)).astype(np.float32)
+    # every device buffer is held in a variable for the whole call: a raw
+    # pointer of a temporary tensor would dangle once the tensor is freed
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    tm, trho, tq, tj, tW, tcnt, toff = T(m), T(rho), T(q), T(j.astype(np.int64)), T(W), T(cnt), T(off)
+    vals = rng.standard_normal((P, 2)).astype(np.float32)
+    tvals = T(vals)
     L = _lib.lib()
     st = _lib.stream_handle(dev)
     out = torch.empty(n, device=dev)
-    _lib.check(L.dsph_op_sph_density(T(m).data_ptr(), T(j.astype(np.int64)).data_ptr(), T(W).data_ptr(), n,
-                                     T(cnt).data_ptr(), T(off).data_ptr(), out.data_ptr(), st))
+    _lib.check(L.dsph_op_sph_density(tm.data_ptr(), n, tj.data_ptr(), tW.data_ptr(), P, n, tcnt.data_ptr(),
+                                     toff.data_ptr(), out.data_ptr(), st))
     ref = np.zeros(n, np.float64)
     np.add.at(ref, i, m[j].astype(np.float64) * W)
     _close(out.cpu().numpy(), ref)
     out3 = torch.empty(n, 3, device=dev)
-    _lib.check(L.dsph_op_sph_interpolation(T(m).data_ptr(), T(rho).data_ptr(), T(q).data_ptr(), 3,
-                                           T(j.astype(np.int64)).data_ptr(), T(W).data_ptr(), n, T(cnt).data_ptr(),
-                                           T(off).data_ptr(), out3.data_ptr(), st))
+    _lib.check(L.dsph_op_sph_interpolation(tm.data_ptr(), trho.data_ptr(), tq.data_ptr(), 3, n, tj.data_ptr(),
+                                           tW.data_ptr(), P, n, tcnt.data_ptr(), toff.data_ptr(), out3.data_ptr(),
+                                           st))
     ref3 = np.zeros((n, 3), np.float64)
     np.add.at(ref3, i, (m[j] / rho[j] * W)[:, None].astype(np.float64) * q[j])
     _close(out3.cpu().numpy(), ref3)
-    vals = rng.standard_normal((P, 2)).astype(np.float32)
     out2 = torch.empty(n, 2, device=dev)
-    _lib.check(L.dsph_op_scatter_add(T(vals).data_ptr(), 2, n, T(cnt).data_ptr(), T(off).data_ptr(),
-                                     out2.data_ptr(), st))
+    _lib.check(L.dsph_op_scatter_add(tvals.data_ptr(), 2, P, n, tcnt.data_ptr(), toff.data_ptr(), out2.data_ptr(),
+                                     st))
     ref2 = np.zeros((n, 2), np.float64)
     np.add.at(ref2, i, vals.astype(np.float64))
     _close(out2.cpu().numpy(), ref2)
+    # out-of-range neighbor indices and an inconsistent CSR are ignored, not faults
+    bad_j = j.astype(np.int64).copy()
+    bad_j[::7] = n + 12345
+    bad_j[1::7] = -3
+    tbad = T(bad_j)
+    big_cnt = cnt.copy()
+    big_cnt[-1] += 1000  # last segment runs past the pair arrays
+    tbig = T(big_cnt)
+    _lib.check(L.dsph_op_sph_density(tm.data_ptr(), n, tbad.data_ptr(), tW.data_ptr(), P, n, tbig.data_ptr(),
+                                     toff.data_ptr(), out.data_ptr(), st))
+    ok = (bad_j >= 0) & (bad_j < n)
+    ref = np.zeros(n, np.float64)
+    np.add.at(ref, i[ok], m[bad_j[ok]].astype(np.float64) * W[ok])
+    _close(out.cpu().numpy(), ref)
 
 
 def test_large_search_properties():
