@@ -99,12 +99,12 @@ def main():
     reps = args.kernel_reps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for _ in range(3):
-        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.vol), _lib.ptr(solver.rho), _lib.ptr(p),
-                                         _lib.ptr(acc), sh))
+        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
+                                         _lib.ptr(solver.rho), _lib.ptr(p), _lib.ptr(acc), sh))
     e0.record(stream)
     for _ in range(reps):
-        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.vol), _lib.ptr(solver.rho), _lib.ptr(p),
-                                         _lib.ptr(acc), sh))
+        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
+                                         _lib.ptr(solver.rho), _lib.ptr(p), _lib.ptr(acc), sh))
     e1.record(stream)
     e1.synchronize()
     t_a = e0.elapsed_time(e1) / reps * 1e-3
@@ -115,7 +115,8 @@ def main():
     res = torch.empty(n, device=dev)
     it_reps = max(reps // 2, 5)
     e0.record(stream)
-    _lib.check(L.dsph_dfsph_iterate(C.byref(pairs), C.byref(prm), _lib.ptr(solver.rho), _lib.ptr(solver.vol),
+    _lib.check(L.dsph_dfsph_iterate(C.byref(pairs), C.byref(prm), _lib.ptr(solver.kgrad), _lib.ptr(solver.rho),
+                                    _lib.ptr(solver.vol),
                                     _lib.ptr(solver.alpha_den), _lib.ptr(solver.div_src), _lib.ptr(p), _lib.ptr(acc),
                                     _lib.ptr(res), it_reps, None, _lib.ptr(ws), ws.numel(), sh))
     e1.record(stream)
@@ -123,12 +124,13 @@ def main():
     t_iter = e0.elapsed_time(e1) / it_reps * 1e-3
     t_b = max(t_iter - t_a, 1e-9)
     P = solver.nl.num_pairs
-    # Algorithmic bytes (each per-particle array once per launch, each CSR
-    # entry once; gathered neighbor fields not counted):
-    # sweep A: x 8 + p 4 + V 4 + rho 4 + offset 4 + a(write) 8 = 32 B/particle + 4 B/pair
-    # sweep B: x 8 + a 8 + V 4 + p 4(r)+4(w) + s 4 + alpha 4 + offset 4 + residual 4(w) = 44 B/particle + 4 B/pair
-    bytes_a = 32 * n + 4 * P
-    bytes_b = 44 * n + 4 * P
+    # Algorithmic bytes (each per-particle array once per launch, each pair
+    # entry once; gathered neighbor fields and list padding not counted):
+    # sweep A: p 4 + rho 4 + count 4 + a(write) 8 = 20 B/particle;
+    #          j 4 + V_j gradW_ij 8 = 12 B/pair
+    # sweep B: a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 32 B/particle; 12 B/pair
+    bytes_a = 20 * n + 12 * P
+    bytes_b = 32 * n + 12 * P
     gbs_a = bytes_a / t_a / 1e9
     gbs_b = bytes_b / t_b / 1e9
     log(f"sweep A (pressure accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "

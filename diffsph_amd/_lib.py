@@ -38,6 +38,8 @@ class Pairs(C.Structure):
         ("h_uniform", C.c_float),
         ("mode", C.c_int32),
         ("domain", Domain),
+        ("counts", C.c_void_p),
+        ("slice_off", C.c_void_p),
     ]
 
 
@@ -91,7 +93,11 @@ SIGNATURES = {
     "dsph_grid_build_workspace": (SZ, [I64, P(Grid)]),
     "dsph_grid_build": (C.c_int, [P(Domain), P(Grid), V, V, I64, V, V, V, V, V, V, SZ, V]),
     "dsph_neighbor_count": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, V, V]),
-    "dsph_neighbor_fill": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, V, V, I64, V]),
+    "dsph_neighbor_fill": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, V, V, I64, V, I64,
+                                     V]),
+    "dsph_sell_workspace": (SZ, [I64]),
+    "dsph_sell_layout": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
+    "dsph_sell_to_csr": (C.c_int, [P(Pairs), V, V]),
     "dsph_wrap_positions": (C.c_int, [P(Domain), V, I64, V, V]),
     "dsph_scan_workspace": (SZ, [I64]),
     "dsph_scan_exclusive": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
@@ -101,11 +107,13 @@ SIGNATURES = {
     "dsph_pairs_export_workspace": (SZ, [I64]),
     "dsph_density": (C.c_int, [P(Pairs), V, V, V, V, V]),
     "dsph_sph_operation": (C.c_int, [P(Pairs), I32, I32, V, V, V, V, V, V, I32, V, V]),
-    "dsph_dfsph_prepare": (C.c_int, [P(Pairs), V, V, V, V, V, F32, F32, P(ViscParams), V, V, V, V, V]),
+    "dsph_dfsph_prepare": (C.c_int, [P(Pairs), V, V, V, V, V, F32, F32, P(ViscParams), V, V, V, V, V, V]),
     "dsph_dfsph_workspace": (SZ, [I64]),
-    "dsph_dfsph_solve": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V, V, P(SolveStats), V, SZ, V]),
-    "dsph_pressure_accel": (C.c_int, [P(Pairs), V, V, V, V, V]),
-    "dsph_dfsph_iterate": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, I32, P(C.c_double), V, SZ, V]),
+    "dsph_dfsph_solve": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V, V, V, P(SolveStats), V, SZ,
+                                   V]),
+    "dsph_pressure_accel": (C.c_int, [P(Pairs), V, V, V, V, V, V]),
+    "dsph_dfsph_iterate": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, I32, P(C.c_double), V, SZ,
+                                     V]),
     "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),
     "dsph_integrate_semi_implicit": (C.c_int, [V, V, V, V, V, V, V, I64, F32, V, V]),
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),

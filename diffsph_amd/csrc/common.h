@@ -134,6 +134,8 @@ struct PairsDev {
   float h0;
   int mode;
   Dom dom;
+  const int32_t* cnt;
+  const int32_t* slice;
 };
 
 inline PairsDev make_pairs(const dsph_pairs& p) {
@@ -149,7 +151,31 @@ inline PairsDev make_pairs(const dsph_pairs& p) {
   o.h0 = p.h_uniform;
   o.mode = p.mode;
   o.dom = make_dom(p.domain);
+  o.cnt = p.counts;
+  o.slice = p.slice_off;
   return o;
+}
+
+// Where the entries of row i live: entry n is at base + n*step (sliced ELL:
+// step 64, consecutive lanes of a wave read consecutive words; CSR: step 1).
+struct RowRef {
+  int base;
+  int step;
+  int cnt;
+};
+
+__device__ __forceinline__ RowRef row_ref(const PairsDev& P, int i) {
+  RowRef r;
+  if (P.slice) {
+    r.base = P.slice[i >> 6] + (i & 63);
+    r.step = 64;
+    r.cnt = P.cnt[i];
+  } else {
+    r.base = P.off[i];
+    r.step = 1;
+    r.cnt = P.off[i + 1] - P.off[i];
+  }
+  return r;
 }
 
 __device__ __forceinline__ float h_of(const float* arr, int idx, float h0) {

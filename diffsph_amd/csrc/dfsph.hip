@@ -1,17 +1,24 @@
 // DFSPH relaxed-Jacobi pressure solve (v2/modules/dfsph.py) as gather
 // kernels with device-side loop control.
 //
+// The prepare sweep (once per step, positions fixed for the whole solve)
+// evaluates the geometry of every pair and stores kV_ij = V_j gradW_ij -- the
+// product the reference forms as (m_j/rho_j) * gradKernels
+// (v2/sphOps.py:50,93) -- in the neighbor list's layout.  Every later sweep
+// of both solves streams kV instead of recomputing sqrt/div per pair; the
+// stored value is bit-identical to the one the on-the-fly evaluation gives.
+//
 // Per iteration two pair sweeps:
-//   A: a_i = -(1/rho_i) sum_j V_j (p_i + p_j) gradW_ij
-//   B: K_i = -dtf sum_j V_j (a_j - a_i) . gradW_ij ; r_i = K_i - s_i ;
+//   A: a_i = -(1/rho_i) sum_j (p_i + p_j) kV_ij
+//   B: K_i = -dtf sum_j (a_j - a_i) . kV_ij ; r_i = K_i - s_i ;
 //      p_i += omega (s_i - K_i)/alpha_i (clamped at 0) -- in place, a thread
 //      only touches its own p_i in B.
 // B also reduces mean(clamp(r_i/rho0, min=-thr)) in fp64: every block
-// writes a partial, the last block to finish (ticket counter) sums the
-// partials in a fixed order, advances the iteration counter and sets `done`
-// with the reference's loop condition.  Every kernel of the loop returns at
-// once when `done` is set, so the host may queue several iterations ahead
-// and synchronise only once per chunk.
+// writes a partial; a one-block kernel queued after B sums the partials in a
+// fixed order, advances the iteration counter and sets `done` with the
+// reference's loop condition (no grid-wide fence or atomic inside B).  Every
+// kernel of the loop returns at once when `done` is set, so the host may
+// queue several iterations ahead and synchronise only once per chunk.
 #include "common.h"
 
 namespace dsph {
@@ -48,17 +55,25 @@ __device__ __forceinline__ double block_sum_d(double v, double* sh) {
   return t;
 }
 
+// V_j gradW_ij of entry k: stored (kv != NULL) or evaluated on the fly.
+__device__ __forceinline__ float2 kv_of(const PairsDev& P, const float2* __restrict__ kv, int k, int j, float2 xi,
+                                        float hi, const float* __restrict__ vol) {
+  if (kv) return kv[k];
+  const float hj = h_of(P.rh, j, P.h0);
+  Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+  const float Vj = vol[j];
+  return make_float2(Vj * g.gx, Vj * g.gy);
+}
+
 // ---------------------------------------------------------------------------
 // Coefficients for both solves + divergence of v + viscosity, one sweep.
 // ---------------------------------------------------------------------------
 struct PrepArgs {
   const float2* vel;
-  const float* mass;   // unused by alpha (reference uses areas*rho0) but kept for V
   const float* areas;
   const float* vol;    // m/rho
   const float* rho;
   float dt, dt2, rho0;
-  // viscosity
   int visc_on;
   int visc_scheme;
   float visc_alpha, visc_cs, visc_rho0, visc_eps;
@@ -68,28 +83,29 @@ struct PrepArgs {
 
 __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
                                                           float* __restrict__ alpha_den, float* __restrict__ div_src,
-                                                          float2* __restrict__ visc_acc) {
+                                                          float2* __restrict__ visc_acc, float2* __restrict__ kv_out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
   const float2 vi = A.vel[i];
   float k1x = 0.f, k1y = 0.f, k2 = 0.f, dv = 0.f, vsx = 0.f, vsy = 0.f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
     const float h = pick_h(P.mode, hi, hj);
     Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
     const float Vj = A.vol[j];
     const float2 vj = A.vel[j];
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    if (kv_out) kv_out[k] = make_float2(kx, ky);
     // alpha terms (dfsph.py:13-20)
-    k1x += Vj * g.gx;
-    k1y += Vj * g.gy;
+    k1x += kx;
+    k1y += ky;
     const float c2 = (Vj * Vj) / (A.areas[j] * A.rho0);
     k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
     // divergence 'difference' of v (sphOps.py:92-101)
-    const float kx = Vj * g.gx, ky = Vj * g.gy;
     dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
     if (A.visc_on) {
       // delta-SPH viscosity (viscosity.py:141-166 / 205-218)
@@ -125,13 +141,15 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
   }
 }
 
-// Density source: s_i = 1 - areas_i/V_i - dt * div(v*) with v* = v + dt adv
-// (dfsph.py:57, 62-63).
-__global__ void __launch_bounds__(kBlock) k_dfsph_density_source(PairsDev P, const float2* __restrict__ vel,
-                                                                 const float2* __restrict__ adv,
-                                                                 const float* __restrict__ vol,
-                                                                 const float* __restrict__ areas, float dt,
-                                                                 float* __restrict__ src) {
+// Source of a solve with advection adv (may be NULL = 0): div(v*) with
+// v* = v + dt adv; density source 1 - areas_i/V_i - dt div(v*)
+// (dfsph.py:57, 62-65).
+__global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float2* __restrict__ kv,
+                                                         const float2* __restrict__ vel,
+                                                         const float2* __restrict__ adv,
+                                                         const float* __restrict__ vol,
+                                                         const float* __restrict__ areas, float dt, int density,
+                                                         float* __restrict__ src) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -139,46 +157,20 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_density_source(PairsDev P, con
   const float2 a_i = adv ? adv[i] : make_float2(0.f, 0.f);
   const float2 vi = make_float2(vel[i].x + dt * a_i.x, vel[i].y + dt * a_i.y);
   float dv = 0.f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
-    const float hj = h_of(P.rh, j, P.h0);
-    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+    const float2 c = kv_of(P, kv, k, j, xi, hi, vol);
     const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
     const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
-    const float Vj = vol[j];
-    const float kx = Vj * g.gx, ky = Vj * g.gy;
-    dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
+    dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
   }
-  src[i] = (1.0f - areas[i] / vol[i]) - dt * dv;
-}
-
-// Divergence source with advection: s_i = div(v + dt adv) (dfsph.py:65).
-__global__ void __launch_bounds__(kBlock) k_dfsph_divergence_source(PairsDev P, const float2* __restrict__ vel,
-                                                                    const float2* __restrict__ adv,
-                                                                    const float* __restrict__ vol, float dt,
-                                                                    float* __restrict__ src) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nq) return;
-  const float2 xi = P.qpos[i];
-  const float hi = h_of(P.qh, i, P.h0);
-  const float2 vi = make_float2(vel[i].x + dt * adv[i].x, vel[i].y + dt * adv[i].y);
-  float dv = 0.f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
-    const int j = P.nbr[k];
-    const float hj = h_of(P.rh, j, P.h0);
-    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
-    const float2 vj = make_float2(vel[j].x + dt * adv[j].x, vel[j].y + dt * adv[j].y);
-    const float Vj = vol[j];
-    const float kx = Vj * g.gx, ky = Vj * g.gy;
-    dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
-  }
-  src[i] = dv;
+  src[i] = density ? (1.0f - areas[i] / vol[i]) - dt * dv : dv;
 }
 
 // Sweep A (dfsph.py:35-37).  `ctrl` NULL -> unconditional.
-__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ vol,
+__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float2* __restrict__ kv,
+                                                          const float* __restrict__ vol,
                                                           const float* __restrict__ rho, const float* __restrict__ p,
                                                           float2* __restrict__ acc, const SolveCtrl* ctrl) {
   if (ctrl && ctrl->done) return;
@@ -188,21 +180,20 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
   const float hi = h_of(P.qh, i, P.h0);
   const float pi = p[i];
   float ax = 0.f, ay = 0.f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
-    const float hj = h_of(P.rh, j, P.h0);
-    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
-    const float Vj = vol[j];
+    const float2 c = kv_of(P, kv, k, j, xi, hi, vol);
     const float s = pi + p[j];
-    ax += s * (Vj * g.gx);
-    ay += s * (Vj * g.gy);
+    ax += s * c.x;
+    ay += s * c.y;
   }
   const float r = rho[i];
   acc[i] = make_float2(-ax / r, -ay / r);
 }
 
 struct SweepBArgs {
+  const float2* kv;
   const float* vol;
   const float* src;
   const float* alpha;
@@ -213,7 +204,6 @@ struct SweepBArgs {
   SolveCtrl* ctrl;
   float dtf;
   float omega;
-  float inv_rho0_dummy;
   float rho0;
   float neg_thr;
   int clamp;
@@ -222,7 +212,6 @@ struct SweepBArgs {
 
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs B) {
   __shared__ double sh[kBlock / 64];
-  __shared__ int last;
   if (B.ctrl->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double err = 0.0;
@@ -231,48 +220,43 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     const float hi = h_of(P.qh, i, P.h0);
     const float2 ai = B.acc[i];
     float sum = 0.f;
-    const int b = P.off[i], e = P.off[i + 1];
-    for (int k = b; k < e; ++k) {
+    const RowRef rr = row_ref(P, i);
+    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
-      const float hj = h_of(P.rh, j, P.h0);
-      Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
-      const float Vj = B.vol[j];
+      const float2 c = kv_of(P, B.kv, k, j, xi, hi, B.vol);
       const float2 aj = B.acc[j];
-      sum += (aj.x - ai.x) * (Vj * g.gx) + (aj.y - ai.y) * (Vj * g.gy);
+      sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
     }
     const float K = -B.dtf * sum;
     const float s = B.src[i];
     const float res = K - s;
     float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
-    if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0) / std::max order
+    if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
     B.p[i] = pn;
     if (B.residual) B.residual[i] = res;
-    const float rr = res / B.rho0;
-    err = (double)(rr < B.neg_thr ? B.neg_thr : rr);
+    const float rr0 = res / B.rho0;
+    err = (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
   }
   double bs = block_sum_d(err, sh);
-  if (threadIdx.x == 0) {
-    B.partials[blockIdx.x] = bs;
-    __threadfence();
-    unsigned int t = atomicAdd(&B.ctrl->ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // Last block: fixed-order reduction of all partials.
+  if (threadIdx.x == 0) B.partials[blockIdx.x] = bs;
+}
+
+// One block: fixed-order sum of the sweep-B partials, the reference's loop
+// condition (dfsph.py:88), `done` for the kernels queued after it.
+__global__ void __launch_bounds__(kBlock) k_dfsph_reduce(const double* __restrict__ partials, int nb, int n,
+                                                         SolveCtrl* ctrl) {
+  __shared__ double sh[kBlock / 64];
+  if (ctrl->done) return;
   double v = 0.0;
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) v += __builtin_nontemporal_load(&B.partials[k]);
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) v += partials[k];
   double tot = block_sum_d(v, sh);
   if (threadIdx.x == 0) {
-    SolveCtrl* c = B.ctrl;
-    double mean = tot / (double)B.n;
-    c->iter += 1;
-    c->err = mean;
-    if (c->iter == 1) c->err0 = mean;
-    c->ticket = 0;
-    bool go = c->iter < c->max_iters && (c->iter < c->min_iters || mean > c->thr);
-    c->done = go ? 0 : 1;
+    double mean = tot / (double)n;
+    ctrl->iter += 1;
+    ctrl->err = mean;
+    if (ctrl->iter == 1) ctrl->err0 = mean;
+    bool go = ctrl->iter < ctrl->max_iters && (ctrl->iter < ctrl->min_iters || mean > ctrl->thr);
+    ctrl->done = go ? 0 : 1;
   }
 }
 
@@ -295,6 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_density_error(const float* __restric
                                                           double* __restrict__ partial_sum,
                                                           double* __restrict__ partial_max) {
   __shared__ double sh[kBlock / 64];
+  __shared__ double shm[kBlock / 64];
   double s = 0.0, m = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double e = fabs((double)rho[i] / (double)rho0 - 1.0);
@@ -302,10 +287,8 @@ __global__ void __launch_bounds__(kBlock) k_density_error(const float* __restric
     m = fmax(m, e);
   }
   double bs = block_sum_d(s, sh);
-  // max: shuffle tree
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
-  __shared__ double shm[kBlock / 64];
   if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -370,6 +353,28 @@ int check_pairs(const dsph_pairs* p);
 
 inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
+SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const float* vol, const float* src,
+                        const float* alpha, float* p, float* residual, const float* accel, double* partials,
+                        SolveCtrl* ctrl, int64_t n) {
+  SweepBArgs B;
+  B.kv = reinterpret_cast<const float2*>(kv);
+  B.vol = vol;
+  B.src = src;
+  B.alpha = alpha;
+  B.p = p;
+  B.residual = residual;
+  B.acc = reinterpret_cast<const float2*>(accel);
+  B.partials = partials;
+  B.ctrl = ctrl;
+  B.dtf = prm->source_density ? (float)((double)prm->dt * (double)prm->dt) : prm->dt;
+  B.omega = prm->omega;
+  B.rho0 = prm->rho0;
+  B.neg_thr = -prm->error_threshold;
+  B.clamp = prm->clamp_pressure;
+  B.n = (int)n;
+  return B;
+}
+
 }  // namespace dsph
 
 using namespace dsph;
@@ -377,17 +382,17 @@ using namespace dsph;
 extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, const float* mass, const float* areas,
                                   const float* vol, const float* rho, float dt, float rho0,
                                   const dsph_visc_params* visc, float* alpha_div, float* alpha_den, float* div_src,
-                                  float* visc_acc, void* stream) {
+                                  float* visc_acc, float* kgrad, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
   if (pairs->nq == 0) return DSPH_OK;
   if (!vel || !areas || !vol || !rho || !alpha_div || !alpha_den || !div_src) return DSPH_ERR_ARG;
   if (visc && !visc_acc) return DSPH_ERR_ARG;
+  (void)mass;
   PairsDev P = make_pairs(*pairs);
   PrepArgs A;
   A.vel = reinterpret_cast<const float2*>(vel);
-  A.mass = mass;
   A.areas = areas;
   A.vol = vol;
   A.rho = rho;
@@ -402,8 +407,8 @@ extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, con
   A.visc_eps = visc ? visc->eps : 0.f;
   A.pi_switch = visc ? visc->pi_switch : 0;
   A.visc_factor = visc ? (float)(2.0 * (2 + 2) * (double)visc->alpha * (double)visc->rho0) : 0.f;
-  k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, A, alpha_div, alpha_den, div_src,
-                                                                    reinterpret_cast<float2*>(visc_acc));
+  k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+      P, A, alpha_div, alpha_den, div_src, reinterpret_cast<float2*>(visc_acc), reinterpret_cast<float2*>(kgrad));
   return launch_status();
 }
 
@@ -412,10 +417,10 @@ extern "C" size_t dsph_dfsph_workspace(int64_t n) {
   return al256(sizeof(SolveCtrl)) + al256(nb * sizeof(double)) + al256(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
 }
 
-extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* vel,
-                                const float* adv, const float* rho, const float* vol, const float* areas,
-                                const float* alpha, const float* src0, float* p, float* accel, float* residual,
-                                dsph_solve_stats* stats, void* ws, size_t ws_bytes, void* stream) {
+extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
+                                const float* vel, const float* adv, const float* rho, const float* vol,
+                                const float* areas, const float* alpha, const float* src0, float* p, float* accel,
+                                float* residual, dsph_solve_stats* stats, void* ws, size_t ws_bytes, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (!prm || pairs->nq != pairs->nr) return DSPH_ERR_ARG;
@@ -441,39 +446,20 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
   double* partials = reinterpret_cast<double*>(w);
   w += al256((size_t)nb * sizeof(double));
   float* src = reinterpret_cast<float*>(w);
+  const float2* kv = reinterpret_cast<const float2*>(kgrad);
 
-  const float dt = prm->dt;
-  const float dtf = prm->source_density ? (float)((double)dt * (double)dt) : dt;
   // Source term.
-  if (prm->source_density) {
-    k_dfsph_density_source<<<nb, kBlock, 0, st>>>(P, reinterpret_cast<const float2*>(vel),
-                                                  reinterpret_cast<const float2*>(adv), vol, areas, dt, src);
-  } else if (adv) {
-    k_dfsph_divergence_source<<<nb, kBlock, 0, st>>>(P, reinterpret_cast<const float2*>(vel),
-                                                     reinterpret_cast<const float2*>(adv), vol, dt, src);
+  if (prm->source_density || adv) {
+    k_dfsph_source<<<nb, kBlock, 0, st>>>(P, kv, reinterpret_cast<const float2*>(vel),
+                                          reinterpret_cast<const float2*>(adv), vol, areas, prm->dt,
+                                          prm->source_density, src);
   } else {
     DSPH_CHECK_HIP(hipMemcpyAsync(src, src0, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
   }
   DSPH_LAUNCH_CHECK();
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, prm->min_iters, prm->max_iters, (double)prm->error_threshold);
   DSPH_LAUNCH_CHECK();
-
-  SweepBArgs B;
-  B.vol = vol;
-  B.src = src;
-  B.alpha = alpha;
-  B.p = p;
-  B.residual = residual;
-  B.acc = reinterpret_cast<const float2*>(accel);
-  B.partials = partials;
-  B.ctrl = ctrl;
-  B.dtf = dtf;
-  B.omega = prm->omega;
-  B.inv_rho0_dummy = 0.f;
-  B.rho0 = prm->rho0;
-  B.neg_thr = -prm->error_threshold;
-  B.clamp = prm->clamp_pressure;
-  B.n = (int)n;
+  SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);
 
   const int chunk = prm->check_every > 0 ? prm->check_every : 8;
   SolveCtrl host;
@@ -483,8 +469,9 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     int k = chunk;
     if (launched + k > prm->max_iters) k = prm->max_iters - launched;
     for (int t = 0; t < k; ++t) {
-      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
       k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+      k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
     }
     DSPH_LAUNCH_CHECK();
     launched += k;
@@ -497,7 +484,7 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     DSPH_CHECK_HIP(hipStreamSynchronize(st));
   }
   // Final pressure acceleration (dfsph.py:105).
-  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
+  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
   DSPH_LAUNCH_CHECK();
   if (stats) {
     stats->iterations = host.iter;
@@ -508,23 +495,23 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
   return DSPH_OK;
 }
 
-extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p,
-                                   float* accel, void* stream) {
+extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, const float* vol, const float* rho,
+                                   const float* p, float* accel, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
   if (pairs->nq == 0) return DSPH_OK;
   if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel),
-                                                                    nullptr);
+  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+      P, reinterpret_cast<const float2*>(kgrad), vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
   return launch_status();
 }
 
-extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* rho,
-                                  const float* vol, const float* alpha, const float* src, float* p, float* accel,
-                                  float* residual, int32_t iterations, double* final_error, void* ws, size_t ws_bytes,
-                                  void* stream) {
+extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
+                                  const float* rho, const float* vol, const float* alpha, const float* src, float* p,
+                                  float* accel, float* residual, int32_t iterations, double* final_error, void* ws,
+                                  size_t ws_bytes, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (!prm || pairs->nq != pairs->nr || iterations < 0) return DSPH_ERR_ARG;
@@ -539,27 +526,14 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
   w += al256(sizeof(SolveCtrl));
   double* partials = reinterpret_cast<double*>(w);
+  const float2* kv = reinterpret_cast<const float2*>(kgrad);
   // Fixed count: the loop condition can never stop it early.
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, iterations, iterations, 0.0);
-  SweepBArgs B;
-  B.vol = vol;
-  B.src = src;
-  B.alpha = alpha;
-  B.p = p;
-  B.residual = residual;
-  B.acc = reinterpret_cast<const float2*>(accel);
-  B.partials = partials;
-  B.ctrl = ctrl;
-  B.dtf = prm->source_density ? (float)((double)prm->dt * (double)prm->dt) : prm->dt;
-  B.omega = prm->omega;
-  B.inv_rho0_dummy = 0.f;
-  B.rho0 = prm->rho0;
-  B.neg_thr = -prm->error_threshold;
-  B.clamp = prm->clamp_pressure;
-  B.n = (int)n;
+  SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);
   for (int t = 0; t < iterations; ++t) {
-    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
     k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+      k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
   }
   DSPH_LAUNCH_CHECK();
   if (final_error) {

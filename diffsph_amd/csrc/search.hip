@@ -150,23 +150,56 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
   counts[i] = c;
 }
 
+// Fill row i.  CSR (slice == NULL): entries at off[i] + n.  Sliced ELL:
+// entries at slice[i/64] + 64 n + i%64, the rest of the slice's columns
+// padded with -1.  Writes stay below `cap` and inside the row's own range.
 __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ slice,
                                                           int32_t* __restrict__ nbr, int64_t cap) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   float2 qi = a.qpos[i];
   if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
   float hi = h_of(a.qh, i, a.h0);
-  int64_t o = off[i];
-  const int64_t end = off[i + 1];
+  int64_t base, step, len;
+  if (slice) {
+    const int s = i >> 6;
+    base = (int64_t)slice[s] + (i & 63);
+    step = 64;
+    len = ((int64_t)slice[s + 1] - slice[s]) >> 6;
+  } else {
+    base = off[i];
+    step = 1;
+    len = (int64_t)off[i + 1] - off[i];
+  }
+  int64_t n = 0;
   for_candidates(a, qi, [&](int b, int e) {
     for (int j = b; j < e; ++j) {
       if (keep_pair(qi, hi, a.rpos[j], h_of(a.rh, j, a.h0), a)) {
-        if (o < end && o < cap) nbr[o] = j;
-        ++o;
+        const int64_t o = base + n * step;
+        if (n < len && o < cap) nbr[o] = j;
+        ++n;
       }
     }
   });
+  for (; n < len; ++n) {
+    const int64_t o = base + n * step;
+    if (o < cap) nbr[o] = -1;
+  }
+}
+
+// Longest row of every 64-row slice, times 64 (one wave per slice).
+__global__ void __launch_bounds__(kBlock) k_slice_len(const int32_t* __restrict__ counts, int nq,
+                                                      int32_t* __restrict__ slen) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = i < nq ? counts[i] : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int t = __shfl_xor(v, o, 64);
+    v = t > v ? t : v;
+  }
+  const int nslices = (nq + 63) >> 6;
+  if ((threadIdx.x & 63) == 0 && (i >> 6) < nslices) slen[i >> 6] = v * 64;
 }
 
 inline GridDev make_grid(const dsph_grid& g) {
@@ -322,15 +355,45 @@ extern "C" int dsph_neighbor_count(const dsph_domain* domain, const dsph_grid* g
 extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
                                   const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos,
                                   const float* qh, int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode,
-                                  const int32_t* offsets, int32_t* nbr, int64_t capacity, void* stream) {
+                                  const int32_t* offsets, const int32_t* slice_off, int64_t layout_total,
+                                  int32_t* nbr, int64_t capacity, void* stream) {
   SearchArgs a;
   int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
                       h_uniform, mode);
   if (s != DSPH_OK) return s;
   if (nq == 0) return DSPH_OK;
-  if (!offsets || (capacity > 0 && !nbr) || capacity < 0) return DSPH_ERR_ARG;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, nbr, capacity);
+  if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
+  if (layout_total > capacity) return DSPH_ERR_CAPACITY;
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity);
   return launch_status();
+}
+
+extern "C" size_t dsph_sell_workspace(int64_t nq) {
+  int64_t ns = (nq + 63) / 64;
+  return align256((size_t)(ns > 0 ? ns : 1) * 4) + scan_ws(ns);
+}
+
+extern "C" int dsph_sell_layout(const int32_t* counts, int64_t nq, int32_t* slice_off, int64_t* total_host,
+                                void* ws, size_t ws_bytes, void* stream) {
+  if (nq < 0 || nq >= INT32_MAX || !slice_off || (nq > 0 && !counts)) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_sell_workspace(nq)) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  const int64_t ns = (nq + 63) / 64;
+  int32_t* slen = static_cast<int32_t*>(ws);
+  char* scan_w = static_cast<char*>(ws) + align256((size_t)(ns > 0 ? ns : 1) * 4);
+  if (nq > 0) {
+    k_slice_len<<<grid_for(nq), kBlock, 0, st>>>(counts, (int)nq, slen);
+    DSPH_LAUNCH_CHECK();
+  }
+  int s = scan_exclusive(slen, ns, slice_off, scan_w, scan_ws(ns), st);
+  if (s != DSPH_OK) return s;
+  if (total_host) {
+    int32_t t = 0;
+    DSPH_CHECK_HIP(hipMemcpyAsync(&t, slice_off + ns, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    DSPH_CHECK_HIP(hipStreamSynchronize(st));
+    *total_host = t;
+  }
+  return DSPH_OK;
 }
 
 extern "C" int dsph_wrap_positions(const dsph_domain* domain, const float* pos, int64_t n, float* out, void* stream) {

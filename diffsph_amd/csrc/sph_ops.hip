@@ -19,8 +19,8 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
   float acc = 0.0f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
     const float2 xj = P.rpos[j];
@@ -58,8 +58,8 @@ __global__ void __launch_bounds__(kBlock) k_sph_operation(PairsDev P, int op, in
   float acc[W * 2];
 #pragma unroll
   for (int c = 0; c < W * 2; ++c) acc[c] = 0.0f;
-  const int b = P.off[i], e = P.off[i + 1];
-  for (int k = b; k < e; ++k) {
+  const RowRef row = row_ref(P, i);
+  for (int n = 0, k = row.base; n < row.cnt; ++n, k += row.step) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
     Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
@@ -128,13 +128,21 @@ __global__ void __launch_bounds__(kBlock) k_sph_operation(PairsDev P, int op, in
 // ---------------------------------------------------------------------------
 // Export of the reference's per-pair neighborDict arrays.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_export_counts(int nq, const int32_t* __restrict__ off,
-                                                          const int32_t* __restrict__ q_rank,
+__global__ void __launch_bounds__(kBlock) k_export_counts(PairsDev P, const int32_t* __restrict__ q_rank,
                                                           int32_t* __restrict__ cnt) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+  if (q >= P.nq) return;
   int row = q_rank ? q_rank[q] : q;
-  cnt[q] = off[row + 1] - off[row];
+  cnt[q] = row_ref(P, row).cnt;
+}
+
+// Sliced list -> CSR order.
+__global__ void __launch_bounds__(kBlock) k_sell_to_csr(PairsDev P, int32_t* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const RowRef rr = row_ref(P, i);
+  const int o = P.off[i];
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) out[o + n] = P.nbr[k];
 }
 
 __global__ void __launch_bounds__(kBlock) k_export_pairs(PairsDev P, const int32_t* __restrict__ q_sort,
@@ -148,9 +156,9 @@ __global__ void __launch_bounds__(kBlock) k_export_pairs(PairsDev P, const int32
   const int row = q_rank ? q_rank[q] : q;
   const float2 xi = P.qpos[row];
   const float hi = h_of(P.qh, row, P.h0);
-  const int b = P.off[row], e = P.off[row + 1];
+  const RowRef rr = row_ref(P, row);
   int64_t o = out_off[q];
-  for (int k = b; k < e; ++k, ++o) {
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step, ++o) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
     const float h = pick_h(P.mode, hi, hj);
@@ -258,7 +266,8 @@ __global__ void __launch_bounds__(kBlock) k_op_scatter_add(const float* __restri
 int check_pairs(const dsph_pairs* p) {
   if (!p || p->nq < 0 || p->nr < 0 || p->nq >= INT32_MAX || p->nr >= INT32_MAX) return DSPH_ERR_ARG;
   if (p->mode < 0 || p->mode > 2) return DSPH_ERR_ARG;
-  if (p->nq > 0 && (!p->offsets || !p->qpos)) return DSPH_ERR_ARG;
+  if (p->nq > 0 && (!p->qpos || (!p->slice_off && !p->offsets))) return DSPH_ERR_ARG;
+  if (p->nq > 0 && p->slice_off && !p->counts) return DSPH_ERR_ARG;
   if (p->nr > 0 && !p->rpos) return DSPH_ERR_ARG;
   if (((p->mode != DSPH_MODE_GATHER && !p->rh) || (p->mode != DSPH_MODE_SCATTER && !p->qh)) && !(p->h_uniform > 0.0f))
     return DSPH_ERR_ARG;
@@ -334,7 +343,7 @@ extern "C" int dsph_pairs_export(const dsph_pairs* pairs, const int32_t* q_sort_
   int32_t* cnt = static_cast<int32_t*>(ws);
   char* scan_w = static_cast<char*>(ws) + ((size_t)(pairs->nq > 0 ? pairs->nq : 1) * 4 + 255) / 256 * 256;
   if (P.nq > 0) {
-    k_export_counts<<<grid_for(P.nq), kBlock, 0, st>>>(P.nq, P.off, q_rank, cnt);
+    k_export_counts<<<grid_for(P.nq), kBlock, 0, st>>>(P, q_rank, cnt);
     DSPH_LAUNCH_CHECK();
   }
   s = scan_exclusive(cnt, P.nq, out_offsets, scan_w, scan_ws(P.nq), st);
@@ -343,6 +352,17 @@ extern "C" int dsph_pairs_export(const dsph_pairs* pairs, const int32_t* q_sort_
   k_export_pairs<<<grid_for(P.nq), kBlock, 0, st>>>(P, q_sort_index, q_rank, r_sort_index, out_offsets, out_i, out_j,
                                                    out_distances, out_vectors, out_supports, out_kernels,
                                                    out_gradients);
+  return launch_status();
+}
+
+extern "C" int dsph_sell_to_csr(const dsph_pairs* pairs, int32_t* out_nbr, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!pairs->offsets) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!out_nbr || !pairs->nbr) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_sell_to_csr<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, out_nbr);
   return launch_status();
 }
 

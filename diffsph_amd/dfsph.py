@@ -96,9 +96,11 @@ class DFSPHSolver:
         self.adv = torch.empty(n, 2, **f)
         self.dudt = torch.empty(n, 2, **f)
         self.residual = torch.empty(n, **f)
+        self.kgrad = None   # V_j gradW_ij per neighbor-list entry, [entries, 2]
         self.grid = None
         self.nl = None
         self.step_count = 0
+        self._last_iters = {}
         self.hooks = {}  # optional: 'den_init' -> callable(solver) returning an initial pressure
 
     # -- helpers ---------------------------------------------------------
@@ -127,6 +129,9 @@ class DFSPHSolver:
         self.p_den = self._permute(self.p_den, si)
         self.grid = grid
         self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True)
+        need = max(self.nl.entries, 1)
+        if self.kgrad is None or self.kgrad.shape[0] < need:
+            self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
         return self.nl
 
     def _visc_params(self):
@@ -153,12 +158,16 @@ class DFSPHSolver:
         prm.max_iters = cfg.max_iters
         prm.clamp_pressure = 1 if cfg.clamp_pressure else 0
         prm.source_density = 1 if density else 0
-        prm.check_every = cfg.check_every
+        # Queue as many iterations as the previous solve of this kind needed
+        # before the first host check (iterations past convergence are no-ops).
+        prev = self._last_iters.get(density, 0)
+        prm.check_every = cfg.check_every if cfg.check_every > 0 else max(4, min(prev + 2, cfg.max_iters))
         stats = _lib.SolveStats()
         L = lib()
         ws = WORKSPACE.get(L.dsph_dfsph_workspace(self.n), p.device)
         alpha = self.alpha_den if density else self.alpha_div
-        check(L.dsph_dfsph_solve(C.byref(pairs), C.byref(prm), ptr(self.vel), ptr(adv), ptr(self.rho),
+        check(L.dsph_dfsph_solve(C.byref(pairs), C.byref(prm), ptr(self.kgrad), ptr(self.vel), ptr(adv),
+                                 ptr(self.rho),
                                  ptr(self.vol), ptr(self.areas), ptr(alpha), ptr(self.div_src), ptr(p),
                                  ptr(accel), ptr(self.residual), C.byref(stats), ptr(ws), ws.numel(),
                                  _lib.stream_handle(p.device)), "dfsph_solve")
@@ -178,7 +187,8 @@ class DFSPHSolver:
         check(L.dsph_dfsph_prepare(C.byref(pairs), ptr(self.vel), ptr(self.mass), ptr(self.areas), ptr(self.vol),
                                    ptr(self.rho), C.c_float(cfg.dt), C.c_float(cfg.rho0),
                                    C.byref(vp) if vp is not None else None, ptr(self.alpha_div),
-                                   ptr(self.alpha_den), ptr(self.div_src), ptr(self.visc) if vp else None, st),
+                                   ptr(self.alpha_den), ptr(self.div_src), ptr(self.visc) if vp else None,
+                                   ptr(self.kgrad), st),
               "dfsph_prepare")
         if vp is None:
             self.visc.zero_()
@@ -186,6 +196,7 @@ class DFSPHSolver:
         p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
         s = self._solve(pairs, False, p_div, self.a_div, None)
         out.div_iters, out.div_error = s.iterations, s.final_error
+        self._last_iters[False] = s.iterations
         self.p_div = p_div
         # advection for the density solve: g + visc + a_div (dfsph.py:199)
         adv = self.adv
@@ -199,12 +210,13 @@ class DFSPHSolver:
             p_den = (cfg.warm_start * self.p_den) if self.p_den is not None else torch.zeros_like(self.rho)
         s = self._solve(pairs, True, p_den, self.a_den, adv)
         out.den_iters, out.den_error, out.den_error0 = s.iterations, s.final_error, s.initial_error
+        self._last_iters[True] = s.iterations
         self.p_den = p_den
         g = None
         if cfg.gravity is not None:
-            g = torch.tensor(cfg.gravity, dtype=torch.float32, device=self.pos.device)
+            g = (C.c_float * 2)(*[float(v) for v in cfg.gravity])  # host pointer
         check(L.dsph_integrate_semi_implicit(ptr(self.pos), ptr(self.vel), ptr(self.a_den), ptr(self.visc),
-                                             ptr(self.a_div), None, ptr(g), self.n, C.c_float(cfg.dt),
+                                             ptr(self.a_div), None, g, self.n, C.c_float(cfg.dt),
                                              ptr(self.dudt), st), "integrate")
         self.step_count += 1
         return out

@@ -118,15 +118,21 @@ class CellGrid:
 
 @dataclass
 class NeighborList:
-    """CSR neighbor list; rows are query particles, entries sorted reference slots."""
+    """Neighbor list; rows are query particles, entries sorted reference slots.
+
+    ``nbr`` is in the sliced-ELL layout (64 rows per slice, column-major,
+    padded with -1) when ``slice_off`` is set, else CSR.  ``offsets`` is the
+    CSR scan of ``counts`` in both cases."""
     counts: torch.Tensor    # int32 [nq]
     offsets: torch.Tensor   # int32 [nq+1]
-    nbr: torch.Tensor       # int32 [P]
+    nbr: torch.Tensor       # int32 [entries]
     num_pairs: int
     qpos: torch.Tensor      # wrapped query positions [nq,2] (row order)
     qh: torch.Tensor | None
     grid: CellGrid
     mode: int
+    slice_off: torch.Tensor | None = None
+    entries: int = 0        # length of the layout (pairs + padding)
 
     def pairs(self) -> _lib.Pairs:
         p = _lib.Pairs()
@@ -141,16 +147,28 @@ class NeighborList:
         p.h_uniform = self.grid.h_uniform
         p.mode = self.mode
         p.domain = self.grid.domain
+        p.counts = ptr(self.counts)
+        p.slice_off = ptr(self.slice_off)
         return p
+
+    def csr_nbr(self) -> torch.Tensor:
+        """Neighbor indices in CSR order (rows back to back)."""
+        if self.slice_off is None:
+            return self.nbr
+        out = torch.empty(max(self.num_pairs, 1), dtype=torch.int32, device=self.nbr.device)
+        pr = self.pairs()
+        check(lib().dsph_sell_to_csr(C.byref(pr), ptr(out), _lib.stream_handle(out.device)), "sell_to_csr")
+        return out[: self.num_pairs]
 
 
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
-           qpos_wrapped: bool = False, capacity: int | None = None) -> NeighborList:
+           qpos_wrapped: bool = False, layout: str = "sell") -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
-    Counts, scans (one host read of the pair total, as the reference's
-    `.item()` does) and fills.  With ``qpos_wrapped`` the queries are the
-    grid's own wrapped sorted positions."""
+    Counts, scans (host reads of the pair and entry totals, as the
+    reference's `.item()` does) and fills.  With ``qpos_wrapped`` the
+    queries are the grid's own wrapped sorted positions.  ``layout`` is
+    "sell" (sliced ELL, the layout the pair kernels stream) or "csr"."""
     _check_f32(qpos, "queryPositions", (2,))
     m = MODES[mode] if isinstance(mode, str) else int(mode)
     nq = qpos.shape[0]
@@ -167,14 +185,26 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     total = C.c_int64(0)
     check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), C.byref(total), ptr(ws), ws.numel(), st), "scan")
     npairs = int(total.value)
-    cap = max(npairs, 1) if capacity is None else capacity
+    slice_off = None
+    entries = npairs
+    if layout == "sell":
+        slice_off = torch.empty((nq + 63) // 64 + 1, dtype=torch.int32, device=dev)
+        ws2 = WORKSPACE.get(L.dsph_sell_workspace(nq), dev)
+        tot2 = C.c_int64(0)
+        check(L.dsph_sell_layout(ptr(counts), nq, ptr(slice_off), C.byref(tot2), ptr(ws2), ws2.numel(), st),
+              "sell_layout")
+        entries = int(tot2.value)
+    elif layout != "csr":
+        raise ValueError(f"unknown layout {layout}")
+    cap = max(entries, 1)
     nbr = torch.empty(cap, dtype=torch.int32, device=dev)
-    check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(nbr), cap, st), "neighbor_fill")
+    check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(slice_off), entries, ptr(nbr), cap, st), "neighbor_fill")
     if qpos_wrapped:
         wq = qpos
     else:
         wq = wrap_positions(qpos, grid.domain)
-    return NeighborList(counts, offsets, nbr[:npairs], npairs, wq, qh, grid, m)
+    return NeighborList(counts, offsets, nbr[:max(entries, 1)] if layout == "sell" else nbr[:npairs], npairs, wq, qh,
+                        grid, m, slice_off, entries)
 
 
 def wrap_positions(pos: torch.Tensor, domain: _lib.Domain) -> torch.Tensor:

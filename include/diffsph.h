@@ -62,14 +62,21 @@ typedef struct dsph_grid {
   int32_t pad_;
 } dsph_grid;
 
-/* A CSR neighbor list plus what is needed to evaluate every pair on the fly.
- * Row i (a query particle) holds reference indices nbr[offsets[i] ..
- * offsets[i+1]).  Positions are the *wrapped* positions the search used. */
+/* A neighbor list plus what is needed to evaluate every pair on the fly.
+ * Two layouts of the same rows:
+ *  - CSR (slice_off == NULL): row i holds nbr[offsets[i] .. offsets[i+1]);
+ *  - sliced ELL, 64 rows per slice (slice_off != NULL): entry n of row i is
+ *    nbr[slice_off[i/64] + 64*n + i%64] for n < counts[i]; a slice is padded
+ *    to its longest row.  One slice is one wave64: every load of a
+ *    neighbor index (or of a per-pair value stored in the same layout) is a
+ *    contiguous 256-byte access.
+ * Entries of a row are in the same order in both layouts.  Positions are
+ * the *wrapped* positions the search used. */
 typedef struct dsph_pairs {
   int64_t nq;               /* query particles (rows)          */
   int64_t nr;               /* reference particles             */
-  const int32_t* offsets;   /* [nq+1]                          */
-  const int32_t* nbr;       /* [offsets[nq]]                   */
+  const int32_t* offsets;   /* [nq+1] CSR row offsets          */
+  const int32_t* nbr;       /* neighbor indices (either layout) */
   const float* qpos;        /* [nq,2] wrapped query positions  */
   const float* rpos;        /* [nr,2] wrapped reference pos.   */
   const float* qh;          /* [nq] or NULL -> h_uniform       */
@@ -77,7 +84,21 @@ typedef struct dsph_pairs {
   float h_uniform;
   int32_t mode;             /* DSPH_MODE_*                     */
   dsph_domain domain;
+  const int32_t* counts;    /* [nq] row lengths (sliced layout) */
+  const int32_t* slice_off; /* [ceil(nq/64)+1] or NULL (CSR)    */
 } dsph_pairs;
+
+/* Sliced-ELL layout from row counts: slice_off[s] = 64 * sum over earlier
+ * slices of their longest row; slice_off[nslices] = padded entry count,
+ * also copied to *total_host when not NULL (synchronises the stream). */
+size_t dsph_sell_workspace(int64_t nq);
+int dsph_sell_layout(const int32_t* counts, int64_t nq, int32_t* slice_off,
+                     int64_t* total_host, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* Copy a sliced list into CSR order: out_nbr[offsets[i] + n] = entry n of
+ * row i. */
+int dsph_sell_to_csr(const dsph_pairs* pairs, int32_t* out_nbr, void* stream);
 
 const char* dsph_status_string(int status);
 int dsph_abi_version(void);
@@ -123,17 +144,21 @@ int dsph_neighbor_count(const dsph_domain* domain, const dsph_grid* grid,
                         int32_t qpos_wrapped, float h_uniform, int32_t mode,
                         int32_t* counts, void* stream);
 
-/* Fill the CSR list: row i receives the sorted reference indices of its
- * neighbors, cells visited dy-major (-1,0,+1) then dx, particles in sorted
- * order (replaces buildNeighborList, neighborhood.cpp:253-435).
- * offsets[nq+1] is the exclusive scan of the counts; entries past
- * `capacity` are not written and DSPH_ERR_CAPACITY is returned. */
+/* Fill the neighbor list: row i receives the sorted reference indices of
+ * its neighbors, cells visited dy-major (-1,0,+1) then dx, particles in
+ * sorted order (replaces buildNeighborList, neighborhood.cpp:253-435).
+ * With slice_off == NULL the list is CSR (offsets[nq+1] = exclusive scan of
+ * the counts); otherwise it is the sliced-ELL layout of dsph_sell_layout
+ * (offsets unused, padding entries set to -1).  The list must hold
+ * `capacity` entries; a layout needing more is rejected with
+ * DSPH_ERR_CAPACITY before anything is written. */
 int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
                        const int32_t* cell_start, const float* sorted_pos,
                        const float* sorted_h, int64_t nref,
                        const float* qpos, const float* qh, int64_t nq,
                        int32_t qpos_wrapped, float h_uniform, int32_t mode,
-                       const int32_t* offsets, int32_t* nbr, int64_t capacity,
+                       const int32_t* offsets, const int32_t* slice_off,
+                       int64_t layout_total, int32_t* nbr, int64_t capacity,
                        void* stream);
 
 /* out = getPeriodicPosition(pos) (v2/modules/neighborhood.py:407-430):
@@ -240,12 +265,16 @@ typedef struct dsph_visc_params {
   int32_t scheme; /* 0 deltaSPH_inviscid, 1 deltaSPH_viscid (alpha = nu) */
 } dsph_visc_params;
 
+/* kgrad (optional, NULL = not stored): V_j gradW_ij of every list entry,
+ * [entries][2] in the list's layout (CSR or sliced), bit-identical to the
+ * on-the-fly value; the solve sweeps stream it instead of re-evaluating the
+ * pair geometry. */
 int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel,
                        const float* mass, const float* areas, const float* vol,
                        const float* rho, float dt, float rho0,
                        const dsph_visc_params* visc,
                        float* alpha_div, float* alpha_den, float* div_src,
-                       float* visc_acc, void* stream);
+                       float* visc_acc, float* kgrad, void* stream);
 
 size_t dsph_dfsph_workspace(int64_t n);
 
@@ -258,7 +287,7 @@ size_t dsph_dfsph_workspace(int64_t n);
  * final p.  Convergence is decided on the device (fp64 mean residual);
  * the host is synchronised once every `check_every` iterations. */
 int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
-                     const float* vel, const float* adv, const float* rho,
+                     const float* kgrad, const float* vel, const float* adv, const float* rho,
                      const float* vol, const float* areas, const float* alpha,
                      const float* src0, float* p, float* accel,
                      float* residual, dsph_solve_stats* stats,
@@ -266,7 +295,8 @@ int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
 
 /* Pressure acceleration a_i = -(1/rho_i) sum_j V_j (p_i + p_j) gradW_ij
  * (computePressureAcceleration, v2/modules/dfsph.py:35-37); one pair sweep. */
-int dsph_pressure_accel(const dsph_pairs* pairs, const float* vol,
+int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad,
+                        const float* vol,
                         const float* rho, const float* p, float* accel,
                         void* stream);
 
@@ -275,7 +305,7 @@ int dsph_pressure_accel(const dsph_pairs* pairs, const float* vol,
  * (updatePressure, v2/modules/dfsph.py:39-50).  final_error (host, may be
  * NULL -> no synchronisation) receives the error metric of the last one. */
 int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
-                       const float* rho, const float* vol, const float* alpha,
+                       const float* kgrad, const float* rho, const float* vol, const float* alpha,
                        const float* src, float* p, float* accel,
                        float* residual, int32_t iterations,
                        double* final_error, void* workspace,
@@ -289,6 +319,7 @@ int dsph_density_error(const float* rho, int64_t n, float rho0, double* out2,
 /* Semi-implicit Euler (v2/modules/integration.py:111-138):
  * v += dt*dudt; x += dt*v.  dudt = sum of up to 4 acceleration fields
  * (NULL entries skipped), as dfsph.py:209 a_p + g + visc + a_div. */
+/* gravity2 is a HOST pointer to (gx, gy), or NULL. */
 int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
                                  const float* a1, const float* a2,
                                  const float* a3, const float* gravity2,

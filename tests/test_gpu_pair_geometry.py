@@ -40,7 +40,7 @@ def test_pair_geometry_bitexact(nside, jit):
     nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True)
     ex = export_pairs(nl)
     off = nl.offsets.cpu().numpy()
-    nbr = nl.nbr.cpu().numpy()
+    nbr = nl.csr_nbr().cpu().numpy()
     sp = g.sorted_pos.cpu().numpy()
     i = np.repeat(np.arange(len(off) - 1), np.diff(off))
     j = nbr

@@ -49,7 +49,7 @@ def test_search_matches_oracle_bitexact(orc, nside, seed, jit):
     nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True)
     np.testing.assert_array_equal(nl.counts.cpu().numpy(), c_o)
     np.testing.assert_array_equal(nl.offsets.cpu().numpy(), off_o)
-    np.testing.assert_array_equal(nl.nbr.cpu().numpy(), nbr_o)
+    np.testing.assert_array_equal(nl.csr_nbr().cpu().numpy(), nbr_o)
 
 
 def test_search_matches_brute_force_sets_small(orc):
@@ -74,7 +74,7 @@ def test_search_matches_brute_force_sets_small(orc):
             g = CellGrid.build(tp, th, dom, float(h.max()))
             nl = search(g, tp, th, mode)
             off = nl.offsets.cpu().numpy()
-            nbr = nl.nbr.cpu().numpy()
+            nbr = nl.csr_nbr().cpu().numpy()
             si = g.sort_index.cpu().numpy()
             got = set()
             for i in range(n):
@@ -252,7 +252,7 @@ def test_large_search_properties():
     # symmetry (uniform h): count pairs (i,j) with i<j equals those with i>j
     off = nl.offsets.long()
     rows = torch.repeat_interleave(torch.arange(g.n, device=dev), cnt.long())
-    cols = nl.nbr.long()
+    cols = nl.csr_nbr().long()
     assert int((rows < cols).sum()) == int((rows > cols).sum())
     assert int((rows == cols).sum()) == g.n
     # symmetric set: sorted keys of (i,j) equal sorted keys of (j,i)
