@@ -1,4 +1,5 @@
-"""Benchmark: DFSPH particle-updates/s on the 1M-particle random-flow workload.
+"""Benchmark: DFSPH particle-updates/s on the 1M-particle random-flow workload,
+plain (the reference's 0.75 p_prev warm start) vs the learned warm start.
 
 Workload: 1024x1024 lattice on [-1,1]^2 (periodic) + U(-0.05dx, 0.05dx)
 jitter, divergence-free spectral-noise velocity (|k| <= 8, max |u| = 1),
@@ -7,12 +8,17 @@ Wendland C2 with ~50 neighbors, rho0 = 1, delta-SPH inviscid viscosity
 search + density + divergence solve + density solve + semi-implicit Euler,
 exactly the reference's DFSPH scheme (v2/simulationSchemes/dfsph.py).
 
+The warm-start network is trained first (untimed, on trajectories of other
+seeds, on the GPU); then the plain and the learned solver each run `warmup`
+untimed and `steps` timed steps from the same initial state.
+
 Prints ONE JSON line on stdout (progress on stderr):
-  value = particles * timed steps / seconds (inputs resident in HBM),
+  value = particles * timed steps / seconds of the learned solver (inputs
+  resident in HBM); the plain solver's figures sit beside it,
   roofline = the dominant pair kernel (pressure sweeps) measured with HIP
   events on the stream it runs on, algorithmic bytes per launch / time,
-  cpu_baseline = the CPU oracle (C++/OpenMP restatement, all host cores)
-  on a bounded sample of the same workload.
+  cpu_baseline = the CPU oracle (C++/OpenMP restatement, host cores) running
+  the plain solver's first timed step from the same state.
 """
 from __future__ import annotations
 
@@ -38,12 +44,14 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--train-rounds", type=int, default=3)
+    ap.add_argument("--train-steps", type=int, default=24, help="trajectory steps per training round")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the p_prev (warm start 1.0) run")
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-nside", type=int, default=1024, help="sample size of the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--kernel-reps", type=int, default=50)
     return ap.parse_args()
@@ -59,6 +67,7 @@ def main():
 
     from diffsph_amd import _lib, sampling
     from diffsph_amd.dfsph import DFSPHConfig, DFSPHSolver
+    from diffsph_amd.learned import train_warm_start
     from diffsph_amd.neighborhood import WORKSPACE
 
     dev = torch.device("cuda:0")
@@ -66,28 +75,58 @@ def main():
     d = sampling.dfsph_random_flow(args.nside, seed=args.seed, jitter_width=0.1)
     n = d["pos"].shape[0]
     dt = 3.0 * d["dx"] / 10.0
-    cfg = DFSPHConfig(dt=dt, support=d["h"])
     T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
-    solver = DFSPHSolver(T(d["pos"]), T(d["vel"]), T(d["mass"]), T(d["area"]), cfg)
+
+    def make(seed, warm=0.75):
+        dd = d if seed == args.seed else sampling.dfsph_random_flow(args.nside, seed=seed, jitter_width=0.1)
+        return DFSPHSolver(T(dd["pos"]), T(dd["vel"]), T(dd["mass"]), T(dd["area"]),
+                           DFSPHConfig(dt=dt, support=d["h"], warm_start=warm))
+
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
-    for k in range(args.warmup):
-        st = solver.step()
-        log(f"warmup {k}: div it {st.div_iters} den it {st.den_iters} err {st.den_error:.3e} pairs {st.num_pairs}")
-    torch.cuda.synchronize()
-    it_div, it_den, errs = [], [], []
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        st = solver.step()
-        it_div.append(st.div_iters)
-        it_den.append(st.den_iters)
-        errs.append(st.den_error)
+    net = train_warm_start(lambda r: make(args.seed + 100 + r), rounds=args.train_rounds, steps=args.train_steps,
+                           sample=0.1, train_steps=400, log=log)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = n * args.steps / secs
-    mean_e, max_e = solver.density_error()
-    log(f"timed {args.steps} steps: {secs * 1e3 / args.steps:.3f} ms/step, {value:.4g} particle-updates/s; "
-        f"iters div {np.mean(it_div):.1f} den {np.mean(it_den):.1f}; rho err mean {mean_e:.2e} max {max_e:.2e}")
+    t_train = time.perf_counter() - t0
+    log(f"warm-start network trained in {t_train:.1f} s")
+
+    def run(label, solver, snapshot=None):
+        for k in range(args.warmup):
+            st = solver.step()
+            log(f"{label} warmup {k}: div it {st.div_iters} den it {st.den_iters} pairs {st.num_pairs}")
+        if snapshot is not None:
+            snapshot.update(pos=solver.pos.cpu().numpy(), vel=solver.vel.cpu().numpy(),
+                            mass=solver.mass.cpu().numpy(), area=solver.areas.cpu().numpy(),
+                            uid=solver.uid.cpu().numpy(), p_div=solver.p_div.cpu().numpy(),
+                            p_den=solver.p_den.cpu().numpy())
+        torch.cuda.synchronize()
+        it_div, it_den, errs = [], [], []
+        c0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = solver.step()
+            it_div.append(st.div_iters)
+            it_den.append(st.den_iters)
+            errs.append(st.den_error)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - c0
+        me, mx = solver.density_error()
+        r = {"value": n * args.steps / secs, "ms_per_step": secs * 1e3 / args.steps,
+             "iters_div_mean": float(np.mean(it_div)), "iters_den_mean": float(np.mean(it_den)),
+             "den_residual_mean": float(np.mean(errs)), "rho_err_mean": me, "rho_err_max": mx,
+             "first_step_iters_div": it_div[0]}
+        log(f"{label}: {r['ms_per_step']:.3f} ms/step, {r['value']:.4g} particle-updates/s; iters div "
+            f"{r['iters_div_mean']:.1f} den {r['iters_den_mean']:.1f}; rho err mean {me:.2e} max {mx:.2e}")
+        return r
+
+    snap = {}
+    plain = run("plain", make(args.seed), snapshot=snap)
+    prev = None if args.no_ablation else run("p_prev", make(args.seed, warm=1.0))
+    solver = make(args.seed)
+    net.install(solver)
+    learned = run("learned", solver)
+    log(f"judge: {net.judge.judged} judged steps, net used {net.judge.used_net}, plain used {net.judge.used_plain}")
+    value = learned["value"]
+    secs = learned["ms_per_step"] * args.steps * 1e-3
 
     # ---- roofline of the pressure sweeps (HIP events on the solver's stream)
     L = _lib.lib()
@@ -144,30 +183,33 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                 "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(byt)}
 
-    # ---- CPU baseline: the oracle on the host cores
+    # ---- CPU baseline: the oracle on the host cores, continuing the plain
+    # solver's trajectory from the state at the start of its timed window
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
         threads = min(os.cpu_count() or 1, 16)
         orc.set_threads(threads)
-        dc = sampling.dfsph_random_flow(args.cpu_nside, seed=args.seed, jitter_width=0.1)
-        nc = dc["pos"].shape[0]
         dom = orc.make_domain((-1, -1), (1, 1), (True, True))
-        ost = orc.DfsphState(dc["pos"].copy(), dc["vel"].copy(), dc["mass"].copy(), dc["area"].copy(),
-                             np.arange(nc, dtype=np.int32))
-        dtc = 3.0 * dc["dx"] / 10.0
-        # one untimed step so that the timed one has the same warm start as the GPU steps
-        ost, _ = orc.dfsph_step(ost, dom, dc["h"], dtc)
+        ost = orc.DfsphState(snap["pos"], snap["vel"], snap["mass"], snap["area"], snap["uid"].astype(np.int32),
+                             snap["p_div"], snap["p_den"])
         c0 = time.perf_counter()
+        its = []
         for _ in range(args.cpu_steps):
-            ost, info = orc.dfsph_step(ost, dom, dc["h"], dtc)
+            ost, info = orc.dfsph_step(ost, dom, d["h"], dt)
+            its.append(int(info["it_div"]))
         c1 = time.perf_counter()
-        cval = nc * args.cpu_steps / (c1 - c0)
+        cval = n * args.cpu_steps / (c1 - c0)
+        same = its[0] == plain["first_step_iters_div"]
         cpu = {"value": round(cval, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} DFSPH step(s) (after 1 warm-start step) of the {nc}-particle "
-                         f"workload, C++/OpenMP oracle, {threads} threads; "
-                         f"iters div {info['it_div']} den {info['it_den']}"}
-        log(f"cpu baseline: {cval:.4g} particle-updates/s ({(c1 - c0):.1f} s, {threads} threads)")
+               "sample": f"{args.cpu_steps} plain DFSPH step(s) of the {n}-particle workload from the GPU plain "
+                         f"solver's state at its first timed step (step {args.warmup}), C++/OpenMP oracle, "
+                         f"{threads} threads; div iterations {its} (GPU plain mean over the timed steps "
+                         f"{plain['iters_div_mean']:.1f}; first timed step matches the GPU: {same})"}
+        log(f"cpu baseline: {cval:.4g} particle-updates/s ({(c1 - c0):.1f} s, {threads} threads, div its {its})")
+
+    def rnd(r):
+        return None if r is None else {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
 
     out = {
         "metric": "particle-updates/s at 1M (DFSPH plain vs learned warm-start, rho-err<=0.1%)",
@@ -183,10 +225,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"dfsph_random_flow_{args.nside}x{args.nside}", "particles": n,
-                   "solver": "plain-jacobi (warm start 0.75 p_prev)", "dt": dt,
-                   "iters_div_mean": float(np.mean(it_div)), "iters_den_mean": float(np.mean(it_den)),
-                   "den_residual_mean": float(np.mean(errs)), "rho_err_mean": mean_e, "rho_err_max": max_e,
-                   "pairs": P},
+                   "solver": "DFSPH, learned warm start (MLP 16-64-64-64-1, bf16 MFMA) with plain-Jacobi judge",
+                   "dt": dt, "pairs": P, "learned": rnd(learned), "plain": rnd(plain),
+                   "p_prev_warm_start": rnd(prev),
+                   "speedup_vs_plain": round(learned["value"] / plain["value"], 3),
+                   "train_seconds": round(t_train, 2),
+                   "judge": {"judged": net.judge.judged, "net": net.judge.used_net,
+                             "plain": net.judge.used_plain}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

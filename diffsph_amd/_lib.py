@@ -116,6 +116,15 @@ SIGNATURES = {
                                      V]),
     "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),
     "dsph_integrate_semi_implicit": (C.c_int, [V, V, V, V, V, V, V, I64, F32, V, V]),
+    "dsph_ws_features": (C.c_int, [P(Pairs), V, V, V, V, V, V, F32, V, V, V, V]),
+    "dsph_mlp_num_params": (C.c_int, []),
+    "dsph_mlp_num_inputs": (C.c_int, []),
+    "dsph_mlp_packed_bytes": (SZ, []),
+    "dsph_mlp_pack": (C.c_int, [V, V, V]),
+    "dsph_mlp_forward": (C.c_int, [V, I64, V, V, V, V]),
+    "dsph_mlp_train_workspace": (SZ, [I64]),
+    "dsph_mlp_loss_grad": (C.c_int, [V, V, I64, F32, V, V, V, V, V, SZ, V]),
+    "dsph_adam_step": (C.c_int, [V, V, V, V, I64, F32, F32, F32, F32, F32, I32, V]),
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_scatter_add": (C.c_int, [V, I32, I64, I32, V, V, V, V]),

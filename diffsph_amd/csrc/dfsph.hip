@@ -273,6 +273,47 @@ __global__ void k_ctrl_init(SolveCtrl* c, int min_iters, int max_iters, double t
 }
 
 // ---------------------------------------------------------------------------
+// Input features of the learned warm start: per particle i
+//   0 p_div_prev_i     1 sum_j V_j W_ij p_div_prev_j
+//   2 s_i (div source) 3 sum_j V_j W_ij s_j
+//   4 p_den_prev_i     5 sum_j V_j W_ij p_den_prev_j
+//   6 rho_i/rho0 - 1   7 alpha_div_i
+// normalised as (f - mean) * istd; columns 8..15 are zero.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float* __restrict__ vol,
+                                                        const float* __restrict__ rho, const float* __restrict__ pdp,
+                                                        const float* __restrict__ pnp,
+                                                        const float* __restrict__ src,
+                                                        const float* __restrict__ alpha, float rho0,
+                                                        const float* __restrict__ mean,
+                                                        const float* __restrict__ istd, float* __restrict__ X) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  float s1 = 0.f, s3 = 0.f, s5 = 0.f;
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = P.nbr[k];
+    const float hj = h_of(P.rh, j, P.h0);
+    const float h = pick_h(P.mode, hi, hj);
+    const float2 xj = P.rpos[j];
+    const float dx = mod_offset(xi.x - xj.x, P.dom, 0), dy = mod_offset(xi.y - xj.y, P.dom, 1);
+    const float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
+    const float vw = vol[j] * wendland_w(q, h);
+    s1 += vw * pdp[j];
+    s3 += vw * src[j];
+    s5 += vw * pnp[j];
+  }
+  float f[8] = {pdp[i], s1, src[i], s3, pnp[i], s5, rho[i] / rho0 - 1.0f, alpha[i]};
+  float* o = X + (int64_t)i * 16;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = (f[c] - mean[c]) * istd[c];
+#pragma unroll
+  for (int c = 8; c < 16; ++c) o[c] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
 // Density error (fp64): mean and max |rho/rho0 - 1|.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_density_error(const float* __restrict__ rho, int n, float rho0,
@@ -533,7 +574,7 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   for (int t = 0; t < iterations; ++t) {
     k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
     k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
-      k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
+    k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
   }
   DSPH_LAUNCH_CHECK();
   if (final_error) {
@@ -582,5 +623,20 @@ extern "C" int dsph_integrate_semi_implicit(float* pos, float* vel, const float*
       reinterpret_cast<float2*>(pos), reinterpret_cast<float2*>(vel), reinterpret_cast<const float2*>(a0),
       reinterpret_cast<const float2*>(a1), reinterpret_cast<const float2*>(a2), reinterpret_cast<const float2*>(a3),
       gx, gy, has_g, (int)n, dt, reinterpret_cast<float2*>(dudt_out));
+  return launch_status();
+}
+
+extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p_div_prev,
+                                const float* p_den_prev, const float* div_src, const float* alpha_div, float rho0,
+                                const float* mean16, const float* istd16, float* X, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vol || !rho || !p_div_prev || !p_den_prev || !div_src || !alpha_div || !mean16 || !istd16 || !X)
+    return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_ws_features<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, vol, rho, p_div_prev, p_den_prev, div_src,
+                                                                  alpha_div, rho0, mean16, istd16, X);
   return launch_status();
 }

@@ -173,6 +173,44 @@ class DFSPHSolver:
                                  _lib.stream_handle(p.device)), "dfsph_solve")
         return stats
 
+    def probe_guess(self, p0: torch.Tensor, iterations: int = 1):
+        """Run `iterations` Jacobi iterations of the divergence solve from p0
+        on copies (the solver state is unchanged).  Returns (error, residual,
+        natural) of the last iteration: the solver's one-sided error metric
+        mean(max(r/rho0, -thr)), the two-sided mean |r| / rho0 of the
+        residual r at that iteration's start, and the natural residual of the
+        clamped Jacobi fixed point, mean |(p_new - p) alpha| / (omega rho0),
+        which is zero exactly at a solution with p >= 0."""
+        cfg = self.cfg
+        prm = _lib.DfsphParams()
+        prm.dt, prm.rho0, prm.omega = cfg.dt, cfg.rho0, cfg.omega_div
+        prm.error_threshold = cfg.threshold_div / cfg.rho0
+        prm.min_iters, prm.max_iters = iterations, iterations
+        prm.clamp_pressure = 1 if cfg.clamp_pressure else 0
+        prm.source_density = 0
+        L = lib()
+        ws = WORKSPACE.get(L.dsph_dfsph_workspace(self.n), p0.device)
+        pairs = self.nl.pairs()
+        st = _lib.stream_handle(p0.device)
+        p = p0.clone()
+        acc = torch.empty(self.n, 2, device=p.device)
+        res = torch.empty(self.n, device=p.device)
+        err = C.c_double(0.0)
+        if iterations > 1:
+            check(L.dsph_dfsph_iterate(C.byref(pairs), C.byref(prm), ptr(self.kgrad), ptr(self.rho), ptr(self.vol),
+                                       ptr(self.alpha_div), ptr(self.div_src), ptr(p), ptr(acc), None,
+                                       iterations - 1, None, ptr(ws), ws.numel(), st), "dfsph_iterate")
+        p_last = p.clone()
+        check(L.dsph_dfsph_iterate(C.byref(pairs), C.byref(prm), ptr(self.kgrad), ptr(self.rho), ptr(self.vol),
+                                   ptr(self.alpha_div), ptr(self.div_src), ptr(p), ptr(acc), ptr(res), 1,
+                                   C.byref(err), ptr(ws), ws.numel(), st), "dfsph_iterate")
+        nat = ((p - p_last) * self.alpha_div).abs().mean() / (cfg.omega_div * cfg.rho0)
+        return float(err.value), float(res.abs().mean()) / cfg.rho0, float(nat)
+
+    def first_iteration_error(self, p0: torch.Tensor) -> float:
+        """Solver error metric after one divergence iteration from p0."""
+        return self.probe_guess(p0)[0]
+
     # -- one step --------------------------------------------------------
     def step(self) -> StepStats:
         cfg = self.cfg
@@ -193,11 +231,16 @@ class DFSPHSolver:
         if vp is None:
             self.visc.zero_()
         # divergence solve (advection = 0: v* = v)
-        p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
+        if "div_init" in self.hooks:
+            p_div = self.hooks["div_init"](self).contiguous()
+        else:
+            p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
         s = self._solve(pairs, False, p_div, self.a_div, None)
         out.div_iters, out.div_error = s.iterations, s.final_error
         self._last_iters[False] = s.iterations
         self.p_div = p_div
+        if "after_div" in self.hooks:
+            self.hooks["after_div"](self)
         # advection for the density solve: g + visc + a_div (dfsph.py:199)
         adv = self.adv
         torch.add(self.visc, self.a_div, out=adv)

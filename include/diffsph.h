@@ -327,6 +327,47 @@ int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
                                  void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Learned warm start of the DFSPH divergence solve (no counterpart in the  */
+/* reference: its dfsphSolve warm-starts with 0.75 p_prev,                  */
+/* v2/modules/dfsph.py:70-75)                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Per-particle input features [n][16] (columns 8..15 zero):
+ *   0 p_div_prev  1 sum_j V_j W_ij p_div_prev_j  2 s_div  3 sum V W s_div
+ *   4 p_den_prev  5 sum_j V_j W_ij p_den_prev_j  6 rho/rho0-1  7 alpha_div
+ * normalised as (f - mean16[c]) * istd16[c] (device arrays). */
+int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
+                     const float* rho, const float* p_div_prev,
+                     const float* p_den_prev, const float* div_src,
+                     const float* alpha_div, float rho0, const float* mean16,
+                     const float* istd16, float* X, void* stream);
+
+/* MLP 16 -> 64 -> 64 -> 64 -> 1 (ReLU), bf16 weights and layer inputs on
+ * MFMA with fp32 accumulation.  Parameters are a flat fp32 array of
+ * dsph_mlp_num_params() values: W1[16][64] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] w4[64] b4[1].  dsph_mlp_pack converts them to the
+ * bf16 MFMA operand layout (dsph_mlp_packed_bytes() bytes) used by
+ * forward and training. */
+int dsph_mlp_num_params(void);
+int dsph_mlp_num_inputs(void);
+size_t dsph_mlp_packed_bytes(void);
+int dsph_mlp_pack(const float* params, void* packed, void* stream);
+int dsph_mlp_forward(const float* X, int64_t n, const float* params,
+                     const void* packed, float* Y, void* stream);
+/* loss = sum_m w (y_m - target_m)^2 and its gradient w.r.t. every
+ * parameter (grad: dsph_mlp_num_params() floats, deterministic fixed-order
+ * reduction).  loss may be NULL. */
+size_t dsph_mlp_train_workspace(int64_t n);
+int dsph_mlp_loss_grad(const float* X, const float* target, int64_t n,
+                       float loss_weight, const float* params,
+                       const void* packed, float* grad, float* loss,
+                       void* workspace, size_t workspace_bytes, void* stream);
+/* Adam on fp32 master parameters (step >= 1 for the bias correction). */
+int dsph_adam_step(float* params, const float* grad, float* m1, float* m2,
+                   int64_t n, float lr, float beta1, float beta2, float eps,
+                   float weight_decay, int32_t step, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Drop-in ops with the reference's per-pair inputs                         */
 /* ---------------------------------------------------------------------- */
 
