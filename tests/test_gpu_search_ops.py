@@ -239,6 +239,43 @@ def test_dropin_ops_match_torch_reference():
     _close(out.cpu().numpy(), ref)
 
 
+def test_dropin_module_on_neighbor_dict(orc):
+    """diffsph_amd.dropin with the reference's calling convention on a
+    neighborDict: density and interpolation of a linear field vs the
+    oracle's CSR density / interpolation, and the getAccessor-style errors."""
+    from diffsph_amd import dropin
+    d = sampling.dfsph_random_flow(48, seed=6, jitter_width=0.2)
+    dev = torch.device("cuda:0")
+    n = d["pos"].shape[0]
+    state = {"positions": torch.from_numpy(d["pos"]).to(dev), "supports": torch.full((n,), d["h"], device=dev)}
+    config = {"domain": {"minExtent": [-1.0, -1.0], "maxExtent": [1.0, 1.0], "periodicity": [True, True]}}
+    _, nd = neighborSearch(state, state, config)
+    m = torch.from_numpy(d["mass"]).to(dev)
+    idx = (nd["indices"][0].long().contiguous(), nd["indices"][1].long().contiguous())
+    W = nd["kernels"].contiguous()
+    cnt, off = nd["numNeighbors"].int().contiguous(), nd["neighborOffsets"].int().contiguous()
+    rho = dropin.sphDensity((m, m), (m, m), (m, m), idx, W, n, cnt, off)
+    dom_o = orc.make_domain((-1, -1), (1, 1), (True, True))
+    g = orc.grid_build(dom_o, d["pos"], d["h"])
+    wpos = np.empty_like(g.sorted_pos)
+    wpos[g.sort_index] = g.sorted_pos
+    _, off_b, nbr_b = orc.neighbors_brute(dom_o, wpos, wpos, h0=d["h"])
+    rho_ref, _ = orc.density(dom_o, off_b, nbr_b, wpos, wpos, d["mass"], d["mass"], h0=d["h"])
+    _close(rho.cpu().numpy(), rho_ref)
+    q = torch.stack([torch.ones(n, device=dev), 2.0 * torch.ones(n, device=dev)], 1).contiguous()
+    qi = dropin.sphInterpolation((m, m), (rho, rho), (q, q), idx, W, n, cnt, off)
+    # interpolation of a constant field = its Shepard sum; here V_j = m_j / rho_j
+    shep = dropin.scatterAdd((m[idx[1]] / rho[idx[1]] * W).contiguous(), n, cnt, off)
+    _close(qi[:, 0].cpu().numpy(), shep.cpu().numpy())
+    _close(qi[:, 1].cpu().numpy(), 2.0 * shep.cpu().numpy())
+    with pytest.raises(RuntimeError, match="kernels"):
+        dropin.sphDensity((m, m), (m, m), (m, m), idx, W.double(), n, cnt, off)
+    with pytest.raises(RuntimeError, match="GPU"):
+        dropin.scatterAdd(W.cpu(), n, cnt, off)
+    with pytest.raises(RuntimeError, match="numNeighbors"):
+        dropin.scatterAdd(W, n, cnt.long(), off)
+
+
 def test_large_search_properties():
     """1M particles (the DFSPH workload size): size-independent properties."""
     d = sampling.dfsph_random_flow(1024, seed=0, jitter_width=0.1)
