@@ -78,6 +78,32 @@ class ViscParams(C.Structure):
     ]
 
 
+class DeltaSPHParams(C.Structure):
+    _fields_ = [
+        ("rho0", C.c_float),
+        ("cs", C.c_float),
+        ("delta", C.c_float),
+        ("kernel_scale", C.c_float),
+        ("visc_coeff", C.c_float),
+        ("visc_eps", C.c_float),
+        ("eos_kappa", C.c_float),
+        ("gravity", C.c_float * 2),
+        ("viscosity", C.c_int32),
+        ("pi_switch", C.c_int32),
+        ("eos", C.c_int32),
+        ("diffusion", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
+class RK4Stages(C.Structure):
+    _fields_ = [
+        ("dxdt", C.c_void_p * 4),
+        ("dudt", C.c_void_p * 4),
+        ("drhodt", C.c_void_p * 4),
+    ]
+
+
 V = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -125,6 +151,14 @@ SIGNATURES = {
     "dsph_mlp_train_workspace": (SZ, [I64]),
     "dsph_mlp_loss_grad": (C.c_int, [V, V, I64, F32, V, V, V, V, V, SZ, V]),
     "dsph_adam_step": (C.c_int, [V, V, V, V, I64, F32, F32, F32, F32, F32, I32, V]),
+    "dsph_remap_neighbors": (C.c_int, [V, I64, V, I64, V]),
+    "dsph_deltasph_normalization": (C.c_int, [P(Pairs), V, V, V, V, V, V]),
+    "dsph_deltasph_rhs": (C.c_int, [P(Pairs), P(DeltaSPHParams), V, V, V, V, V, V, V, V]),
+    "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),
+    "dsph_rk4_combine": (C.c_int, [I64, P(RK4Stages), V, V, V, F32, V, V, V, V]),
+    "dsph_max_norm_workspace": (SZ, []),
+    "dsph_max_norm2": (C.c_int, [V, I64, P(C.c_float), V, SZ, V]),
+    "dsph_sum_squares": (C.c_int, [V, I64, P(C.c_double), V, SZ, V]),
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_scatter_add": (C.c_int, [V, I32, I64, I32, V, V, V, V]),

@@ -194,6 +194,10 @@ inline int launch_status() {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Validates a dsph_pairs descriptor (sizes, layout pointers, supports);
+// defined in sph_ops.hip.
+int check_pairs(const dsph_pairs* p);
+
 }  // namespace dsph
 
 #define DSPH_CHECK_HIP(expr)                          \

@@ -1,0 +1,406 @@
+// delta-SPH right-hand side, RK4 stage arithmetic and the time-step reduction.
+//
+// One right-hand side (v2/simulationSchemes/deltaPlus.py:176-422, fluid only)
+// is two pair kernels over the neighbor list:
+//
+//   k_dsph_normalization  per particle i, two passes over its row:
+//       M_i = sum_j V_j (x_j - x_i) (x) grad W_ij      (normalizationMatrices.py:18-23)
+//       M_i = I if the row has fewer than 4 entries     (deltaPlus.py:245)
+//       L_i = pinv(M_i)                                 (math.py:10-57)
+//       grad^L rho_i = sum_j V_j (rho_i - rho_j) L_i grad W_ij   (densityDiffusion.py:7-26)
+//   k_dsph_rhs            per particle i, one pass:
+//       density diffusion  D_i   (densityDiffusion.py:49-73, scheme deltaSPH)
+//       viscosity          (viscosity.py:123-170 inviscid, :200-218 viscid)
+//       continuity         (momentumEquation.py:6-39)
+//       EOS + pressure     (pressureEOS.py:28-64, pressureForce.py:10-12)
+//       dudt = a_p + g + visc, drho/dt = continuity + D   (deltaPlus.py:349-350)
+//
+// Pair geometry (d = x_i - x_j min-image, r, xhat, q, W, grad W) is evaluated
+// on the fly with the bit-reproducible helpers of common.h; V_j = m_j / rho_j
+// per pair.  Every expression is written in the order of the reference's
+// tensor expressions, and the CPU oracle evaluates the same expressions.
+#include "common.h"
+#include "pmath.h"
+
+namespace dsph {
+
+__global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, const float* __restrict__ mass,
+                                                               const float* __restrict__ rho,
+                                                               float4* __restrict__ L_out,
+                                                               float2* __restrict__ ev_out,
+                                                               float2* __restrict__ grad_rho) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const RowRef rr = row_ref(P, i);
+  float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = P.nbr[k];
+    const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const float Vj = mass[j] / rho[j];
+    const float s = -(g.q * g.h);  // -(distances * supports)
+    const float qx = s * g.ex, qy = s * g.ey;
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    m00 += qx * kx;
+    m01 += qx * ky;
+    m10 += qy * kx;
+    m11 += qy * ky;
+  }
+  if (rr.cnt < 4) {
+    m00 = 1.f;
+    m01 = 0.f;
+    m10 = 0.f;
+    m11 = 1.f;
+  }
+  const dsph_pm::Pinv2 Lm = dsph_pm::pinv2x2(m00, m01, m10, m11);
+  if (L_out) L_out[i] = make_float4(Lm.l00, Lm.l01, Lm.l10, Lm.l11);
+  if (ev_out) ev_out[i] = make_float2(Lm.ev0, Lm.ev1);
+  const float rho_i = rho[i];
+  float ax = 0.f, ay = 0.f;
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = P.nbr[k];
+    const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const float Vj = mass[j] / rho[j];
+    const float nx = Lm.l00 * g.gx + Lm.l01 * g.gy;
+    const float ny = Lm.l10 * g.gx + Lm.l11 * g.gy;
+    const float kx = Vj * nx, ky = Vj * ny;
+    const float d = rho_i - rho[j];
+    ax += d * kx;
+    ay += d * ky;
+  }
+  grad_rho[i] = make_float2(ax, ay);
+}
+
+struct RhsArgs {
+  const float2* vel;
+  const float* mass;
+  const float* rho;
+  const float2* grad_rho;
+  float2* dudt;
+  float* drhodt;
+  float* terms;  // optional [n, 8]: a_p.xy, visc.xy, continuity, diffusion, pressure, 0
+  float rho0;
+  float eos_c;     // cs^2 (isothermal) or kappa (Tait)
+  float delta, kscale, cs;
+  float visc_c;    // viscid: 2 (d+2) nu rho0; inviscid: alpha
+  float eps;
+  float gx, gy;
+  int visc_scheme;  // -1 none, 0 inviscid, 1 viscid
+  int pi_switch;
+  int diffusion;
+};
+
+__device__ __forceinline__ float eos_p(float rho, float rho0, float c) { return c * (rho - rho0); }
+
+__global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, RhsArgs A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float2 ui = A.vel[i];
+  const float rho_i = A.rho[i];
+  const float2 gri = A.diffusion ? A.grad_rho[i] : make_float2(0.f, 0.f);
+  const float p_i = eos_p(rho_i, A.rho0, A.eos_c);
+  float accD = 0.f, accVx = 0.f, accVy = 0.f, accC = 0.f, accPx = 0.f, accPy = 0.f;
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = P.nbr[k];
+    const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const float2 uj = A.vel[j];
+    const float rho_j = A.rho[j];
+    const float m_j = A.mass[j];
+    const float Vj = m_j / rho_j;
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    const float rij = g.q * g.h;  // distances * supports
+    if (A.diffusion) {
+      const float2 grj = A.grad_rho[j];
+      const float r_ij = (2.0f * (rho_j - rho_i)) / (rij + 1e-6f * g.h);
+      const float psx = (-r_ij) * g.ex + (gri.x + grj.x);
+      const float psy = (-r_ij) * g.ey + (gri.y + grj.y);
+      accD += psx * kx + psy * ky;
+    }
+    if (A.visc_scheme >= 0) {
+      const float vx = ui.x - uj.x, vy = ui.y - uj.y;
+      const float vr = vx * g.ex + vy * g.ey;
+      float pi = vr / (rij + A.eps * (g.h * g.h));
+      if (A.pi_switch) pi = vr < 0.0f ? pi : 0.0f;
+      const float vs = Vj * pi;
+      accVx += vs * g.gx;
+      accVy += vs * g.gy;
+    }
+    {
+      const float qx = (uj.x - ui.x) * m_j, qy = (uj.y - ui.y) * m_j;
+      float kq = qx * g.gx + qy * g.gy;
+      kq = (kq * rho_i) / rho_j;
+      accC += kq;
+    }
+    {
+      const float ps = p_i + eos_p(rho_j, A.rho0, A.eos_c);
+      accPx += ps * kx;
+      accPy += ps * ky;
+    }
+  }
+  const float apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
+  float vsx = 0.f, vsy = 0.f;
+  if (A.visc_scheme == 1) {
+    const float s = A.visc_c / rho_i;
+    vsx = s * accVx;
+    vsy = s * accVy;
+  } else if (A.visc_scheme == 0) {
+    const float s = (((A.visc_c * hi) * A.cs) * A.rho0) / rho_i;
+    vsx = s * accVx;
+    vsy = s * accVy;
+  }
+  const float cont = -accC;
+  const float D = A.diffusion ? (((A.delta * hi) / A.kscale) * A.cs) * accD : 0.0f;
+  A.dudt[i] = make_float2((apx + A.gx) + vsx, (apy + A.gy) + vsy);
+  A.drhodt[i] = cont + D;
+  if (A.terms) {
+    float* t = A.terms + (int64_t)i * 8;
+    t[0] = apx;
+    t[1] = apy;
+    t[2] = vsx;
+    t[3] = vsy;
+    t[4] = cont;
+    t[5] = D;
+    t[6] = p_i;
+    t[7] = 0.0f;
+  }
+}
+
+// out = base + k * c for the three state fields (any may be absent).
+__global__ void __launch_bounds__(kBlock) k_rk_update(int n, const float* __restrict__ x0, const float* __restrict__ kx,
+                                                      float* __restrict__ x, const float* __restrict__ u0,
+                                                      const float* __restrict__ ku, float* __restrict__ u,
+                                                      const float* __restrict__ r0, const float* __restrict__ kr,
+                                                      float* __restrict__ r, float c) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (x) {
+    x[2 * i] = x0[2 * i] + kx[2 * i] * c;
+    x[2 * i + 1] = x0[2 * i + 1] + kx[2 * i + 1] * c;
+  }
+  if (u) {
+    u[2 * i] = u0[2 * i] + ku[2 * i] * c;
+    u[2 * i + 1] = u0[2 * i + 1] + ku[2 * i + 1] * c;
+  }
+  if (r) r[i] = r0[i] + kr[i] * c;
+}
+
+// (k1 + 2 k2 + 2 k3 + k4) / 6, then state += that * dt (integration.py:286-307).
+__device__ __forceinline__ float rk4_avg(float a, float b, float c, float d) {
+  return (((a + 2.0f * b) + 2.0f * c) + d) / 6.0f;
+}
+
+struct Rk4Args {
+  const float* k[3][4];  // [field][stage]: dxdt (2n), dudt (2n), drhodt (n)
+  float* state[3];       // x, u, rho (in/out)
+  float* avg[3];         // optional averaged derivatives
+  float dt;
+  int n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_rk4_combine(Rk4Args A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  for (int f = 0; f < 3; ++f) {
+    if (!A.state[f]) continue;
+    const int w = f == 2 ? 1 : 2;
+    for (int c = 0; c < w; ++c) {
+      const int64_t o = (int64_t)i * w + c;
+      const float d = rk4_avg(A.k[f][0][o], A.k[f][1][o], A.k[f][2][o], A.k[f][3][o]);
+      A.state[f][o] = A.state[f][o] + d * A.dt;
+      if (A.avg[f]) A.avg[f][o] = d;
+    }
+  }
+}
+
+// max_i |v_i| (float32 norm with separately rounded squares): per-block
+// maxima, then one block.  The maximum does not depend on the order.
+__global__ void __launch_bounds__(kBlock) k_max_norm(const float2* __restrict__ v, int n, float* __restrict__ part) {
+  __shared__ float sh[kBlock / 64];
+  float m = 0.0f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float2 a = v[i];
+    const float s = sqrt_rn(dist2_exact(a.x, a.y));
+    m = s > m ? s : m;  // NaN entries are skipped (isnan-filtered in the reference)
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float t = __shfl_down(m, o, 64);
+    m = t > m ? t : m;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = 0.0f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = sh[w] > b ? sh[w] : b;
+    part[blockIdx.x] = b;
+  }
+}
+
+__global__ void k_max_final(const float* part, int nb, float* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float m = 0.0f;
+  for (int b = 0; b < nb; ++b) m = part[b] > m ? part[b] : m;
+  *out = m;
+}
+
+// sum of squares of n floats in float64 (NaN entries skipped): per-block
+// partials in a fixed tree order, then one block, so repeated runs give the
+// same bits.
+__global__ void __launch_bounds__(kBlock) k_sumsq(const float* __restrict__ v, int64_t n, double* __restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = (double)v[i];
+    if (a == a) s += a * a;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b += sh[w];
+    part[blockIdx.x] = b;
+  }
+}
+
+__global__ void k_sum_final(const double* part, int nb, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[b];
+  *out = s;
+}
+
+static int check_same_set(const dsph_pairs* pairs) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  return DSPH_OK;
+}
+
+}  // namespace dsph
+
+using namespace dsph;
+
+extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const float* mass, const float* rho, float* L,
+                                           float* eigvals, float* grad_rho, void* stream) {
+  int s = check_same_set(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!mass || !rho || !grad_rho) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_dsph_normalization<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+      P, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
+      reinterpret_cast<float2*>(grad_rho));
+  return launch_status();
+}
+
+extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_deltasph_params* prm, const float* vel,
+                                 const float* mass, const float* rho, const float* grad_rho, float* dudt,
+                                 float* drhodt, float* terms, void* stream) {
+  int s = check_same_set(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vel || !mass || !rho || !dudt || !drhodt) return DSPH_ERR_ARG;
+  if (prm->diffusion && !grad_rho) return DSPH_ERR_ARG;
+  if (prm->viscosity < -1 || prm->viscosity > 1 || prm->eos < 0 || prm->eos > 1) return DSPH_ERR_ARG;
+  if (prm->diffusion && !(prm->kernel_scale > 0.0f)) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  RhsArgs A;
+  A.vel = reinterpret_cast<const float2*>(vel);
+  A.mass = mass;
+  A.rho = rho;
+  A.grad_rho = reinterpret_cast<const float2*>(grad_rho);
+  A.dudt = reinterpret_cast<float2*>(dudt);
+  A.drhodt = drhodt;
+  A.terms = terms;
+  A.rho0 = prm->rho0;
+  A.eos_c = prm->eos == 0 ? prm->cs * prm->cs : prm->eos_kappa;
+  A.delta = prm->delta;
+  A.kscale = prm->kernel_scale;
+  A.cs = prm->cs;
+  // viscid prefactor 2 (d+2) nu rho0 with d = 2, formed in double as the
+  // reference forms it from Python scalars (viscosity.py:218)
+  A.visc_c = prm->viscosity == 1 ? (float)(2.0 * 4.0 * (double)prm->visc_coeff * (double)prm->rho0) : prm->visc_coeff;
+  A.eps = prm->visc_eps;
+  A.gx = prm->gravity[0];
+  A.gy = prm->gravity[1];
+  A.visc_scheme = prm->viscosity;
+  A.pi_switch = prm->pi_switch;
+  A.diffusion = prm->diffusion;
+  k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, A);
+  return launch_status();
+}
+
+extern "C" int dsph_rk_update(int64_t n, const float* x0, const float* dxdt, float* x, const float* u0,
+                              const float* dudt, float* u, const float* rho0, const float* drhodt, float* rho,
+                              float c, void* stream) {
+  if (n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if ((x && (!x0 || !dxdt)) || (u && (!u0 || !dudt)) || (rho && (!rho0 || !drhodt))) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  k_rk_update<<<grid_for(n), kBlock, 0, as_stream(stream)>>>((int)n, x0, dxdt, x, u0, dudt, u, rho0, drhodt, rho,
+                                                              c);
+  return launch_status();
+}
+
+extern "C" int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u, float* rho, float dt,
+                                float* dxdt_avg, float* dudt_avg, float* drhodt_avg, void* stream) {
+  if (!k || n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  Rk4Args A;
+  const float* const* src[3] = {k->dxdt, k->dudt, k->drhodt};
+  float* st[3] = {x, u, rho};
+  float* av[3] = {dxdt_avg, dudt_avg, drhodt_avg};
+  for (int f = 0; f < 3; ++f) {
+    A.state[f] = st[f];
+    A.avg[f] = av[f];
+    for (int s = 0; s < 4; ++s) {
+      A.k[f][s] = src[f][s];
+      if (st[f] && !src[f][s]) return DSPH_ERR_ARG;
+    }
+  }
+  A.dt = dt;
+  A.n = (int)n;
+  k_rk4_combine<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(A);
+  return launch_status();
+}
+
+extern "C" size_t dsph_max_norm_workspace(void) { return sizeof(double) * 1040; }
+
+extern "C" int dsph_max_norm2(const float* v, int64_t n, float* out_host, void* ws, size_t ws_bytes, void* stream) {
+  if (!out_host || n < 0 || n >= INT32_MAX || (n > 0 && !v)) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_max_norm_workspace()) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  const int nb = 1024;
+  float* part = static_cast<float*>(ws);
+  float* out = part + nb;
+  k_max_norm<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(v), (int)n, part);
+  k_max_final<<<1, 64, 0, st>>>(part, nb, out);
+  DSPH_LAUNCH_CHECK();
+  DSPH_CHECK_HIP(hipMemcpyAsync(out_host, out, sizeof(float), hipMemcpyDeviceToHost, st));
+  DSPH_CHECK_HIP(hipStreamSynchronize(st));
+  return DSPH_OK;
+}
+
+extern "C" int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* ws, size_t ws_bytes, void* stream) {
+  if (!out_host || n < 0 || (n > 0 && !v)) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_max_norm_workspace()) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  const int nb = 1024;
+  double* part = static_cast<double*>(ws);
+  double* out = part + nb;
+  k_sumsq<<<nb, kBlock, 0, st>>>(v, n, part);
+  k_sum_final<<<1, 64, 0, st>>>(part, nb, out);
+  DSPH_LAUNCH_CHECK();
+  DSPH_CHECK_HIP(hipMemcpyAsync(out_host, out, sizeof(double), hipMemcpyDeviceToHost, st));
+  DSPH_CHECK_HIP(hipStreamSynchronize(st));
+  return DSPH_OK;
+}

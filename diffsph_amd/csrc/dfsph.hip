@@ -390,8 +390,6 @@ __global__ void __launch_bounds__(kBlock) k_integrate_sie(float2* __restrict__ p
   if (dudt_out) dudt_out[i] = d;
 }
 
-int check_pairs(const dsph_pairs* p);
-
 inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
 SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const float* vol, const float* src,

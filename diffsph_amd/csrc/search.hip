@@ -404,3 +404,22 @@ extern "C" int dsph_wrap_positions(const dsph_domain* domain, const float* pos, 
                                                         make_dom(*domain), reinterpret_cast<float2*>(out));
   return launch_status();
 }
+
+namespace dsph {
+__global__ void __launch_bounds__(kBlock) k_remap(int32_t* __restrict__ nbr, int64_t entries,
+                                                  const int32_t* __restrict__ map, int nmap) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= entries) return;
+  const int32_t j = nbr[e];
+  nbr[e] = (j >= 0 && j < nmap) ? map[j] : -1;
+}
+}  // namespace dsph
+
+extern "C" int dsph_remap_neighbors(int32_t* nbr, int64_t entries, const int32_t* map, int64_t nmap, void* stream) {
+  if (entries < 0 || nmap < 0 || nmap >= INT32_MAX || (entries > 0 && (!nbr || !map))) return DSPH_ERR_ARG;
+  if (entries == 0) return DSPH_OK;
+  const int64_t nb = (entries + kBlock - 1) / kBlock;
+  if (nb >= INT32_MAX) return DSPH_ERR_ARG;
+  k_remap<<<(int)nb, kBlock, 0, as_stream(stream)>>>(nbr, entries, map, (int)nmap);
+  return launch_status();
+}

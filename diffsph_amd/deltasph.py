@@ -1,0 +1,278 @@
+"""delta-SPH (weakly compressible) on the GPU: the reference's deltaPlus
+scheme over the HIP pair kernels.
+
+Follows v2/simulationSchemes/deltaPlus.py:176-422 (fluid only, density by
+continuity) integrated with RK4 (v2/modules/integration.py:231-307) and the
+adaptive time step of v2/modules/timestep.py:8-39:
+
+1. neighbor search on the stage positions (rows in the state's order,
+   entries mapped back to state indices),
+2. normalization matrices, their pseudo-inverses L and the renormalized
+   density gradient (one kernel, two passes over each row),
+3. one fused pass: density diffusion, viscosity, continuity, EOS, pressure
+   acceleration, gravity -> dudt, drho/dt (dx/dt is the stage velocity),
+4. RK4 stage updates and the final combination,
+5. dt = computeTimestep(max |dudt|) for the next step.
+
+The state is re-sorted into cell order at the start of every step and
+carries a UID so callers can map back to the input order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr
+from .neighborhood import WORKSPACE, CellGrid, search, wrap_positions
+
+VISCOSITY = {None: -1, "none": -1, "deltaSPH_inviscid": 0, "deltaSPH_viscid": 1}
+EOS = {"isoThermal": 0, "Tait": 1}
+
+
+@dataclass
+class DeltaSPHConfig:
+    support: float                  # h (uniform)
+    dx: float                       # particle spacing (kernelScale = h / (2 dx), parameters.py:205)
+    rho0: float = 1.0
+    cs: float = 10.0
+    domain_min: tuple = (-1.0, -1.0)
+    domain_max: tuple = (1.0, 1.0)
+    periodic: tuple = (True, True)
+    delta: float = 0.1              # density diffusion (densityDiffusion.py:82)
+    diffusion: bool = True
+    viscosity: str | None = "deltaSPH_viscid"
+    nu: float = 0.01
+    alpha: float = 0.01
+    eps: float = 1e-6
+    pi_switch: bool = False
+    eos: str = "isoThermal"
+    kappa: float = 1.3
+    gravity: tuple | None = None
+    # computeTimestep (timestep.py:46-54 defaults)
+    dt: float | None = None         # None: the first step uses computeTimestep without the acceleration term
+    adaptive: bool = True
+    cfl: float = 1.5
+    max_dt: float = 0.1
+    min_dt: float = 0.001
+    viscosity_constraint: bool = True
+    acceleration_constraint: bool = True
+    acoustic_constraint: bool = True
+    # "global": |dudt| over all components, as computeTimestep evaluates
+    # torch.linalg.norm(dudt[~isnan(dudt)]) on a flattened tensor
+    # (timestep.py:22-25); "max": the largest per-particle |dudt|
+    accel_norm: str = "global"
+
+    @property
+    def kernel_scale(self) -> float:
+        return float(np.float32(self.support) / (np.float32(2.0) * np.float32(self.dx)))
+
+    def params(self) -> _lib.DeltaSPHParams:
+        p = _lib.DeltaSPHParams()
+        if self.viscosity not in VISCOSITY:
+            raise ValueError(f"unsupported viscosity scheme {self.viscosity}")
+        if self.eos not in EOS:
+            raise ValueError(f"unsupported EOS {self.eos}")
+        p.rho0, p.cs, p.delta, p.kernel_scale = self.rho0, self.cs, self.delta, self.kernel_scale
+        p.visc_coeff = self.nu if self.viscosity == "deltaSPH_viscid" else self.alpha
+        p.visc_eps = self.eps
+        p.eos_kappa = self.kappa
+        g = self.gravity or (0.0, 0.0)
+        p.gravity[0], p.gravity[1] = float(g[0]), float(g[1])
+        p.viscosity = VISCOSITY[self.viscosity]
+        p.pi_switch = 1 if self.pi_switch else 0
+        p.eos = EOS[self.eos]
+        p.diffusion = 1 if self.diffusion else 0
+        return p
+
+    def timestep_nu(self) -> float:
+        """The viscosity constraint's nu (timestep.py:13-14)."""
+        if self.viscosity == "deltaSPH_viscid":
+            return self.nu
+        return self.alpha * self.cs * self.support / (2 * (2 + 2))
+
+
+def compute_timestep(cfg: DeltaSPHConfig, dt_prev: float, max_accel: float | None) -> float:
+    """computeTimestep (v2/modules/timestep.py:8-39) in float32 arithmetic."""
+    F = np.float32
+    h, ks = F(cfg.support), F(cfg.kernel_scale)
+    dt_v = F(0.125) * h * h / F(cfg.timestep_nu()) / (ks * ks)
+    dt_c = F(cfg.cfl) * h / F(cfg.cs) / ks
+    dt_a = F(cfg.max_dt) if max_accel is None else F(0.25) * np.sqrt(h / (F(max_accel) + F(1e-7))) / ks
+    new = F(dt_prev)
+    if cfg.viscosity_constraint:
+        new = dt_v
+    if cfg.acceleration_constraint:
+        new = min(new, dt_a)
+    if cfg.acoustic_constraint:
+        new = min(new, dt_c)
+    new = min(new, F(cfg.max_dt))
+    new = max(new, F(cfg.min_dt))
+    return float(F(new))
+
+
+@dataclass
+class RHS:
+    dudt: torch.Tensor
+    drhodt: torch.Tensor
+    grad_rho: torch.Tensor
+    num_pairs: int
+
+
+class DeltaSPHSolver:
+    """Owns the particle state (cell-ordered, on the GPU) and steps it with RK4."""
+
+    def __init__(self, positions, velocities, masses, densities, cfg: DeltaSPHConfig, uid=None):
+        if not positions.is_cuda:
+            raise RuntimeError("DeltaSPHSolver needs GPU tensors")
+        self.cfg = cfg
+        self.n = positions.shape[0]
+        dev = positions.device
+        f = dict(dtype=torch.float32, device=dev)
+        self.domain = _lib.make_domain(cfg.domain_min, cfg.domain_max, cfg.periodic)
+        self.pos = positions.detach().to(**f).contiguous().clone()
+        self.vel = velocities.detach().to(**f).contiguous().clone()
+        self.mass = masses.detach().to(**f).contiguous().clone()
+        self.rho = densities.detach().to(**f).contiguous().clone()
+        self.uid = (uid if uid is not None else torch.arange(self.n, device=dev, dtype=torch.int32)).to(
+            torch.int32).contiguous().clone()
+        self.prm = cfg.params()
+        self.dt = cfg.dt if cfg.dt is not None else compute_timestep(cfg, cfg.max_dt, None)
+        self.time = 0.0
+        self.step_count = 0
+        self.dudt = None            # RK4-averaged derivatives of the last step
+        self.drhodt = None
+        self.last_pairs = 0
+        n = self.n
+        self._grad = torch.empty(n, 2, **f)
+        self._k = [(torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f)) for _ in range(4)]
+        self._stage = (torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f))
+
+    # -- pieces ------------------------------------------------------------
+    def neighbors(self, pos: torch.Tensor):
+        """Neighbor list of `pos` with rows in the state's order and entries
+        mapped to state indices; returns (NeighborList, pairs descriptor,
+        wrapped positions)."""
+        cfg = self.cfg
+        grid = CellGrid.build(pos, None, self.domain, cfg.support, h_uniform=cfg.support)
+        wpos = wrap_positions(pos, self.domain)
+        nl = search(grid, wpos, None, "scatter", qpos_wrapped=True)
+        check(lib().dsph_remap_neighbors(ptr(nl.nbr), nl.nbr.numel(), ptr(grid.sort_index), self.n,
+                                         _lib.stream_handle(pos.device)), "remap_neighbors")
+        pr = nl.pairs()
+        pr.rpos = ptr(wpos)
+        pr.rh = None
+        return nl, pr, wpos
+
+    def rhs(self, pos, vel, rho, dudt, drhodt, terms=None, L=None, eigvals=None) -> RHS:
+        """One right-hand side at (pos, vel, rho) into dudt [n,2], drhodt [n]."""
+        st = _lib.stream_handle(pos.device)
+        nl, pr, wpos = self.neighbors(pos)
+        L_ = lib()
+        if self.cfg.diffusion or L is not None or eigvals is not None:
+            check(L_.dsph_deltasph_normalization(C.byref(pr), ptr(self.mass), ptr(rho), ptr(L), ptr(eigvals),
+                                                 ptr(self._grad), st), "deltasph_normalization")
+        check(L_.dsph_deltasph_rhs(C.byref(pr), C.byref(self.prm), ptr(vel), ptr(self.mass), ptr(rho),
+                                   ptr(self._grad), ptr(dudt), ptr(drhodt), ptr(terms), st), "deltasph_rhs")
+        # keep the list alive until the kernels that read it have been queued
+        self._keep = (nl, wpos)
+        return RHS(dudt, drhodt, self._grad, nl.num_pairs)
+
+    def _sort(self):
+        grid = CellGrid.build(self.pos, None, self.domain, self.cfg.support, h_uniform=self.cfg.support)
+        si = grid.sort_index
+        st = _lib.stream_handle(self.pos.device)
+
+        def perm(t):
+            out = torch.empty_like(t)
+            if t.dtype == torch.float32:
+                w = 1 if t.dim() == 1 else t.shape[1]
+                check(lib().dsph_gather_f32(ptr(t), ptr(si), self.n, w, ptr(out), st), "gather")
+                return out
+            return t[si.long()]
+
+        self.pos, self.vel, self.mass, self.rho, self.uid = (perm(t) for t in
+                                                             (self.pos, self.vel, self.mass, self.rho, self.uid))
+
+    # -- one step ------------------------------------------------------------
+    def step(self) -> float:
+        """One RK4 step with the current dt; returns the dt used."""
+        L_ = lib()
+        st = _lib.stream_handle(self.pos.device)
+        self._sort()
+        dt = float(np.float32(self.dt))
+        half = float(np.float32(dt) * np.float32(0.5))
+        x0, u0, r0 = self.pos, self.vel, self.rho
+        xs, us, rs = self._stage
+        x, u, r = x0, u0, r0
+        for s, c in enumerate([half, half, dt, None]):
+            kx, ku, kr = self._k[s]
+            kx.copy_(u)                     # dx/dt of this stage = its velocity
+            res = self.rhs(x, u, r, ku, kr)
+            self.last_pairs = res.num_pairs
+            if c is not None:
+                check(L_.dsph_rk_update(self.n, ptr(x0), ptr(kx), ptr(xs), ptr(u0), ptr(ku), ptr(us), ptr(r0),
+                                        ptr(kr), ptr(rs), C.c_float(c), st), "rk_update")
+                x, u, r = xs, us, rs
+        stages = _lib.RK4Stages()
+        for s in range(4):
+            stages.dxdt[s] = ptr(self._k[s][0])
+            stages.dudt[s] = ptr(self._k[s][1])
+            stages.drhodt[s] = ptr(self._k[s][2])
+        if self.dudt is None:
+            self.dudt = torch.empty_like(self.vel)
+            self.drhodt = torch.empty_like(self.rho)
+        check(L_.dsph_rk4_combine(self.n, C.byref(stages), ptr(self.pos), ptr(self.vel), ptr(self.rho),
+                                  C.c_float(dt), None, ptr(self.dudt), ptr(self.drhodt), st), "rk4_combine")
+        self.time += dt
+        self.step_count += 1
+        if self.cfg.adaptive:
+            self.dt = compute_timestep(self.cfg, self.dt, self.max_accel())
+        return dt
+
+    def max_accel(self) -> float:
+        """The acceleration of the time-step condition (see accel_norm)."""
+        ws = WORKSPACE.get(lib().dsph_max_norm_workspace(), self.vel.device)
+        st = _lib.stream_handle(self.vel.device)
+        if self.cfg.accel_norm == "global":
+            out = C.c_double(0.0)
+            check(lib().dsph_sum_squares(ptr(self.dudt), 2 * self.n, C.byref(out), ptr(ws), ws.numel(), st),
+                  "sum_squares")
+            return float(np.float32(math.sqrt(out.value)))
+        out = C.c_float(0.0)
+        check(lib().dsph_max_norm2(ptr(self.dudt), self.n, C.byref(out), ptr(ws), ws.numel(), st), "max_norm2")
+        return float(out.value)
+
+    def kinetic_energy(self) -> float:
+        """sum 0.5 m |u|^2 (runner.py:44-45 uses areas * densities = m)."""
+        return float(0.5 * (self.mass.double() * (self.vel.double() ** 2).sum(1)).sum())
+
+    def state_by_uid(self):
+        order = torch.argsort(self.uid.long())
+        return self.pos[order], self.vel[order], self.rho[order]
+
+
+def taylor_green_state(n_side: int, device, u_max: float = 1.0, target_neighbors: float = 50.0):
+    """Config of the reference's Taylor-Green benchmark: n_side^2 cell-centred
+    lattice on [-1,1]^2 (sampling.py:55-74), area dx^2, m = area rho0,
+    h = sqrt(n dx^2 / pi) (math.py:96-105), rho = rho0, u = sin(pi x) cos(pi y),
+    v = -cos(pi x) sin(pi y), c_s = 10 u_max."""
+    from . import sampling
+    pos, dx = sampling.sample_regular(n_side)
+    area = dx * dx
+    h = sampling.volume_to_support(area, target_neighbors)
+    vel = sampling.taylor_green(pos, u_max)
+    n = pos.shape[0]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(device)  # noqa: E731
+    cfg = DeltaSPHConfig(support=h, dx=dx, cs=10.0 * u_max)
+    return (T(pos), T(vel), T(np.full(n, area * cfg.rho0)), T(np.full(n, cfg.rho0)), cfg)
+
+
+def analytic_tgv_energy_ratio(t: float, nu: float) -> float:
+    """E_k(t)/E_k(0) of the Taylor-Green vortex with wavenumber pi per axis:
+    exp(-4 nu pi^2 t)."""
+    return math.exp(-4.0 * nu * math.pi ** 2 * t)

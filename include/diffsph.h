@@ -368,6 +368,101 @@ int dsph_adam_step(float* params, const float* grad, float* m1, float* m2,
                    float weight_decay, int32_t step, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* delta-SPH right-hand side, RK4 and time step                             */
+/* ---------------------------------------------------------------------- */
+
+/* Map neighbor entries from sorted slots to the query set's own indices:
+ * nbr[e] = map[nbr[e]] for 0 <= nbr[e] < nmap, padding (-1) and
+ * out-of-range entries become -1.  Used when the rows are queried in the
+ * state's order against a grid sorted on the same positions (the
+ * reference maps j through sortIndex the same way, neighborhood.cpp:253-435
+ * and v2/modules/neighborhood.py:526). */
+int dsph_remap_neighbors(int32_t* nbr, int64_t entries, const int32_t* map,
+                         int64_t nmap, void* stream);
+
+/* Parameters of one delta-SPH right-hand side (fluid only,
+ * v2/simulationSchemes/deltaPlus.py:176-422). */
+typedef struct dsph_deltasph_params {
+  float rho0;
+  float cs;            /* speed of sound */
+  float delta;         /* density diffusion delta (densityDiffusion.py:73) */
+  float kernel_scale;  /* h / (2 dx) (parameters.py:205) */
+  float visc_coeff;    /* nu (viscid) or alpha (inviscid) */
+  float visc_eps;      /* eps of pi_ij = v.x / (r + eps h^2) */
+  float eos_kappa;     /* Tait kappa (eos == 1) */
+  float gravity[2];    /* constant body acceleration (gravity.py:26-29) */
+  int32_t viscosity;   /* -1 none, 0 deltaSPH_inviscid, 1 deltaSPH_viscid */
+  int32_t pi_switch;   /* viscosity only for approaching pairs */
+  int32_t eos;         /* 0 isoThermal p = cs^2 (rho - rho0), 1 Tait p = kappa (rho - rho0) */
+  int32_t diffusion;   /* 0 off, 1 deltaSPH density diffusion */
+  int32_t pad_;
+} dsph_deltasph_params;
+
+/* Normalization matrices, their pseudo-inverses and the renormalized
+ * density gradient of one particle set against itself:
+ * M_i = sum_j V_j (x_j - x_i) (x) grad W_ij (identity for rows with < 4
+ * entries), L_i = pinv(M_i) by the closed-form 2x2 SVD, and
+ * grad^L rho_i = sum_j V_j (rho_i - rho_j) L_i grad W_ij, V_j = m_j/rho_j.
+ * Replaces computeCovarianceMatrices (v2/modules/normalizationMatrices.py:
+ * 18-23), pinv2x2 (v2/math.py:10-57), deltaPlus.py:245-248 and
+ * renormalizedDensityGradient (v2/modules/densityDiffusion.py:7-26).
+ * L [n,4] row-major and eigvals [n,2] may be NULL; grad_rho is [n,2]. */
+int dsph_deltasph_normalization(const dsph_pairs* pairs, const float* mass,
+                                const float* rho, float* L, float* eigvals,
+                                float* grad_rho, void* stream);
+
+/* One fused pass: delta-SPH density diffusion (densityDiffusion.py:49-73),
+ * viscosity (viscosity.py:123-170 / 200-218), continuity
+ * (momentumEquation.py:6-39), EOS (pressureEOS.py:28-64) and pressure
+ * acceleration (pressureForce.py:10-12), combined as deltaPlus.py:349-350:
+ * dudt = a_p + g + visc [n,2], drhodt = continuity + diffusion [n].
+ * terms [n,8] (may be NULL) receives a_p.xy, visc.xy, continuity,
+ * diffusion, pressure, 0 for inspection. */
+int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_deltasph_params* prm,
+                      const float* vel, const float* mass, const float* rho,
+                      const float* grad_rho, float* dudt, float* drhodt,
+                      float* terms, void* stream);
+
+/* Stage state of the reference's RK4 (v2/modules/integration.py:23-40,
+ * 231-276): x = x0 + dxdt c, u = u0 + dudt c, rho = rho0 + drhodt c, each
+ * field skipped when its output is NULL. */
+int dsph_rk_update(int64_t n, const float* x0, const float* dxdt, float* x,
+                   const float* u0, const float* dudt, float* u,
+                   const float* rho0, const float* drhodt, float* rho,
+                   float c, void* stream);
+
+/* Stage derivatives for the RK4 combination; a field whose state pointer is
+ * NULL in dsph_rk4_combine may leave its entries NULL. */
+typedef struct dsph_rk4_stages {
+  const float* dxdt[4];    /* [n,2] each */
+  const float* dudt[4];    /* [n,2] each */
+  const float* drhodt[4];  /* [n]   each */
+} dsph_rk4_stages;
+
+/* state += (k1 + 2 k2 + 2 k3 + k4)/6 * dt per field
+ * (v2/modules/integration.py:286-307); the averaged derivatives are also
+ * written to *_avg when not NULL (the reference keeps them as dxdt, dudt,
+ * drhodt, :489-494). */
+int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u,
+                     float* rho, float dt, float* dxdt_avg, float* dudt_avg,
+                     float* drhodt_avg, void* stream);
+
+/* max_i |v_i| of an [n,2] field (float32 norm).  Writes the host float
+ * *out_host (synchronises the stream).  The workspace size serves both
+ * reductions below. */
+size_t dsph_max_norm_workspace(void);
+int dsph_max_norm2(const float* v, int64_t n, float* out_host, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* sum of squares of n floats in float64, NaN entries skipped, in a fixed
+ * reduction order.  The acceleration condition of computeTimestep
+ * (v2/modules/timestep.py:22-25) takes torch.linalg.norm of
+ * dudt[~isnan(dudt)], a flattened 1-D tensor: the norm of all components,
+ * i.e. sqrt of this sum.  Writes *out_host (synchronises the stream). */
+int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Drop-in ops with the reference's per-pair inputs                         */
 /* ---------------------------------------------------------------------- */
 

@@ -20,6 +20,13 @@
 #include <cstring>
 #include <vector>
 
+// Portable float32 atan2/sin/cos (float64 series, IEEE arithmetic only) and
+// the closed-form 2x2 pseudo-inverse, shared with the HIP build so that the
+// float32 normalization matrices agree bit for bit.  The functions are
+// pinned against numpy (atan2, sin, cos, linalg.pinv) in tests/test_oracle.py;
+// the float64 path below restates the pseudo-inverse with <cmath>.
+#include "../diffsph_amd/csrc/pmath.h"
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -119,8 +126,8 @@ G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
   g.r = std::sqrt(a + b);
   g.h = h;
   g.q = g.r / h;
-  g.ex = g.dx / (g.r + (T)1e-7f);
-  g.ey = g.dy / (g.r + (T)1e-7f);
+  g.ex = g.dx / (g.r + (T)1e-7);
+  g.ey = g.dy / (g.r + (T)1e-7);
   g.w = kernelW<T>(g.q, h);
   T dw = kernelDW<T>(g.q, h);
   g.gx = g.ex * dw;
@@ -568,6 +575,195 @@ int dfsph_solve(const Csr<T>& c, const DomT<T>& d, T dt, T rho0, T omega, T thr,
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// delta-SPH right-hand side (v2/simulationSchemes/deltaPlus.py:176-422).
+// ---------------------------------------------------------------------------
+inline float atan2_t(float y, float x) { return dsph_pm::atan2f_pm(y, x); }
+inline double atan2_t(double y, double x) { return std::atan2(y, x); }
+inline void sincos_t(float x, float* s, float* c) { dsph_pm::sincosf_pm(x, s, c); }
+inline void sincos_t(double x, double* s, double* c) {
+  *s = std::sin(x);
+  *c = std::cos(x);
+}
+template <class T>
+T sgn_t(T v) { return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0)); }
+
+// pinv2x2, v2/math.py:10-57, in the working type.
+template <class T>
+void pinv2(T a, T b, T c, T d, T* L, T* ev) {
+  const T a2 = a * a, b2 = b * b, c2 = c * c, d2 = d * d;
+  const T theta = T(0.5) * atan2_t(T(2) * a * c + T(2) * b * d, a2 + b2 - c2 - d2);
+  T st, ct;
+  sincos_t(theta, &st, &ct);
+  const T S1 = a2 + b2 + c2 + d2;
+  const T e = a2 + b2 - c2 - d2;
+  const T f = a * c + b * d;
+  const T S2 = std::sqrt(e * e + T(4) * (f * f));
+  const T o1 = std::sqrt((S1 + S2) / T(2));
+  const T o2 = std::sqrt(std::max(S1 - S2 + (T)1e-7, (T)1e-7) / T(2));
+  const T phi = T(0.5) * atan2_t(T(2) * a * b + T(2) * c * d, a2 - b2 + c2 - d2);
+  T sp, cp;
+  sincos_t(phi, &sp, &cp);
+  const T s11 = sgn_t<T>((a * ct + c * st) * cp + (b * ct + d * st) * sp);
+  const T s22 = sgn_t<T>((a * st - c * ct) * sp + (-b * st + d * ct) * cp);
+  const T v00 = cp * s11, v01 = -sp * s22, v10 = sp * s11, v11 = cp * s22;
+  const T i1 = std::fabs(o1) > (T)1e-5 ? T(1) / o1 : T(0);
+  const T i2 = std::fabs(o2) > (T)1e-5 ? T(1) / o2 : T(0);
+  const T w00 = v00 * i1, w01 = v01 * i2, w10 = v10 * i1, w11 = v11 * i2;
+  L[0] = w00 * ct + w01 * (-st);
+  L[1] = w00 * st + w01 * ct;
+  L[2] = w10 * ct + w11 * (-st);
+  L[3] = w10 * st + w11 * ct;
+  const bool swap = std::fabs(o2) > std::fabs(o1);
+  ev[0] = swap ? o2 : o1;
+  ev[1] = swap ? o1 : o2;
+}
+
+// computeCovarianceMatrices (normalizationMatrices.py:18-23), identity for
+// rows with < 4 entries (deltaPlus.py:245), pinv2x2, then
+// renormalizedDensityGradient (densityDiffusion.py:7-26).
+template <class T>
+void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, T* Lout, T* evout,
+                        T* grad_rho) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T m00 = 0, m01 = 0, m10 = 0, m11 = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T Vj = mass[j] / rho[j];
+      T s = -(g.q * g.h);
+      T qx = s * g.ex, qy = s * g.ey;
+      T kx = Vj * g.gx, ky = Vj * g.gy;
+      m00 += qx * kx;
+      m01 += qx * ky;
+      m10 += qy * kx;
+      m11 += qy * ky;
+    }
+    if (c.off[i + 1] - c.off[i] < 4) {
+      m00 = 1;
+      m01 = 0;
+      m10 = 0;
+      m11 = 1;
+    }
+    T L[4], ev[2];
+    pinv2<T>(m00, m01, m10, m11, L, ev);
+    if (Lout)
+      for (int q = 0; q < 4; ++q) Lout[4 * i + q] = L[q];
+    if (evout) {
+      evout[2 * i] = ev[0];
+      evout[2 * i + 1] = ev[1];
+    }
+    T ax = 0, ay = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      T Vj = mass[j] / rho[j];
+      T nx = L[0] * g.gx + L[1] * g.gy;
+      T ny = L[2] * g.gx + L[3] * g.gy;
+      T kx = Vj * nx, ky = Vj * ny;
+      T dr = rho[i] - rho[j];
+      ax += dr * kx;
+      ay += dr * ky;
+    }
+    grad_rho[2 * i] = ax;
+    grad_rho[2 * i + 1] = ay;
+  }
+}
+
+struct DeltaP {
+  double rho0, cs, delta, kscale, visc_coeff, eps, kappa, gx, gy;
+  int viscosity, pi_switch, eos, diffusion;
+};
+
+// Fused delta-SPH terms: densityDiffusion.py:49-73 (deltaSPH scheme),
+// viscosity.py:123-170 / 200-218, momentumEquation.py:6-39,
+// pressureEOS.py:28-64, pressureForce.py:10-12, combined as
+// deltaPlus.py:349-350.
+template <class T>
+void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, const T* mass, const T* rho,
+              const T* grad_rho, T* dudt, T* drhodt, T* terms) {
+  const T rho0 = (T)P.rho0;
+  const T eos_c = P.eos == 0 ? (T)P.cs * (T)P.cs : (T)P.kappa;
+  const T cs = (T)P.cs;
+  const T visc_c = P.viscosity == 1 ? (T)(2.0 * 4.0 * P.visc_coeff * P.rho0) : (T)P.visc_coeff;
+  const T eps = (T)P.eps;
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    const T hi = c.qh ? c.qh[i] : c.h0;
+    const T uix = vel[2 * i], uiy = vel[2 * i + 1];
+    const T rho_i = rho[i];
+    const T grix = P.diffusion ? grad_rho[2 * i] : T(0), griy = P.diffusion ? grad_rho[2 * i + 1] : T(0);
+    const T p_i = eos_c * (rho_i - rho0);
+    T accD = 0, accVx = 0, accVy = 0, accC = 0, accPx = 0, accPy = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T ujx = vel[2 * j], ujy = vel[2 * j + 1];
+      const T rho_j = rho[j];
+      const T m_j = mass[j];
+      const T Vj = m_j / rho_j;
+      const T kx = Vj * g.gx, ky = Vj * g.gy;
+      const T rij = g.q * g.h;
+      if (P.diffusion) {
+        const T r_ij = (T(2) * (rho_j - rho_i)) / (rij + (T)1e-6 * g.h);
+        const T psx = (-r_ij) * g.ex + (grix + grad_rho[2 * j]);
+        const T psy = (-r_ij) * g.ey + (griy + grad_rho[2 * j + 1]);
+        accD += psx * kx + psy * ky;
+      }
+      if (P.viscosity >= 0) {
+        const T vx = uix - ujx, vy = uiy - ujy;
+        const T vr = vx * g.ex + vy * g.ey;
+        T pi = vr / (rij + eps * (g.h * g.h));
+        if (P.pi_switch) pi = vr < T(0) ? pi : T(0);
+        const T vs = Vj * pi;
+        accVx += vs * g.gx;
+        accVy += vs * g.gy;
+      }
+      {
+        const T qx = (ujx - uix) * m_j, qy = (ujy - uiy) * m_j;
+        T kq = qx * g.gx + qy * g.gy;
+        kq = (kq * rho_i) / rho_j;
+        accC += kq;
+      }
+      {
+        const T ps = p_i + eos_c * (rho_j - rho0);
+        accPx += ps * kx;
+        accPy += ps * ky;
+      }
+    }
+    const T apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
+    T vsx = 0, vsy = 0;
+    if (P.viscosity == 1) {
+      const T s = visc_c / rho_i;
+      vsx = s * accVx;
+      vsy = s * accVy;
+    } else if (P.viscosity == 0) {
+      const T s = (((visc_c * hi) * cs) * rho0) / rho_i;
+      vsx = s * accVx;
+      vsy = s * accVy;
+    }
+    const T cont = -accC;
+    const T D = P.diffusion ? ((((T)P.delta * hi) / (T)P.kscale) * cs) * accD : T(0);
+    dudt[2 * i] = (apx + (T)P.gx) + vsx;
+    dudt[2 * i + 1] = (apy + (T)P.gy) + vsy;
+    drhodt[i] = cont + D;
+    if (terms) {
+      T* t = terms + 8 * i;
+      t[0] = apx;
+      t[1] = apy;
+      t[2] = vsx;
+      t[3] = vsy;
+      t[4] = cont;
+      t[5] = D;
+      t[6] = p_i;
+      t[7] = 0;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -671,6 +867,67 @@ int orc_integrate_sie_f32(float* pos, float* vel, const float* a0, const float* 
       pos[2 * i + k] = pos[2 * i + k] + v * dt;
     }
   }
+  return 0;
+}
+
+// delta-SPH exports (same-set neighbor lists: rpos = qpos, rh = qh).
+int orc_deltasph_normalization_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                                   const float* qpos, const float* qh, float h0, int32_t mode, const float* mass,
+                                   const float* rho, float* L, float* ev, float* grad_rho) {
+  const float* rpos = qpos;
+  const float* rh = qh;
+  dsph_normalization<float>(ORC_CSR(float), DomT<float>(*dom), mass, rho, L, ev, grad_rho);
+  return 0;
+}
+int orc_deltasph_normalization_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                                   const double* qpos, const double* qh, double h0, int32_t mode, const double* mass,
+                                   const double* rho, double* L, double* ev, double* grad_rho) {
+  const double* rpos = qpos;
+  const double* rh = qh;
+  dsph_normalization<double>(ORC_CSR(double), DomT<double>(*dom), mass, rho, L, ev, grad_rho);
+  return 0;
+}
+
+// prm: rho0, cs, delta, kernel_scale, visc_coeff, eps, kappa, gx, gy;
+// iprm: viscosity, pi_switch, eos, diffusion.
+int orc_deltasph_rhs_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
+                         const float* qh, float h0, int32_t mode, const double* prm, const int32_t* iprm,
+                         const float* vel, const float* mass, const float* rho, const float* grad_rho, float* dudt,
+                         float* drhodt, float* terms) {
+  const float* rpos = qpos;
+  const float* rh = qh;
+  DeltaP P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8], iprm[0], iprm[1], iprm[2], iprm[3]};
+  dsph_rhs<float>(ORC_CSR(float), DomT<float>(*dom), P, vel, mass, rho, grad_rho, dudt, drhodt, terms);
+  return 0;
+}
+int orc_deltasph_rhs_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                         const double* qpos, const double* qh, double h0, int32_t mode, const double* prm,
+                         const int32_t* iprm, const double* vel, const double* mass, const double* rho,
+                         const double* grad_rho, double* dudt, double* drhodt, double* terms) {
+  const double* rpos = qpos;
+  const double* rh = qh;
+  DeltaP P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8], iprm[0], iprm[1], iprm[2], iprm[3]};
+  dsph_rhs<double>(ORC_CSR(double), DomT<double>(*dom), P, vel, mass, rho, grad_rho, dudt, drhodt, terms);
+  return 0;
+}
+
+// The 2x2 pseudo-inverse on its own ([n,4] row-major in, L [n,4], ev [n,2]).
+int orc_pinv2x2_f32(const float* M, int64_t n, float* L, float* ev) {
+  for (int64_t i = 0; i < n; ++i) pinv2<float>(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3], L + 4 * i, ev + 2 * i);
+  return 0;
+}
+int orc_pinv2x2_f64(const double* M, int64_t n, double* L, double* ev) {
+  for (int64_t i = 0; i < n; ++i)
+    pinv2<double>(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3], L + 4 * i, ev + 2 * i);
+  return 0;
+}
+// The portable float32 atan2 and sin/cos.
+int orc_atan2_pm(const float* y, const float* x, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = dsph_pm::atan2f_pm(y[i], x[i]);
+  return 0;
+}
+int orc_sincos_pm(const float* x, int64_t n, float* s, float* c) {
+  for (int64_t i = 0; i < n; ++i) dsph_pm::sincosf_pm(x[i], s + i, c + i);
   return 0;
 }
 

@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "build", "liboracle.so")
 SRC = os.path.join(HERE, "oracle.cpp")
+DEPS = [SRC, os.path.join(HERE, "..", "diffsph_amd", "csrc", "pmath.h")]
 
 
 class Domain(C.Structure):
@@ -34,7 +35,7 @@ def make_domain(mn, mx, periodic) -> Domain:
 
 
 def build(force: bool = False) -> str:
-    stale = not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC)
+    stale = not os.path.exists(LIB) or os.path.getmtime(LIB) < max(os.path.getmtime(p) for p in DEPS)
     if force or stale:
         subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB
@@ -296,3 +297,189 @@ def wendland(q, h):
     dw = np.zeros_like(q)
     lib().orc_wendland(_p(q), _p(h), C.c_int64(q.size), _p(w), _p(dw))
     return w, dw
+
+
+# ---------------------------------------------------------------------------
+# delta-SPH (v2/simulationSchemes/deltaPlus.py:176-422) with RK4
+# (v2/modules/integration.py:231-307) and computeTimestep
+# (v2/modules/timestep.py:8-39).
+# ---------------------------------------------------------------------------
+def pinv2x2(M, dtype=np.float32):
+    """Closed-form 2x2 pseudo-inverse of v2/math.py:10-57 for M [n,2,2]."""
+    cv = f32 if dtype == np.float32 else f64
+    M = cv(np.asarray(M).reshape(-1, 4))
+    n = M.shape[0]
+    L = np.zeros((n, 4), dtype)
+    ev = np.zeros((n, 2), dtype)
+    fn = lib().orc_pinv2x2_f32 if dtype == np.float32 else lib().orc_pinv2x2_f64
+    fn(_p(M), C.c_int64(n), _p(L), _p(ev))
+    return L.reshape(n, 2, 2), ev
+
+
+def atan2_pm(y, x):
+    y, x = f32(y), f32(x)
+    out = np.zeros(y.shape, np.float32)
+    lib().orc_atan2_pm(_p(y), _p(x), C.c_int64(y.size), _p(out))
+    return out
+
+
+def sincos_pm(x):
+    x = f32(x)
+    s = np.zeros(x.shape, np.float32)
+    c = np.zeros(x.shape, np.float32)
+    lib().orc_sincos_pm(_p(x), C.c_int64(x.size), _p(s), _p(c))
+    return s, c
+
+
+@dataclass
+class DeltaParams:
+    rho0: float = 1.0
+    cs: float = 10.0
+    delta: float = 0.1
+    kernel_scale: float = 1.0
+    viscosity: int = 1          # -1 none, 0 deltaSPH_inviscid, 1 deltaSPH_viscid
+    visc_coeff: float = 0.01    # nu (viscid) or alpha (inviscid)
+    eps: float = 1e-6
+    kappa: float = 1.3
+    gravity: tuple = (0.0, 0.0)
+    pi_switch: bool = False
+    eos: int = 0                # 0 isoThermal, 1 Tait
+    diffusion: bool = True
+
+    def arrays(self):
+        prm = np.array([self.rho0, self.cs, self.delta, self.kernel_scale, self.visc_coeff, self.eps, self.kappa,
+                        self.gravity[0], self.gravity[1]], np.float64)
+        if not (self.viscosity in (-1, 0, 1)):
+            raise ValueError("viscosity must be -1, 0 or 1")
+        iprm = np.array([self.viscosity, 1 if self.pi_switch else 0, self.eos, 1 if self.diffusion else 0], np.int32)
+        return prm, iprm
+
+
+def deltasph_normalization(dom, off, nbr, wpos, mass, rho, h0, dtype=np.float32):
+    n = wpos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    wpos, mass, rho = cv(wpos), cv(mass), cv(rho)
+    L = np.zeros((n, 4), dtype)
+    ev = np.zeros((n, 2), dtype)
+    g = np.zeros((n, 2), dtype)
+    fn = lib().orc_deltasph_normalization_f32 if dtype == np.float32 else lib().orc_deltasph_normalization_f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(wpos), None, hc(h0), C.c_int32(0), _p(mass), _p(rho), _p(L),
+       _p(ev), _p(g))
+    return L, ev, g
+
+
+def deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad_rho, prm: DeltaParams, h0, dtype=np.float32):
+    n = wpos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    wpos, vel, mass, rho, grad_rho = cv(wpos), cv(vel), cv(mass), cv(rho), cv(grad_rho)
+    dudt = np.zeros((n, 2), dtype)
+    drhodt = np.zeros(n, dtype)
+    terms = np.zeros((n, 8), dtype)
+    pa, ia = prm.arrays()
+    fn = lib().orc_deltasph_rhs_f32 if dtype == np.float32 else lib().orc_deltasph_rhs_f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(wpos), None, hc(h0), C.c_int32(0), _p(pa), _p(ia), _p(vel),
+       _p(mass), _p(rho), _p(grad_rho), _p(dudt), _p(drhodt), _p(terms))
+    return dudt, drhodt, terms
+
+
+def neighbors_state_order(dom, pos, h):
+    """Neighbor list with rows in the state's order and entries as state
+    indices (cell-scan order within a row), on wrapped positions: the grid is
+    built on `pos`, queried with the wrapped positions in state order, and
+    the sorted slots are mapped back through sortIndex."""
+    g = grid_build(dom, pos, h)
+    wpos = np.empty_like(g.sorted_pos)
+    wpos[g.sort_index] = g.sorted_pos
+    cnt, off, nbr_s = neighbors_cell(dom, g, wpos, h0=h)
+    return off, g.sort_index[nbr_s].astype(np.int32), wpos
+
+
+def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float32):
+    """One right-hand side: (dudt, drhodt, extras); dxdt is the velocity."""
+    off, nbr, wpos = neighbors_state_order(dom, pos, h)
+    L, ev, grad = deltasph_normalization(dom, off, nbr, wpos, mass, rho, h, dtype)
+    dudt, drhodt, terms = deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad, prm, h, dtype)
+    return dudt, drhodt, dict(L=L, ev=ev, grad_rho=grad, terms=terms, off=off, nbr=nbr)
+
+
+@dataclass
+class TimestepParams:
+    support: float
+    kernel_scale: float
+    cs: float
+    nu: float                  # the viscosity constraint's nu (timestep.py:13-14)
+    cfl: float = 1.5
+    max_dt: float = 0.1
+    min_dt: float = 0.001
+    viscosity_constraint: bool = True
+    acceleration_constraint: bool = True
+    acoustic_constraint: bool = True
+
+
+def compute_timestep(tp: TimestepParams, dt_prev: float, max_accel=None) -> float:
+    """computeTimestep (v2/modules/timestep.py:8-39) in float32 arithmetic;
+    `max_accel` = max |dudt| (None before the first step: dt_a = maxDt)."""
+    F = np.float32
+    h, ks = F(tp.support), F(tp.kernel_scale)
+    dt_v = F(0.125) * h * h / F(tp.nu) / (ks * ks)
+    dt_c = F(tp.cfl) * h / F(tp.cs) / ks
+    if max_accel is None:
+        dt_a = F(tp.max_dt)
+    else:
+        dt_a = F(0.25) * np.sqrt(h / (F(max_accel) + F(1e-7))) / ks
+    new = F(dt_prev)
+    if tp.viscosity_constraint:
+        new = dt_v
+    if tp.acceleration_constraint:
+        new = min(new, dt_a)
+    if tp.acoustic_constraint:
+        new = min(new, dt_c)
+    new = min(new, F(tp.max_dt))
+    new = max(new, F(tp.min_dt))
+    return float(F(new))
+
+
+@dataclass
+class DeltaState:
+    pos: np.ndarray
+    vel: np.ndarray
+    rho: np.ndarray
+    mass: np.ndarray
+    uid: np.ndarray
+
+
+def rk4_step(st: DeltaState, dom, h, prm: DeltaParams, dt: float, reorder=True, dtype=np.float32, rhs=None):
+    """One RK4 step of the delta-SPH scheme (integration.py:231-307): the
+    state is first sorted into cell order (as the GPU solver keeps it), then
+    k1..k4 with stage states x0 + k dt/2, x0 + k dt/2, x0 + k dt, and
+    state += (k1 + 2 k2 + 2 k3 + k4)/6 dt for x (k = velocity), u, rho.
+    `rhs(dom, x, u, rho, mass, h, prm, dtype) -> (dudt, drhodt, extras)`
+    defaults to :func:`deltaplus_rhs`.  Returns (state, averaged dudt)."""
+    rhs = rhs or deltaplus_rhs
+    F = np.float32 if dtype == np.float32 else np.float64
+    if reorder:
+        si = grid_build(dom, st.pos, h).sort_index
+        st = DeltaState(st.pos[si], st.vel[si], st.rho[si], st.mass[si], st.uid[si])
+    x0, u0, r0 = st.pos.astype(F), st.vel.astype(F), st.rho.astype(F)
+    dtf = F(dt)
+    half = F(dtf * F(0.5))
+    ks = []
+    x, u, r = x0, u0, r0
+    for s, c in enumerate([half, half, dtf, None]):
+        dudt, drhodt, _ = rhs(dom, x, u, r, st.mass.astype(F), h, prm, dtype)
+        ks.append((u.copy(), dudt, drhodt))
+        if c is not None:
+            x = x0 + ks[-1][0] * c
+            u = u0 + dudt * c
+            r = r0 + drhodt * c
+
+    def avg(a, b, c_, d):
+        return (((a + F(2) * b) + F(2) * c_) + d) / F(6)
+
+    dx = avg(*[k[0] for k in ks])
+    du = avg(*[k[1] for k in ks])
+    dr = avg(*[k[2] for k in ks])
+    new = DeltaState(x0 + dx * dtf, u0 + du * dtf, r0 + dr * dtf, st.mass, st.uid)
+    return new, du

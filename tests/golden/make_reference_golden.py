@@ -1,0 +1,160 @@
+"""Write golden vectors of the reference's delta-SPH and DFSPH pair modules.
+
+Run in a checkout that has the reference at /root/reference (not needed
+by the tests, which only read the .npz this writes):
+
+    python tests/golden/make_reference_golden.py
+
+It loads the reference's own module files by path (src/diffSPH/v2/...),
+skipping the package __init__ of v2/modules (which would import the
+unavailable torchCompactRadius), builds a small periodic particle set and
+its neighborDict per-pair arrays with the reference's `mod` and Wendland C2
+functions (the pair list itself comes from the brute-force predicate
+r/h <= 1, as v2/modules/neighborhood.py:433-460 filters), and records the
+modules' outputs in float32:
+
+  computeCovarianceMatrices + identity for < 4 neighbors + pinv2x2
+  renormalizedDensityGradient, computeDensityDeltaTerm (deltaSPH)
+  computeViscosity (deltaSPH_viscid and deltaSPH_inviscid)
+  computeMomentumEquation, computeEOS (isoThermal), computePressureAccel
+  computeTimestep
+  DFSPH: computeAlpha, computeSourceTerm (divergence and density),
+         computePressureAcceleration + updatePressure (one iteration)
+"""
+import importlib
+import os
+import sys
+import types
+
+import numpy as np
+import torch
+
+REF = "/root/reference/src"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_modules_16.npz")
+
+
+def load():
+    sys.path.insert(0, REF)
+    pkg = types.ModuleType("diffSPH.v2.modules")
+    pkg.__path__ = [os.path.join(REF, "diffSPH", "v2", "modules")]
+    importlib.import_module("diffSPH.v2")
+    sys.modules["diffSPH.v2.modules"] = pkg
+    mods = {}
+    for name in ["normalizationMatrices", "densityDiffusion", "viscosity", "momentumEquation", "pressureEOS",
+                 "pressureForce", "timestep", "dfsph"]:
+        mods[name] = importlib.import_module("diffSPH.v2.modules." + name)
+    mods["math"] = importlib.import_module("diffSPH.v2.math")
+    mods["W2"] = importlib.import_module("diffSPH.kernelFunctions.Wendland2")
+    return mods
+
+
+def main():
+    torch.manual_seed(0)
+    rng = np.random.default_rng(16)
+    M = load()
+    ns = 16
+    dx = 2.0 / ns
+    xs = np.linspace(-1 + dx / 2, 1 - dx / 2, ns, dtype=np.float32)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+    pos = (pos + rng.uniform(-0.15 * dx, 0.15 * dx, pos.shape)).astype(np.float32)
+    n = pos.shape[0]
+    h = np.float32(np.sqrt(50.0 * dx * dx / np.pi))
+    vel = (np.stack([np.sin(np.pi * pos[:, 0]) * np.cos(np.pi * pos[:, 1]),
+                     -np.cos(np.pi * pos[:, 0]) * np.sin(np.pi * pos[:, 1])], 1)
+           + 0.1 * rng.standard_normal((n, 2))).astype(np.float32)
+    rho = (1.0 + 0.01 * rng.standard_normal(n)).astype(np.float32)
+    mass = np.full(n, dx * dx, np.float32)
+
+    T = torch.from_numpy
+    p = T(pos)
+    # brute-force pair list (i ascending, j ascending) with the reference's
+    # pair geometry (neighborhood.py:445-458)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i_all, j_all = T(ii.ravel()), T(jj.ravel())
+    xij = p[i_all] - p[j_all]
+    xij = torch.stack([M["math"].mod(xij[:, k], -1.0, 1.0) for k in range(2)], -1)
+    rij = torch.linalg.norm(xij, dim=-1)
+    xij = xij / (rij + 1e-7).view(-1, 1)
+    hv = torch.full((n,), float(h))
+    rij = rij / hv[j_all]
+    keep = rij <= 1.0
+    i, j = i_all[keep], j_all[keep]
+    rij, xij, hij = rij[keep], xij[keep], hv[j][:]
+    W = M["W2"].kernel(rij, hij, 2)
+    gW = M["W2"].kernelGradient(rij, xij, hij, 2)
+    numNeighbors = torch.bincount(i, minlength=n).to(torch.int32)
+    nd = {"indices": (i, j), "distances": rij, "vectors": xij, "supports": hij, "kernels": W, "gradients": gW,
+          "numNeighbors": numNeighbors}
+    ks = torch.tensor(float(h / (2 * dx)))
+    cfg = {"diffusion": {"delta": 0.1, "densityScheme": "deltaSPH", "velocityScheme": "deltaSPH_viscid",
+                         "nu": 0.01, "alpha": 0.01, "eps": 1e-6, "pi-switch": False},
+           "kernel": {"kernelScale": ks}, "fluid": {"rho0": 1.0, "cs": 10.0}, "domain": {"dim": 2},
+           "EOS": {"type": "isoThermal"},
+           "timestep": {"active": True, "dt": 0.001, "CFL": 1.5, "maxDt": 0.1, "minDt": 0.001,
+                        "viscosityConstraint": True, "accelerationConstraint": True,
+                        "acousticConstraint": True},
+           "particle": {"support": torch.tensor(float(h))}}
+    st = {"positions": p, "velocities": T(vel), "masses": T(mass), "densities": T(rho), "supports": hv,
+          "numParticles": n, "numNeighbors": numNeighbors}
+    out = {}
+    C = M["normalizationMatrices"].computeCovarianceMatrices(st, st, nd, cfg)
+    C[numNeighbors < 4] = torch.eye(2)[None]
+    L, ev = M["math"].pinv2x2(C)
+    st["L"] = L
+    out["covariance"], out["L"], out["L_ev"] = C, L, ev
+    st["gradRho^L"] = M["densityDiffusion"].renormalizedDensityGradient(st, st, nd, cfg)
+    out["grad_rho"] = st["gradRho^L"]
+    out["diffusion"] = M["densityDiffusion"].computeDensityDeltaTerm(st, st, nd, cfg)
+    out["visc_viscid"] = M["viscosity"].computeViscosity(st, st, nd, cfg)
+    cfg["diffusion"]["velocityScheme"] = "deltaSPH_inviscid"
+    out["visc_inviscid"] = M["viscosity"].computeViscosity(st, st, nd, cfg)
+    out["continuity"] = M["momentumEquation"].computeMomentumEquation(st, st, nd, cfg)
+    st["pressures"] = M["pressureEOS"].computeEOS(st, cfg)
+    out["pressure"] = st["pressures"]
+    out["pressure_accel"] = M["pressureForce"].computePressureAccel(st, st, nd, cfg)
+    cfg["diffusion"]["velocityScheme"] = "deltaSPH_viscid"
+    dudt = out["pressure_accel"] + out["visc_viscid"]
+    state = {"fluid": {"dudt": dudt}}
+    out["dt_next"] = torch.tensor(float(M["timestep"].computeTimestep(state, cfg)))
+    out["dt_first"] = torch.tensor(float(M["timestep"].computeTimestep({"fluid": {}}, cfg)))
+    # DFSPH pieces (v2/modules/dfsph.py): alpha, sources, one Jacobi
+    # iteration of each solve and the pressure acceleration after it
+    D = M["dfsph"]
+    dt = float(np.float32(3.0 * dx / 10.0))
+    cfg["timestep"]["dt"] = dt
+    cfg["dfsph"] = {"omega": 0.3, "clampPressure": True}
+    adv = T((0.5 * rng.standard_normal((n, 2))).astype(np.float32))
+    p0 = T((0.5 + 0.1 * rng.standard_normal(n)).astype(np.float32))
+    d = dict(st)
+    d["areas"] = T(mass) / 1.0
+    d["actualArea"] = d["masses"] / d["densities"]
+    for solve in ("div", "den"):
+        dens = solve == "den"
+        d["predictedVelocities"] = d["velocities"] + dt * adv if dens else d["velocities"]
+        alpha = D.computeAlpha(d, d, cfg, nd, density=dens)
+        if dens:
+            src = 1 - d["areas"] / d["actualArea"] - D.computeSourceTerm(d, d, cfg, nd, density=True)
+        else:
+            src = D.computeSourceTerm(d, d, cfg, nd, density=False)
+        d["alpha"], d["sourceTerm"] = alpha, src
+        d["pressureB"], d["pressureA"] = p0.clone(), p0.clone()
+        d["pressureAccel"] = D.computePressureAcceleration(d, d, cfg, nd)
+        p1, res = D.updatePressure(d, d, cfg, nd, density=dens)
+        d["pressureB"] = p1
+        out[f"dfsph_{solve}_alpha"] = alpha
+        out[f"dfsph_{solve}_source"] = src
+        out[f"dfsph_{solve}_p1"] = p1
+        out[f"dfsph_{solve}_residual"] = res
+        out[f"dfsph_{solve}_accel"] = D.computePressureAcceleration(d, d, cfg, nd)
+    out["dfsph_adv"] = adv
+    out["dfsph_p0"] = p0
+    out["dfsph_dt"] = torch.tensor(dt)
+    np_out = {k: v.detach().numpy().astype(np.float32) for k, v in out.items()}
+    np.savez(OUT, pos=pos, vel=vel, rho=rho, mass=mass, h=np.float32(h), dx=np.float32(dx),
+             kernel_scale=np.float32(float(ks)), i=i.numpy().astype(np.int32), j=j.numpy().astype(np.int32), **np_out)
+    print("wrote", OUT, {k: v.shape for k, v in np_out.items()})
+
+
+if __name__ == "__main__":
+    main()

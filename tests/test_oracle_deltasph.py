@@ -1,0 +1,242 @@
+"""Pins of the oracle's delta-SPH restatement against what mathematics fixes:
+portable atan2/sin/cos vs numpy, the 2x2 pseudo-inverse vs the SVD closed
+form, first-order consistency of the renormalized gradient, conservation
+(antisymmetric pair sums), vanishing terms for uniform/rigid states, signs
+under compression, RK4 on a linear ODE, and the time-step formula."""
+import math
+
+import numpy as np
+import pytest
+
+from diffsph_amd import sampling
+from oracle import oracle as orc
+
+F = np.float32
+
+
+def _ulp_diff(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+def test_portable_atan2_within_one_ulp():
+    rng = np.random.default_rng(0)
+    y = (rng.standard_normal(200000) * np.exp(rng.uniform(-8, 8, 200000))).astype(F)
+    x = (rng.standard_normal(200000) * np.exp(rng.uniform(-8, 8, 200000))).astype(F)
+    got = orc.atan2_pm(y, x)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64)).astype(F)
+    d = _ulp_diff(got, ref)
+    assert d.max() <= 1 and (d == 0).mean() > 0.999
+    # signed zeros and axes (C99 atan2)
+    sy = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.0, 1.0], F)
+    sx = np.array([0.0, 0.0, -0.0, -0.0, 0.0, 0.0, -2.0, 1.0], F)
+    got = orc.atan2_pm(sy, sx)
+    ref = np.arctan2(sy.astype(np.float64), sx.astype(np.float64)).astype(F)
+    assert np.array_equal(got.view(np.int32), ref.view(np.int32))
+    assert np.isnan(orc.atan2_pm(np.array([np.nan], F), np.array([1.0], F))[0])
+
+
+def test_portable_sincos_within_one_ulp():
+    x = np.linspace(-math.pi / 2 - 0.01, math.pi / 2 + 0.01, 300001).astype(F)
+    s, c = orc.sincos_pm(x)
+    rs = np.sin(x.astype(np.float64)).astype(F)
+    rc = np.cos(x.astype(np.float64)).astype(F)
+    assert _ulp_diff(s, rs).max() <= 1 and _ulp_diff(c, rc).max() <= 1
+    assert (_ulp_diff(s, rs) == 0).mean() > 0.999 and (_ulp_diff(c, rc) == 0).mean() > 0.999
+
+
+def _pinv_closed_form(M):
+    """V diag(1/s1, 1/sqrt(s2^2 + 5e-8)) U^T: the reference regularises the
+    smaller singular value as sqrt((S1 - S2 + 1e-7)/2) with S1 - S2 = 2 s2^2."""
+    U, S, Vt = np.linalg.svd(M)
+    inv = np.stack([1.0 / S[:, 0], 1.0 / np.sqrt(S[:, 1] ** 2 + 5e-8)], 1)
+    return np.einsum("nji,nj,nkj->nik", Vt, inv, U), S
+
+
+def test_pinv2x2_matches_svd_closed_form():
+    rng = np.random.default_rng(1)
+    M = rng.standard_normal((4000, 2, 2))
+    R, S = _pinv_closed_form(M)
+    L64, ev64 = orc.pinv2x2(M, np.float64)
+    rel = np.abs(L64 - R).max(axis=(1, 2)) / np.abs(R).max(axis=(1, 2))
+    assert rel.max() < 1e-7
+    np.testing.assert_allclose(ev64[:, 0], S[:, 0], rtol=1e-12)
+    # float32: the closed form loses ~eps cond^2 in S1 - S2; near-identity
+    # matrices (the SPH case) must match to float32 accuracy
+    A = np.eye(2)[None] + 0.2 * rng.standard_normal((4000, 2, 2))
+    R, S = _pinv_closed_form(A)
+    L32, _ = orc.pinv2x2(A.astype(F))
+    cond = S[:, 0] / S[:, 1]
+    ok = cond < 4
+    rel = np.abs(L32 - R).max(axis=(1, 2)) / np.abs(R).max(axis=(1, 2))
+    assert ok.sum() > 3000 and rel[ok].max() < 5e-6
+
+
+def test_pinv2x2_special_cases():
+    L, ev = orc.pinv2x2(np.eye(2, dtype=F)[None])
+    np.testing.assert_allclose(L[0], np.eye(2), atol=1e-7)
+    # exactly rank one: sign(0) = 0 removes the null direction -> the true pseudo-inverse
+    L, ev = orc.pinv2x2(np.array([[[2.0, 0.0], [0.0, 0.0]]]), np.float64)
+    np.testing.assert_allclose(L[0], np.linalg.pinv(np.array([[2.0, 0.0], [0.0, 0.0]])), atol=1e-12)
+    np.testing.assert_allclose(ev[0], [2.0, math.sqrt(5e-8)], rtol=1e-9)
+    # nearly singular: the small value is regularised to sqrt(s2^2 + 5e-8)
+    L, ev = orc.pinv2x2(np.array([[[2.0, 0.0], [0.0, 1e-6]]]), np.float64)
+    np.testing.assert_allclose(L[0], [[0.5, 0.0], [0.0, 1.0 / math.sqrt(1e-12 + 5e-8)]], rtol=1e-6)
+    # a reflection (negative determinant) with distinct singular values is inverted
+    M = np.array([[[0.0, 2.0], [1.0, 0.0]]])
+    L, _ = orc.pinv2x2(M, np.float64)
+    np.testing.assert_allclose(L[0], np.linalg.inv(M[0]), atol=1e-9)
+    # equal singular values with zero diagonal: both angles are atan2(0, 0) = 0
+    # and both column signs sign(0) = 0, so the formula returns 0 (as written
+    # in v2/math.py:21-46); SPH normalization matrices are near the identity
+    L, _ = orc.pinv2x2(np.array([[[0.0, 1.0], [1.0, 0.0]]]), np.float64)
+    assert np.all(L == 0)
+
+
+def _jittered(n_side, seed, jit=0.3, periodic=True):
+    pos, dx = sampling.sample_regular(n_side)
+    pos = sampling.jitter(pos, dx, jit, seed)
+    h = sampling.volume_to_support(dx * dx, 50.0)
+    dom = orc.make_domain((-1, -1), (1, 1), (periodic, periodic))
+    return pos, dx, h, dom
+
+
+def test_renormalized_gradient_exact_for_linear_density():
+    """sum_j V_j (rho_i - rho_j) L_i gradW_ij = -grad rho for any particle
+    set when rho is linear (first-order consistency; the reference's
+    'difference' gradient carries the minus sign)."""
+    pos, dx, h, dom = _jittered(40, 3, periodic=False)
+    n = pos.shape[0]
+    b = np.array([0.3, -0.7])
+    rho = (1.0 + pos.astype(np.float64) @ b)
+    mass = np.full(n, dx * dx)
+    off, nbr, wpos = orc.neighbors_state_order(dom, pos, h)
+    L, ev, g = orc.deltasph_normalization(dom, off, nbr, wpos, mass, rho, h, np.float64)
+    interior = np.all(np.abs(pos) < 1.0 - 1.2 * h, axis=1)
+    assert interior.sum() > 500
+    np.testing.assert_allclose(g[interior], np.broadcast_to(-b, g[interior].shape), rtol=0, atol=1e-6)
+    # float32 oracle close to the float64 one
+    _, _, g32 = orc.deltasph_normalization(dom, off, nbr, wpos, mass, rho, h, np.float32)
+    assert np.abs(g32[interior] + b).max() < 2e-4
+    # with rho = rho0 and V = dx^2 the normalization matrix is ~ I
+    L1, _, _ = orc.deltasph_normalization(dom, off, nbr, wpos, mass, np.ones(n), h, np.float64)
+    Lm = L1.reshape(-1, 2, 2)[interior]
+    assert np.abs(Lm - np.eye(2)).max() < 0.15  # jitter 0.3 dx
+
+
+def _prm(**kw):
+    d = dict(rho0=1.0, cs=10.0, delta=0.1, kernel_scale=1.9947, viscosity=1, visc_coeff=0.01)
+    d.update(kw)
+    return orc.DeltaParams(**d)
+
+
+def test_rhs_vanishes_for_uniform_rest_and_rigid_translation():
+    pos, dx, h, dom = _jittered(32, 4)
+    n = pos.shape[0]
+    mass = np.full(n, dx * dx, F)
+    rho = np.full(n, 1.0, F)
+    vel = np.tile(np.array([[0.4, -1.3]], F), (n, 1))
+    for visc in (0, 1):
+        dudt, drhodt, ex = orc.deltaplus_rhs(dom, pos, vel, rho, mass, h, _prm(viscosity=visc))
+        t = ex["terms"]
+        assert np.all(t[:, 0:2] == 0) and np.all(t[:, 2:4] == 0)  # p = 0, v_ij = 0
+        assert np.all(t[:, 4] == 0) and np.all(t[:, 5] == 0)      # u_j - u_i = 0, rho uniform
+        assert np.all(dudt == 0) and np.all(drhodt == 0)
+    g = (0.0, -9.81)
+    dudt, _, _ = orc.deltaplus_rhs(dom, pos, vel, rho, mass, h, _prm(gravity=g))
+    np.testing.assert_array_equal(dudt, np.tile(np.array([g], F), (n, 1)))
+
+
+def test_rhs_conservation_of_pair_sums():
+    """Antisymmetric pair terms (uniform h): sum_i m_i a_p,i = 0,
+    sum_i m_i visc_i = 0, sum_i V_i D_i = 0 for arbitrary rho, u."""
+    pos, dx, h, dom = _jittered(32, 5)
+    n = pos.shape[0]
+    rng = np.random.default_rng(5)
+    mass = np.full(n, dx * dx)
+    rho = 1.0 + 0.01 * rng.standard_normal(n)
+    vel = rng.standard_normal((n, 2))
+    dudt, drhodt, ex = orc.deltaplus_rhs(dom, pos, vel, rho, mass, h, _prm(), np.float64)
+    t = ex["terms"]
+    for cols in ((0, 1), (2, 3)):
+        s = (mass[:, None] * t[:, cols]).sum(0)
+        scale = np.abs(mass[:, None] * t[:, cols]).sum()
+        assert np.abs(s).max() < 1e-12 * scale
+    V = mass / rho
+    assert abs((V * t[:, 5]).sum()) < 1e-12 * np.abs(V * t[:, 5]).sum()
+
+
+def test_rhs_signs_under_compression():
+    """u = -c x compresses: continuity raises rho by ~ 2 c rho; the
+    pressure of a denser centre pushes outward."""
+    pos, dx, h, dom = _jittered(40, 6, jit=0.0, periodic=False)
+    n = pos.shape[0]
+    c = 0.5
+    vel = (-c * pos).astype(np.float64)
+    mass = np.full(n, dx * dx)
+    rho = np.full(n, 1.0)
+    dudt, drhodt, ex = orc.deltaplus_rhs(dom, pos, vel, rho, mass, h, _prm(), np.float64)
+    inner = np.all(np.abs(pos) < 1.0 - 1.2 * h, axis=1)
+    t = ex["terms"]
+    # tr(M) = 2 on the lattice to ~1%: drho/dt = c rho tr(M)
+    assert np.abs(t[inner, 4] - 2 * c).max() < 0.03
+    # a density bump -> outward pressure acceleration
+    rho_b = 1.0 + 0.01 * np.exp(-(pos.astype(np.float64) ** 2).sum(1) / 0.1)
+    _, _, ex = orc.deltaplus_rhs(dom, pos, np.zeros((n, 2)), rho_b, mass, h, _prm(), np.float64)
+    a = ex["terms"][:, 0:2]
+    r = np.linalg.norm(pos, axis=1)
+    ring = (r > 0.2) & (r < 0.5)
+    assert np.all((a[ring] * pos[ring]).sum(1) > 0)
+
+
+def test_rk4_on_linear_ode():
+    """RK4 composition with dudt = lam u, drho/dt = mu rho, dx/dt = u:
+    u and rho follow the 4th-order Taylor polynomial, x the matching
+    weights of the stage velocities."""
+    lam, mu, dt = -0.7, 0.3, 0.1
+    x0 = np.array([[0.1, 0.2]])
+    u0 = np.array([[1.0, -2.0]])
+    r0 = np.array([1.5])
+    st = orc.DeltaState(x0, u0, r0, np.ones(1), np.zeros(1, np.int32))
+
+    def rhs(dom, x, u, r, m, h, prm, dtype):
+        return lam * u, mu * r, {}
+
+    new, du = orc.rk4_step(st, None, 0.0, None, dt, reorder=False, dtype=np.float64, rhs=rhs)
+    z = lam * dt
+    np.testing.assert_allclose(new.vel, u0 * (1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24), rtol=1e-14)
+    w = mu * dt
+    np.testing.assert_allclose(new.rho, r0 * (1 + w + w * w / 2 + w ** 3 / 6 + w ** 4 / 24), rtol=1e-14)
+    np.testing.assert_allclose(new.pos, x0 + u0 * dt * (1 + z / 2 + z * z / 6 + z ** 3 / 24), rtol=1e-14)
+
+
+def test_timestep_formula():
+    tp = orc.TimestepParams(support=0.1247, kernel_scale=1.9947, cs=10.0, nu=0.01)
+    h, ks = 0.1247, 1.9947
+    dt_v = 0.125 * h * h / 0.01 / ks ** 2
+    dt_c = 1.5 * h / 10.0 / ks
+    assert orc.compute_timestep(tp, 0.001) == pytest.approx(min(dt_v, dt_c, 0.1), rel=1e-6)
+    dt_a = 0.25 * math.sqrt(h / (400.0 + 1e-7)) / ks
+    assert orc.compute_timestep(tp, 0.001, 400.0) == pytest.approx(min(dt_v, dt_c, dt_a), rel=1e-6)
+    assert orc.compute_timestep(tp, 0.001, 1e12) == pytest.approx(0.001, rel=1e-6)  # floored at minDt
+
+
+@pytest.mark.parametrize("scheme", [1, 0])
+def test_viscosity_is_nu_laplacian_on_divergence_free_field(scheme):
+    """On the Taylor-Green field (div u = 0, lap u = -2 pi^2 u) the viscid
+    term is nu lap u and the inviscid one is alpha h c_s / (2 (d+2)) lap u
+    (the timestep module's equivalent nu), to the kernel's discretisation
+    error (~4 % at 50 neighbors); continuity vanishes."""
+    pos, dx = sampling.sample_regular(64)
+    h = sampling.volume_to_support(dx * dx, 50.0)
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = pos.shape[0]
+    vel = sampling.taylor_green(pos).astype(np.float64)
+    prm = _prm(kernel_scale=h / (2 * dx), viscosity=scheme, visc_coeff=0.01)
+    _, _, ex = orc.deltaplus_rhs(dom, pos, vel, np.ones(n), np.full(n, dx * dx), h, prm, np.float64)
+    t = ex["terms"]
+    nu = 0.01 if scheme == 1 else 0.01 * h * 10.0 / 8.0
+    ref = -2.0 * math.pi ** 2 * nu * vel
+    assert np.abs(t[:, 2:4] - ref).max() < 0.06 * np.abs(ref).max()
+    assert np.abs(t[:, 4]).max() < 1e-5
