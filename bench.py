@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (reference kernel for PMC byte counters)")
     return ap.parse_args()
 
 
@@ -68,7 +70,6 @@ def main():
     from diffsph_amd import _lib, sampling
     from diffsph_amd.dfsph import DFSPHConfig, DFSPHSolver
     from diffsph_amd.learned import train_warm_start
-    from diffsph_amd.neighborhood import WORKSPACE
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -128,6 +129,14 @@ def main():
     value = learned["value"]
     secs = learned["ms_per_step"] * args.steps * 1e-3
 
+    if args.copy_calibration:
+        src = torch.empty(64 << 20, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # ---- roofline of the pressure sweeps (HIP events on the solver's stream)
     L = _lib.lib()
     pairs = solver.nl.pairs()
@@ -150,18 +159,22 @@ def main():
     prm = _lib.DfsphParams()
     prm.dt, prm.rho0, prm.omega, prm.error_threshold = dt, 1.0, 0.3, 5e-4
     prm.min_iters, prm.max_iters, prm.clamp_pressure, prm.source_density = 1, 1, 1, 1
-    ws = WORKSPACE.get(L.dsph_dfsph_workspace(n), dev)
     res = torch.empty(n, device=dev)
-    it_reps = max(reps // 2, 5)
+    parts = torch.empty((n + 255) // 256, dtype=torch.float64, device=dev)
+
+    def sweep_b():
+        _lib.check(L.dsph_pressure_update(C.byref(pairs), C.byref(prm), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
+                                          _lib.ptr(solver.div_src), _lib.ptr(solver.alpha_den), _lib.ptr(acc),
+                                          _lib.ptr(p), _lib.ptr(res), _lib.ptr(parts), sh))
+
+    for _ in range(3):
+        sweep_b()
     e0.record(stream)
-    _lib.check(L.dsph_dfsph_iterate(C.byref(pairs), C.byref(prm), _lib.ptr(solver.kgrad), _lib.ptr(solver.rho),
-                                    _lib.ptr(solver.vol),
-                                    _lib.ptr(solver.alpha_den), _lib.ptr(solver.div_src), _lib.ptr(p), _lib.ptr(acc),
-                                    _lib.ptr(res), it_reps, None, _lib.ptr(ws), ws.numel(), sh))
+    for _ in range(reps):
+        sweep_b()
     e1.record(stream)
     e1.synchronize()
-    t_iter = e0.elapsed_time(e1) / it_reps * 1e-3
-    t_b = max(t_iter - t_a, 1e-9)
+    t_b = e0.elapsed_time(e1) / reps * 1e-3
     P = solver.nl.num_pairs
     # Algorithmic bytes (each per-particle array once per launch, each pair
     # entry once; gathered neighbor fields and list padding not counted):
@@ -172,16 +185,33 @@ def main():
     bytes_b = 32 * n + 12 * P
     gbs_a = bytes_a / t_a / 1e9
     gbs_b = bytes_b / t_b / 1e9
-    log(f"sweep A (pressure accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
-        f"sweep B (pressure update): {t_b * 1e6:.1f} us, {bytes_b / 1e6:.1f} MB -> {gbs_b:.0f} GB/s; "
+    log(f"sweep A (dsph_pressure_accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
+        f"sweep B (dsph_pressure_update): {t_b * 1e6:.1f} us, {bytes_b / 1e6:.1f} MB -> {gbs_b:.0f} GB/s; "
         f"pairs {P} ({P / n:.1f}/particle)")
     if t_b >= t_a:
         kname, ach, tb, byt = "k_dfsph_sweep_b", gbs_b, t_b, bytes_b
     else:
         kname, ach, tb, byt = "k_dfsph_sweep_a", gbs_a, t_a, bytes_a
+    # HBM bytes per launch from the committed PMC profile of the same kernel
+    # (FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy): its ratio to the
+    # algorithmic bytes of the profiled launch, applied to this launch.
+    traffic, ratio = None, None
+    pmc_path = os.path.join(ROOT, "profiles", "dfsph_sweeps_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+        prof_bytes = pmc["traffic_bytes"].get(kname)
+        prof_alg = pmc.get("algorithmic_bytes", {}).get(kname)
+        if prof_bytes and prof_alg:
+            ratio = prof_bytes / prof_alg
+            traffic = int(round(ratio * byt))
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(byt)}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+                "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(byt),
+                "traffic_over_algorithmic": None if ratio is None else round(ratio, 4),
+                "other_sweep": {"kernel": "k_dfsph_sweep_a" if kname == "k_dfsph_sweep_b" else "k_dfsph_sweep_b",
+                                "achieved": round(gbs_a if kname == "k_dfsph_sweep_b" else gbs_b, 1),
+                                "kernel_us": round((t_a if kname == "k_dfsph_sweep_b" else t_b) * 1e6, 2)}}
 
     # ---- CPU baseline: the oracle on the host cores, continuing the plain
     # solver's trajectory from the state at the start of its timed window

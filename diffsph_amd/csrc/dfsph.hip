@@ -168,7 +168,22 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float
   src[i] = density ? (1.0f - areas[i] / vol[i]) - dt * dv : dv;
 }
 
-// Sweep A (dfsph.py:35-37).  `ctrl` NULL -> unconditional.
+// Streamed per-entry loads of the sweeps: the neighbor index and V_j gradW_ij
+// are read exactly once per launch, so they bypass the caches
+// (non-temporal) and leave L2 to the gathered neighbor fields.
+__device__ __forceinline__ int ld_stream(const int32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+  return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+
+// Entries per unrolled group: the group's index and coefficient loads are
+// issued together, then its gathers, then the additions in entry order (the
+// sum is accumulated exactly as the one-at-a-time loop does).
+constexpr int kUnroll = 4;
+
+// Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij.
+// `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
+// else evaluated on the fly.
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float2* __restrict__ kv,
                                                           const float* __restrict__ vol,
                                                           const float* __restrict__ rho, const float* __restrict__ p,
@@ -176,17 +191,46 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
   if (ctrl && ctrl->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
-  const float2 xi = P.qpos[i];
-  const float hi = h_of(P.qh, i, P.h0);
   const float pi = p[i];
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
-    const float2 c = kv_of(P, kv, k, j, xi, hi, vol);
-    const float s = pi + p[j];
-    ax += s * c.x;
-    ay += s * c.y;
+  if (kv) {
+    int n = 0, k = rr.base;
+    for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
+      int j[kUnroll];
+      float2 c[kUnroll];
+      float pj[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        j[u] = ld_stream(P.nbr + k + u * rr.step);
+        c[u] = ld_stream(kv + k + u * rr.step);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) pj[u] = p[j[u]];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const float s = pi + pj[u];
+        ax += s * c[u].x;
+        ay += s * c[u].y;
+      }
+    }
+    for (; n < rr.cnt; ++n, k += rr.step) {
+      const int j = ld_stream(P.nbr + k);
+      const float2 c = ld_stream(kv + k);
+      const float s = pi + p[j];
+      ax += s * c.x;
+      ay += s * c.y;
+    }
+  } else {
+    const float2 xi = P.qpos[i];
+    const float hi = h_of(P.qh, i, P.h0);
+    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+      const int j = P.nbr[k];
+      const float2 c = kv_of(P, nullptr, k, j, xi, hi, vol);
+      const float s = pi + p[j];
+      ax += s * c.x;
+      ay += s * c.y;
+    }
   }
   const float r = rho[i];
   acc[i] = make_float2(-ax / r, -ay / r);
@@ -210,22 +254,50 @@ struct SweepBArgs {
   int n;
 };
 
+// Sweep B (dfsph.py:39-50): K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij,
+// p_i += omega (s_i - K_i) / alpha_i (clamped at 0, in place: a thread
+// touches only its own p_i), residual K_i - s_i, per-block float64 partial
+// of max(r_i/rho0, -thr).
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs B) {
   __shared__ double sh[kBlock / 64];
-  if (B.ctrl->done) return;
+  if (B.ctrl && B.ctrl->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double err = 0.0;
   if (i < P.nq) {
-    const float2 xi = P.qpos[i];
-    const float hi = h_of(P.qh, i, P.h0);
     const float2 ai = B.acc[i];
     float sum = 0.f;
     const RowRef rr = row_ref(P, i);
-    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-      const int j = P.nbr[k];
-      const float2 c = kv_of(P, B.kv, k, j, xi, hi, B.vol);
-      const float2 aj = B.acc[j];
-      sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
+    if (B.kv) {
+      int n = 0, k = rr.base;
+      for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
+        int j[kUnroll];
+        float2 c[kUnroll];
+        float2 aj[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          j[u] = ld_stream(P.nbr + k + u * rr.step);
+          c[u] = ld_stream(B.kv + k + u * rr.step);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) aj[u] = B.acc[j[u]];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) sum += (aj[u].x - ai.x) * c[u].x + (aj[u].y - ai.y) * c[u].y;
+      }
+      for (; n < rr.cnt; ++n, k += rr.step) {
+        const int j = ld_stream(P.nbr + k);
+        const float2 c = ld_stream(B.kv + k);
+        const float2 aj = B.acc[j];
+        sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
+      }
+    } else {
+      const float2 xi = P.qpos[i];
+      const float hi = h_of(P.qh, i, P.h0);
+      for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+        const int j = P.nbr[k];
+        const float2 c = kv_of(P, nullptr, k, j, xi, hi, B.vol);
+        const float2 aj = B.acc[j];
+        sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
+      }
     }
     const float K = -B.dtf * sum;
     const float s = B.src[i];
@@ -544,6 +616,21 @@ extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, 
   PairsDev P = make_pairs(*pairs);
   k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
       P, reinterpret_cast<const float2*>(kgrad), vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
+  return launch_status();
+}
+
+extern "C" int dsph_pressure_update(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
+                                    const float* vol, const float* src, const float* alpha, const float* accel,
+                                    float* p, float* residual, double* block_partials, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm || pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  const int64_t n = pairs->nq;
+  if (n == 0) return DSPH_OK;
+  if (!src || !alpha || !accel || !p || !block_partials || (!kgrad && !vol)) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, block_partials, nullptr, n);
+  k_dfsph_sweep_b<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(P, B);
   return launch_status();
 }
 

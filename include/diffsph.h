@@ -293,6 +293,18 @@ int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
                      float* residual, dsph_solve_stats* stats,
                      void* workspace, size_t workspace_bytes, void* stream);
 
+/* One pressure update (updatePressure, v2/modules/dfsph.py:39-50):
+ * K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij with dtf = dt^2 for the
+ * density solve (prm->source_density) else dt, p_i += omega (s_i - K_i) /
+ * alpha_i (clamped at 0 when prm->clamp_pressure), residual_i = K_i - s_i
+ * (may be NULL), and per 256-particle block the float64 sum of
+ * max(residual_i / rho0, -error_threshold) into block_partials
+ * [ceil(n/256)].  One pair sweep; dsph_dfsph_solve runs it in its loop. */
+int dsph_pressure_update(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
+                         const float* kgrad, const float* vol, const float* src,
+                         const float* alpha, const float* accel, float* p,
+                         float* residual, double* block_partials, void* stream);
+
 /* Pressure acceleration a_i = -(1/rho_i) sum_j V_j (p_i + p_j) gradW_ij
  * (computePressureAcceleration, v2/modules/dfsph.py:35-37); one pair sweep. */
 int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad,
