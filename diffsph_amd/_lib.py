@@ -96,6 +96,19 @@ class DeltaSPHParams(C.Structure):
     ]
 
 
+class Boundary(C.Structure):
+    _fields_ = [
+        ("pairs", C.POINTER(Pairs)),
+        ("mass", C.c_void_p),
+        ("rho", C.c_void_p),
+        ("vel", C.c_void_p),
+        ("visc_coeff", C.c_float),
+        ("visc_eps", C.c_float),
+        ("viscosity", C.c_int32),
+        ("pi_switch", C.c_int32),
+    ]
+
+
 class RK4Stages(C.Structure):
     _fields_ = [
         ("dxdt", C.c_void_p * 4),
@@ -153,8 +166,10 @@ SIGNATURES = {
     "dsph_mlp_loss_grad": (C.c_int, [V, V, I64, F32, V, V, V, V, V, SZ, V]),
     "dsph_adam_step": (C.c_int, [V, V, V, V, I64, F32, F32, F32, F32, F32, I32, V]),
     "dsph_remap_neighbors": (C.c_int, [V, I64, V, I64, V]),
-    "dsph_deltasph_normalization": (C.c_int, [P(Pairs), V, V, V, V, V, V]),
-    "dsph_deltasph_rhs": (C.c_int, [P(Pairs), P(DeltaSPHParams), V, V, V, V, V, V, V, V]),
+    "dsph_deltasph_normalization": (C.c_int, [P(Pairs), P(Boundary), V, V, V, V, V, V]),
+    "dsph_deltasph_rhs": (C.c_int, [P(Pairs), P(Boundary), P(DeltaSPHParams), V, V, V, V, V, V, V, V]),
+    "dsph_boundary_ghosts": (C.c_int, [V, V, V, I64, F32, V, V]),
+    "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, V, V, V]),
     "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),
     "dsph_rk4_combine": (C.c_int, [I64, P(RK4Stages), V, V, V, F32, V, V, V, V]),
     "dsph_max_norm_workspace": (SZ, []),

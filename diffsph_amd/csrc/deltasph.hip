@@ -24,7 +24,51 @@
 
 namespace dsph {
 
-__global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, const float* __restrict__ mass,
+// The wall particles seen by the fluid rows: pair list with rows = fluid
+// particles (same order as the fluid list) and entries = boundary indices,
+// rpos = boundary positions; the boundary's mass, mDBC density, velocity
+// and its viscosity parameters (boundaryDiffusion, boundaryViscosity.py).
+struct BoundaryDev {
+  PairsDev P;
+  const float* mass;
+  const float* rho;
+  const float2* vel;
+  float visc_coeff, visc_eps;
+  float visc_c_viscid;  // 2 (d+2) nu_b rho0 for the viscid wall term
+  int viscosity;
+  int pi_switch;
+  int on;
+};
+
+inline BoundaryDev make_boundary(const dsph_boundary* b) {
+  BoundaryDev B;
+  B.on = b != nullptr;
+  if (!b) {
+    B.mass = B.rho = nullptr;
+    B.vel = nullptr;
+    B.visc_coeff = B.visc_eps = B.visc_c_viscid = 0.f;
+    B.viscosity = -1;
+    B.pi_switch = 0;
+    return B;
+  }
+  B.P = make_pairs(*b->pairs);
+  B.mass = b->mass;
+  B.rho = b->rho;
+  B.vel = reinterpret_cast<const float2*>(b->vel);
+  B.visc_coeff = b->visc_coeff;
+  B.visc_eps = b->visc_eps;
+  B.visc_c_viscid = 0.f;
+  B.viscosity = b->viscosity;
+  B.pi_switch = b->pi_switch;
+  return B;
+}
+
+// With a boundary: M_i = M_ff + M_bf ('fluidwBoundary', deltaPlus.py:241),
+// the identity test uses the fluid row length (the fluid numNeighbors), and
+// the wall term of the density gradient is the plain 'difference' gradient
+// sum_b V_b (rho_i - rho_b) gradW_ib (densityGradient, deltaPlus.py:262-263).
+__global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, BoundaryDev B,
+                                                               const float* __restrict__ mass,
                                                                const float* __restrict__ rho,
                                                                float4* __restrict__ L_out,
                                                                float2* __restrict__ ev_out,
@@ -48,6 +92,27 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, const
     m10 += qy * kx;
     m11 += qy * ky;
   }
+  if (B.on) {
+    const RowRef br = row_ref(B.P, i);
+    float b00 = 0.f, b01 = 0.f, b10 = 0.f, b11 = 0.f;
+    for (int n = 0, k = br.base; n < br.cnt; ++n, k += br.step) {
+      const int j = B.P.nbr[k];
+      const float h = pick_h(B.P.mode, hi, h_of(B.P.rh, j, B.P.h0));
+      const Geom g = pair_geom(xi, B.P.rpos[j], h, B.P.dom);
+      const float Vj = B.mass[j] / B.rho[j];
+      const float s = -(g.q * g.h);
+      const float qx = s * g.ex, qy = s * g.ey;
+      const float kx = Vj * g.gx, ky = Vj * g.gy;
+      b00 += qx * kx;
+      b01 += qx * ky;
+      b10 += qy * kx;
+      b11 += qy * ky;
+    }
+    m00 = m00 + b00;
+    m01 = m01 + b01;
+    m10 = m10 + b10;
+    m11 = m11 + b11;
+  }
   if (rr.cnt < 4) {
     m00 = 1.f;
     m01 = 0.f;
@@ -70,6 +135,22 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, const
     const float d = rho_i - rho[j];
     ax += d * kx;
     ay += d * ky;
+  }
+  if (B.on) {
+    const RowRef br = row_ref(B.P, i);
+    float bx = 0.f, by = 0.f;
+    for (int n = 0, k = br.base; n < br.cnt; ++n, k += br.step) {
+      const int j = B.P.nbr[k];
+      const float h = pick_h(B.P.mode, hi, h_of(B.P.rh, j, B.P.h0));
+      const Geom g = pair_geom(xi, B.P.rpos[j], h, B.P.dom);
+      const float Vj = B.mass[j] / B.rho[j];
+      const float kx = Vj * g.gx, ky = Vj * g.gy;
+      const float d = rho_i - B.rho[j];
+      bx += d * kx;
+      by += d * ky;
+    }
+    ax = ax + bx;
+    ay = ay + by;
   }
   grad_rho[i] = make_float2(ax, ay);
 }
@@ -95,7 +176,12 @@ struct RhsArgs {
 
 __device__ __forceinline__ float eos_p(float rho, float rho0, float c) { return c * (rho - rho0); }
 
-__global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, RhsArgs A) {
+// With a boundary (deltaPlus.py:281-329, direction 'all' / 'boundaryToFluid'
+// for the fluid rows): continuity and pressure add the wall pairs as a
+// second sum (ftf + btf), the viscosity adds computeViscosity_Boundary
+// (boundaryViscosity.py:123-170 inviscid with V_b = m_b/(rho_i + rho_b),
+// :198-218 viscid with V_b = m_b/rho_b) with its own coefficients.
+__global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, RhsArgs A) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -144,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, RhsArgs A) {
       accPy += ps * ky;
     }
   }
-  const float apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
+  float apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
   float vsx = 0.f, vsy = 0.f;
   if (A.visc_scheme == 1) {
     const float s = A.visc_c / rho_i;
@@ -155,7 +241,54 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, RhsArgs A) {
     vsx = s * accVx;
     vsy = s * accVy;
   }
-  const float cont = -accC;
+  float cont = -accC;
+  if (B.on) {
+    float bVx = 0.f, bVy = 0.f, bC = 0.f, bPx = 0.f, bPy = 0.f;
+    const RowRef br = row_ref(B.P, i);
+    for (int n = 0, k = br.base; n < br.cnt; ++n, k += br.step) {
+      const int j = B.P.nbr[k];
+      const float h = pick_h(B.P.mode, hi, h_of(B.P.rh, j, B.P.h0));
+      const Geom g = pair_geom(xi, B.P.rpos[j], h, B.P.dom);
+      const float2 uj = B.vel[j];
+      const float rho_j = B.rho[j];
+      const float m_j = B.mass[j];
+      const float rij = g.q * g.h;
+      if (B.viscosity >= 0) {
+        const float vx = ui.x - uj.x, vy = ui.y - uj.y;
+        const float vr = vx * g.ex + vy * g.ey;
+        float pi = vr / (rij + B.visc_eps * (g.h * g.h));
+        if (B.pi_switch) pi = vr < 0.0f ? pi : 0.0f;
+        const float Vb = B.viscosity == 0 ? m_j / (rho_i + rho_j) : m_j / rho_j;
+        const float vs = Vb * pi;
+        bVx += vs * g.gx;
+        bVy += vs * g.gy;
+      }
+      {
+        const float qx = (uj.x - ui.x) * m_j, qy = (uj.y - ui.y) * m_j;
+        float kq = qx * g.gx + qy * g.gy;
+        kq = (kq * rho_i) / rho_j;
+        bC += kq;
+      }
+      {
+        const float Vj = m_j / rho_j;
+        const float ps = p_i + eos_p(rho_j, A.rho0, A.eos_c);
+        bPx += ps * (Vj * g.gx);
+        bPy += ps * (Vj * g.gy);
+      }
+    }
+    apx = apx + (-bPx) / rho_i;
+    apy = apy + (-bPy) / rho_i;
+    cont = cont + (-bC);
+    if (B.viscosity == 1) {
+      const float s = B.visc_c_viscid / rho_i;
+      vsx = vsx + s * bVx;
+      vsy = vsy + s * bVy;
+    } else if (B.viscosity == 0) {
+      const float s = (((B.visc_coeff * hi) * A.cs) * A.rho0) / rho_i;
+      vsx = vsx + s * bVx;
+      vsy = vsy + s * bVy;
+    }
+  }
   const float D = A.diffusion ? (((A.delta * hi) / A.kscale) * A.cs) * accD : 0.0f;
   A.dudt[i] = make_float2((apx + A.gx) + vsx, (apy + A.gy) + vsy);
   A.drhodt[i] = cont + D;
@@ -278,6 +411,79 @@ __global__ void k_sum_final(const double* part, int nb, double* out) {
   *out = s;
 }
 
+// Ghost positions of wall particles (buildBoundaryGhostParticles,
+// v2/modules/mDBC.py:9-17): x_g = x_b + (d + max(d, -h)) n with the signed
+// distance d (negative inside the wall) and the normal n pointing into it.
+__global__ void __launch_bounds__(kBlock) k_ghosts(const float2* __restrict__ bpos, const float* __restrict__ dist,
+                                                   const float2* __restrict__ nrm, int nb, float h,
+                                                   float2* __restrict__ gpos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const float d = dist[i];
+  const float md = d + (d > -h ? d : -h);
+  const float2 x = bpos[i], nn = nrm[i];
+  gpos[i] = make_float2(x.x + md * nn.x, x.y + md * nn.y);
+}
+
+// mDBC wall density (mDBCDensity, v2/modules/mDBC.py:50-136): per ghost g
+// over its fluid neighbors j (V_j = m_j / rho_j, x_gj = x_j - x_g):
+//   Shepard  rho_S = sum m_j W / (sum V_j W + 1e-7)
+//   A = [[sum V W, sum V W x_gj^T], [sum V gradW, sum V x_gj (x) gradW]^T-layout as the
+//        reference fills A_g], b = [sum m_j W, sum m_j gradW]
+//   rho_b = rho0 (no neighbors), rho_S (1..5 neighbors), else
+//   c = pinv(A) b (pinv only for > 4 neighbors) and rho_b = c0 + (x_b - x_g) . c[1:].
+__global__ void __launch_bounds__(kBlock) k_mdbc(PairsDev G, const float* __restrict__ mass,
+                                                 const float* __restrict__ rho, const float2* __restrict__ bpos,
+                                                 float rho0, float support, float* __restrict__ rho_b,
+                                                 float* __restrict__ shepard) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nq) return;
+  const float2 xg = G.qpos[i];
+  const float hi = h_of(G.qh, i, G.h0);
+  float nom = 0.f, vol = 0.f, gsx = 0.f, gsy = 0.f, vgx = 0.f, vgy = 0.f, psx = 0.f, psy = 0.f;
+  float p00 = 0.f, p01 = 0.f, p10 = 0.f, p11 = 0.f;
+  const RowRef rr = row_ref(G, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = G.nbr[k];
+    const float h = pick_h(G.mode, hi, h_of(G.rh, j, G.h0));
+    const Geom g = pair_geom(xg, G.rpos[j], h, G.dom);
+    const float m_j = mass[j], rho_j = rho[j];
+    const float Vj = m_j / rho_j;
+    nom += m_j * g.w;
+    const float kw = Vj * g.w;
+    vol += 1.0f * kw;
+    const float kx = Vj * g.gx, ky = Vj * g.gy;
+    gsx += rho_j * kx;
+    gsy += rho_j * ky;
+    vgx += 1.0f * kx;
+    vgy += 1.0f * ky;
+    const float xx = ((-g.ex) * g.q) * support, xy = ((-g.ey) * g.q) * support;
+    psx += xx * kw;
+    psy += xy * kw;
+    p00 += xx * kx;
+    p01 += xx * ky;
+    p10 += xy * kx;
+    p11 += xy * ky;
+  }
+  const float shep = nom / (vol + 1e-7f);
+  float out = rho0;
+  if (rr.cnt > 0) out = shep;
+  if (rr.cnt > 5) {
+    const double A[9] = {vol, psx, psy, vgx, p00, p01, vgy, p10, p11};
+    double Ai[9];
+    dsph_pm::pinv3x3(A, 3.0 * 1.1920928955078125e-07, Ai);
+    const double b0 = nom, b1 = gsx, b2 = gsy;
+    const float c0 = (float)(Ai[0] * b0 + Ai[1] * b1 + Ai[2] * b2);
+    const float c1 = (float)(Ai[3] * b0 + Ai[4] * b1 + Ai[5] * b2);
+    const float c2 = (float)(Ai[6] * b0 + Ai[7] * b1 + Ai[8] * b2);
+    const float2 xb = bpos[i];
+    const float rx = xb.x - xg.x, ry = xb.y - xg.y;
+    out = c0 + (rx * c1 + ry * c2);
+  }
+  rho_b[i] = out;
+  if (shepard) shepard[i] = shep;
+}
+
 static int check_same_set(const dsph_pairs* pairs) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
@@ -289,23 +495,40 @@ static int check_same_set(const dsph_pairs* pairs) {
 
 using namespace dsph;
 
-extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const float* mass, const float* rho, float* L,
-                                           float* eigvals, float* grad_rho, void* stream) {
+static int check_boundary(const dsph_boundary* b, const dsph_pairs* fluid) {
+  if (!b) return DSPH_OK;
+  int s = check_pairs(b->pairs);
+  if (s != DSPH_OK) return s;
+  if (b->pairs->nq != fluid->nq) return DSPH_ERR_ARG;
+  if (b->pairs->nr > 0 && (!b->mass || !b->rho || !b->vel)) return DSPH_ERR_ARG;
+  if (b->viscosity < -1 || b->viscosity > 1) return DSPH_ERR_ARG;
+  return DSPH_OK;
+}
+
+extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_boundary* boundary, const float* mass,
+                                           const float* rho, float* L, float* eigvals, float* grad_rho,
+                                           void* stream) {
   int s = check_same_set(pairs);
+  if (s != DSPH_OK) return s;
+  s = check_boundary(boundary, pairs);
   if (s != DSPH_OK) return s;
   if (pairs->nq == 0) return DSPH_OK;
   if (!mass || !rho || !grad_rho) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
+  BoundaryDev B = make_boundary(boundary);
   k_dsph_normalization<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
+      P, B, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
       reinterpret_cast<float2*>(grad_rho));
   return launch_status();
 }
 
-extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_deltasph_params* prm, const float* vel,
-                                 const float* mass, const float* rho, const float* grad_rho, float* dudt,
-                                 float* drhodt, float* terms, void* stream) {
+extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                 const dsph_deltasph_params* prm, const float* vel, const float* mass,
+                                 const float* rho, const float* grad_rho, float* dudt, float* drhodt, float* terms,
+                                 void* stream) {
   int s = check_same_set(pairs);
+  if (s != DSPH_OK) return s;
+  s = check_boundary(boundary, pairs);
   if (s != DSPH_OK) return s;
   if (!prm) return DSPH_ERR_ARG;
   if (pairs->nq == 0) return DSPH_OK;
@@ -336,7 +559,9 @@ extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_deltasph_pa
   A.visc_scheme = prm->viscosity;
   A.pi_switch = prm->pi_switch;
   A.diffusion = prm->diffusion;
-  k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, A);
+  BoundaryDev B = make_boundary(boundary);
+  if (B.on && B.viscosity == 1) B.visc_c_viscid = (float)(2.0 * 4.0 * (double)B.visc_coeff * (double)prm->rho0);
+  k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, B, A);
   return launch_status();
 }
 
@@ -403,4 +628,28 @@ extern "C" int dsph_sum_squares(const float* v, int64_t n, double* out_host, voi
   DSPH_CHECK_HIP(hipMemcpyAsync(out_host, out, sizeof(double), hipMemcpyDeviceToHost, st));
   DSPH_CHECK_HIP(hipStreamSynchronize(st));
   return DSPH_OK;
+}
+
+extern "C" int dsph_boundary_ghosts(const float* bpos, const float* distances, const float* normals, int64_t nb,
+                                    float support, float* ghost_pos, void* stream) {
+  if (nb < 0 || nb >= INT32_MAX || (nb > 0 && (!bpos || !distances || !normals || !ghost_pos))) return DSPH_ERR_ARG;
+  if (nb == 0) return DSPH_OK;
+  k_ghosts<<<grid_for(nb), kBlock, 0, as_stream(stream)>>>(reinterpret_cast<const float2*>(bpos), distances,
+                                                            reinterpret_cast<const float2*>(normals), (int)nb,
+                                                            support, reinterpret_cast<float2*>(ghost_pos));
+  return launch_status();
+}
+
+extern "C" int dsph_mdbc(const dsph_pairs* ghost_pairs, const float* fluid_mass, const float* fluid_rho,
+                         const float* boundary_pos, float rho0, float support, float* boundary_rho, float* shepard,
+                         void* stream) {
+  int s = check_pairs(ghost_pairs);
+  if (s != DSPH_OK) return s;
+  if (ghost_pairs->nq == 0) return DSPH_OK;
+  if (!boundary_pos || !boundary_rho || (ghost_pairs->nr > 0 && (!fluid_mass || !fluid_rho))) return DSPH_ERR_ARG;
+  PairsDev G = make_pairs(*ghost_pairs);
+  k_mdbc<<<grid_for(G.nq), kBlock, 0, as_stream(stream)>>>(G, fluid_mass, fluid_rho,
+                                                            reinterpret_cast<const float2*>(boundary_pos), rho0,
+                                                            support, boundary_rho, shepard);
+  return launch_status();
 }

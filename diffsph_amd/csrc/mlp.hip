@@ -390,21 +390,38 @@ __global__ void __launch_bounds__(kBlock) k_mlp_train(const float* __restrict__ 
   }
 }
 
-// Sum block partials in fixed order: g[p] = sum_b gpart[b][p].
-__global__ void __launch_bounds__(kBlock) k_mlp_reduce(const float* __restrict__ gpart,
-                                                       const float* __restrict__ lpart, int nb,
-                                                       float* __restrict__ grad, float* __restrict__ loss) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum block partials in a fixed two-level order: group g of kRedGroups sums
+// its contiguous run of blocks, then the group sums are added in order.
+// Column kNumParams of a partial row is the block's loss.
+constexpr int kRedGroups = 32;
+
+__global__ void __launch_bounds__(kBlock) k_mlp_reduce_groups(const float* __restrict__ gpart,
+                                                              const float* __restrict__ lpart, int nb,
+                                                              float* __restrict__ gmid) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (p > kNumParams) return;
+  const int chunk = (nb + kRedGroups - 1) / kRedGroups;
+  const int b0 = g * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  float s = 0.f;
   if (p < kNumParams) {
-    float s = 0.f;
-    for (int b = 0; b < nb; ++b) s += gpart[(int64_t)b * kNumParams + p];
+    for (int b = b0; b < b1; ++b) s += gpart[(int64_t)b * kNumParams + p];
+  } else {
+    for (int b = b0; b < b1; ++b) s += lpart[b];
+  }
+  gmid[(int64_t)g * (kNumParams + 1) + p] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mlp_reduce_final(const float* __restrict__ gmid,
+                                                             float* __restrict__ grad, float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kNumParams) return;
+  float s = 0.f;
+  for (int g = 0; g < kRedGroups; ++g) s += gmid[(int64_t)g * (kNumParams + 1) + p];
+  if (p < kNumParams)
     grad[p] = s;
-  }
-  if (p == 0 && loss) {
-    float s = 0.f;
-    for (int b = 0; b < nb; ++b) s += lpart[b];
+  else if (loss)
     loss[0] = s;
-  }
 }
 
 // Adam (fp32 master weights), bias-corrected.
@@ -452,7 +469,7 @@ extern "C" int dsph_mlp_forward(const float* X, int64_t n, const float* params, 
 }
 
 extern "C" size_t dsph_mlp_train_workspace(int64_t n) {
-  return ((size_t)mlp_grid(n) * (kNumParams + 1) * sizeof(float) + 255) / 256 * 256;
+  return (((size_t)mlp_grid(n) + kRedGroups) * (kNumParams + 1) * sizeof(float) + 255) / 256 * 256;
 }
 
 extern "C" int dsph_mlp_loss_grad(const float* X, const float* target, int64_t n, float loss_weight,
@@ -472,7 +489,9 @@ extern "C" int dsph_mlp_loss_grad(const float* X, const float* target, int64_t n
   }
   k_mlp_train<<<nb, kBlock, 0, st>>>(X, target, (int)n, loss_weight, params, static_cast<const short4v*>(packed),
                                      gpart, lpart);
-  k_mlp_reduce<<<grid_for(kNumParams), kBlock, 0, st>>>(gpart, lpart, nb, grad, loss);
+  float* gmid = lpart + nb;
+  k_mlp_reduce_groups<<<dim3(grid_for(kNumParams + 1), kRedGroups), kBlock, 0, st>>>(gpart, lpart, nb, gmid);
+  k_mlp_reduce_final<<<grid_for(kNumParams + 1), kBlock, 0, st>>>(gmid, grad, loss);
   return launch_status();
 }
 

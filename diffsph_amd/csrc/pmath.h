@@ -114,4 +114,65 @@ DSPH_HD inline Pinv2 pinv2x2(float a, float b, float c, float d) {
   return r;
 }
 
+// Pseudo-inverse of a 3x3 matrix (row-major a[9]) as torch.linalg.pinv
+// computes it: SVD, singular values <= rtol * sigma_max dropped (rtol is
+// max(m, n) * eps of float32 for a float32 input).  Evaluated in float64:
+// cyclic Jacobi on A^T A = V diag(l) V^T (16 sweeps, far past convergence),
+// then pinv = sum over kept i of v_i (A v_i)^T / l_i.  Returns the number of
+// singular values kept.
+DSPH_HD inline int pinv3x3(const double* a, double rtol, double* out) {
+  double S[3][3], V[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += a[3 * k + r] * a[3 * k + c];
+      S[r][c] = s;
+      V[r][c] = r == c ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    for (int pq = 0; pq < 3; ++pq) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      const double apq = S[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (S[q][q] - S[p][p]) / (2.0 * apq);
+      const double at = theta < 0.0 ? -theta : theta;
+      const double t = (theta < 0.0 ? -1.0 : 1.0) / (at + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      for (int k = 0; k < 3; ++k) {  // S <- S J (columns p, q)
+        const double skp = S[k][p], skq = S[k][q];
+        S[k][p] = c * skp - s * skq;
+        S[k][q] = s * skp + c * skq;
+      }
+      for (int k = 0; k < 3; ++k) {  // S <- J^T S (rows p, q)
+        const double spk = S[p][k], sqk = S[q][k];
+        S[p][k] = c * spk - s * sqk;
+        S[q][k] = s * spk + c * sqk;
+      }
+      for (int k = 0; k < 3; ++k) {  // V <- V J
+        const double vkp = V[k][p], vkq = V[k][q];
+        V[k][p] = c * vkp - s * vkq;
+        V[k][q] = s * vkp + c * vkq;
+      }
+    }
+  }
+  double sig[3], smax = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double l = S[i][i] > 0.0 ? S[i][i] : 0.0;
+    sig[i] = sqrt(l);
+    smax = sig[i] > smax ? sig[i] : smax;
+  }
+  for (int k = 0; k < 9; ++k) out[k] = 0.0;
+  int kept = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (!(sig[i] > rtol * smax) || sig[i] == 0.0) continue;
+    ++kept;
+    double av[3];
+    for (int r = 0; r < 3; ++r) av[r] = a[3 * r] * V[0][i] + a[3 * r + 1] * V[1][i] + a[3 * r + 2] * V[2][i];
+    const double inv = 1.0 / (sig[i] * sig[i]);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) out[3 * r + c] += V[r][i] * av[c] * inv;
+  }
+  return kept;
+}
+
 }  // namespace dsph_pm

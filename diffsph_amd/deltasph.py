@@ -62,6 +62,12 @@ class DeltaSPHConfig:
     viscosity_constraint: bool = True
     acceleration_constraint: bool = True
     acoustic_constraint: bool = True
+    # wall viscosity (boundaryDiffusion, boundaryViscosity.py:219-241)
+    boundary_viscosity: str | None = "deltaSPH_inviscid"
+    boundary_alpha: float = 0.01
+    boundary_nu: float = 0.01
+    boundary_eps: float = 1e-6
+    boundary_pi_switch: bool = False
     # "global": |dudt| over all components, as computeTimestep evaluates
     # torch.linalg.norm(dudt[~isnan(dudt)]) on a flattened tensor
     # (timestep.py:22-25); "max": the largest per-particle |dudt|
@@ -115,6 +121,37 @@ def compute_timestep(cfg: DeltaSPHConfig, dt_prev: float, max_accel: float | Non
     return float(F(new))
 
 
+class Walls:
+    """Static wall particles with mDBC densities (v2/modules/mDBC.py).
+
+    The set is sorted once into its own cell order (so sorted slots are wall
+    indices), its ghosts are mirrored into the fluid once, and its density
+    is re-extrapolated from the fluid at every right-hand side."""
+
+    def __init__(self, positions, distances, normals, areas, cfg: "DeltaSPHConfig", domain):
+        dev = positions.device
+        f = dict(dtype=torch.float32, device=dev)
+        pos = positions.detach().to(**f).contiguous()
+        self.grid = CellGrid.build(pos, None, domain, cfg.support, h_uniform=cfg.support)
+        si = self.grid.sort_index.long()
+        self.n = pos.shape[0]
+        self.pos = self.grid.sorted_pos
+        self.dist = distances.detach().to(**f)[si].contiguous()
+        self.normal = normals.detach().to(**f)[si].contiguous()
+        self.mass = (areas.detach().to(**f).expand(self.n)[si] * cfg.rho0).contiguous()
+        self.vel = torch.zeros(self.n, 2, **f)
+        self.rho = torch.full((self.n,), float(cfg.rho0), **f)
+        self.shepard = torch.empty(self.n, **f)
+        self.ghost = torch.empty(self.n, 2, **f)
+        check(lib().dsph_boundary_ghosts(ptr(self.pos), ptr(self.dist), ptr(self.normal), self.n,
+                                         C.c_float(cfg.support), ptr(self.ghost), _lib.stream_handle(dev)),
+              "boundary_ghosts")
+        self.viscosity = VISCOSITY[cfg.boundary_viscosity]
+        self.visc_coeff = cfg.boundary_nu if cfg.boundary_viscosity == "deltaSPH_viscid" else cfg.boundary_alpha
+        self.eps = cfg.boundary_eps
+        self.pi_switch = 1 if cfg.boundary_pi_switch else 0
+
+
 @dataclass
 class RHS:
     dudt: torch.Tensor
@@ -126,7 +163,9 @@ class RHS:
 class DeltaSPHSolver:
     """Owns the particle state (cell-ordered, on the GPU) and steps it with RK4."""
 
-    def __init__(self, positions, velocities, masses, densities, cfg: DeltaSPHConfig, uid=None):
+    def __init__(self, positions, velocities, masses, densities, cfg: DeltaSPHConfig, uid=None, walls=None):
+        """`walls`: optional dict(pos, dist, normal, area) of static wall
+        particles handled with mDBC."""
         if not positions.is_cuda:
             raise RuntimeError("DeltaSPHSolver needs GPU tensors")
         self.cfg = cfg
@@ -141,6 +180,11 @@ class DeltaSPHSolver:
         self.uid = (uid if uid is not None else torch.arange(self.n, device=dev, dtype=torch.int32)).to(
             torch.int32).contiguous().clone()
         self.prm = cfg.params()
+        self.walls = None
+        if walls is not None:
+            T = lambda a: a if isinstance(a, torch.Tensor) else torch.as_tensor(a)  # noqa: E731
+            self.walls = Walls(T(walls["pos"]).to(dev), T(walls["dist"]).to(dev), T(walls["normal"]).to(dev),
+                               T(walls["area"]).to(dev), cfg, self.domain)
         self.dt = cfg.dt if cfg.dt is not None else compute_timestep(cfg, cfg.max_dt, None)
         self.time = 0.0
         self.step_count = 0
@@ -156,7 +200,7 @@ class DeltaSPHSolver:
     def neighbors(self, pos: torch.Tensor):
         """Neighbor list of `pos` with rows in the state's order and entries
         mapped to state indices; returns (NeighborList, pairs descriptor,
-        wrapped positions)."""
+        wrapped positions, grid)."""
         cfg = self.cfg
         grid = CellGrid.build(pos, None, self.domain, cfg.support, h_uniform=cfg.support)
         wpos = wrap_positions(pos, self.domain)
@@ -166,20 +210,46 @@ class DeltaSPHSolver:
         pr = nl.pairs()
         pr.rpos = ptr(wpos)
         pr.rh = None
-        return nl, pr, wpos
+        return nl, pr, wpos, grid
+
+    def wall_pairs(self, wpos, grid, rho):
+        """Fluid->wall list (rows fluid, entries wall indices), the ghost->
+        fluid list, and the mDBC wall densities at the fluid state `rho`.
+        Returns (boundary descriptor, objects to keep alive)."""
+        W = self.walls
+        st = _lib.stream_handle(wpos.device)
+        nl_b = search(W.grid, wpos, None, "scatter", qpos_wrapped=True)
+        pr_b = nl_b.pairs()
+        nl_g = search(grid, W.ghost, None, "scatter", qpos_wrapped=True)
+        check(lib().dsph_remap_neighbors(ptr(nl_g.nbr), nl_g.nbr.numel(), ptr(grid.sort_index), self.n, st),
+              "remap_neighbors")
+        pr_g = nl_g.pairs()
+        pr_g.rpos = ptr(wpos)
+        pr_g.rh = None
+        check(lib().dsph_mdbc(C.byref(pr_g), ptr(self.mass), ptr(rho), ptr(W.pos), C.c_float(self.cfg.rho0),
+                              C.c_float(self.cfg.support), ptr(W.rho), ptr(W.shepard), st), "mdbc")
+        b = _lib.Boundary()
+        b.pairs = C.pointer(pr_b)
+        b.mass, b.rho, b.vel = ptr(W.mass), ptr(W.rho), ptr(W.vel)
+        b.visc_coeff, b.visc_eps = W.visc_coeff, W.eps
+        b.viscosity, b.pi_switch = W.viscosity, W.pi_switch
+        return b, (nl_b, pr_b, nl_g, pr_g)
 
     def rhs(self, pos, vel, rho, dudt, drhodt, terms=None, L=None, eigvals=None) -> RHS:
         """One right-hand side at (pos, vel, rho) into dudt [n,2], drhodt [n]."""
         st = _lib.stream_handle(pos.device)
-        nl, pr, wpos = self.neighbors(pos)
+        nl, pr, wpos, grid = self.neighbors(pos)
+        bnd, keep = (None, None)
+        if self.walls is not None:
+            bnd, keep = self.wall_pairs(wpos, grid, rho)
+        bp = C.byref(bnd) if bnd is not None else None
         L_ = lib()
         if self.cfg.diffusion or L is not None or eigvals is not None:
-            check(L_.dsph_deltasph_normalization(C.byref(pr), ptr(self.mass), ptr(rho), ptr(L), ptr(eigvals),
+            check(L_.dsph_deltasph_normalization(C.byref(pr), bp, ptr(self.mass), ptr(rho), ptr(L), ptr(eigvals),
                                                  ptr(self._grad), st), "deltasph_normalization")
-        check(L_.dsph_deltasph_rhs(C.byref(pr), C.byref(self.prm), ptr(vel), ptr(self.mass), ptr(rho),
+        check(L_.dsph_deltasph_rhs(C.byref(pr), bp, C.byref(self.prm), ptr(vel), ptr(self.mass), ptr(rho),
                                    ptr(self._grad), ptr(dudt), ptr(drhodt), ptr(terms), st), "deltasph_rhs")
-        # keep the list alive until the kernels that read it have been queued
-        self._keep = (nl, wpos)
+        self._keep = (nl, wpos, keep)
         return RHS(dudt, drhodt, self._grad, nl.num_pairs)
 
     def _sort(self):
@@ -276,3 +346,26 @@ def analytic_tgv_energy_ratio(t: float, nu: float) -> float:
     """E_k(t)/E_k(0) of the Taylor-Green vortex with wavenumber pi per axis:
     exp(-4 nu pi^2 t)."""
     return math.exp(-4.0 * nu * math.pi ** 2 * t)
+
+
+def dam_break_state(res: int, device, g: float = 9.81, column_height: float = 2.0):
+    """The reference's dam-break benchmark: a 1 x 2 water column in the
+    closed 4 x 2 box, dx = 1/res, three wall layers, gravity -g, c_s =
+    20 sqrt(2 g H), delta-SPH inviscid viscosity alpha = 0.01 for fluid and
+    wall, density diffusion delta = 0.1, RK4 with the CFL time step.  The
+    minimum time step is lowered below the acoustic one (the reference's
+    default floor of 1e-3 would exceed it).  Returns (positions,
+    velocities, masses, densities, config, walls)."""
+    from . import sampling
+    d = sampling.dam_break(res)
+    n = d["fluid_pos"].shape[0]
+    dx = d["dx"]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(device)  # noqa: E731
+    cs = 20.0 * math.sqrt(2.0 * g * column_height)
+    cfg = DeltaSPHConfig(support=d["h"], dx=dx, cs=cs, domain_min=(-4 * dx, -4 * dx),
+                         domain_max=(4.0 + 4 * dx, 2.0 + 4 * dx), periodic=(False, False),
+                         viscosity="deltaSPH_inviscid", alpha=0.01, gravity=(0.0, -g), min_dt=1e-7)
+    walls = dict(pos=T(d["wall_pos"]), dist=T(d["wall_dist"]), normal=T(d["wall_normal"]),
+                 area=torch.tensor(d["area"], dtype=torch.float32, device=device))
+    return (T(d["fluid_pos"]), torch.zeros(n, 2, device=device), T(np.full(n, d["area"] * cfg.rho0)),
+            T(np.full(n, cfg.rho0)), cfg, walls)

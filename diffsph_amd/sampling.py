@@ -128,3 +128,43 @@ def dfsph_random_flow(n_side: int, seed: int = 0, jitter_width: float = 0.1, kma
     mass = (area * np.float32(rho0)).astype(np.float32)
     h = volume_to_support(vol, target_neighbors)
     return dict(pos=pos, vel=vel, area=area, mass=mass, dx=dx, h=h)
+
+
+def box_walls(dx: float, box_min=(0.0, 0.0), box_max=(4.0, 2.0), layers: int = 3):
+    """Wall particles on the cell-centred lattice of spacing dx in the
+    `layers`-thick frame around the box: positions, signed distances
+    (negative inside the wall, -sdf of the box) and normals (gradient of the
+    box's sdf, pointing into the wall), as processBoundarySDFs
+    (v2/sampling.py:900-941) sets 'distances' and 'normals'."""
+    x0, y0 = box_min
+    x1, y1 = box_max
+    ix = np.arange(int(round((x0 - layers * dx) / dx)), int(round((x1 + layers * dx) / dx)))
+    iy = np.arange(int(round((y0 - layers * dx) / dx)), int(round((y1 + layers * dx) / dx)))
+    X, Y = np.meshgrid((ix + 0.5) * dx, (iy + 0.5) * dx, indexing="xy")
+    p = np.stack([X.ravel(), Y.ravel()], 1)
+    inside = (p[:, 0] > x0) & (p[:, 0] < x1) & (p[:, 1] > y0) & (p[:, 1] < y1)
+    p = p[~inside]
+    c = np.array([(x0 + x1) / 2, (y0 + y1) / 2])
+    e = np.array([(x1 - x0) / 2, (y1 - y0) / 2])
+    d = p - c
+    q = np.abs(d) - e
+    qp = np.maximum(q, 0.0)
+    sdf = np.sqrt((qp ** 2).sum(1))
+    nrm = qp * np.sign(d) / sdf[:, None]
+    return p.astype(np.float32), (-sdf).astype(np.float32), nrm.astype(np.float32)
+
+
+def dam_break(res: int = 256, column=(1.0, 2.0), box=(4.0, 2.0), target_neighbors: float = 50.0, layers: int = 3):
+    """Dam break: a column [0,w]x[0,H] of water (dx = 1/res, cell-centred
+    lattice) in the closed box [0,4]x[0,2] with `layers` wall rows outside
+    each side.  Returns dict(fluid_pos, wall_pos, wall_dist, wall_normal,
+    dx, h, area)."""
+    dx = 1.0 / res
+    nx, ny = int(round(column[0] * res)), int(round(column[1] * res))
+    xs = (np.arange(nx) + 0.5) * dx
+    ys = (np.arange(ny) + 0.5) * dx
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    fluid = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+    wp, wd, wn = box_walls(dx, (0.0, 0.0), box, layers)
+    h = volume_to_support(dx * dx, target_neighbors)
+    return dict(fluid_pos=fluid, wall_pos=wp, wall_dist=wd, wall_normal=wn, dx=dx, h=h, area=dx * dx)

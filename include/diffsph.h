@@ -410,6 +410,22 @@ typedef struct dsph_deltasph_params {
   int32_t pad_;
 } dsph_deltasph_params;
 
+/* Wall particles seen by the fluid: `pairs` has the fluid particles as rows
+ * (same nq and row order as the fluid list) and boundary indices as
+ * entries (rpos = boundary positions); mass, rho (the mDBC densities of
+ * dsph_mdbc), vel [nb,2]; the wall viscosity of computeViscosity_Boundary
+ * (v2/modules/boundaryViscosity.py:123-218, config boundaryDiffusion). */
+typedef struct dsph_boundary {
+  const dsph_pairs* pairs;
+  const float* mass;
+  const float* rho;
+  const float* vel;
+  float visc_coeff;    /* nu (viscid) or alpha (inviscid) */
+  float visc_eps;
+  int32_t viscosity;   /* -1 none, 0 deltaSPH_inviscid, 1 deltaSPH_viscid */
+  int32_t pi_switch;
+} dsph_boundary;
+
 /* Normalization matrices, their pseudo-inverses and the renormalized
  * density gradient of one particle set against itself:
  * M_i = sum_j V_j (x_j - x_i) (x) grad W_ij (identity for rows with < 4
@@ -418,22 +434,46 @@ typedef struct dsph_deltasph_params {
  * Replaces computeCovarianceMatrices (v2/modules/normalizationMatrices.py:
  * 18-23), pinv2x2 (v2/math.py:10-57), deltaPlus.py:245-248 and
  * renormalizedDensityGradient (v2/modules/densityDiffusion.py:7-26).
+ * With `boundary` (may be NULL): M_i also sums the wall pairs
+ * ('fluidwBoundary', deltaPlus.py:241) and the gradient adds the plain
+ * wall term sum_b V_b (rho_i - rho_b) gradW_ib (densityGradient,
+ * deltaPlus.py:262-263).
  * L [n,4] row-major and eigvals [n,2] may be NULL; grad_rho is [n,2]. */
-int dsph_deltasph_normalization(const dsph_pairs* pairs, const float* mass,
-                                const float* rho, float* L, float* eigvals,
-                                float* grad_rho, void* stream);
+int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                const float* mass, const float* rho, float* L,
+                                float* eigvals, float* grad_rho, void* stream);
 
 /* One fused pass: delta-SPH density diffusion (densityDiffusion.py:49-73),
  * viscosity (viscosity.py:123-170 / 200-218), continuity
  * (momentumEquation.py:6-39), EOS (pressureEOS.py:28-64) and pressure
  * acceleration (pressureForce.py:10-12), combined as deltaPlus.py:349-350:
  * dudt = a_p + g + visc [n,2], drhodt = continuity + diffusion [n].
+ * With `boundary` (may be NULL) continuity and pressure add the wall pairs
+ * (direction 'all', ftf + btf) and the viscosity adds the wall viscosity.
  * terms [n,8] (may be NULL) receives a_p.xy, visc.xy, continuity,
  * diffusion, pressure, 0 for inspection. */
-int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_deltasph_params* prm,
-                      const float* vel, const float* mass, const float* rho,
-                      const float* grad_rho, float* dudt, float* drhodt,
-                      float* terms, void* stream);
+int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                      const dsph_deltasph_params* prm, const float* vel,
+                      const float* mass, const float* rho, const float* grad_rho,
+                      float* dudt, float* drhodt, float* terms, void* stream);
+
+/* Ghost particles of the wall (buildBoundaryGhostParticles,
+ * v2/modules/mDBC.py:9-17): x_g = x_b + (d + max(d, -h)) n, d the signed
+ * distance (negative inside the wall), n the normal pointing into it. */
+int dsph_boundary_ghosts(const float* boundary_pos, const float* distances,
+                         const float* normals, int64_t nb, float support,
+                         float* ghost_pos, void* stream);
+
+/* mDBC wall densities (mDBCDensity, v2/modules/mDBC.py:50-136): rows of
+ * `ghost_pairs` are the ghosts (qpos = ghost positions, one per wall
+ * particle, in wall order), entries fluid indices (rpos = fluid
+ * positions).  Shepard density for 1..5 neighbors, rho0 for none, else the
+ * first-order MLS extrapolation c0 + (x_b - x_g) . c[1:] with
+ * c = pinv(A_g) b (3x3 pseudo-inverse, singular values <= 3 eps_f32
+ * sigma_max dropped as torch.linalg.pinv does).  shepard [nb] may be NULL. */
+int dsph_mdbc(const dsph_pairs* ghost_pairs, const float* fluid_mass,
+              const float* fluid_rho, const float* boundary_pos, float rho0,
+              float support, float* boundary_rho, float* shepard, void* stream);
 
 /* Stage state of the reference's RK4 (v2/modules/integration.py:23-40,
  * 231-276): x = x0 + dxdt c, u = u0 + dudt c, rho = rho0 + drhodt c, each

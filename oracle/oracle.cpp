@@ -619,12 +619,26 @@ void pinv2(T a, T b, T c, T d, T* L, T* ev) {
   ev[1] = swap ? o1 : o2;
 }
 
-// computeCovarianceMatrices (normalizationMatrices.py:18-23), identity for
-// rows with < 4 entries (deltaPlus.py:245), pinv2x2, then
-// renormalizedDensityGradient (densityDiffusion.py:7-26).
+// Wall particles as seen by the fluid rows (rows = fluid, entries = wall
+// indices, rpos = wall positions).
+template <class T>
+struct BSet {
+  Csr<T> c;
+  const T* mass;
+  const T* rho;
+  const T* vel;
+  double visc_coeff, eps;
+  int viscosity, pi_switch;
+};
+
+// computeCovarianceMatrices (normalizationMatrices.py:18-23) over fluid
+// (+ wall: 'fluidwBoundary', deltaPlus.py:241), identity for rows with < 4
+// fluid neighbors (deltaPlus.py:245), pinv2x2, then
+// renormalizedDensityGradient (densityDiffusion.py:7-26) + the plain wall
+// gradient (densityGradient, deltaPlus.py:262-263).
 template <class T>
 void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, T* Lout, T* evout,
-                        T* grad_rho) {
+                        T* grad_rho, const BSet<T>* B) {
   set_threads();
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < c.nq; ++i) {
@@ -640,6 +654,25 @@ void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const 
       m01 += qx * ky;
       m10 += qy * kx;
       m11 += qy * ky;
+    }
+    if (B) {
+      T b00 = 0, b01 = 0, b10 = 0, b11 = 0;
+      for (int k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
+        int32_t j = B->c.nbr[k];
+        G<T> g = pg(B->c, i, j, d);
+        T Vj = B->mass[j] / B->rho[j];
+        T s = -(g.q * g.h);
+        T qx = s * g.ex, qy = s * g.ey;
+        T kx = Vj * g.gx, ky = Vj * g.gy;
+        b00 += qx * kx;
+        b01 += qx * ky;
+        b10 += qy * kx;
+        b11 += qy * ky;
+      }
+      m00 = m00 + b00;
+      m01 = m01 + b01;
+      m10 = m10 + b10;
+      m11 = m11 + b11;
     }
     if (c.off[i + 1] - c.off[i] < 4) {
       m00 = 1;
@@ -667,6 +700,20 @@ void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const 
       ax += dr * kx;
       ay += dr * ky;
     }
+    if (B) {
+      T bx = 0, by = 0;
+      for (int k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
+        int32_t j = B->c.nbr[k];
+        G<T> g = pg(B->c, i, j, d);
+        T Vj = B->mass[j] / B->rho[j];
+        T kx = Vj * g.gx, ky = Vj * g.gy;
+        T dr = rho[i] - B->rho[j];
+        bx += dr * kx;
+        by += dr * ky;
+      }
+      ax = ax + bx;
+      ay = ay + by;
+    }
     grad_rho[2 * i] = ax;
     grad_rho[2 * i + 1] = ay;
   }
@@ -683,7 +730,7 @@ struct DeltaP {
 // deltaPlus.py:349-350.
 template <class T>
 void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, const T* mass, const T* rho,
-              const T* grad_rho, T* dudt, T* drhodt, T* terms) {
+              const T* grad_rho, T* dudt, T* drhodt, T* terms, const BSet<T>* B) {
   const T rho0 = (T)P.rho0;
   const T eos_c = P.eos == 0 ? (T)P.cs * (T)P.cs : (T)P.kappa;
   const T cs = (T)P.cs;
@@ -734,7 +781,7 @@ void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, 
         accPy += ps * ky;
       }
     }
-    const T apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
+    T apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
     T vsx = 0, vsy = 0;
     if (P.viscosity == 1) {
       const T s = visc_c / rho_i;
@@ -745,7 +792,52 @@ void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, 
       vsx = s * accVx;
       vsy = s * accVy;
     }
-    const T cont = -accC;
+    T cont = -accC;
+    if (B) {
+      T bVx = 0, bVy = 0, bC = 0, bPx = 0, bPy = 0;
+      for (int k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
+        int32_t j = B->c.nbr[k];
+        G<T> g = pg(B->c, i, j, d);
+        const T ujx = B->vel[2 * j], ujy = B->vel[2 * j + 1];
+        const T rho_j = B->rho[j];
+        const T m_j = B->mass[j];
+        const T rij = g.q * g.h;
+        if (B->viscosity >= 0) {
+          const T vx = uix - ujx, vy = uiy - ujy;
+          const T vr = vx * g.ex + vy * g.ey;
+          T pi = vr / (rij + (T)B->eps * (g.h * g.h));
+          if (B->pi_switch) pi = vr < T(0) ? pi : T(0);
+          const T Vb = B->viscosity == 0 ? m_j / (rho_i + rho_j) : m_j / rho_j;
+          const T vs = Vb * pi;
+          bVx += vs * g.gx;
+          bVy += vs * g.gy;
+        }
+        {
+          const T qx = (ujx - uix) * m_j, qy = (ujy - uiy) * m_j;
+          T kq = qx * g.gx + qy * g.gy;
+          kq = (kq * rho_i) / rho_j;
+          bC += kq;
+        }
+        {
+          const T Vj = m_j / rho_j;
+          const T ps = p_i + eos_c * (rho_j - rho0);
+          bPx += ps * (Vj * g.gx);
+          bPy += ps * (Vj * g.gy);
+        }
+      }
+      apx = apx + (-bPx) / rho_i;
+      apy = apy + (-bPy) / rho_i;
+      cont = cont + (-bC);
+      if (B->viscosity == 1) {
+        const T s = (T)(2.0 * 4.0 * B->visc_coeff * P.rho0) / rho_i;
+        vsx = vsx + s * bVx;
+        vsy = vsy + s * bVy;
+      } else if (B->viscosity == 0) {
+        const T s = ((((T)B->visc_coeff * hi) * cs) * rho0) / rho_i;
+        vsx = vsx + s * bVx;
+        vsy = vsy + s * bVy;
+      }
+    }
     const T D = P.diffusion ? ((((T)P.delta * hi) / (T)P.kscale) * cs) * accD : T(0);
     dudt[2 * i] = (apx + (T)P.gx) + vsx;
     dudt[2 * i + 1] = (apy + (T)P.gy) + vsy;
@@ -761,6 +853,56 @@ void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, 
       t[6] = p_i;
       t[7] = 0;
     }
+  }
+}
+
+// mDBCDensity (v2/modules/mDBC.py:50-136); rows of `c` are ghosts.
+template <class T>
+void mdbc(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, const T* bpos, T rho0, T support,
+          T* rho_b, T* shep_out) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T nom = 0, vol = 0, gsx = 0, gsy = 0, vgx = 0, vgy = 0, psx = 0, psy = 0, p00 = 0, p01 = 0, p10 = 0, p11 = 0;
+    const int cnt = c.off[i + 1] - c.off[i];
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T m_j = mass[j], rho_j = rho[j];
+      const T Vj = m_j / rho_j;
+      nom += m_j * g.w;
+      const T kw = Vj * g.w;
+      vol += T(1) * kw;
+      const T kx = Vj * g.gx, ky = Vj * g.gy;
+      gsx += rho_j * kx;
+      gsy += rho_j * ky;
+      vgx += T(1) * kx;
+      vgy += T(1) * ky;
+      const T xx = ((-g.ex) * g.q) * support, xy = ((-g.ey) * g.q) * support;
+      psx += xx * kw;
+      psy += xy * kw;
+      p00 += xx * kx;
+      p01 += xx * ky;
+      p10 += xy * kx;
+      p11 += xy * ky;
+    }
+    const T shep = nom / (vol + (T)1e-7);
+    T out = rho0;
+    if (cnt > 0) out = shep;
+    if (cnt > 5) {
+      const double A[9] = {(double)vol, (double)psx, (double)psy, (double)vgx, (double)p00,
+                           (double)p01, (double)vgy, (double)p10, (double)p11};
+      double Ai[9];
+      dsph_pm::pinv3x3(A, 3.0 * 1.1920928955078125e-07, Ai);
+      const double b0 = (double)nom, b1 = (double)gsx, b2 = (double)gsy;
+      const T c0 = (T)(Ai[0] * b0 + Ai[1] * b1 + Ai[2] * b2);
+      const T c1 = (T)(Ai[3] * b0 + Ai[4] * b1 + Ai[5] * b2);
+      const T c2 = (T)(Ai[6] * b0 + Ai[7] * b1 + Ai[8] * b2);
+      const T rx = bpos[2 * i] - c.qpos[2 * i], ry = bpos[2 * i + 1] - c.qpos[2 * i + 1];
+      out = c0 + (rx * c1 + ry * c2);
+    }
+    rho_b[i] = out;
+    if (shep_out) shep_out[i] = shep;
   }
 }
 
@@ -870,46 +1012,88 @@ int orc_integrate_sie_f32(float* pos, float* vel, const float* a0, const float* 
   return 0;
 }
 
-// delta-SPH exports (same-set neighbor lists: rpos = qpos, rh = qh).
-int orc_deltasph_normalization_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
-                                   const float* qpos, const float* qh, float h0, int32_t mode, const float* mass,
-                                   const float* rho, float* L, float* ev, float* grad_rho) {
-  const float* rpos = qpos;
-  const float* rh = qh;
-  dsph_normalization<float>(ORC_CSR(float), DomT<float>(*dom), mass, rho, L, ev, grad_rho);
-  return 0;
+// delta-SPH exports (same-set neighbor lists: rpos = qpos, rh = qh); the
+// optional wall set: CSR (b_off over the fluid rows, b_nbr wall indices),
+// wall positions, mass, density, velocity, viscosity (coefficient, eps,
+// scheme, pi-switch) and the wall support.
+struct orc_boundary {
+  const int32_t* off;
+  const int32_t* nbr;
+  const void* pos;
+  const void* mass;
+  const void* rho;
+  const void* vel;
+  double visc_coeff, eps, h0;
+  int32_t viscosity, pi_switch;
+};
+
+}  // extern "C"
+
+namespace {
+template <class T>
+BSet<T> make_bset(const orc_boundary* b, int64_t nq, const T* qpos, const T* qh, T h0q) {
+  BSet<T> B;
+  B.c = make_csr<T>(nq, b->off, b->nbr, qpos, static_cast<const T*>(b->pos), qh, nullptr, (T)b->h0, 0);
+  (void)h0q;
+  B.mass = static_cast<const T*>(b->mass);
+  B.rho = static_cast<const T*>(b->rho);
+  B.vel = static_cast<const T*>(b->vel);
+  B.visc_coeff = b->visc_coeff;
+  B.eps = b->eps;
+  B.viscosity = b->viscosity;
+  B.pi_switch = b->pi_switch;
+  return B;
 }
-int orc_deltasph_normalization_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
-                                   const double* qpos, const double* qh, double h0, int32_t mode, const double* mass,
-                                   const double* rho, double* L, double* ev, double* grad_rho) {
-  const double* rpos = qpos;
-  const double* rh = qh;
-  dsph_normalization<double>(ORC_CSR(double), DomT<double>(*dom), mass, rho, L, ev, grad_rho);
-  return 0;
-}
+}  // namespace
+
+extern "C" {
+
+#define ORC_NORM(T, SUF)                                                                                         \
+  int orc_deltasph_normalization_##SUF(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,  \
+                                       const T* qpos, const T* qh, T h0, int32_t mode, const T* mass, const T* rho, \
+                                       T* L, T* ev, T* grad_rho, const orc_boundary* bnd) {                     \
+    const T* rpos = qpos;                                                                                        \
+    const T* rh = qh;                                                                                            \
+    BSet<T> B;                                                                                                   \
+    if (bnd) B = make_bset<T>(bnd, nq, qpos, qh, h0);                                                            \
+    dsph_normalization<T>(ORC_CSR(T), DomT<T>(*dom), mass, rho, L, ev, grad_rho, bnd ? &B : nullptr);           \
+    return 0;                                                                                                    \
+  }
+ORC_NORM(float, f32)
+ORC_NORM(double, f64)
 
 // prm: rho0, cs, delta, kernel_scale, visc_coeff, eps, kappa, gx, gy;
 // iprm: viscosity, pi_switch, eos, diffusion.
-int orc_deltasph_rhs_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
-                         const float* qh, float h0, int32_t mode, const double* prm, const int32_t* iprm,
-                         const float* vel, const float* mass, const float* rho, const float* grad_rho, float* dudt,
-                         float* drhodt, float* terms) {
-  const float* rpos = qpos;
-  const float* rh = qh;
-  DeltaP P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8], iprm[0], iprm[1], iprm[2], iprm[3]};
-  dsph_rhs<float>(ORC_CSR(float), DomT<float>(*dom), P, vel, mass, rho, grad_rho, dudt, drhodt, terms);
-  return 0;
-}
-int orc_deltasph_rhs_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
-                         const double* qpos, const double* qh, double h0, int32_t mode, const double* prm,
-                         const int32_t* iprm, const double* vel, const double* mass, const double* rho,
-                         const double* grad_rho, double* dudt, double* drhodt, double* terms) {
-  const double* rpos = qpos;
-  const double* rh = qh;
-  DeltaP P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8], iprm[0], iprm[1], iprm[2], iprm[3]};
-  dsph_rhs<double>(ORC_CSR(double), DomT<double>(*dom), P, vel, mass, rho, grad_rho, dudt, drhodt, terms);
-  return 0;
-}
+#define ORC_RHS(T, SUF)                                                                                          \
+  int orc_deltasph_rhs_##SUF(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,          \
+                             const T* qpos, const T* qh, T h0, int32_t mode, const double* prm, const int32_t* iprm, \
+                             const T* vel, const T* mass, const T* rho, const T* grad_rho, T* dudt, T* drhodt,    \
+                             T* terms, const orc_boundary* bnd) {                                                \
+    const T* rpos = qpos;                                                                                        \
+    const T* rh = qh;                                                                                            \
+    DeltaP P{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8], iprm[0], iprm[1], iprm[2],   \
+             iprm[3]};                                                                                           \
+    BSet<T> B;                                                                                                   \
+    if (bnd) B = make_bset<T>(bnd, nq, qpos, qh, h0);                                                            \
+    dsph_rhs<T>(ORC_CSR(T), DomT<T>(*dom), P, vel, mass, rho, grad_rho, dudt, drhodt, terms, bnd ? &B : nullptr); \
+    return 0;                                                                                                    \
+  }
+ORC_RHS(float, f32)
+ORC_RHS(double, f64)
+
+// mDBC: rows of (off, nbr) are the ghosts at qpos; entries fluid indices at rpos.
+#define ORC_MDBC(T, SUF)                                                                                         \
+  int orc_mdbc_##SUF(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const T* qpos,   \
+                     const T* rpos, T h0, const T* mass, const T* rho, const T* bpos, T rho0, T support, T* rho_b, \
+                     T* shep) {                                                                                  \
+    const T* qh = nullptr;                                                                                       \
+    const T* rh = nullptr;                                                                                       \
+    int32_t mode = 0;                                                                                            \
+    mdbc<T>(ORC_CSR(T), DomT<T>(*dom), mass, rho, bpos, rho0, support, rho_b, shep);                              \
+    return 0;                                                                                                    \
+  }
+ORC_MDBC(float, f32)
+ORC_MDBC(double, f64)
 
 // The 2x2 pseudo-inverse on its own ([n,4] row-major in, L [n,4], ev [n,2]).
 int orc_pinv2x2_f32(const float* M, int64_t n, float* L, float* ev) {

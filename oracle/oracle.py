@@ -355,22 +355,57 @@ class DeltaParams:
         return prm, iprm
 
 
-def deltasph_normalization(dom, off, nbr, wpos, mass, rho, h0, dtype=np.float32):
+class OrcBoundary(C.Structure):
+    _fields_ = [("off", C.c_void_p), ("nbr", C.c_void_p), ("pos", C.c_void_p), ("mass", C.c_void_p),
+                ("rho", C.c_void_p), ("vel", C.c_void_p), ("visc_coeff", C.c_double), ("eps", C.c_double),
+                ("h0", C.c_double), ("viscosity", C.c_int32), ("pi_switch", C.c_int32)]
+
+
+@dataclass
+class WallSet:
+    """Wall particles seen by the fluid rows: CSR (off over the fluid rows,
+    nbr wall indices), wall positions, mass, density, velocity, support and
+    the wall viscosity (boundaryDiffusion)."""
+    off: np.ndarray
+    nbr: np.ndarray
+    pos: np.ndarray
+    mass: np.ndarray
+    rho: np.ndarray
+    vel: np.ndarray
+    h: float
+    viscosity: int = 0
+    visc_coeff: float = 0.01
+    eps: float = 1e-6
+    pi_switch: bool = False
+
+    def struct(self, dtype):
+        cv = f32 if dtype == np.float32 else f64
+        self._keep = [np.ascontiguousarray(self.off, np.int32), np.ascontiguousarray(self.nbr, np.int32),
+                      cv(self.pos), cv(self.mass), cv(self.rho), cv(self.vel)]
+        k = self._keep
+        return OrcBoundary(_p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), _p(k[5]), self.visc_coeff, self.eps,
+                           self.h, self.viscosity, 1 if self.pi_switch else 0)
+
+
+def deltasph_normalization(dom, off, nbr, wpos, mass, rho, h0, dtype=np.float32, wall: WallSet | None = None):
     n = wpos.shape[0]
     cv = f32 if dtype == np.float32 else f64
     wpos, mass, rho = cv(wpos), cv(mass), cv(rho)
+    bs = wall.struct(dtype) if wall is not None else None
     L = np.zeros((n, 4), dtype)
     ev = np.zeros((n, 2), dtype)
     g = np.zeros((n, 2), dtype)
     fn = lib().orc_deltasph_normalization_f32 if dtype == np.float32 else lib().orc_deltasph_normalization_f64
     hc = C.c_float if dtype == np.float32 else C.c_double
     fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(wpos), None, hc(h0), C.c_int32(0), _p(mass), _p(rho), _p(L),
-       _p(ev), _p(g))
+       _p(ev), _p(g), C.byref(bs) if bs is not None else None)
     return L, ev, g
 
 
-def deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad_rho, prm: DeltaParams, h0, dtype=np.float32):
+def deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad_rho, prm: DeltaParams, h0, dtype=np.float32,
+                 wall: WallSet | None = None):
     n = wpos.shape[0]
+    bs = wall.struct(dtype) if wall is not None else None
     cv = f32 if dtype == np.float32 else f64
     wpos, vel, mass, rho, grad_rho = cv(wpos), cv(vel), cv(mass), cv(rho), cv(grad_rho)
     dudt = np.zeros((n, 2), dtype)
@@ -380,8 +415,30 @@ def deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad_rho, prm: DeltaParams
     fn = lib().orc_deltasph_rhs_f32 if dtype == np.float32 else lib().orc_deltasph_rhs_f64
     hc = C.c_float if dtype == np.float32 else C.c_double
     fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(wpos), None, hc(h0), C.c_int32(0), _p(pa), _p(ia), _p(vel),
-       _p(mass), _p(rho), _p(grad_rho), _p(dudt), _p(drhodt), _p(terms))
+       _p(mass), _p(rho), _p(grad_rho), _p(dudt), _p(drhodt), _p(terms), C.byref(bs) if bs is not None else None)
     return dudt, drhodt, terms
+
+
+def boundary_ghosts(bpos, dist, normals, h):
+    """x_g = x_b + (d + max(d, -h)) n (v2/modules/mDBC.py:9-17), float32."""
+    bpos, dist, normals = f32(bpos), f32(dist), f32(normals)
+    hf = np.float32(h)
+    md = dist + np.maximum(dist, -hf)
+    return (bpos + md[:, None] * normals).astype(np.float32)
+
+
+def mdbc(dom, off, nbr, ghost_pos, fluid_pos, h0, mass, rho, bpos, rho0, support, dtype=np.float32):
+    """mDBCDensity (v2/modules/mDBC.py:50-136): (wall density, Shepard)."""
+    cv = f32 if dtype == np.float32 else f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    gp, fp, mass, rho, bpos = cv(ghost_pos), cv(fluid_pos), cv(mass), cv(rho), cv(bpos)
+    nb = gp.shape[0]
+    out = np.zeros(nb, dtype)
+    shep = np.zeros(nb, dtype)
+    fn = lib().orc_mdbc_f32 if dtype == np.float32 else lib().orc_mdbc_f64
+    fn(C.byref(dom), C.c_int64(nb), _p(off), _p(nbr), _p(gp), _p(fp), hc(h0), _p(mass), _p(rho), _p(bpos), hc(rho0),
+       hc(support), _p(out), _p(shep))
+    return out, shep
 
 
 def neighbors_state_order(dom, pos, h):
@@ -396,12 +453,58 @@ def neighbors_state_order(dom, pos, h):
     return off, g.sort_index[nbr_s].astype(np.int32), wpos
 
 
-def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float32):
-    """One right-hand side: (dudt, drhodt, extras); dxdt is the velocity."""
-    off, nbr, wpos = neighbors_state_order(dom, pos, h)
-    L, ev, grad = deltasph_normalization(dom, off, nbr, wpos, mass, rho, h, dtype)
-    dudt, drhodt, terms = deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad, prm, h, dtype)
-    return dudt, drhodt, dict(L=L, ev=ev, grad_rho=grad, terms=terms, off=off, nbr=nbr)
+@dataclass
+class WallState:
+    """Static wall particles in their own cell order, with ghosts."""
+    pos: np.ndarray
+    dist: np.ndarray
+    normal: np.ndarray
+    mass: np.ndarray
+    vel: np.ndarray
+    ghost: np.ndarray
+    h: float
+    viscosity: int = 0
+    visc_coeff: float = 0.01
+    eps: float = 1e-6
+    pi_switch: bool = False
+
+
+def make_walls(dom, pos, dist, normal, area, rho0, h, viscosity=0, visc_coeff=0.01, eps=1e-6, pi_switch=False):
+    g = grid_build(dom, pos, h)
+    si = g.sort_index
+    p = g.sorted_pos.copy()
+    d, nrm = f32(dist)[si], f32(normal)[si]
+    n = p.shape[0]
+    mass = (np.broadcast_to(f32(area), (n,))[si] * np.float32(rho0)).astype(np.float32)
+    return WallState(p, d, nrm, mass, np.zeros((n, 2), np.float32), boundary_ghosts(p, d, nrm, h), h, viscosity,
+                     visc_coeff, eps, pi_switch)
+
+
+def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float32, walls: WallState | None = None):
+    """One right-hand side: (dudt, drhodt, extras); dxdt is the velocity.
+    With `walls`: the fluid->wall list, the ghost->fluid list, mDBC wall
+    densities and the wall terms (v2/simulationSchemes/deltaPlus.py:186-329)."""
+    g = grid_build(dom, pos, h)
+    wpos = np.empty_like(g.sorted_pos)
+    wpos[g.sort_index] = g.sorted_pos
+    _, off, nbr_s = neighbors_cell(dom, g, wpos, h0=h)
+    nbr = g.sort_index[nbr_s].astype(np.int32)
+    wall = None
+    extras = {}
+    if walls is not None:
+        gw = grid_build(dom, walls.pos, h)
+        _, off_b, nbr_bs = neighbors_cell(dom, gw, wpos, h0=h)
+        nbr_b = gw.sort_index[nbr_bs].astype(np.int32)
+        _, off_g, nbr_gs = neighbors_cell(dom, g, walls.ghost, h0=h)
+        nbr_g = g.sort_index[nbr_gs].astype(np.int32)
+        rb, shep = mdbc(dom, off_g, nbr_g, walls.ghost, wpos, h, mass, rho, walls.pos, prm.rho0, h, dtype)
+        wall = WallSet(off_b, nbr_b, walls.pos, walls.mass, rb, walls.vel, h, walls.viscosity, walls.visc_coeff,
+                       walls.eps, walls.pi_switch)
+        extras.update(wall_rho=rb, shepard=shep)
+    L, ev, grad = deltasph_normalization(dom, off, nbr, wpos, mass, rho, h, dtype, wall=wall)
+    dudt, drhodt, terms = deltasph_rhs(dom, off, nbr, wpos, vel, mass, rho, grad, prm, h, dtype, wall=wall)
+    extras.update(L=L, ev=ev, grad_rho=grad, terms=terms, off=off, nbr=nbr)
+    return dudt, drhodt, extras
 
 
 @dataclass
@@ -450,14 +553,17 @@ class DeltaState:
     uid: np.ndarray
 
 
-def rk4_step(st: DeltaState, dom, h, prm: DeltaParams, dt: float, reorder=True, dtype=np.float32, rhs=None):
+def rk4_step(st: DeltaState, dom, h, prm: DeltaParams, dt: float, reorder=True, dtype=np.float32, rhs=None,
+             walls: WallState | None = None):
     """One RK4 step of the delta-SPH scheme (integration.py:231-307): the
     state is first sorted into cell order (as the GPU solver keeps it), then
     k1..k4 with stage states x0 + k dt/2, x0 + k dt/2, x0 + k dt, and
     state += (k1 + 2 k2 + 2 k3 + k4)/6 dt for x (k = velocity), u, rho.
     `rhs(dom, x, u, rho, mass, h, prm, dtype) -> (dudt, drhodt, extras)`
     defaults to :func:`deltaplus_rhs`.  Returns (state, averaged dudt)."""
-    rhs = rhs or deltaplus_rhs
+    if rhs is None:
+        def rhs(dom_, x_, u_, r_, m_, h_, prm_, dtype_):
+            return deltaplus_rhs(dom_, x_, u_, r_, m_, h_, prm_, dtype_, walls=walls)
     F = np.float32 if dtype == np.float32 else np.float64
     if reorder:
         si = grid_build(dom, st.pos, h).sort_index

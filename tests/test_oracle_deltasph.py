@@ -240,3 +240,76 @@ def test_viscosity_is_nu_laplacian_on_divergence_free_field(scheme):
     ref = -2.0 * math.pi ** 2 * nu * vel
     assert np.abs(t[:, 2:4] - ref).max() < 0.06 * np.abs(ref).max()
     assert np.abs(t[:, 4]).max() < 1e-5
+
+
+def _half_plane(ns=48, layers=3):
+    """Fluid lattice on [0,1] x [0,1] above a wall of `layers` rows below y=0
+    (normals pointing into the wall, distances negative)."""
+    dx = 1.0 / ns
+    xs = (np.arange(ns) + 0.5) * dx
+    fx, fy = np.meshgrid(xs, xs, indexing="xy")
+    fluid = np.stack([fx.ravel(), fy.ravel()], 1).astype(np.float32)
+    wy = -(np.arange(layers) + 0.5) * dx
+    bx, by = np.meshgrid(xs, wy, indexing="xy")
+    wall = np.stack([bx.ravel(), by.ravel()], 1).astype(np.float32)
+    normals = np.tile(np.array([[0.0, -1.0]], np.float32), (wall.shape[0], 1))
+    dist = wall[:, 1].copy()
+    h = sampling.volume_to_support(dx * dx, 50.0)
+    return fluid, wall, normals, dist, dx, h
+
+
+def test_mdbc_ghosts_mirror_and_linear_extrapolation():
+    """Ghosts mirror the wall particles into the fluid; the first-order MLS
+    of mDBC reproduces a linear density field exactly at the wall particle;
+    ghosts with 1..5 neighbors take the Shepard density, none -> rho0."""
+    fluid, wall, normals, dist, dx, h = _half_plane()
+    ghosts = orc.boundary_ghosts(wall, dist, normals, h)
+    np.testing.assert_allclose(ghosts[:, 1], -wall[:, 1], atol=1e-7)
+    np.testing.assert_array_equal(ghosts[:, 0], wall[:, 0])
+    dom = orc.make_domain((-0.2, -0.2), (1.2, 1.2), (False, False))
+    _, off, nbr = orc.neighbors_brute(dom, ghosts, fluid, h0=h)
+    a, b = 1.0, np.array([0.05, -0.2])
+    rho = (a + fluid.astype(np.float64) @ b)
+    mass = np.full(fluid.shape[0], dx * dx)
+    rb, shep = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho, wall, 1.0, h, np.float64)
+    cnt = np.diff(off)
+    inner = (wall[:, 0] > h) & (wall[:, 0] < 1 - h) & (cnt > 5)
+    assert inner.sum() > 50
+    # exact up to the 1e-7 regularisation of xhat = d / (r + 1e-7)
+    np.testing.assert_allclose(rb[inner], a + wall[inner].astype(np.float64) @ b, rtol=0, atol=5e-7)
+    # Shepard of a uniform field is the field; few / no neighbors
+    rho1 = np.full(fluid.shape[0], 1.3)
+    rb1, shep1 = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho1, wall, 1.0, h, np.float64)
+    np.testing.assert_allclose(shep1[cnt > 0], 1.3, rtol=1e-6)
+    few = np.array([3], np.int32)
+    off_f = np.array([0, 3, 3], np.int32)
+    nbr_f = nbr[off[few[0]]:off[few[0]] + 3].astype(np.int32)
+    rbf, shf = orc.mdbc(dom, off_f, nbr_f, ghosts[[3, 4]], fluid, h, mass, rho1, wall[[3, 4]], 1.0, h, np.float64)
+    assert rbf[0] == pytest.approx(shf[0]) and rbf[1] == 1.0
+
+
+def test_wall_terms_hydrostatic_balance_sign():
+    """A fluid layer at rest on a wall with a hydrostatic density column:
+    the wall pressure pushes the bottom rows up (a_y > 0 from the wall
+    pairs), continuity is zero at rest, the wall viscosity is zero."""
+    fluid, wall, normals, dist, dx, h = _half_plane(32)
+    n, nb = fluid.shape[0], wall.shape[0]
+    dom = orc.make_domain((-0.2, -0.2), (1.2, 1.2), (False, False))
+    g, cs = 1.0, 10.0
+    rho = (1.0 + g * (1.0 - fluid[:, 1].astype(np.float64)) / cs ** 2)
+    mass = np.full(n, dx * dx)
+    _, off_b, nbr_b = orc.neighbors_brute(dom, fluid, wall, h0=h)
+    ghosts = orc.boundary_ghosts(wall, dist, normals, h)
+    _, off_g, nbr_g = orc.neighbors_brute(dom, ghosts, fluid, h0=h)
+    rb, _ = orc.mdbc(dom, off_g, nbr_g, ghosts, fluid, h, mass, rho, wall, 1.0, h, np.float64)
+    wall_set = orc.WallSet(off_b, nbr_b, wall, np.full(nb, dx * dx), rb, np.zeros((nb, 2)), h, viscosity=0)
+    prm = orc.DeltaParams(rho0=1.0, cs=cs, delta=0.1, kernel_scale=h / (2 * dx), viscosity=0, visc_coeff=0.01,
+                          gravity=(0.0, -g))
+    _, off, nbr = orc.neighbors_brute(dom, fluid, fluid, h0=h)
+    L, _, grad = orc.deltasph_normalization(dom, off, nbr, fluid, mass, rho, h, np.float64, wall=wall_set)
+    _, _, t_w = orc.deltasph_rhs(dom, off, nbr, fluid, np.zeros((n, 2)), mass, rho, grad, prm, h, np.float64,
+                                 wall=wall_set)
+    _, _, t_0 = orc.deltasph_rhs(dom, off, nbr, fluid, np.zeros((n, 2)), mass, rho, grad, prm, h, np.float64)
+    bottom = (fluid[:, 1] < 1.5 * dx) & (fluid[:, 0] > h) & (fluid[:, 0] < 1 - h)
+    assert np.all(t_w[bottom, 1] - t_0[bottom, 1] > 0)
+    assert np.all(t_w[:, 4] == 0) and np.all(t_w[:, 2:4] == 0)
