@@ -175,6 +175,8 @@ SIGNATURES = {
     "dsph_max_norm_workspace": (SZ, []),
     "dsph_max_norm2": (C.c_int, [V, I64, P(C.c_float), V, SZ, V]),
     "dsph_sum_squares": (C.c_int, [V, I64, P(C.c_double), V, SZ, V]),
+    "dsph_deltasph_vjp_workspace": (SZ, [I64]),
+    "dsph_deltasph_rhs_vjp": (C.c_int, [P(Pairs), P(DeltaSPHParams), V, V, V, V, V, V, V, V, V, V, V, SZ, V]),
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_scatter_add": (C.c_int, [V, I32, I64, I32, V, V, V, V]),

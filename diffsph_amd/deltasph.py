@@ -191,6 +191,8 @@ class DeltaSPHSolver:
         self.dudt = None            # RK4-averaged derivatives of the last step
         self.drhodt = None
         self.last_pairs = 0
+        self.last_sort = None
+        self.on_sorted = None       # optional callback(solver) after the per-step sort
         n = self.n
         self._grad = torch.empty(n, 2, **f)
         self._k = [(torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f)) for _ in range(4)]
@@ -255,6 +257,7 @@ class DeltaSPHSolver:
     def _sort(self):
         grid = CellGrid.build(self.pos, None, self.domain, self.cfg.support, h_uniform=self.cfg.support)
         si = grid.sort_index
+        self.last_sort = si
         st = _lib.stream_handle(self.pos.device)
 
         def perm(t):
@@ -274,6 +277,8 @@ class DeltaSPHSolver:
         L_ = lib()
         st = _lib.stream_handle(self.pos.device)
         self._sort()
+        if self.on_sorted is not None:
+            self.on_sorted(self)
         dt = float(np.float32(self.dt))
         half = float(np.float32(dt) * np.float32(0.5))
         x0, u0, r0 = self.pos, self.vel, self.rho

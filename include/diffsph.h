@@ -515,6 +515,31 @@ int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* workspac
                      size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Adjoint of the delta-SPH right-hand side                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Vector-Jacobian product of one fluid-only delta-SPH right-hand side
+ * (dsph_deltasph_rhs without walls) with respect to positions, velocities
+ * and densities: given the cotangents of dx/dt (= u, may be NULL), dudt
+ * [n,2] and drhodt [n], writes dl/dx [n,2], dl/du [n,2], dl/drho [n].  It
+ * differentiates every pair term, including the normalization matrix
+ * through dL = -L dM L and the renormalized density gradient; the neighbor
+ * set, the time step and the masses are constants.  The reference obtains
+ * these gradients by torch autograd through its modules
+ * (v2/simulationSchemes/deltaPlus.py:176-422, v2/modules/integration.py:
+ * 231-307); it has no hand-written adjoint.  Requires one uniform support
+ * in scatter mode (symmetric neighbor sets): the kernels gather both sides
+ * of every pair, no atomics.  grad_rho is the forward's
+ * dsph_deltasph_normalization output at the same state. */
+size_t dsph_deltasph_vjp_workspace(int64_t n);
+int dsph_deltasph_rhs_vjp(const dsph_pairs* pairs, const dsph_deltasph_params* prm,
+                          const float* vel, const float* mass, const float* rho,
+                          const float* grad_rho, const float* cot_dxdt,
+                          const float* cot_dudt, const float* cot_drhodt,
+                          float* grad_pos, float* grad_vel, float* grad_rho_out,
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Drop-in ops with the reference's per-pair inputs                         */
 /* ---------------------------------------------------------------------- */
 

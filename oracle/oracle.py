@@ -260,7 +260,7 @@ def dfsph_step(st: DfsphState, dom: Domain, h: float, dt: float, rho0: float = 1
     wpos = np.empty_like(g.sorted_pos)
     wpos[g.sort_index] = g.sorted_pos
     cv = f32 if dtype == np.float32 else f64
-    wpos = cv(wpos)
+    wpos = cv(wpos) if dtype == np.float32 else wrap_positions(dom, st.pos, np.float64)
     rho, vol = density(dom, off, nbr, wpos, wpos, st.mass, st.mass, h0=h, dtype=dtype)
     ad, an, ds, vs = dfsph_prepare(dom, off, nbr, wpos, st.vel, st.areas, vol, rho, dt, rho0, visc, h, dtype)
     if visc is None:
@@ -441,6 +441,19 @@ def mdbc(dom, off, nbr, ghost_pos, fluid_pos, h0, mass, rho, bpos, rho0, support
     return out, shep
 
 
+def wrap_positions(dom, pos, dtype=np.float64):
+    """getPeriodicPosition (v2/modules/neighborhood.py:407-430) in `dtype`:
+    remainder(x - min, L) + min on periodic axes (numpy's remainder has
+    torch.remainder's sign convention)."""
+    p = np.array(pos, dtype=dtype, copy=True)
+    for k in range(2):
+        if dom.periodic[k]:
+            mn = dtype(dom.min[k])
+            L = dtype(dom.max[k]) - mn
+            p[:, k] = np.remainder(p[:, k] - mn, L) + mn
+    return p
+
+
 def neighbors_state_order(dom, pos, h):
     """Neighbor list with rows in the state's order and entries as state
     indices (cell-scan order within a row), on wrapped positions: the grid is
@@ -489,6 +502,10 @@ def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float3
     wpos[g.sort_index] = g.sorted_pos
     _, off, nbr_s = neighbors_cell(dom, g, wpos, h0=h)
     nbr = g.sort_index[nbr_s].astype(np.int32)
+    if dtype != np.float32:
+        # the float64 error reference wraps in float64 (the list itself keeps
+        # the float32 predicate)
+        wpos = wrap_positions(dom, pos, np.float64)
     wall = None
     extras = {}
     if walls is not None:

@@ -20,6 +20,12 @@ modules' outputs in float32:
   computeTimestep
   DFSPH: computeAlpha, computeSourceTerm (divergence and density),
          computePressureAcceleration + updatePressure (one iteration)
+
+and, in a second file, the reference's autograd gradient of the final
+kinetic energy with respect to the initial velocities and positions through
+a few RK4 steps of the Taylor-Green delta-SPH scheme (float64, fixed time
+steps), composing those modules per stage exactly as deltaPlus.py:176-350
+and integration.py:231-307 do for a periodic fluid without walls.
 """
 import importlib
 import os
@@ -31,6 +37,7 @@ import torch
 
 REF = "/root/reference/src"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_modules_16.npz")
+OUT_GRAD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_rollout_grad_16.npz")
 
 
 def load():
@@ -156,5 +163,86 @@ def main():
     print("wrote", OUT, {k: v.shape for k, v in np_out.items()})
 
 
+def rollout_grad(steps=4):
+    """Autograd gradient of E_k(T) = sum 0.5 m |u|^2 through `steps` RK4
+    steps, float64."""
+    M = load()
+    torch.set_default_dtype(torch.float64)
+    rng = np.random.default_rng(7)
+    ns = 16
+    dx = 2.0 / ns
+    xs = np.linspace(-1 + dx / 2, 1 - dx / 2, ns)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel()], 1)
+    pos = (pos + rng.uniform(-0.05 * dx, 0.05 * dx, pos.shape)).astype(np.float32).astype(np.float64)
+    n = pos.shape[0]
+    h = float(np.float32(np.sqrt(50.0 * dx * dx / np.pi)))
+    vel = np.stack([np.sin(np.pi * pos[:, 0]) * np.cos(np.pi * pos[:, 1]),
+                    -np.cos(np.pi * pos[:, 0]) * np.sin(np.pi * pos[:, 1])], 1).astype(np.float32).astype(np.float64)
+    mass = np.full(n, dx * dx)
+    ks = h / (2 * dx)
+    cfg = {"diffusion": {"delta": 0.1, "densityScheme": "deltaSPH", "velocityScheme": "deltaSPH_viscid",
+                         "nu": 0.01, "alpha": 0.01, "eps": 1e-6, "pi-switch": False},
+           "kernel": {"kernelScale": ks}, "fluid": {"rho0": 1.0, "cs": 10.0}, "domain": {"dim": 2},
+           "EOS": {"type": "isoThermal"}}
+    hv = torch.full((n,), h)
+    m = torch.tensor(mass)
+
+    def rhs(x, u, rho):
+        xw = torch.remainder(x + 1.0, 2.0) - 1.0
+        with torch.no_grad():
+            d = xw[:, None, :] - xw[None, :, :]
+            d = torch.stack([M["math"].mod(d[..., k], -1.0, 1.0) for k in range(2)], -1)
+            keep = torch.linalg.norm(d, dim=-1) / h <= 1.0
+            i, j = torch.nonzero(keep, as_tuple=True)
+        xij = xw[i] - xw[j]
+        xij = torch.stack([M["math"].mod(xij[:, k], -1.0, 1.0) for k in range(2)], -1)
+        rij = torch.linalg.norm(xij, dim=-1)
+        xij = xij / (rij + 1e-7).view(-1, 1)
+        rij = rij / hv[j]
+        hij = hv[j]
+        nd = {"indices": (i, j), "distances": rij, "vectors": xij, "supports": hij,
+              "kernels": M["W2"].kernel(rij, hij, 2), "gradients": M["W2"].kernelGradient(rij, xij, hij, 2)}
+        nn = torch.bincount(i, minlength=n)
+        st = {"positions": xw, "velocities": u, "masses": m, "densities": rho, "supports": hv, "numParticles": n,
+              "numNeighbors": nn}
+        Cm = M["normalizationMatrices"].computeCovarianceMatrices(st, st, nd, cfg)
+        Cm = torch.where((nn < 4)[:, None, None], torch.eye(2)[None], Cm)
+        st["L"], _ = M["math"].pinv2x2(Cm)
+        st["gradRho^L"] = M["densityDiffusion"].renormalizedDensityGradient(st, st, nd, cfg)
+        D = M["densityDiffusion"].computeDensityDeltaTerm(st, st, nd, cfg)
+        visc = M["viscosity"].computeViscosity(st, st, nd, cfg)
+        cont = M["momentumEquation"].computeMomentumEquation(st, st, nd, cfg)
+        st["pressures"] = M["pressureEOS"].computeEOS(st, cfg)
+        ap = M["pressureForce"].computePressureAccel(st, st, nd, cfg)
+        rhs.terms = dict(ap=ap, visc=visc, cont=cont, D=D, grad=st["gradRho^L"], L=st["L"])
+        return u, ap + visc, cont + D
+
+    x0 = torch.tensor(pos, requires_grad=True)
+    u0 = torch.tensor(vel, requires_grad=True)
+    r0 = torch.ones(n, requires_grad=True)
+    with torch.no_grad():
+        rhs(x0, u0, r0)
+    first = {f"rhs0_{k}": v.detach().numpy() for k, v in rhs.terms.items()}
+    dts = [0.01, 0.01, 0.008, 0.008, 0.006, 0.006, 0.006, 0.006][:steps]
+    x, u, r = x0, u0, r0
+    for dt in dts:
+        k1 = rhs(x, u, r)
+        k2 = rhs(x + dt / 2 * k1[0], u + dt / 2 * k1[1], r + dt / 2 * k1[2])
+        k3 = rhs(x + dt / 2 * k2[0], u + dt / 2 * k2[1], r + dt / 2 * k2[2])
+        k4 = rhs(x + dt * k3[0], u + dt * k3[1], r + dt * k3[2])
+        x = x + dt * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0]) / 6
+        u = u + dt * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1]) / 6
+        r = r + dt * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2]) / 6
+    E = 0.5 * (m * (u * u).sum(1)).sum()
+    E.backward()
+    np.savez(OUT_GRAD, pos=pos, vel=vel, mass=mass, h=h, dx=dx, dts=np.array(dts), E=E.item(),
+             grad_u0=u0.grad.numpy(), grad_x0=x0.grad.numpy(), grad_rho0=r0.grad.numpy(),
+             u_T=u.detach().numpy(), x_T=x.detach().numpy(), rho_T=r.detach().numpy(), **first)
+    print("wrote", OUT_GRAD, "E", E.item(), "|grad u0|", float(u0.grad.norm()))
+    torch.set_default_dtype(torch.float32)
+
+
 if __name__ == "__main__":
     main()
+    rollout_grad()

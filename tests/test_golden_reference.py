@@ -95,3 +95,38 @@ def test_golden_dfsph_alpha_source_and_iteration(g, solve):
     _close(p1, g[f"dfsph_{solve}_p1"])
     _close(res, g[f"dfsph_{solve}_residual"], tol=5e-5)
     _close(acc, g[f"dfsph_{solve}_accel"])
+
+
+def test_golden_rollout_forward_matches_oracle():
+    """The oracle's float64 RK4 delta-SPH steps reproduce the reference
+    composition's final state and kinetic energy (same fixed time steps)."""
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_rollout_grad_16.npz")))
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = g["pos"].shape[0]
+    prm = orc.DeltaParams(rho0=1.0, cs=10.0, delta=0.1, kernel_scale=float(g["h"]) / (2 * float(g["dx"])),
+                          viscosity=1, visc_coeff=0.01, eps=1e-6)
+    st = orc.DeltaState(g["pos"].copy(), g["vel"].copy(), np.ones(n), g["mass"].copy(), np.arange(n, dtype=np.int32))
+    for dt in g["dts"]:
+        st, _ = orc.rk4_step(st, dom, float(g["h"]), prm, float(dt), dtype=np.float64)
+    order = np.argsort(st.uid)
+    u = st.vel[order]
+    E = 0.5 * (g["mass"] * (u ** 2).sum(1)).sum()
+    assert E == pytest.approx(float(g["E"]), rel=1e-10)
+    np.testing.assert_allclose(u, g["u_T"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(st.rho[order], g["rho_T"], rtol=0, atol=1e-10)
+
+
+def test_golden_rhs_float64_initial_state():
+    """float64 right-hand side at the rollout's initial state (rho = rho0):
+    the oracle's normalization, viscosity and continuity equal the
+    reference modules' to float64 rounding."""
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_rollout_grad_16.npz")))
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = g["pos"].shape[0]
+    prm = orc.DeltaParams(rho0=1.0, cs=10.0, delta=0.1, kernel_scale=float(g["h"]) / (2 * float(g["dx"])),
+                          viscosity=1, visc_coeff=0.01, eps=1e-6)
+    _, _, ex = orc.deltaplus_rhs(dom, g["pos"], g["vel"], np.ones(n), g["mass"], float(g["h"]), prm, np.float64)
+    _close(ex["L"].reshape(-1, 2, 2), g["rhs0_L"], 1e-12)
+    _close(ex["terms"][:, 2:4], g["rhs0_visc"], 1e-12)
+    _close(ex["terms"][:, 4], g["rhs0_cont"], 1e-12)
+    assert np.all(g["rhs0_ap"] == 0) and np.all(ex["terms"][:, 0:2] == 0)
