@@ -63,11 +63,24 @@ inline BoundaryDev make_boundary(const dsph_boundary* b) {
   return B;
 }
 
+// The periodic box a particle lives in: one for a single domain, or one per
+// particle (its domain's) for a batch of independent domains.
+struct OneDomain {
+  Dom d;
+  __device__ __forceinline__ Dom operator()(int) const { return d; }
+};
+struct DomainPerParticle {
+  const Dom* doms;
+  const int32_t* dom_of;
+  __device__ __forceinline__ Dom operator()(int i) const { return doms[dom_of[i]]; }
+};
+
 // With a boundary: M_i = M_ff + M_bf ('fluidwBoundary', deltaPlus.py:241),
 // the identity test uses the fluid row length (the fluid numNeighbors), and
 // the wall term of the density gradient is the plain 'difference' gradient
 // sum_b V_b (rho_i - rho_b) gradW_ib (densityGradient, deltaPlus.py:262-263).
-__global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, BoundaryDev B,
+template <class DF>
+__global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, BoundaryDev B, DF domain,
                                                                const float* __restrict__ mass,
                                                                const float* __restrict__ rho,
                                                                float4* __restrict__ L_out,
@@ -75,6 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
                                                                float2* __restrict__ grad_rho) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
+  const Dom dm = domain(i);
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
   const RowRef rr = row_ref(P, i);
@@ -82,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
     const float Vj = mass[j] / rho[j];
     const float s = -(g.q * g.h);  // -(distances * supports)
     const float qx = s * g.ex, qy = s * g.ey;
@@ -127,7 +141,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
     const float Vj = mass[j] / rho[j];
     const float nx = Lm.l00 * g.gx + Lm.l01 * g.gy;
     const float ny = Lm.l10 * g.gx + Lm.l11 * g.gy;
@@ -181,9 +195,11 @@ __device__ __forceinline__ float eos_p(float rho, float rho0, float c) { return 
 // second sum (ftf + btf), the viscosity adds computeViscosity_Boundary
 // (boundaryViscosity.py:123-170 inviscid with V_b = m_b/(rho_i + rho_b),
 // :198-218 viscid with V_b = m_b/rho_b) with its own coefficients.
-__global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, RhsArgs A) {
+template <class DF>
+__global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, RhsArgs A, DF domain) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
+  const Dom dm = domain(i);
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
   const float2 ui = A.vel[i];
@@ -195,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, 
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
     const float2 uj = A.vel[j];
     const float rho_j = A.rho[j];
     const float m_j = A.mass[j];
@@ -517,7 +533,7 @@ extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_b
   PairsDev P = make_pairs(*pairs);
   BoundaryDev B = make_boundary(boundary);
   k_dsph_normalization<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, B, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
+      P, B, OneDomain{P.dom}, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
       reinterpret_cast<float2*>(grad_rho));
   return launch_status();
 }
@@ -561,7 +577,7 @@ extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* b
   A.diffusion = prm->diffusion;
   BoundaryDev B = make_boundary(boundary);
   if (B.on && B.viscosity == 1) B.visc_c_viscid = (float)(2.0 * 4.0 * (double)B.visc_coeff * (double)prm->rho0);
-  k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, B, A);
+  k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, B, A, OneDomain{P.dom});
   return launch_status();
 }
 
