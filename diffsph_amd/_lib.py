@@ -117,6 +117,36 @@ class RK4Stages(C.Structure):
     ]
 
 
+class TimestepParams(C.Structure):
+    _fields_ = [
+        ("support", C.c_float),
+        ("kernel_scale", C.c_float),
+        ("dt_visc", C.c_float),
+        ("dt_acoustic", C.c_float),
+        ("max_dt", C.c_float),
+        ("min_dt", C.c_float),
+        ("adaptive", C.c_int32),
+        ("viscosity_constraint", C.c_int32),
+        ("acceleration_constraint", C.c_int32),
+        ("acoustic_constraint", C.c_int32),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("ncell", C.c_int64),
+        ("ndom", C.c_int32),
+        ("width", C.c_int32),
+        ("block", C.c_int32),
+        ("support", C.c_float),
+        ("hsearch", C.c_float),
+        ("pad_", C.c_int32),
+        ("workspace", C.c_void_p),
+        ("workspace_bytes", C.c_size_t),
+    ]
+
+
 V = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -175,6 +205,15 @@ SIGNATURES = {
     "dsph_max_norm_workspace": (SZ, []),
     "dsph_max_norm2": (C.c_int, [V, I64, P(C.c_float), V, SZ, V]),
     "dsph_sum_squares": (C.c_int, [V, I64, P(C.c_double), V, SZ, V]),
+    "dsph_batch_workspace": (SZ, [P(Domain), P(I64), I32, F32, I32]),
+    "dsph_batch_init": (C.c_int, [P(Domain), P(I64), I32, F32, F32, I32, I32, V, SZ, P(Batch)]),
+    "dsph_batch_step": (C.c_int, [P(Batch), P(DeltaSPHParams), P(TimestepParams), V, V, V, V, V, V, V, I32, V]),
+    "dsph_batch_graph": (C.c_int, [P(Batch), P(DeltaSPHParams), P(TimestepParams), V, V, V, V, V, V, V, I32,
+                                   P(C.c_void_p)]),
+    "dsph_graph_launch": (C.c_int, [V, V]),
+    "dsph_graph_destroy": (C.c_int, [V]),
+    "dsph_batch_probe": (C.c_int, [P(Batch), V, P(I32), V]),
+    "dsph_batch_status": (C.c_int, [P(Batch), P(I32), I32, V]),
     "dsph_deltasph_vjp_workspace": (SZ, [I64]),
     "dsph_deltasph_rhs_vjp": (C.c_int, [P(Pairs), P(DeltaSPHParams), V, V, V, V, V, V, V, V, V, V, V, SZ, V]),
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),

@@ -20,6 +20,7 @@
 // per pair.  Every expression is written in the order of the reference's
 // tensor expressions, and the CPU oracle evaluates the same expressions.
 #include "common.h"
+#include "deltasph_impl.h"
 #include "pmath.h"
 
 namespace dsph {
@@ -169,24 +170,6 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   grad_rho[i] = make_float2(ax, ay);
 }
 
-struct RhsArgs {
-  const float2* vel;
-  const float* mass;
-  const float* rho;
-  const float2* grad_rho;
-  float2* dudt;
-  float* drhodt;
-  float* terms;  // optional [n, 8]: a_p.xy, visc.xy, continuity, diffusion, pressure, 0
-  float rho0;
-  float eos_c;     // cs^2 (isothermal) or kappa (Tait)
-  float delta, kscale, cs;
-  float visc_c;    // viscid: 2 (d+2) nu rho0; inviscid: alpha
-  float eps;
-  float gx, gy;
-  int visc_scheme;  // -1 none, 0 inviscid, 1 viscid
-  int pi_switch;
-  int diffusion;
-};
 
 __device__ __forceinline__ float eos_p(float rho, float rho0, float c) { return c * (rho - rho0); }
 
@@ -538,22 +521,13 @@ extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_b
   return launch_status();
 }
 
-extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* boundary,
-                                 const dsph_deltasph_params* prm, const float* vel, const float* mass,
-                                 const float* rho, const float* grad_rho, float* dudt, float* drhodt, float* terms,
-                                 void* stream) {
-  int s = check_same_set(pairs);
-  if (s != DSPH_OK) return s;
-  s = check_boundary(boundary, pairs);
-  if (s != DSPH_OK) return s;
-  if (!prm) return DSPH_ERR_ARG;
-  if (pairs->nq == 0) return DSPH_OK;
-  if (!vel || !mass || !rho || !dudt || !drhodt) return DSPH_ERR_ARG;
+int dsph::make_rhs_args(const dsph_deltasph_params* prm, const float* vel, const float* mass, const float* rho,
+                        const float* grad_rho, float* dudt, float* drhodt, float* terms, RhsArgs* out) {
+  if (!prm || !out) return DSPH_ERR_ARG;
   if (prm->diffusion && !grad_rho) return DSPH_ERR_ARG;
   if (prm->viscosity < -1 || prm->viscosity > 1 || prm->eos < 0 || prm->eos > 1) return DSPH_ERR_ARG;
   if (prm->diffusion && !(prm->kernel_scale > 0.0f)) return DSPH_ERR_ARG;
-  PairsDev P = make_pairs(*pairs);
-  RhsArgs A;
+  RhsArgs& A = *out;
   A.vel = reinterpret_cast<const float2*>(vel);
   A.mass = mass;
   A.rho = rho;
@@ -575,6 +549,43 @@ extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* b
   A.visc_scheme = prm->viscosity;
   A.pi_switch = prm->pi_switch;
   A.diffusion = prm->diffusion;
+  return DSPH_OK;
+}
+
+int dsph::launch_normalization_per_particle(const PairsDev& P, const Dom* doms, const int32_t* dom_of,
+                                            const float* mass, const float* rho, float2* grad_rho, int block,
+                                            hipStream_t st) {
+  if (block != 64 && block != 128 && block != 256) return DSPH_ERR_ARG;
+  if (P.nq == 0) return DSPH_OK;
+  k_dsph_normalization<<<grid_for(P.nq, block), block, 0, st>>>(P, make_boundary(nullptr),
+                                                                DomainPerParticle{doms, dom_of}, mass, rho,
+                                                                nullptr, nullptr, grad_rho);
+  return launch_status();
+}
+
+int dsph::launch_rhs_per_particle(const PairsDev& P, const RhsArgs& A, const Dom* doms, const int32_t* dom_of,
+                                  int block, hipStream_t st) {
+  if (block != 64 && block != 128 && block != 256) return DSPH_ERR_ARG;
+  if (P.nq == 0) return DSPH_OK;
+  k_dsph_rhs<<<grid_for(P.nq, block), block, 0, st>>>(P, make_boundary(nullptr), A, DomainPerParticle{doms, dom_of});
+  return launch_status();
+}
+
+extern "C" int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                 const dsph_deltasph_params* prm, const float* vel, const float* mass,
+                                 const float* rho, const float* grad_rho, float* dudt, float* drhodt, float* terms,
+                                 void* stream) {
+  int s = check_same_set(pairs);
+  if (s != DSPH_OK) return s;
+  s = check_boundary(boundary, pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vel || !mass || !rho || !dudt || !drhodt) return DSPH_ERR_ARG;
+  RhsArgs A;
+  s = make_rhs_args(prm, vel, mass, rho, grad_rho, dudt, drhodt, terms, &A);
+  if (s != DSPH_OK) return s;
+  PairsDev P = make_pairs(*pairs);
   BoundaryDev B = make_boundary(boundary);
   if (B.on && B.viscosity == 1) B.visc_c_viscid = (float)(2.0 * 4.0 * (double)B.visc_coeff * (double)prm->rho0);
   k_dsph_rhs<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, B, A, OneDomain{P.dom});

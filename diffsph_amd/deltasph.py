@@ -72,6 +72,9 @@ class DeltaSPHConfig:
     # torch.linalg.norm(dudt[~isnan(dudt)]) on a flattened tensor
     # (timestep.py:22-25); "max": the largest per-particle |dudt|
     accel_norm: str = "global"
+    # cell side of the search grids = 1.001 * support * cell_scale (>= 1);
+    # changes the order of the neighbor entries, not the neighbor sets
+    cell_scale: float = 1.0
 
     @property
     def kernel_scale(self) -> float:
@@ -102,12 +105,21 @@ class DeltaSPHConfig:
         return self.alpha * self.cs * self.support / (2 * (2 + 2))
 
 
-def compute_timestep(cfg: DeltaSPHConfig, dt_prev: float, max_accel: float | None) -> float:
-    """computeTimestep (v2/modules/timestep.py:8-39) in float32 arithmetic."""
+def timestep_constants(cfg: DeltaSPHConfig):
+    """The state-independent bounds of computeTimestep (timestep.py:13-20)
+    in float32: (viscous dt, acoustic dt)."""
     F = np.float32
     h, ks = F(cfg.support), F(cfg.kernel_scale)
     dt_v = F(0.125) * h * h / F(cfg.timestep_nu()) / (ks * ks)
     dt_c = F(cfg.cfl) * h / F(cfg.cs) / ks
+    return dt_v, dt_c
+
+
+def compute_timestep(cfg: DeltaSPHConfig, dt_prev: float, max_accel: float | None) -> float:
+    """computeTimestep (v2/modules/timestep.py:8-39) in float32 arithmetic."""
+    F = np.float32
+    h, ks = F(cfg.support), F(cfg.kernel_scale)
+    dt_v, dt_c = timestep_constants(cfg)
     dt_a = F(cfg.max_dt) if max_accel is None else F(0.25) * np.sqrt(h / (F(max_accel) + F(1e-7))) / ks
     new = F(dt_prev)
     if cfg.viscosity_constraint:
@@ -204,7 +216,7 @@ class DeltaSPHSolver:
         mapped to state indices; returns (NeighborList, pairs descriptor,
         wrapped positions, grid)."""
         cfg = self.cfg
-        grid = CellGrid.build(pos, None, self.domain, cfg.support, h_uniform=cfg.support)
+        grid = CellGrid.build(pos, None, self.domain, cfg.support * cfg.cell_scale, h_uniform=cfg.support)
         wpos = wrap_positions(pos, self.domain)
         nl = search(grid, wpos, None, "scatter", qpos_wrapped=True)
         check(lib().dsph_remap_neighbors(ptr(nl.nbr), nl.nbr.numel(), ptr(grid.sort_index), self.n,
@@ -255,7 +267,8 @@ class DeltaSPHSolver:
         return RHS(dudt, drhodt, self._grad, nl.num_pairs)
 
     def _sort(self):
-        grid = CellGrid.build(self.pos, None, self.domain, self.cfg.support, h_uniform=self.cfg.support)
+        grid = CellGrid.build(self.pos, None, self.domain, self.cfg.support * self.cfg.cell_scale,
+                              h_uniform=self.cfg.support)
         si = grid.sort_index
         self.last_sort = si
         st = _lib.stream_handle(self.pos.device)

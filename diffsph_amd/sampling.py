@@ -76,6 +76,22 @@ def spectral_velocity(pos: np.ndarray, kmax: int = 8, seed: int = 0, umax: float
     return (u * (umax / mag)).astype(np.float32)
 
 
+def spectral_velocity_torch(pos, kmax: int = 8, seed: int = 0, umax: float = 1.0, period: float = 2.0):
+    """:func:`spectral_velocity` for a torch tensor of positions, evaluated
+    in float64 on the tensor's device (same modes, same formula); returns
+    float32 [n,2] on that device."""
+    import torch
+    ks, a, phi = spectral_modes(kmax, seed)
+    w = 2.0 * math.pi / period
+    x = pos.double()
+    kk = torch.from_numpy(ks).to(x.device)
+    arg = w * (x @ kk.T) + torch.from_numpy(phi).to(x.device)[None, :]
+    sn = torch.sin(arg) * torch.from_numpy(a * w).to(x.device)[None, :]
+    u = torch.stack([-(sn * kk[None, :, 1]).sum(1), (sn * kk[None, :, 0]).sum(1)], dim=1)
+    mag = torch.sqrt((u ** 2).sum(1)).max()
+    return (u * (umax / mag)).float().contiguous()
+
+
 def taylor_green(pos: np.ndarray, umag: float = 1.0) -> np.ndarray:
     x = pos[:, 0].astype(np.float64)
     y = pos[:, 1].astype(np.float64)

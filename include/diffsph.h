@@ -515,6 +515,102 @@ int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* workspac
                      size_t workspace_bytes, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Batched ensemble of independent periodic delta-SPH domains               */
+/* ---------------------------------------------------------------------- */
+
+/* The reference runs several domains only as a Python loop over batch ids
+ * (v2/neighborhoodFallback/neighborhood.py:121-183); each domain's step is
+ * deltaPlus.simulationStep (v2/simulationSchemes/deltaPlus.py:176-422)
+ * integrated with RK4 (v2/modules/integration.py:231-307) and
+ * computeTimestep (v2/modules/timestep.py:8-39).  Here every pass of the
+ * step runs over all domains in one launch: domain d owns the particles
+ * [first_d, first_d + count_d) of the concatenated state, its own periodic
+ * box, cell grid (global cell index = the domain's cell offset + local
+ * cell) and time step.  Each domain's result is bit-identical to stepping
+ * it alone with the single-domain entry points (same grid, same search
+ * order, same float32 expressions and the same float64 reduction order of
+ * the time-step norm). */
+
+/* computeTimestep constants: dt_visc = 0.125 h^2 / nu / kernel_scale^2 and
+ * dt_acoustic = cfl h / cs / kernel_scale, formed by the caller in float32
+ * in that order; the acceleration condition
+ * 0.25 sqrt(h / (|dudt| + 1e-7)) / kernel_scale (|dudt| = norm of all
+ * components of the domain's RK4-averaged dudt) is evaluated per domain on
+ * the device. */
+typedef struct dsph_timestep_params {
+  float support;
+  float kernel_scale;
+  float dt_visc;
+  float dt_acoustic;
+  float max_dt;
+  float min_dt;
+  int32_t adaptive;
+  int32_t viscosity_constraint;
+  int32_t acceleration_constraint;
+  int32_t acoustic_constraint;
+} dsph_timestep_params;
+
+/* A batch laid out in one device workspace by dsph_batch_init. */
+typedef struct dsph_batch {
+  int64_t n;          /* particles over all domains */
+  int64_t ncell;      /* cells over all domains */
+  int32_t ndom;
+  int32_t width;      /* neighbor entries per row (fixed-width sliced ELL) */
+  int32_t block;      /* workgroup size of the per-particle kernels: 64, 128 or 256 */
+  float support;      /* uniform h of every domain */
+  float hsearch;      /* search radius the cell grids are sized for (>= support) */
+  int32_t pad_;
+  void* workspace;
+  size_t workspace_bytes;
+} dsph_batch;
+
+/* Workspace for `ndom` periodic boxes with the given particle counts
+ * (host arrays), grids sized for `hsearch`, `width` entries per row.
+ * Returns 0 on invalid input. */
+size_t dsph_batch_workspace(const dsph_domain* domains, const int64_t* counts, int32_t ndom,
+                            float hsearch, int32_t width);
+
+/* Lays the batch out in `workspace` and uploads the per-domain tables
+ * (boxes, grids, offsets; synchronous).  All domains must be periodic on
+ * both axes with at least 3 cells per axis; support <= hsearch. */
+int dsph_batch_init(const dsph_domain* domains, const int64_t* counts, int32_t ndom,
+                    float support, float hsearch, int32_t width, int32_t block,
+                    void* workspace, size_t workspace_bytes, dsph_batch* out);
+
+/* `steps` RK4 steps of every domain: sort into cell order, four right-hand
+ * sides, RK4 combination, per-domain time step.  State arrays are the
+ * concatenation of the domains (pos, vel [n,2], mass, rho [n], uid [n]),
+ * dt [ndom] is the step each domain takes next (updated when adaptive),
+ * time [ndom] (float64) accumulates.  No host synchronisation and no
+ * allocation: the call may be captured into a graph.  Rows longer than
+ * `width` are truncated and counted (dsph_batch_status). */
+int dsph_batch_step(const dsph_batch* batch, const dsph_deltasph_params* prm,
+                    const dsph_timestep_params* tp, float* pos, float* vel, float* mass,
+                    float* rho, int32_t* uid, float* dt, double* time, int32_t steps,
+                    void* stream);
+
+/* The same `steps` steps captured once into an executable HIP graph
+ * (*graph); replay with dsph_graph_launch on any stream.  The graph keeps
+ * the pointers it was captured with. */
+int dsph_batch_graph(const dsph_batch* batch, const dsph_deltasph_params* prm,
+                     const dsph_timestep_params* tp, float* pos, float* vel, float* mass,
+                     float* rho, int32_t* uid, float* dt, double* time, int32_t steps,
+                     void** graph);
+int dsph_graph_launch(void* graph, void* stream);
+int dsph_graph_destroy(void* graph);
+
+/* Sorts the current positions into cells and builds the neighbor rows
+ * once; *max_count_host = the longest row (synchronises).  Used to size
+ * `width`. */
+int dsph_batch_probe(const dsph_batch* batch, const float* pos, int32_t* max_count_host,
+                     void* stream);
+
+/* status[0] = rows truncated to `width` since the last reset, status[1] =
+ * the longest row seen (synchronises).  reset != 0 clears both after
+ * reading (with a fill kernel). */
+int dsph_batch_status(const dsph_batch* batch, int32_t* status_host, int32_t reset, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Adjoint of the delta-SPH right-hand side                                 */
 /* ---------------------------------------------------------------------- */
 

@@ -63,3 +63,27 @@ def test_grid_dims_match_oracle(orc):
         ncx, ncy, hc = orc.grid_dims(orc.make_domain(mn, mx, per), h)
         assert (g.ncell[0], g.ncell[1], g.hcell) == (ncx, ncy, hc)
         assert g.hcell >= h
+
+
+def test_batch_workspace_host_checks():
+    """The batch layout is computed on the host: invalid boxes, counts and
+    widths give 0 / an error status before any device call."""
+    from diffsph_amd import _lib
+    L = _lib.lib()
+    doms = (_lib.Domain * 2)(_lib.make_domain((-0.25, -0.25), (0.25, 0.25), (True, True)),
+                             _lib.make_domain((-0.5, -0.5), (0.5, 0.5), (True, True)))
+    counts = (C.c_int64 * 2)(1024, 4096)
+    h = 2.0 / 256 * 3.99
+    ws = L.dsph_batch_workspace(doms, counts, 2, C.c_float(h), 64)
+    assert ws > 4 * 64 * 5120
+    assert L.dsph_batch_workspace(doms, counts, 2, C.c_float(h), 0) == 0
+    # fewer than 3 cells per periodic axis
+    assert L.dsph_batch_workspace(doms, counts, 2, C.c_float(0.2), 64) == 0
+    walled = (_lib.Domain * 1)(_lib.make_domain((0, 0), (1, 1), (False, True)))
+    assert L.dsph_batch_workspace(walled, counts, 1, C.c_float(h), 64) == 0
+    b = _lib.Batch()
+    assert L.dsph_batch_init(doms, counts, 2, C.c_float(h), C.c_float(h), 64, 256, None, ws, C.byref(b)) == 1
+    assert L.dsph_batch_init(doms, counts, 2, C.c_float(h), C.c_float(h), 64, 100, C.c_void_p(256), ws,
+                             C.byref(b)) == 1
+    assert L.dsph_batch_init(doms, counts, 2, C.c_float(h), C.c_float(0.5 * h), 64, 256, C.c_void_p(256), ws,
+                             C.byref(b)) == 1
