@@ -56,7 +56,24 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for PMC byte counters)")
-    return ap.parse_args()
+    ap.add_argument("--workload", default="dfsph", choices=["dfsph", "ensemble", "tgv", "dambreak", "rollout"],
+                    help="dfsph: 1M DFSPH plain vs learned (the headline metric); ensemble: 512 periodic "
+                         "delta-SPH domains in one HIP graph; tgv: delta-SPH Taylor-Green vortex; dambreak: "
+                         "delta-SPH dam break with mDBC walls; rollout: adjoint of a Taylor-Green rollout")
+    ap.add_argument("--ndom", type=int, default=512, help="ensemble: number of domains")
+    ap.add_argument("--cell-scale", type=float, default=None, help="ensemble: skip the autotuner, use this cell scale")
+    ap.add_argument("--block", type=int, default=256, help="ensemble: workgroup size with --cell-scale")
+    args = ap.parse_args()
+    defaults = {"ensemble": (128, 2, 0), "tgv": (256, 8, 64), "dambreak": (2000, 20, 256), "rollout": (1, 1, 512)}
+    if args.workload in defaults:
+        st, wu, ns = defaults[args.workload]
+        if "--steps" not in sys.argv:
+            args.steps = st
+        if "--warmup" not in sys.argv:
+            args.warmup = wu
+        if "--nside" not in sys.argv:
+            args.nside = ns
+    return args
 
 
 def main():
@@ -65,6 +82,10 @@ def main():
         print("bench.py: this build measures one GPU only (the workload runs on a single MI355X); "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "dfsph":
+        {"ensemble": bench_ensemble, "tgv": bench_tgv, "dambreak": bench_dambreak,
+         "rollout": bench_rollout}[args.workload](args)
+        return
     import torch
 
     from diffsph_amd import _lib, sampling
@@ -266,6 +287,264 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# delta-SPH workloads (one JSON line each, same keys as the headline line)
+# ---------------------------------------------------------------------------
+
+def _line(metric, value, unit, args, steps, secs, dtype, config, roofline, cpu, higher=True):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": round(secs * 1e3 / max(steps, 1), 4), "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+            "roofline": roofline, "cpu_baseline": cpu}
+
+
+def _time_pass(fn, reps, stream):
+    import torch
+    for _ in range(2):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def _batch_roofline(b, pairs_per_particle, reps):
+    """HIP-event timing of the three pair passes of a batch (re-running them
+    on the lists the last probe left) and their algorithmic bytes:
+      rows (neighbor fill): query pos 8 + domain 4 + count write 4 B/particle, entry write 4 B/pair
+      normalization:        pos 8 + mass 4 + rho 4 + count 4 + domain 4 + grad write 8 B/particle,
+                            entries read twice 8 B/pair
+      right-hand side:      pos 8 + vel 8 + rho 4 + mass 4 + grad 8 + count 4 + domain 4
+                            + dudt 8 + drho/dt 4 B/particle, entry 4 B/pair
+    (gathered neighbor fields are counted once per particle, not per pair)."""
+    import torch
+    from diffsph_amd import _lib
+    L = _lib.lib()
+    stream = torch.cuda.current_stream(b.device)
+    sh = stream.cuda_stream
+    n = b.n
+    P = pairs_per_particle * n
+    out = {}
+    for pas, name, per_p, per_pair in ((0, "k_bfill", 16, 4), (1, "k_dsph_normalization", 32, 8),
+                                       (2, "k_dsph_rhs", 52, 4)):
+        def fn(pas=pas):
+            _lib.check(L.dsph_batch_replay_pass(C.byref(b.batch), C.byref(b.prm), pas, sh), "replay_pass")
+        t = _time_pass(fn, reps, stream)
+        byt = per_p * n + per_pair * P
+        out[name] = {"us": round(t * 1e6, 2), "bytes": int(byt), "gbs": round(byt / t / 1e9, 1)}
+        log(f"{name}: {t * 1e6:.1f} us, {byt / 1e6:.1f} MB algorithmic -> {byt / t / 1e9:.0f} GB/s")
+    dom = max(out, key=lambda k: out[k]["us"])
+    r = out[dom]
+    return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "kernel_us": r["us"],
+            "algorithmic_bytes": r["bytes"], "passes": out}
+
+
+def _oracle_deltasph_rate(states, cfg, steps):
+    """CPU oracle (C++/OpenMP) RK4 delta-SPH steps over a list of
+    (domain, pos, vel, rho, mass) samples; returns (particle-updates/s,
+    threads, particles)."""
+    from oracle import oracle as orc
+    threads = min(os.cpu_count() or 1, 16)
+    orc.set_threads(threads)
+    prm = orc.DeltaParams(rho0=cfg.rho0, cs=cfg.cs, delta=cfg.delta, kernel_scale=cfg.kernel_scale,
+                          viscosity=1 if cfg.viscosity == "deltaSPH_viscid" else 0,
+                          visc_coeff=cfg.nu if cfg.viscosity == "deltaSPH_viscid" else cfg.alpha, eps=cfg.eps)
+    tot = 0
+    c0 = time.perf_counter()
+    for (mn, mx, pos, vel, rho, mass) in states:
+        dom = orc.make_domain(mn, mx, (True, True))
+        n = pos.shape[0]
+        st = orc.DeltaState(pos, vel, rho, mass, np.arange(n, dtype=np.int32))
+        for _ in range(steps):
+            st, _ = orc.rk4_step(st, dom, cfg.support, prm, 1e-3)
+        tot += n * steps
+    secs = time.perf_counter() - c0
+    return tot / secs, threads, tot
+
+
+def bench_ensemble(args):
+    """512 independent periodic delta-SPH domains (N_d log-uniform in
+    [1k, 64k], ~8.8M particles), autotuned cell size and workgroup size,
+    `steps` steps captured into ONE HIP graph and timed as one launch."""
+    import torch
+    from dataclasses import replace
+    from diffsph_amd import batched
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ens = batched.make_ensemble(ndom=args.ndom, seed=args.seed, device=dev)
+    n = sum(ens.counts)
+    log(f"ensemble: {len(ens.counts)} domains, {n} particles (min {min(ens.counts)}, max {max(ens.counts)})")
+    if args.cell_scale is None:
+        best, times = batched.autotune(ens, log=log)
+    else:
+        best, times = (args.cell_scale, args.block), {}
+    cfg = replace(ens.cfg, cell_scale=best[0])
+    b = batched.BatchedDeltaSPH(ens.boxes, ens.counts, ens.pos, ens.vel, ens.mass, ens.rho, cfg, block=best[1])
+    log(f"autotuned: cell_scale {best[0]}, block {best[1]}, width {b.width}")
+    b.step(args.warmup)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    graph = b.capture(args.steps)
+    torch.cuda.synchronize()
+    t_cap = time.perf_counter() - c0
+    c0 = time.perf_counter()
+    graph.launch()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - c0
+    b.check_rows()
+    value = n * args.steps / secs
+    log(f"{args.steps} steps in one graph launch: {secs:.3f} s ({secs * 1e3 / args.steps:.3f} ms/step), "
+        f"{value:.4g} particle-updates/s; capture+instantiate {t_cap:.2f} s")
+    nb, pairs = b.probe()
+    roof = _batch_roofline(b, pairs_per_particle=pairs / n, reps=args.kernel_reps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        order = list(range(len(ens.counts)))
+        states, tot = [], 0
+        for d in order:
+            sl = slice(int(b.first[d]), int(b.first[d + 1]))
+            lo, hi = ens.boxes[d]
+            states.append((lo, hi, ens.pos[sl].cpu().numpy(), ens.vel[sl].cpu().numpy(), ens.rho[sl].cpu().numpy(),
+                           ens.mass[sl].cpu().numpy()))
+            tot += ens.counts[d]
+            if tot >= 400_000:
+                break
+        rate, threads, parts = _oracle_deltasph_rate(states, cfg, 1)
+        cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"one RK4 step of the first {len(states)} ensemble domains ({parts} particles), "
+                         f"C++/OpenMP oracle, {threads} threads"}
+        log(f"cpu baseline: {rate:.4g} particle-updates/s")
+    config = {"workload": f"ensemble_{len(ens.counts)}_periodic_deltasph", "domains": len(ens.counts),
+              "particles": n, "min_domain": min(ens.counts), "max_domain": max(ens.counts),
+              "steps_per_graph_launch": args.steps, "cell_scale": best[0], "block": best[1], "width": b.width,
+              "longest_row": nb, "autotune_ms_per_step": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in times.items()},
+              "capture_seconds": round(t_cap, 3), "dt_mean": float(b.dt.mean().item())}
+    config["pairs"] = pairs
+    print(json.dumps(_line("particle-updates/s, batched ensemble of 512 periodic delta-SPH domains (~8.8M)",
+                           value, "particle-updates/s", args, args.steps, secs, "f32", config, roof, cpu)),
+          flush=True)
+    graph.close()
+
+
+def bench_tgv(args):
+    """delta-SPH Taylor-Green vortex (periodic [-1,1]^2, nside^2 lattice,
+    ~50 neighbors, c_s = 10, nu = 0.01, delta = 0.1, RK4 + CFL dt) as a
+    one-domain batch: `steps` steps in one HIP graph launch."""
+    import torch
+    from diffsph_amd import batched
+    from diffsph_amd.deltasph import analytic_tgv_energy_ratio, taylor_green_state
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    pos, vel, mass, rho, cfg = taylor_green_state(args.nside, dev)
+    n = pos.shape[0]
+    b = batched.BatchedDeltaSPH([((-1.0, -1.0), (1.0, 1.0))], [n], pos, vel, mass, rho, cfg)
+    E0 = 0.5 * float((b.mass.double() * (b.vel.double() ** 2).sum(1)).sum())
+    graph = b.capture(args.steps)
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    graph.launch()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - c0
+    b.check_rows()
+    E = 0.5 * float((b.mass.double() * (b.vel.double() ** 2).sum(1)).sum())
+    t = float(b.time[0].item())
+    ratio, ana = E / E0, analytic_tgv_energy_ratio(t, cfg.nu)
+    value = n * args.steps / secs
+    log(f"TGV {args.nside}^2: {args.steps} steps {secs:.3f} s, {value:.4g} particle-updates/s; t = {t:.4f}, "
+        f"E/E0 {ratio:.5f} (analytic {ana:.5f})")
+    _, pairs = b.probe()
+    roof = _batch_roofline(b, pairs_per_particle=pairs / n, reps=args.kernel_reps)
+    cpu = None
+    if not args.no_cpu_baseline:
+        rate, threads, parts = _oracle_deltasph_rate(
+            [((-1.0, -1.0), (1.0, 1.0), pos.cpu().numpy(), vel.cpu().numpy(), rho.cpu().numpy(),
+              mass.cpu().numpy())], cfg, max(1, min(args.steps, 40)))
+        cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{parts // n} RK4 steps of the {n}-particle Taylor-Green state, C++/OpenMP oracle, "
+                         f"{threads} threads"}
+        log(f"cpu baseline: {rate:.4g} particle-updates/s")
+    config = {"workload": f"deltasph_taylor_green_{args.nside}x{args.nside}", "particles": n,
+              "steps_per_graph_launch": args.steps, "simulated_time": t, "energy_ratio": ratio,
+              "analytic_energy_ratio": ana}
+    print(json.dumps(_line("particle-updates/s, delta-SPH Taylor-Green vortex", value, "particle-updates/s", args,
+                           args.steps, secs, "f32", config, roof, cpu)), flush=True)
+    graph.close()
+
+
+def bench_dambreak(args):
+    """delta-SPH dam break with mDBC walls (4 x 2 box, 1 x 2 column,
+    dx = 1/nside, three wall layers, g = -9.81, c_s = 20 sqrt(2 g H),
+    alpha = 0.01), RK4 with the CFL time step, stepped by the single-domain
+    solver (per-stage host synchronisation for the list sizes)."""
+    import torch
+    from diffsph_amd.deltasph import DeltaSPHSolver, dam_break_state
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    pos, vel, mass, rho, cfg, walls = dam_break_state(args.nside, dev)
+    n = pos.shape[0]
+    s = DeltaSPHSolver(pos, vel, mass, rho, cfg, walls=walls)
+    log(f"dam break: {n} fluid, {s.walls.n} wall particles, h {cfg.support:.5g}, c_s {cfg.cs:.4g}")
+    for _ in range(args.warmup):
+        s.step()
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    for k in range(args.steps):
+        s.step()
+        if (k + 1) % 500 == 0:
+            log(f"step {k + 1}: t = {s.time:.5f}, dt = {s.dt:.3g}")
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - c0
+    value = n * args.steps / secs
+    front = float(s.pos[:, 0].max().item())
+    log(f"dam break: {args.steps} steps {secs:.2f} s, {value:.4g} particle-updates/s; t = {s.time:.4f}, "
+        f"front x = {front:.4f}")
+    config = {"workload": f"deltasph_dam_break_dx_1_{args.nside}", "particles": n, "wall_particles": s.walls.n,
+              "simulated_time": s.time, "front_x": front, "dt_last": s.dt}
+    print(json.dumps(_line("particle-updates/s, delta-SPH dam break with mDBC walls", value, "particle-updates/s",
+                           args, args.steps, secs, "f32", config, None, None)), flush=True)
+
+
+def bench_rollout(args):
+    """Gradient of the final kinetic energy w.r.t. the initial velocity
+    through 64 RK4 delta-SPH steps of the Taylor-Green set-up (nside^2),
+    checkpoint every 8 steps: forward + hand-written adjoint."""
+    import torch
+    from diffsph_amd import adjoint
+    from diffsph_amd.deltasph import taylor_green_state
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    pos, vel, mass, rho, cfg = taylor_green_state(args.nside, dev)
+    n = pos.shape[0]
+    steps = 64
+    fwd, bwd = [], []
+    for rep in range(args.warmup + args.steps):
+        u0 = vel.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        x, u, r = adjoint.rollout(pos, u0, rho, mass, cfg, steps, checkpoint_every=8)
+        E = adjoint.kinetic_energy(u, mass)
+        torch.cuda.synchronize()
+        c1 = time.perf_counter()
+        E.backward()
+        torch.cuda.synchronize()
+        c2 = time.perf_counter()
+        if rep >= args.warmup:
+            fwd.append(c1 - c0)
+            bwd.append(c2 - c1)
+    secs = float(np.mean(fwd) + np.mean(bwd))
+    value = n * steps / secs
+    log(f"rollout {args.nside}^2 x {steps} steps: forward {np.mean(fwd):.3f} s, backward {np.mean(bwd):.3f} s, "
+        f"{value:.4g} particle-steps/s (forward + adjoint)")
+    config = {"workload": f"deltasph_rollout_adjoint_{args.nside}x{args.nside}_64_steps_checkpoint_8",
+              "particles": n, "forward_seconds": float(np.mean(fwd)), "backward_seconds": float(np.mean(bwd))}
+    print(json.dumps(_line("particle-steps/s, forward + adjoint of a 64-step delta-SPH rollout", value,
+                           "particle-steps/s", args, steps, secs, "f32", config, None, None)), flush=True)
 
 
 if __name__ == "__main__":

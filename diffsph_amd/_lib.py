@@ -199,7 +199,7 @@ SIGNATURES = {
     "dsph_deltasph_normalization": (C.c_int, [P(Pairs), P(Boundary), V, V, V, V, V, V]),
     "dsph_deltasph_rhs": (C.c_int, [P(Pairs), P(Boundary), P(DeltaSPHParams), V, V, V, V, V, V, V, V]),
     "dsph_boundary_ghosts": (C.c_int, [V, V, V, I64, F32, V, V]),
-    "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, V, V, V]),
+    "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, I32, V, V, V]),
     "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),
     "dsph_rk4_combine": (C.c_int, [I64, P(RK4Stages), V, V, V, F32, V, V, V, V]),
     "dsph_max_norm_workspace": (SZ, []),
@@ -212,7 +212,8 @@ SIGNATURES = {
                                    P(C.c_void_p)]),
     "dsph_graph_launch": (C.c_int, [V, V]),
     "dsph_graph_destroy": (C.c_int, [V]),
-    "dsph_batch_probe": (C.c_int, [P(Batch), V, P(I32), V]),
+    "dsph_batch_replay_pass": (C.c_int, [P(Batch), P(DeltaSPHParams), I32, V]),
+    "dsph_batch_probe": (C.c_int, [P(Batch), V, P(I32), P(I64), V]),
     "dsph_batch_status": (C.c_int, [P(Batch), P(I32), I32, V]),
     "dsph_deltasph_vjp_workspace": (SZ, [I64]),
     "dsph_deltasph_rhs_vjp": (C.c_int, [P(Pairs), P(DeltaSPHParams), V, V, V, V, V, V, V, V, V, V, V, SZ, V]),

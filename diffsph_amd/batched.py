@@ -135,12 +135,18 @@ class BatchedDeltaSPH:
         self.batch = b
         self.width = width
 
+    def probe(self):
+        """(longest neighbor row, number of pairs) of the current positions;
+        leaves these lists in the workspace."""
+        m = C.c_int32(0)
+        p = C.c_int64(0)
+        check(lib().dsph_batch_probe(C.byref(self.batch), ptr(self.pos), C.byref(m), C.byref(p),
+                                     _lib.stream_handle(self.device)), "batch_probe")
+        return int(m.value), int(p.value)
+
     def max_neighbors(self) -> int:
         """Longest neighbor row of the current positions."""
-        m = C.c_int32(0)
-        check(lib().dsph_batch_probe(C.byref(self.batch), ptr(self.pos), C.byref(m),
-                                     _lib.stream_handle(self.device)), "batch_probe")
-        return int(m.value)
+        return self.probe()[0]
 
     def suggest_width(self, margin: float = 1.3) -> int:
         return int(math.ceil(self.max_neighbors() * margin / 8.0) * 8)

@@ -23,6 +23,7 @@
 //           computeTimestep in float32.
 // No host synchronisation, no allocation, no memset nodes: a sequence of
 // steps can be captured into one HIP graph.
+#include <cstring>
 #include <vector>
 
 #include "common.h"
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_bgather2(int n, const int32_t* __res
 // one contiguous slice), entries mapped through si to state indices (si ==
 // NULL: the sorted order is the state order).  Rows longer than `width`
 // keep their first `width` entries and are counted in status[0]; status[1]
-// is the longest row.
+// is the longest row, status[2..3] (uint64) the number of pairs found.
 __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restrict__ dom_of,
                                                   const Dom* __restrict__ doms, const BGrid* __restrict__ grids,
                                                   const float2* __restrict__ q, const float2* __restrict__ spos,
@@ -215,6 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
                                                   int32_t* __restrict__ cnt, int32_t* __restrict__ status) {
   __shared__ int sh_max[kBlock / 64];
   __shared__ int sh_over[kBlock / 64];
+  __shared__ int sh_sum[kBlock / 64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int total = 0;
   if (i < n) {
@@ -230,8 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
         const float2 rj = spos[j];
         const float dx = mod_offset(qi.x - rj.x, d, 0);
         const float dy = mod_offset(qi.y - rj.y, d, 1);
-        const float r = sqrt_rn(dist2_exact(dx, dy));
-        if (__fdiv_rn(r, h) <= 1.0f) {
+        if (within_support(dist2_exact(dx, dy), h)) {
           if (total < width) nbr[base + (int64_t)total * 64] = si ? si[j] : j;
           ++total;
         }
@@ -260,26 +261,31 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
     }
     cnt[i] = total < width ? total : width;
   }
-  int m = total, over = total > width ? 1 : 0;
+  int m = total, over = total > width ? 1 : 0, sum = total;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const int t = __shfl_xor(m, o, 64);
     m = t > m ? t : m;
     over += __shfl_xor(over, o, 64);
+    sum += __shfl_xor(sum, o, 64);
   }
   if ((threadIdx.x & 63) == 0) {
     sh_max[threadIdx.x >> 6] = m;
     sh_over[threadIdx.x >> 6] = over;
+    sh_sum[threadIdx.x >> 6] = sum;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int bm = 0, bo = 0;
+    unsigned long long bs = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
       bm = sh_max[w] > bm ? sh_max[w] : bm;
       bo += sh_over[w];
+      bs += (unsigned long long)sh_sum[w];
     }
     if (bo) atomicAdd(&status[0], bo);
     atomicMax(&status[1], bm);
+    atomicAdd(reinterpret_cast<unsigned long long*>(status + 2), bs);
   }
 }
 
@@ -739,7 +745,29 @@ extern "C" int dsph_batch_status(const dsph_batch* batch, int32_t* status_host, 
   return DSPH_OK;
 }
 
-extern "C" int dsph_batch_probe(const dsph_batch* batch, const float* pos, int32_t* max_count_host, void* stream) {
+extern "C" int dsph_batch_replay_pass(const dsph_batch* batch, const dsph_deltasph_params* prm, int32_t pass,
+                                      void* stream) {
+  int s = check_batch(batch);
+  if (s != DSPH_OK) return s;
+  if (!prm || pass < 0 || pass > 2) return DSPH_ERR_ARG;
+  const Ctx c = make_ctx(batch, as_stream(stream));
+  const Layout& L = c.L;
+  float2* wq = c.p<float2>(L.wq);
+  if (pass == 0) return fill_rows(c, wq, c.p<float2>(L.spos), c.p<int32_t>(L.si));
+  const PairsDev P = batch_pairs(c, wq);
+  const float* rho = c.p<float>(L.Srho[2]);
+  if (pass == 1)
+    return launch_normalization_per_particle(P, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), c.p<float>(L.Bmass), rho,
+                                             c.p<float2>(L.grad), c.block, c.st);
+  RhsArgs A;
+  s = make_rhs_args(prm, c.p<float>(L.Svel[2]), c.p<float>(L.Bmass), rho, c.p<float>(L.grad), c.p<float>(L.ku[3]),
+                    c.p<float>(L.kr[3]), nullptr, &A);
+  if (s != DSPH_OK) return s;
+  return launch_rhs_per_particle(P, A, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), c.block, c.st);
+}
+
+extern "C" int dsph_batch_probe(const dsph_batch* batch, const float* pos, int32_t* max_count_host,
+                                int64_t* pairs_host, void* stream) {
   int s = check_batch(batch);
   if (s != DSPH_OK) return s;
   if (!pos || !max_count_host) return DSPH_ERR_ARG;
@@ -753,11 +781,16 @@ extern "C" int dsph_batch_probe(const dsph_batch* batch, const float* pos, int32
   k_bgather2<<<c.grid(c.n), c.block, 0, st>>>(c.n, c.p<int32_t>(c.L.si), wq, c.p<float2>(c.L.spos));
   s = fill_rows(c, wq, c.p<float2>(c.L.spos), c.p<int32_t>(c.L.si));
   if (s != DSPH_OK) return s;
-  int32_t h[2] = {0, 0};
+  int32_t h[4] = {0, 0, 0, 0};
   DSPH_CHECK_HIP(hipMemcpyAsync(h, status, sizeof(h), hipMemcpyDeviceToHost, st));
   k_fill_i32<<<1, kBlock, 0, st>>>(status, 4, 0);
   DSPH_LAUNCH_CHECK();
   DSPH_CHECK_HIP(hipStreamSynchronize(st));
   *max_count_host = h[1];
+  if (pairs_host) {
+    uint64_t t;
+    memcpy(&t, h + 2, sizeof(t));
+    *pairs_host = (int64_t)t;
+  }
   return DSPH_OK;
 }

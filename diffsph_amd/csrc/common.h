@@ -69,6 +69,18 @@ __device__ __forceinline__ float dist2_exact(float dx, float dy) {
   return __fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy));
 }
 
+// The neighbor predicate sqrt_rn(d2) /_rn h <= 1, decided from d2 alone when
+// it lies clearly inside or outside h^2 (1 -/+ 1e-6): the rounding of the
+// square root, of the division and of the thresholds themselves is below
+// 3e-7 relative, so the shortcut returns exactly what the full expression
+// returns; only the band between the thresholds evaluates it.
+__device__ __forceinline__ bool within_support(float d2, float h) {
+  const float hh = h * h;
+  if (d2 < hh * 0.999999f) return true;
+  if (d2 > hh * 1.000001f) return false;
+  return __fdiv_rn(sqrt_rn(d2), h) <= 1.0f;
+}
+
 // Wendland C2 in 2-D (kernelFunctions/Wendland2.py): C_2 = 7/pi.
 constexpr float kC2 = (float)(7.0 / 3.14159265358979323846);
 

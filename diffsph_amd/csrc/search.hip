@@ -104,9 +104,7 @@ struct SearchArgs {
 __device__ __forceinline__ bool keep_pair(float2 qi, float hi, float2 rj, float hj, const SearchArgs& a) {
   float dx = mod_offset(qi.x - rj.x, a.d, 0);
   float dy = mod_offset(qi.y - rj.y, a.d, 1);
-  float r = sqrt_rn(dist2_exact(dx, dy));
-  float h = pick_h(a.mode, hi, hj);
-  return __fdiv_rn(r, h) <= 1.0f;
+  return within_support(dist2_exact(dx, dy), pick_h(a.mode, hi, hj));
 }
 
 // Visit the candidate slices of query i in offset order (dy=-1,0,1; dx=-1,0,1).

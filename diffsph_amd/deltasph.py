@@ -32,6 +32,9 @@ from .neighborhood import WORKSPACE, CellGrid, search, wrap_positions
 
 VISCOSITY = {None: -1, "none": -1, "deltaSPH_inviscid": 0, "deltaSPH_viscid": 1}
 EOS = {"isoThermal": 0, "Tait": 1}
+# wall density: "mDBCDensity" = v2/modules/mDBC.py:50-136 (pinv, unclamped);
+# "mDBC_op" = the C++ op v2/cppSrc/mDBC.h:129-244 (Cramer, clamped at rho0)
+MDBC = {"mDBCDensity": 0, "mDBC_op": 1}
 
 
 @dataclass
@@ -68,6 +71,7 @@ class DeltaSPHConfig:
     boundary_nu: float = 0.01
     boundary_eps: float = 1e-6
     boundary_pi_switch: bool = False
+    mdbc: str = "mDBCDensity"
     # "global": |dudt| over all components, as computeTimestep evaluates
     # torch.linalg.norm(dudt[~isnan(dudt)]) on a flattened tensor
     # (timestep.py:22-25); "max": the largest per-particle |dudt|
@@ -241,7 +245,8 @@ class DeltaSPHSolver:
         pr_g.rpos = ptr(wpos)
         pr_g.rh = None
         check(lib().dsph_mdbc(C.byref(pr_g), ptr(self.mass), ptr(rho), ptr(W.pos), C.c_float(self.cfg.rho0),
-                              C.c_float(self.cfg.support), ptr(W.rho), ptr(W.shepard), st), "mdbc")
+                              C.c_float(self.cfg.support), MDBC[self.cfg.mdbc], ptr(W.rho), ptr(W.shepard), st),
+              "mdbc")
         b = _lib.Boundary()
         b.pairs = C.pointer(pr_b)
         b.mass, b.rho, b.vel = ptr(W.mass), ptr(W.rho), ptr(W.vel)
@@ -372,8 +377,11 @@ def dam_break_state(res: int, device, g: float = 9.81, column_height: float = 2.
     20 sqrt(2 g H), delta-SPH inviscid viscosity alpha = 0.01 for fluid and
     wall, density diffusion delta = 0.1, RK4 with the CFL time step.  The
     minimum time step is lowered below the acoustic one (the reference's
-    default floor of 1e-3 would exceed it).  Returns (positions,
-    velocities, masses, densities, config, walls)."""
+    default floor of 1e-3 would exceed it).  Wall densities follow the
+    reference's C++ mDBC op (clamped at rho0): the column reaches the lid,
+    and once the water falls away from it the unclamped extrapolation of
+    mDBCDensity drives the lid densities to ~0 and the run diverges.
+    Returns (positions, velocities, masses, densities, config, walls)."""
     from . import sampling
     d = sampling.dam_break(res)
     n = d["fluid_pos"].shape[0]
@@ -382,7 +390,8 @@ def dam_break_state(res: int, device, g: float = 9.81, column_height: float = 2.
     cs = 20.0 * math.sqrt(2.0 * g * column_height)
     cfg = DeltaSPHConfig(support=d["h"], dx=dx, cs=cs, domain_min=(-4 * dx, -4 * dx),
                          domain_max=(4.0 + 4 * dx, 2.0 + 4 * dx), periodic=(False, False),
-                         viscosity="deltaSPH_inviscid", alpha=0.01, gravity=(0.0, -g), min_dt=1e-7)
+                         viscosity="deltaSPH_inviscid", alpha=0.01, gravity=(0.0, -g), min_dt=1e-7,
+                         mdbc="mDBC_op")
     walls = dict(pos=T(d["wall_pos"]), dist=T(d["wall_dist"]), normal=T(d["wall_normal"]),
                  area=torch.tensor(d["area"], dtype=torch.float32, device=device))
     return (T(d["fluid_pos"]), torch.zeros(n, 2, device=device), T(np.full(n, d["area"] * cfg.rho0)),

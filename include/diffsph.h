@@ -464,16 +464,22 @@ int dsph_boundary_ghosts(const float* boundary_pos, const float* distances,
                          const float* normals, int64_t nb, float support,
                          float* ghost_pos, void* stream);
 
-/* mDBC wall densities (mDBCDensity, v2/modules/mDBC.py:50-136): rows of
- * `ghost_pairs` are the ghosts (qpos = ghost positions, one per wall
- * particle, in wall order), entries fluid indices (rpos = fluid
- * positions).  Shepard density for 1..5 neighbors, rho0 for none, else the
- * first-order MLS extrapolation c0 + (x_b - x_g) . c[1:] with
- * c = pinv(A_g) b (3x3 pseudo-inverse, singular values <= 3 eps_f32
- * sigma_max dropped as torch.linalg.pinv does).  shepard [nb] may be NULL. */
+/* mDBC wall densities: rows of `ghost_pairs` are the ghosts (qpos = ghost
+ * positions, one per wall particle, in wall order), entries fluid indices
+ * (rpos = fluid positions).  shepard [nb] may be NULL.
+ * variant 0 -- mDBCDensity (v2/modules/mDBC.py:50-136): Shepard density for
+ *   1..5 neighbors, rho0 for none, else the first-order MLS extrapolation
+ *   c0 + (x_b - x_g) . c[1:] with c = pinv(A_g) b (3x3 pseudo-inverse,
+ *   singular values <= 3 eps_f32 sigma_max dropped as torch.linalg.pinv
+ *   does); no clamp.
+ * variant 1 -- the C++ mDBC op (mDBCDensityKernel, v2/cppSrc/mDBC.h:129-244,
+ *   host mDBC, v2/cppSrc/mDBC.cpp:3-147): the same system solved by
+ *   Cramer's rule with det + 1e-7, Shepard density for <= 5 neighbors,
+ *   the result clamped from below at rho0. */
 int dsph_mdbc(const dsph_pairs* ghost_pairs, const float* fluid_mass,
               const float* fluid_rho, const float* boundary_pos, float rho0,
-              float support, float* boundary_rho, float* shepard, void* stream);
+              float support, int32_t variant, float* boundary_rho, float* shepard,
+              void* stream);
 
 /* Stage state of the reference's RK4 (v2/modules/integration.py:23-40,
  * 231-276): x = x0 + dxdt c, u = u0 + dudt c, rho = rho0 + drhodt c, each
@@ -600,10 +606,18 @@ int dsph_graph_launch(void* graph, void* stream);
 int dsph_graph_destroy(void* graph);
 
 /* Sorts the current positions into cells and builds the neighbor rows
- * once; *max_count_host = the longest row (synchronises).  Used to size
- * `width`. */
+ * once; *max_count_host = the longest row, *pairs_host (may be NULL) = the
+ * number of pairs (synchronises).  Used to size `width`. */
 int dsph_batch_probe(const dsph_batch* batch, const float* pos, int32_t* max_count_host,
-                     void* stream);
+                     int64_t* pairs_host, void* stream);
+
+/* Re-runs one pass of the last stage of the last step on the data it left
+ * in the workspace (same inputs, same outputs): pass 0 = neighbor rows,
+ * 1 = normalization + density gradient, 2 = right-hand side.  For timing a
+ * single kernel; no counterpart in the reference.  Only valid after
+ * dsph_batch_step (or a graph launch) has run at least one step. */
+int dsph_batch_replay_pass(const dsph_batch* batch, const dsph_deltasph_params* prm,
+                           int32_t pass, void* stream);
 
 /* status[0] = rows truncated to `width` since the last reset, status[1] =
  * the longest row seen (synchronises).  reset != 0 clears both after

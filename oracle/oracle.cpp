@@ -856,13 +856,71 @@ void dsph_rhs(const Csr<T>& c, const DomT<T>& d, const DeltaP& P, const T* vel, 
   }
 }
 
+// 3x3 determinant by cofactors of the first row, as mDBC.h:200-209 writes it.
+template <class T>
+T det3(T m11, T m12, T m13, T m21, T m22, T m23, T m31, T m32, T m33) {
+  const T a = m11 * (m22 * m33 - m23 * m32);
+  const T b = m12 * (m21 * m33 - m23 * m31);
+  const T c = m13 * (m21 * m32 - m22 * m31);
+  return a - b + c;
+}
+
+// The reference's C++ mDBC op (mDBCDensityKernel, v2/cppSrc/mDBC.h:129-244):
+// the same sums as mDBCDensity with x_gj = -(q h) xhat and b = sum m_j
+// [W, gradW]; Cramer's rule with det + 1e-7; Shepard density for <= 5
+// neighbors; the result clamped from below at rho0.
+template <class T>
+T mdbc_op_row(const Csr<T>& c, const DomT<T>& d, int64_t i, const T* mass, const T* rho, const T* bpos, T rho0,
+              T* shep_out) {
+  T nom = 0, den = 0;
+  T m00 = 0, m01 = 0, m02 = 0, m10 = 0, m11 = 0, m12 = 0, m20 = 0, m21 = 0, m22 = 0, bx = 0, by = 0, bz = 0;
+  const int cnt = c.off[i + 1] - c.off[i];
+  for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+    int32_t j = c.nbr[k];
+    G<T> g = pg(c, i, j, d);
+    const T m_j = mass[j], rho_j = rho[j];
+    const T V = m_j / rho_j;
+    const T xx = ((-g.q) * g.h) * g.ex, xy = ((-g.q) * g.h) * g.ey;
+    nom += m_j * g.w;
+    den += V * g.w;
+    const T vw = V * g.w, vgx = V * g.gx, vgy = V * g.gy;
+    m00 += vw;
+    m10 += vgx;
+    m20 += vgy;
+    m01 += vw * xx;
+    m11 += vgx * xx;
+    m21 += vgx * xy;
+    m02 += vw * xy;
+    m12 += vgy * xx;
+    m22 += vgy * xy;
+    bx += m_j * g.w;
+    by += m_j * g.gx;
+    bz += m_j * g.gy;
+  }
+  const T shep = nom / (den + (T)1e-7);
+  if (shep_out) shep_out[i] = shep;
+  if (cnt <= 5) return shep < rho0 ? rho0 : shep;
+  const T d0 = det3(m00, m01, m02, m10, m11, m12, m20, m21, m22) + (T)1e-7;
+  const T rx = det3(bx, m01, m02, by, m11, m12, bz, m21, m22) / d0;
+  const T ry = det3(m00, bx, m02, m10, by, m12, m20, bz, m22) / d0;
+  const T rz = det3(m00, m01, bx, m10, m11, by, m20, m21, bz) / d0;
+  const T px = bpos[2 * i] - c.qpos[2 * i], py = bpos[2 * i + 1] - c.qpos[2 * i + 1];
+  const T out = rx + (px * ry + py * rz);
+  return out < rho0 ? rho0 : out;
+}
+
 // mDBCDensity (v2/modules/mDBC.py:50-136); rows of `c` are ghosts.
+// variant 1: the C++ op (mdbc_op_row).
 template <class T>
 void mdbc(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, const T* bpos, T rho0, T support,
-          T* rho_b, T* shep_out) {
+          T* rho_b, T* shep_out, int variant) {
   set_threads();
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < c.nq; ++i) {
+    if (variant == 1) {
+      rho_b[i] = mdbc_op_row(c, d, i, mass, rho, bpos, rho0, shep_out);
+      continue;
+    }
     T nom = 0, vol = 0, gsx = 0, gsy = 0, vgx = 0, vgy = 0, psx = 0, psy = 0, p00 = 0, p01 = 0, p10 = 0, p11 = 0;
     const int cnt = c.off[i + 1] - c.off[i];
     for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
@@ -1085,11 +1143,11 @@ ORC_RHS(double, f64)
 #define ORC_MDBC(T, SUF)                                                                                         \
   int orc_mdbc_##SUF(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const T* qpos,   \
                      const T* rpos, T h0, const T* mass, const T* rho, const T* bpos, T rho0, T support, T* rho_b, \
-                     T* shep) {                                                                                  \
+                     T* shep, int32_t variant) {                                                                 \
     const T* qh = nullptr;                                                                                       \
     const T* rh = nullptr;                                                                                       \
     int32_t mode = 0;                                                                                            \
-    mdbc<T>(ORC_CSR(T), DomT<T>(*dom), mass, rho, bpos, rho0, support, rho_b, shep);                              \
+    mdbc<T>(ORC_CSR(T), DomT<T>(*dom), mass, rho, bpos, rho0, support, rho_b, shep, variant);                     \
     return 0;                                                                                                    \
   }
 ORC_MDBC(float, f32)

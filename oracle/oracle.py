@@ -427,8 +427,11 @@ def boundary_ghosts(bpos, dist, normals, h):
     return (bpos + md[:, None] * normals).astype(np.float32)
 
 
-def mdbc(dom, off, nbr, ghost_pos, fluid_pos, h0, mass, rho, bpos, rho0, support, dtype=np.float32):
-    """mDBCDensity (v2/modules/mDBC.py:50-136): (wall density, Shepard)."""
+def mdbc(dom, off, nbr, ghost_pos, fluid_pos, h0, mass, rho, bpos, rho0, support, dtype=np.float32, variant=0):
+    """(wall density, Shepard density).  variant 0: mDBCDensity
+    (v2/modules/mDBC.py:50-136; pinv solve, no clamp); variant 1: the C++
+    mDBC op (mDBCDensityKernel, v2/cppSrc/mDBC.h:129-244; Cramer's rule,
+    Shepard for <= 5 neighbors, clamped at rho0)."""
     cv = f32 if dtype == np.float32 else f64
     hc = C.c_float if dtype == np.float32 else C.c_double
     gp, fp, mass, rho, bpos = cv(ghost_pos), cv(fluid_pos), cv(mass), cv(rho), cv(bpos)
@@ -437,7 +440,7 @@ def mdbc(dom, off, nbr, ghost_pos, fluid_pos, h0, mass, rho, bpos, rho0, support
     shep = np.zeros(nb, dtype)
     fn = lib().orc_mdbc_f32 if dtype == np.float32 else lib().orc_mdbc_f64
     fn(C.byref(dom), C.c_int64(nb), _p(off), _p(nbr), _p(gp), _p(fp), hc(h0), _p(mass), _p(rho), _p(bpos), hc(rho0),
-       hc(support), _p(out), _p(shep))
+       hc(support), _p(out), _p(shep), C.c_int32(variant))
     return out, shep
 
 
@@ -480,9 +483,11 @@ class WallState:
     visc_coeff: float = 0.01
     eps: float = 1e-6
     pi_switch: bool = False
+    mdbc_variant: int = 0
 
 
-def make_walls(dom, pos, dist, normal, area, rho0, h, viscosity=0, visc_coeff=0.01, eps=1e-6, pi_switch=False):
+def make_walls(dom, pos, dist, normal, area, rho0, h, viscosity=0, visc_coeff=0.01, eps=1e-6, pi_switch=False,
+               mdbc_variant=0):
     g = grid_build(dom, pos, h)
     si = g.sort_index
     p = g.sorted_pos.copy()
@@ -490,7 +495,7 @@ def make_walls(dom, pos, dist, normal, area, rho0, h, viscosity=0, visc_coeff=0.
     n = p.shape[0]
     mass = (np.broadcast_to(f32(area), (n,))[si] * np.float32(rho0)).astype(np.float32)
     return WallState(p, d, nrm, mass, np.zeros((n, 2), np.float32), boundary_ghosts(p, d, nrm, h), h, viscosity,
-                     visc_coeff, eps, pi_switch)
+                     visc_coeff, eps, pi_switch, mdbc_variant)
 
 
 def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float32, walls: WallState | None = None):
@@ -514,7 +519,8 @@ def deltaplus_rhs(dom, pos, vel, rho, mass, h, prm: DeltaParams, dtype=np.float3
         nbr_b = gw.sort_index[nbr_bs].astype(np.int32)
         _, off_g, nbr_gs = neighbors_cell(dom, g, walls.ghost, h0=h)
         nbr_g = g.sort_index[nbr_gs].astype(np.int32)
-        rb, shep = mdbc(dom, off_g, nbr_g, walls.ghost, wpos, h, mass, rho, walls.pos, prm.rho0, h, dtype)
+        rb, shep = mdbc(dom, off_g, nbr_g, walls.ghost, wpos, h, mass, rho, walls.pos, prm.rho0, h, dtype,
+                        walls.mdbc_variant)
         wall = WallSet(off_b, nbr_b, walls.pos, walls.mass, rb, walls.vel, h, walls.viscosity, walls.visc_coeff,
                        walls.eps, walls.pi_switch)
         extras.update(wall_rho=rb, shepard=shep)

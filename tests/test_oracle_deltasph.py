@@ -313,3 +313,56 @@ def test_wall_terms_hydrostatic_balance_sign():
     bottom = (fluid[:, 1] < 1.5 * dx) & (fluid[:, 0] > h) & (fluid[:, 0] < 1 - h)
     assert np.all(t_w[bottom, 1] - t_0[bottom, 1] > 0)
     assert np.all(t_w[:, 4] == 0) and np.all(t_w[:, 2:4] == 0)
+
+
+def test_mdbc_op_variant_cramer_matches_pinv_and_clamps():
+    """The C++ mDBC op variant (Cramer's rule, det + 1e-7, clamp at rho0):
+    on a linear field that rises toward the wall it extrapolates exactly
+    (like the pinv variant); on a smooth non-linear field the two solution
+    methods agree to rounding; a field falling toward the wall is clamped
+    at rho0 where the unclamped variant goes below; <= 5 neighbors give
+    max(Shepard, rho0)."""
+    fluid, wall, normals, dist, dx, h = _half_plane()
+    ghosts = orc.boundary_ghosts(wall, dist, normals, h)
+    dom = orc.make_domain((-0.2, -0.2), (1.2, 1.2), (False, False))
+    _, off, nbr = orc.neighbors_brute(dom, ghosts, fluid, h0=h)
+    cnt = np.diff(off)
+    inner = (wall[:, 0] > h) & (wall[:, 0] < 1 - h) & (cnt > 5)
+    mass = np.full(fluid.shape[0], dx * dx)
+    f64 = np.float64
+    a, b = 1.05, np.array([0.03, -0.1])
+    rho = a + fluid.astype(f64) @ b
+    exact = a + wall.astype(f64) @ b
+    assert np.all(exact > 1.0)
+    r1, s1 = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho, wall, 1.0, h, f64, variant=1)
+    r0, s0 = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho, wall, 1.0, h, f64, variant=0)
+    # Cramer's det + 1e-7 scales the solution by det / (det + 1e-7) (det of
+    # A_g ~ 0.1 here): a relative bias of ~1e-6 toward zero
+    err = r1[inner] - exact[inner]
+    assert np.all(err < 0) and np.abs(err).max() < 2e-6
+    np.testing.assert_allclose(r0[inner], exact[inner], rtol=0, atol=5e-7)
+    np.testing.assert_array_equal(s1, s0)
+    # smooth non-linear field: Cramer vs pseudo-inverse
+    x = fluid.astype(f64)
+    rho_nl = 1.1 + 0.05 * np.sin(3 * x[:, 0]) * np.cos(2 * x[:, 1]) - 0.02 * x[:, 1] ** 2
+    r1, _ = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho_nl, wall, 1.0, h, f64, variant=1)
+    r0, _ = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho_nl, wall, 1.0, h, f64, variant=0)
+    assert np.all(r0[inner] > 1.0)
+    np.testing.assert_allclose(r1[inner], r0[inner], rtol=2e-6, atol=0)
+    # falling toward the wall: clamped
+    rho_dn = 1.0 + 0.2 * x[:, 1]
+    r1, _ = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho_dn, wall, 1.0, h, f64, variant=1)
+    r0, _ = orc.mdbc(dom, off, nbr, ghosts, fluid, h, mass, rho_dn, wall, 1.0, h, f64, variant=0)
+    assert np.all(r0[inner] < 1.0) and np.all(r1[inner] == 1.0)
+    # few neighbors: max(Shepard, rho0)
+    off_f = np.array([0, 3, 6], np.int32)
+    nbr_f = np.concatenate([nbr[off[3]:off[3] + 3], nbr[off[4]:off[4] + 3]]).astype(np.int32)
+    for val, want in ((1.3, 1.3), (0.9, 1.0)):
+        rr = np.full(fluid.shape[0], val)
+        rb, sh = orc.mdbc(dom, off_f, nbr_f, ghosts[[3, 4]], fluid, h, mass, rr, wall[[3, 4]], 1.0, h, f64,
+                          variant=1)
+        # Shepard's +1e-7 on a 3-neighbor volume sum (~0.1): ~1e-6 relative
+        np.testing.assert_allclose(sh, val, rtol=5e-6)
+        np.testing.assert_allclose(rb, max(want, 1.0), rtol=5e-6)
+        if want == 1.0:
+            assert np.all(rb == 1.0)
