@@ -201,6 +201,12 @@ __global__ void __launch_bounds__(kBlock) k_bgather2(int n, const int32_t* __res
   dst[p] = src[si[p]];
 }
 
+// A candidate: sorted wrapped position and the state index it maps to.
+struct BCand {
+  float2 x;
+  int idx;
+};
+
 // Row i (state order) of the fixed-width sliced ELL: entry k at
 // (i/64)*64*width + 64k + i%64, candidates in the single-domain search's
 // order (cell rows dy = -1, 0, 1; cells dx = -1, 0, 1, a non-wrapping row as
@@ -228,15 +234,15 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
     const int ncx = g.nc[0], ncy = g.nc[1];
     const int cx = bcell(qi.x, d, 0, g), cy = bcell(qi.y, d, 1, g);
     auto visit = [&](int jb, int je) {
-      for (int j = jb; j < je; ++j) {
-        const float2 rj = spos[j];
-        const float dx = mod_offset(qi.x - rj.x, d, 0);
-        const float dy = mod_offset(qi.y - rj.y, d, 1);
-        if (within_support(dist2_exact(dx, dy), h)) {
-          if (total < width) nbr[base + (int64_t)total * 64] = si ? si[j] : j;
-          ++total;
-        }
-      }
+      scan_range<4>(jb, je, [&](int j) { return BCand{spos[j], si ? si[j] : j}; },
+                    [&](int, const BCand& c) {
+                      const float dx = mod_offset(qi.x - c.x.x, d, 0);
+                      const float dy = mod_offset(qi.y - c.x.y, d, 1);
+                      if (within_support(dist2_exact(dx, dy), h)) {
+                        if (total < width) nbr[base + (int64_t)total * 64] = c.idx;
+                        ++total;
+                      }
+                    });
     };
     for (int oy = -1; oy <= 1; ++oy) {
       int yy = cy + oy;

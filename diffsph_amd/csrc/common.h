@@ -190,6 +190,46 @@ __device__ __forceinline__ RowRef row_ref(const PairsDev& P, int i) {
   return r;
 }
 
+// body(j, load(j)) for j = b .. e-1 in order; the loads of U consecutive j
+// are issued before the first body of the group, so U gathers are in
+// flight at once.  The order of the bodies (and of any sums they form) is
+// that of the plain loop.
+template <int U, class Load, class Body>
+__device__ __forceinline__ void scan_range(int b, int e, Load&& load, Body&& body) {
+  int j = b;
+  for (; j + U <= e; j += U) {
+    decltype(load(j)) d[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = load(j + u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) body(j + u, d[u]);
+  }
+  for (; j < e; ++j) body(j, load(j));
+}
+
+// body(j, load(j)) over the entries j of a neighbor row, in row order, with
+// U entries and their gathers in flight (Stream: the entries are read with
+// non-temporal loads, for lists read once).
+template <int U, bool Stream, class Load, class Body>
+__device__ __forceinline__ void scan_row(const int32_t* __restrict__ nbr, const RowRef& rr, Load&& load,
+                                         Body&& body) {
+  int n = 0, k = rr.base;
+  for (; n + U <= rr.cnt; n += U, k += U * rr.step) {
+    int j[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = Stream ? __builtin_nontemporal_load(nbr + k + u * rr.step) : nbr[k + u * rr.step];
+    decltype(load(0)) d[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) d[u] = load(j[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) body(j[u], d[u]);
+  }
+  for (; n < rr.cnt; ++n, k += rr.step) {
+    const int j = nbr[k];
+    body(j, load(j));
+  }
+}
+
 __device__ __forceinline__ float h_of(const float* arr, int idx, float h0) {
   return arr ? arr[idx] : h0;
 }

@@ -64,6 +64,16 @@ inline BoundaryDev make_boundary(const dsph_boundary* b) {
   return B;
 }
 
+// Gathered neighbor data of the pair loops.
+struct NbrXMR {
+  float2 x;
+  float m, r;
+};
+struct NbrRhs {
+  float2 x, u, g;
+  float r, m;
+};
+
 // The periodic box a particle lives in: one for a single domain, or one per
 // particle (its domain's) for a batch of independent domains.
 struct OneDomain {
@@ -94,11 +104,17 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   const float hi = h_of(P.qh, i, P.h0);
   const RowRef rr = row_ref(P, i);
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
+  auto ld = [&](int j) {
+    NbrXMR d;
+    d.x = P.rpos[j];
+    d.m = mass[j];
+    d.r = rho[j];
+    return d;
+  };
+  scan_row<4, false>(P.nbr, rr, ld, [&](int j, const NbrXMR& d) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
-    const float Vj = mass[j] / rho[j];
+    const Geom g = pair_geom(xi, d.x, h, dm);
+    const float Vj = d.m / d.r;
     const float s = -(g.q * g.h);  // -(distances * supports)
     const float qx = s * g.ex, qy = s * g.ey;
     const float kx = Vj * g.gx, ky = Vj * g.gy;
@@ -106,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
     m01 += qx * ky;
     m10 += qy * kx;
     m11 += qy * ky;
-  }
+  });
   if (B.on) {
     const RowRef br = row_ref(B.P, i);
     float b00 = 0.f, b01 = 0.f, b10 = 0.f, b11 = 0.f;
@@ -139,18 +155,17 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   if (ev_out) ev_out[i] = make_float2(Lm.ev0, Lm.ev1);
   const float rho_i = rho[i];
   float ax = 0.f, ay = 0.f;
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
+  scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrXMR& e) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
-    const float Vj = mass[j] / rho[j];
+    const Geom g = pair_geom(xi, e.x, h, dm);
+    const float Vj = e.m / e.r;
     const float nx = Lm.l00 * g.gx + Lm.l01 * g.gy;
     const float ny = Lm.l10 * g.gx + Lm.l11 * g.gy;
     const float kx = Vj * nx, ky = Vj * ny;
-    const float d = rho_i - rho[j];
+    const float d = rho_i - e.r;
     ax += d * kx;
     ay += d * ky;
-  }
+  });
   if (B.on) {
     const RowRef br = row_ref(B.P, i);
     float bx = 0.f, by = 0.f;
@@ -191,18 +206,26 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, 
   const float p_i = eos_p(rho_i, A.rho0, A.eos_c);
   float accD = 0.f, accVx = 0.f, accVy = 0.f, accC = 0.f, accPx = 0.f, accPy = 0.f;
   const RowRef rr = row_ref(P, i);
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
+  auto ld = [&](int j) {
+    NbrRhs d;
+    d.x = P.rpos[j];
+    d.u = A.vel[j];
+    d.r = A.rho[j];
+    d.m = A.mass[j];
+    d.g = A.diffusion ? A.grad_rho[j] : make_float2(0.f, 0.f);
+    return d;
+  };
+  scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrRhs& e) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, P.rpos[j], h, dm);
-    const float2 uj = A.vel[j];
-    const float rho_j = A.rho[j];
-    const float m_j = A.mass[j];
+    const Geom g = pair_geom(xi, e.x, h, dm);
+    const float2 uj = e.u;
+    const float rho_j = e.r;
+    const float m_j = e.m;
     const float Vj = m_j / rho_j;
     const float kx = Vj * g.gx, ky = Vj * g.gy;
     const float rij = g.q * g.h;  // distances * supports
     if (A.diffusion) {
-      const float2 grj = A.grad_rho[j];
+      const float2 grj = e.g;
       const float r_ij = (2.0f * (rho_j - rho_i)) / (rij + 1e-6f * g.h);
       const float psx = (-r_ij) * g.ex + (gri.x + grj.x);
       const float psy = (-r_ij) * g.ey + (gri.y + grj.y);
@@ -228,7 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, 
       accPx += ps * kx;
       accPy += ps * ky;
     }
-  }
+  });
   float apx = (-accPx) / rho_i, apy = (-accPy) / rho_i;
   float vsx = 0.f, vsy = 0.f;
   if (A.visc_scheme == 1) {

@@ -101,6 +101,12 @@ struct SearchArgs {
   int nq;
 };
 
+// A candidate reference particle: wrapped position and support.
+struct Cand {
+  float2 x;
+  float h;
+};
+
 __device__ __forceinline__ bool keep_pair(float2 qi, float hi, float2 rj, float hj, const SearchArgs& a) {
   float dx = mod_offset(qi.x - rj.x, a.d, 0);
   float dy = mod_offset(qi.y - rj.y, a.d, 1);
@@ -143,7 +149,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
   float hi = h_of(a.qh, i, a.h0);
   int c = 0;
   for_candidates(a, qi, [&](int b, int e) {
-    for (int j = b; j < e; ++j) c += keep_pair(qi, hi, a.rpos[j], h_of(a.rh, j, a.h0), a) ? 1 : 0;
+    scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
+                  [&](int, const Cand& r) { c += keep_pair(qi, hi, r.x, r.h, a) ? 1 : 0; });
   });
   counts[i] = c;
 }
@@ -172,13 +179,14 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   }
   int64_t n = 0;
   for_candidates(a, qi, [&](int b, int e) {
-    for (int j = b; j < e; ++j) {
-      if (keep_pair(qi, hi, a.rpos[j], h_of(a.rh, j, a.h0), a)) {
-        const int64_t o = base + n * step;
-        if (n < len && o < cap) nbr[o] = j;
-        ++n;
-      }
-    }
+    scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
+                  [&](int j, const Cand& r) {
+                    if (keep_pair(qi, hi, r.x, r.h, a)) {
+                      const int64_t o = base + n * step;
+                      if (n < len && o < cap) nbr[o] = j;
+                      ++n;
+                    }
+                  });
   });
   for (; n < len; ++n) {
     const int64_t o = base + n * step;
