@@ -54,7 +54,7 @@ class DfsphParams(C.Structure):
         ("clamp_pressure", C.c_int32),
         ("source_density", C.c_int32),
         ("check_every", C.c_int32),
-        ("pad_", C.c_int32),
+        ("scheme", C.c_int32),
     ]
 
 
@@ -114,6 +114,15 @@ class RK4Stages(C.Structure):
         ("dxdt", C.c_void_p * 4),
         ("dudt", C.c_void_p * 4),
         ("drhodt", C.c_void_p * 4),
+    ]
+
+
+class BicgstabStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("status", C.c_int32),
+        ("residual_norm", C.c_double),
+        ("rhs_norm", C.c_double),
     ]
 
 
@@ -205,6 +214,15 @@ SIGNATURES = {
     "dsph_max_norm_workspace": (SZ, []),
     "dsph_max_norm2": (C.c_int, [V, I64, P(C.c_float), V, SZ, V]),
     "dsph_sum_squares": (C.c_int, [V, I64, P(C.c_double), V, SZ, V]),
+    "dsph_isph_laplacian": (C.c_int, [P(Pairs), V, V, F32, V, V, V, V]),
+    "dsph_spmv": (C.c_int, [P(Pairs), V, V, V, V]),
+    "dsph_bicgstab_workspace": (SZ, [I64]),
+    "dsph_bicgstab": (C.c_int, [P(Pairs), V, V, V, V, F32, C.c_double, C.c_double, I32, I32, I32,
+                                P(BicgstabStats), V, SZ, V]),
+    "dsph_isph_source_workspace": (SZ, [I64]),
+    "dsph_isph_source": (C.c_int, [P(Pairs), V, V, V, V, F32, F32, V, V, SZ, V]),
+    "dsph_isph_jacobi_init": (C.c_int, [I64, V, F32, V, V, V, V]),
+    "dsph_isph_close": (C.c_int, [P(Pairs), V, V, V, V, V, V, F32, F32, V, V, V, V, V]),
     "dsph_batch_workspace": (SZ, [P(Domain), P(I64), I32, F32, I32]),
     "dsph_batch_init": (C.c_int, [P(Domain), P(I64), I32, F32, F32, I32, I32, V, SZ, P(Batch)]),
     "dsph_batch_step": (C.c_int, [P(Batch), P(DeltaSPHParams), P(TimestepParams), V, V, V, V, V, V, V, I32, V]),

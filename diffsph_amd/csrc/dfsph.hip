@@ -181,17 +181,21 @@ __device__ __forceinline__ float2 ld_stream(const float2* p) {
 // sum is accumulated exactly as the one-at-a-time loop does).
 constexpr int kUnroll = 4;
 
-// Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij.
+// Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij;
+// with `diff` the ISPH form (incompressible.py:276-285, gradient
+// 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
 // `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
 // else evaluated on the fly.
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float2* __restrict__ kv,
                                                           const float* __restrict__ vol,
                                                           const float* __restrict__ rho, const float* __restrict__ p,
-                                                          float2* __restrict__ acc, const SolveCtrl* ctrl) {
+                                                          float2* __restrict__ acc, const SolveCtrl* ctrl, int diff) {
   if (ctrl && ctrl->done) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float pi = p[i];
+  // -p_j turns the sum into the difference form; negation is exact
+  const float sj = diff ? -1.0f : 1.0f;
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
   if (kv) {
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
         c[u] = ld_stream(kv + k + u * rr.step);
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) pj[u] = p[j[u]];
+      for (int u = 0; u < kUnroll; ++u) pj[u] = sj * p[j[u]];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const float s = pi + pj[u];
@@ -217,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
     for (; n < rr.cnt; ++n, k += rr.step) {
       const int j = ld_stream(P.nbr + k);
       const float2 c = ld_stream(kv + k);
-      const float s = pi + p[j];
+      const float s = pi + sj * p[j];
       ax += s * c.x;
       ay += s * c.y;
     }
@@ -227,7 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
       const float2 c = kv_of(P, nullptr, k, j, xi, hi, vol);
-      const float s = pi + p[j];
+      const float s = pi + sj * p[j];
       ax += s * c.x;
       ay += s * c.y;
     }
@@ -252,6 +256,7 @@ struct SweepBArgs {
   float neg_thr;
   int clamp;
   int n;
+  int isph;  // error metric |r| (incompressible.py:357) instead of max(r/rho0, -thr)
 };
 
 // Sweep B (dfsph.py:39-50): K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij,
@@ -306,8 +311,12 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
     B.p[i] = pn;
     if (B.residual) B.residual[i] = res;
-    const float rr0 = res / B.rho0;
-    err = (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
+    if (B.isph) {
+      err = (double)fabsf(res);
+    } else {
+      const float rr0 = res / B.rho0;
+      err = (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
+    }
   }
   double bs = block_sum_d(err, sh);
   if (threadIdx.x == 0) B.partials[blockIdx.x] = bs;
@@ -483,6 +492,7 @@ SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const flo
   B.neg_thr = -prm->error_threshold;
   B.clamp = prm->clamp_pressure;
   B.n = (int)n;
+  B.isph = prm->scheme == 1;
   return B;
 }
 
@@ -580,7 +590,8 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     int k = chunk;
     if (launched + k > prm->max_iters) k = prm->max_iters - launched;
     for (int t = 0; t < k; ++t) {
-      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl,
+                                             B.isph);
       k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
       k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
     }
@@ -595,7 +606,7 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     DSPH_CHECK_HIP(hipStreamSynchronize(st));
   }
   // Final pressure acceleration (dfsph.py:105).
-  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
+  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, B.isph);
   DSPH_LAUNCH_CHECK();
   if (stats) {
     stats->iterations = host.iter;
@@ -615,7 +626,7 @@ extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, 
   if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, reinterpret_cast<const float2*>(kgrad), vol, rho, p, reinterpret_cast<float2*>(accel), nullptr);
+      P, reinterpret_cast<const float2*>(kgrad), vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0);
   return launch_status();
 }
 
@@ -657,7 +668,7 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, iterations, iterations, 0.0);
   SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);
   for (int t = 0; t < iterations; ++t) {
-    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl);
+    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl, B.isph);
     k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
     k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
   }

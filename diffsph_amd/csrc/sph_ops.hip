@@ -84,7 +84,8 @@ __global__ void __launch_bounds__(kBlock) k_sph_operation(PairsDev P, int op, in
         acc[0] += (qi[0] * iri + qj[0] * irj) * kx + (qi[W > 1 ? 1 : 0] * iri + qj[W > 1 ? 1 : 0] * irj) * ky;
       }
     } else {
-      const float V = rm[j] / rr[j];
+      // divergenceConsistent (v2/sphOps.py:108-149) weighs with m_j / rho_i
+      const float V = op == DSPH_OP_DIVERGENCE_CONSISTENT ? rm[j] / rho_i : rm[j] / rr[j];
       const float kx = V * g.gx, ky = V * g.gy;
       if (op == DSPH_OP_GRADIENT) {
 #pragma unroll
@@ -309,9 +310,10 @@ extern "C" int dsph_sph_operation(const dsph_pairs* pairs, int32_t op, int32_t g
                                   const float* rq, int32_t width, float* out, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
-  if (op < 0 || op > 2 || grad_mode < 0 || grad_mode > 3) return DSPH_ERR_ARG;
+  if (op < 0 || op > 3 || grad_mode < 0 || grad_mode > 3) return DSPH_ERR_ARG;
   if (width < 1 || width > 2) return DSPH_ERR_DIM;
-  if (op == DSPH_OP_DIVERGENCE && width != 2) return DSPH_ERR_DIM;
+  if ((op == DSPH_OP_DIVERGENCE || op == DSPH_OP_DIVERGENCE_CONSISTENT) && width != 2) return DSPH_ERR_DIM;
+  if (op == DSPH_OP_DIVERGENCE_CONSISTENT && !qrho) return DSPH_ERR_ARG;
   if (pairs->nq == 0) return DSPH_OK;
   if (!rmass || !rrho || !rq || !out) return DSPH_ERR_ARG;
   const bool needs_qi = op != DSPH_OP_INTERPOLATE && grad_mode != DSPH_GRAD_NAIVE;

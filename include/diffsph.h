@@ -214,9 +214,11 @@ int dsph_density(const dsph_pairs* pairs, const float* rmass,
  * op = 0 interpolate  sum_j m_j/rho_j q_j W_ij
  *      1 gradient     (mode: 0 naive, 1 difference, 2 summation, 3 symmetric)
  *      2 divergence   (mode as above; q is [n,2])
+ *      3 divergenceConsistent (sphOps.py:108-149): the divergence with
+ *        m_j / rho_i in place of m_j / rho_j (needs qrho)
  * `width` = components of q (1 or 2).  out is [nq, width] (interpolate),
  * [nq, width, 2] (gradient), [nq] (divergence). */
-enum { DSPH_OP_INTERPOLATE = 0, DSPH_OP_GRADIENT = 1, DSPH_OP_DIVERGENCE = 2 };
+enum { DSPH_OP_INTERPOLATE = 0, DSPH_OP_GRADIENT = 1, DSPH_OP_DIVERGENCE = 2, DSPH_OP_DIVERGENCE_CONSISTENT = 3 };
 enum { DSPH_GRAD_NAIVE = 0, DSPH_GRAD_DIFFERENCE = 1, DSPH_GRAD_SUMMATION = 2, DSPH_GRAD_SYMMETRIC = 3 };
 int dsph_sph_operation(const dsph_pairs* pairs, int32_t op, int32_t grad_mode,
                        const float* qmass, const float* rmass,
@@ -238,7 +240,11 @@ typedef struct dsph_dfsph_params {
   int32_t clamp_pressure; /* p = max(p, 0) after every update */
   int32_t source_density; /* 1: density source, 0: divergence source */
   int32_t check_every;    /* iterations launched between host checks (0 = 8) */
-  int32_t pad_;
+  int32_t scheme;         /* 0 DFSPH (v2/modules/dfsph.py); 1 ISPH relaxed Jacobi
+                             (solveIncompressible_relaxedJacobi,
+                             v2/modules/incompressible.py:308-364): pressure
+                             gradient 'difference', error mean(|r|); the caller
+                             passes alpha/dt, omega 0.5, no clamp, density source */
 } dsph_dfsph_params;
 
 typedef struct dsph_solve_stats {
@@ -519,6 +525,73 @@ int dsph_max_norm2(const float* v, int64_t n, float* out_host, void* workspace,
  * i.e. sqrt of this sum.  Writes *out_host (synchronises the stream). */
 int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* workspace,
                      size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* ISPH (v2/modules/incompressible.py, v2/sparse.py)                        */
+/* ---------------------------------------------------------------------- */
+
+/* Pressure Laplacian of projectVelocities (computeLaplacian,
+ * v2/modules/incompressible.py:102-133), one value per list entry in the
+ * list's own layout (values[k] belongs to nbr[k]): off-diagonal
+ * t_ij = ((2 dt) (m_j / rho_j)) (|gradW_ij| / (q_ij h + 1e-7 h)), the
+ * row's self entry -a_ii with a_ii = sum_j t_ij.  aii [n] and the Jacobi
+ * preconditioner precond = -1 / a_ii (incompressible.py:178) may be NULL.
+ * Replaces the COO tuple (A_, (i, j), n). */
+int dsph_isph_laplacian(const dsph_pairs* pairs, const float* mass, const float* rho,
+                        float dt, float* values, float* aii, float* precond, void* stream);
+
+/* y = A x with A in the list layout (matvec_sparse_coo, v2/sparse.py:8-10). */
+int dsph_spmv(const dsph_pairs* pairs, const float* values, const float* x, float* y,
+              void* stream);
+
+typedef struct dsph_bicgstab_stats {
+  int32_t iterations;      /* completed iterations */
+  int32_t status;          /* 0 |r| < atol, 1 |s| < atol (both converged), -10 rho
+                              breakdown, -11 omega / (rtilde.v) breakdown, 2 maxiter */
+  double residual_norm;    /* |r| (|s| after the s-test) at the last test (float64) */
+  double rhs_norm;         /* |b| */
+} dsph_bicgstab_stats;
+
+/* Preconditioned BiCGStab of v2/sparse.py:141-230 (the scipy algorithm):
+ * solves A x = b with M = diag(precond), x0 = x0_scale * x (in place),
+ * stopping at |r| < max(atol, rtol |b|), the s-test, a breakdown or
+ * maxiter.  fp64 != 0 runs the vectors in float64 (projectVelocities'
+ * solverPrecision = float64, incompressible.py:173-200: A, b, M and x0
+ * promoted, x rounded back); else float32 as the reference's default.
+ * Dot products are float64 in both cases, with a fixed reduction order;
+ * the loop control runs on the device and the host reads it every
+ * check_every iterations (0 = 8).  Synchronises the stream. */
+size_t dsph_bicgstab_workspace(int64_t n);
+int dsph_bicgstab(const dsph_pairs* pairs, const float* values, const float* precond,
+                  const float* b, float* x, float x0_scale, double rtol, double atol,
+                  int32_t maxiter, int32_t check_every, int32_t fp64,
+                  dsph_bicgstab_stats* stats, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
+/* Source of the divergence-free system (computeDivergenceSourceTerm,
+ * incompressible.py:148-165): B = rho0 * divergenceConsistent('difference')
+ * of v* = vel + adv dt (adv may be NULL), minus its mean (float64 mean). */
+size_t dsph_isph_source_workspace(int64_t n);
+int dsph_isph_source(const dsph_pairs* pairs, const float* vel, const float* adv,
+                     const float* mass, const float* rho, float dt, float rho0, float* B,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Inputs of the relaxed-Jacobi solve (incompressible.py:269, 335-337):
+ * alpha = alpha_den / dt (alpha_den: dsph_dfsph_prepare's density alpha),
+ * p = 0.5 p_prev (p_prev NULL -> 0).  The solve itself is dsph_dfsph_solve
+ * with scheme = 1. */
+int dsph_isph_jacobi_init(int64_t n, const float* alpha_den, float dt, const float* p_prev,
+                          float* alpha, float* p, void* stream);
+
+/* Closing passes of the ISPH step (incompressible.py:398-460):
+ *   a = -grad_summation(p_star) / rho0, v' = v* + a dt (v* = vel_in + adv dt),
+ *   shift = -dt^2 grad_symmetric(p_ss) / rho0,
+ *   vel = v' + grad_difference(v') . shift, pos = pos + v' dt + shift.
+ * vprime, shift [n,2] receive v' and the shift; vel may alias vel_in. */
+int dsph_isph_close(const dsph_pairs* pairs, const float* mass, const float* rho,
+                    const float* vel_in, const float* adv, const float* p_star,
+                    const float* p_ss, float dt, float rho0, float* vprime, float* shift,
+                    float* pos, float* vel, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batched ensemble of independent periodic delta-SPH domains               */

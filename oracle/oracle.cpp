@@ -379,7 +379,8 @@ void sph_operation(const Csr<T>& c, const DomT<T>& d, int op, int gm, const T* q
           acc[0] += (qi[0] * iri + qj[0] * irj) * kx + (qi[1] * iri + qj[1] * irj) * ky;
         }
       } else {
-        T V = rm[j] / rr[j];
+        // op 3: divergenceConsistent (sphOps.py:108-149), m_j / rho_i
+        T V = op == 3 ? rm[j] / rho_i : rm[j] / rr[j];
         T kx = V * g.gx, ky = V * g.gy;
         if (op == 1) {
           for (int cc = 0; cc < W; ++cc) {
@@ -462,8 +463,10 @@ void dfsph_prepare(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* are
 }
 
 // computePressureAcceleration (v2/modules/dfsph.py:35-37).
+// isph: the 'difference' form of solveIncompressible_relaxedJacobi
+// (incompressible.py:276-285), sum_j (p_i - p_j) V_j gradW_ij.
 template <class T>
-void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, const T* p, T* acc) {
+void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, const T* p, T* acc, int isph = 0) {
   set_threads();
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < c.nq; ++i) {
@@ -471,7 +474,7 @@ void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, cons
     for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
       int32_t j = c.nbr[k];
       G<T> g = pg(c, i, j, d);
-      T s = p[i] + p[j];
+      T s = isph ? p[i] - p[j] : p[i] + p[j];
       ax += s * (vol[j] * g.gx);
       ay += s * (vol[j] * g.gy);
     }
@@ -482,9 +485,10 @@ void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, cons
 
 // updatePressure (v2/modules/dfsph.py:39-50); returns sum of
 // clamp(r/rho0, min=-thr) (accumulated in double).
+// isph: error |r| (incompressible.py:357) instead of max(r/rho0, -thr).
 template <class T>
 double sweep_b(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* acc, const T* src, const T* alpha, T dtf,
-               T omega, T rho0, T thr, int clampp, T* p, T* resid) {
+               T omega, T rho0, T thr, int clampp, T* p, T* resid, int isph = 0) {
   double tot = 0.0;
   set_threads();
 #pragma omp parallel for schedule(static) reduction(+ : tot)
@@ -502,7 +506,7 @@ double sweep_b(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* acc, co
     if (clampp) pn = std::max(pn, T(0));
     p[i] = pn;
     if (resid) resid[i] = res;
-    tot += (double)std::max(res / rho0, -thr);
+    tot += isph ? (double)std::fabs(res) : (double)std::max(res / rho0, -thr);
   }
   return tot;
 }
@@ -549,7 +553,7 @@ template <class T>
 int dfsph_solve(const Csr<T>& c, const DomT<T>& d, T dt, T rho0, T omega, T thr, int min_iters, int max_iters,
                 int clampp, int density_src, const T* vel, const T* adv, const T* rho, const T* vol, const T* areas,
                 const T* alpha, const T* src0, T* p, T* acc, T* resid, int32_t* iters_out, double* err_out,
-                double* err0_out) {
+                double* err0_out, int isph = 0) {
   const int64_t n = c.nq;
   std::vector<T> src(n);
   if (density_src || adv)
@@ -562,17 +566,275 @@ int dfsph_solve(const Csr<T>& c, const DomT<T>& d, T dt, T rho0, T omega, T thr,
   int it = 0;
   double err = 0.0, err0 = 0.0;
   while (it < max_iters && (it < min_iters || err > (double)thr)) {
-    sweep_a<T>(c, d, vol, rho, p, acc);
-    double tot = sweep_b<T>(c, d, vol, acc, src.data(), alpha, dtf, omega, rho0, thr, clampp, p, resid);
+    sweep_a<T>(c, d, vol, rho, p, acc, isph);
+    double tot = sweep_b<T>(c, d, vol, acc, src.data(), alpha, dtf, omega, rho0, thr, clampp, p, resid, isph);
     err = tot / (double)n;
     if (it == 0) err0 = err;
     ++it;
   }
-  sweep_a<T>(c, d, vol, rho, p, acc);
+  sweep_a<T>(c, d, vol, rho, p, acc, isph);
   *iters_out = it;
   *err_out = err;
   *err0_out = err0;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// ISPH (v2/modules/incompressible.py, v2/sparse.py).
+// ---------------------------------------------------------------------------
+
+// The GPU's fixed float64 reduction: rows in blocks of 256; in a block each
+// 64-lane wave adds by the shuffle-down tree (lane l += lane l+o for
+// o = 32..1) and the four wave sums are added in order; then one block adds
+// the block partials: lane t sums partials t, t+256, ... in order and the
+// same block tree combines the lanes.
+static double block_tree256(double* v) {
+  double t = 0.0;
+  for (int w = 0; w < 4; ++w) {
+    double* a = v + 64 * w;
+    for (int o = 32; o > 0; o >>= 1)
+      for (int l = 0; l < o; ++l) a[l] = a[l] + a[l + o];
+    t += a[0];
+  }
+  return t;
+}
+
+static double final_sum_tree(const std::vector<double>& part) {
+  double lanes[256];
+  for (int t = 0; t < 256; ++t) {
+    double v = 0.0;
+    for (size_t k = t; k < part.size(); k += 256) v += part[k];
+    lanes[t] = v;
+  }
+  return block_tree256(lanes);
+}
+
+// sum_i f(i) over n rows with the GPU's tree.
+template <class F>
+double tree_sum(int64_t n, F&& f) {
+  const int64_t nb = (n + 255) / 256;
+  std::vector<double> part((size_t)(nb > 0 ? nb : 1), 0.0);
+  for (int64_t b = 0; b < nb; ++b) {
+    double v[256];
+    for (int t = 0; t < 256; ++t) {
+      const int64_t i = b * 256 + t;
+      v[t] = i < n ? f(i) : 0.0;
+    }
+    part[b] = block_tree256(v);
+  }
+  return final_sum_tree(part);
+}
+
+// computeLaplacian (incompressible.py:102-133); values in CSR order.
+template <class T>
+void isph_laplacian(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, T c2dt, T* val, T* aii, T* pre) {
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T a = 0;
+    int64_t self_k = -1;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      const int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T gx2 = g.gx * g.gx, gy2 = g.gy * g.gy;
+      const T gn = std::sqrt(gx2 + gy2);
+      const T quot = g.q * g.h + (T)1e-7 * g.h;
+      const T t = (c2dt * (mass[j] / rho[j])) * (gn / quot);
+      val[k] = t;
+      a += t;
+      if (j == i) self_k = k;
+    }
+    if (self_k >= 0) val[self_k] = -a;
+    aii[i] = a;
+    pre[i] = T(-1) / a;
+  }
+}
+
+template <class T>
+void spmv_csr(const Csr<T>& c, const T* val, const T* x, T* y) {
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T acc = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) acc += val[k] * x[c.nbr[k]];
+    y[i] = acc;
+  }
+}
+
+// BiCGStab (sparse.py:141-230) in the GPU's arithmetic: float32 vectors,
+// float64 dots and scalars, the scalars rounded to float32 where they
+// scale a vector.  Returns the status; *iters the completed iterations.
+template <class V>
+void spmv_promote(const Csr<float>& c, const float* val, const V* x, V* y) {
+  for (int64_t i = 0; i < c.nq; ++i) {
+    V acc = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) acc += (V)val[k] * x[c.nbr[k]];
+    y[i] = acc;
+  }
+}
+
+// Vectors of type T (float, or double for solverPrecision = float64); the
+// float inputs are promoted and x is rounded back.
+template <class T>
+int bicgstab(const Csr<float>& c, const float* val, const float* pre_f, const float* b_f, float* x_f, float x0_scale,
+             double rtol, double atol, int maxiter, int32_t* iters, double* rnorm_out, double* bnorm_out) {
+  const int64_t n = c.nq;
+  auto dot = [&](const T* u, const T* w) { return tree_sum(n, [&](int64_t i) { return (double)u[i] * (double)w[i]; }); };
+  std::vector<T> r(n), rt(n), p(n), ph(n), v(n), sh(n), t(n), ax(n), bv(n), xv(n), prev(n);
+  for (int64_t i = 0; i < n; ++i) {
+    bv[i] = (T)b_f[i];
+    xv[i] = (T)(x_f[i] * x0_scale);
+    prev[i] = (T)pre_f[i];
+  }
+  const T* b = bv.data();
+  const T* pre = prev.data();
+  T* x = xv.data();
+  struct Store {
+    std::vector<T>& xv;
+    float* x_f;
+    ~Store() {
+      for (size_t i = 0; i < xv.size(); ++i) x_f[i] = (float)xv[i];
+    }
+  } store{xv, x_f};
+  spmv_promote<T>(c, val, x, ax.data());
+  for (int64_t i = 0; i < n; ++i) r[i] = rt[i] = b[i] - ax[i];
+  const double bnorm = std::sqrt(dot(b, b));
+  double rr = dot(r.data(), r.data());
+  const double atol2 = std::max(atol, rtol * bnorm);
+  const double rhotol = 1.1920928955078125e-07 * 1.1920928955078125e-07;
+  double rnorm = std::sqrt(rr);
+  *bnorm_out = bnorm;
+  *iters = 0;
+  *rnorm_out = rnorm;
+  if (bnorm == 0.0) {
+    for (int64_t i = 0; i < n; ++i) x[i] = b[i];
+    return 0;
+  }
+  if (maxiter <= 0) return 2;
+  if (rnorm < atol2) return 0;
+  double rho = rr, rho_prev = 0.0, alpha = 0.0, omega = 0.0, beta = 0.0;
+  if (std::fabs(rho) < rhotol) return -10;
+  int it = 0;
+  for (;;) {
+    for (int64_t i = 0; i < n; ++i) {
+      const T pi = it == 0 ? r[i] : ((p[i] - (T)omega * v[i]) * (T)beta) + r[i];
+      p[i] = pi;
+      ph[i] = pre[i] * pi;
+    }
+    spmv_promote<T>(c, val, ph.data(), v.data());
+    const double rv = dot(rt.data(), v.data());
+    if (rv == 0.0) {
+      *iters = it;
+      return -11;
+    }
+    alpha = rho / rv;
+    for (int64_t i = 0; i < n; ++i) {
+      r[i] = r[i] - (T)alpha * v[i];
+      sh[i] = pre[i] * r[i];
+    }
+    const double ss = dot(r.data(), r.data());
+    if (std::sqrt(ss) < atol2) {
+      for (int64_t i = 0; i < n; ++i) x[i] = x[i] + (T)alpha * ph[i];
+      *iters = it;
+      *rnorm_out = std::sqrt(ss);
+      return 1;
+    }
+    spmv_promote<T>(c, val, sh.data(), t.data());
+    omega = dot(t.data(), r.data()) / dot(t.data(), t.data());
+    for (int64_t i = 0; i < n; ++i) {
+      T xi = x[i] + (T)alpha * ph[i];
+      xi = xi + (T)omega * sh[i];
+      x[i] = xi;
+      r[i] = r[i] - (T)omega * t[i];
+    }
+    rr = dot(r.data(), r.data());
+    const double rtr = dot(rt.data(), r.data());
+    rho_prev = rho;
+    ++it;
+    *iters = it;
+    if (it >= maxiter) return 2;
+    rnorm = std::sqrt(rr);
+    *rnorm_out = rnorm;
+    if (rnorm < atol2) return 0;
+    rho = rtr;
+    if (std::fabs(rho) < rhotol) return -10;
+    if (std::fabs(omega) < rhotol) return -11;
+    beta = (rho / rho_prev) * (alpha / omega);
+  }
+}
+
+// computeDivergenceSourceTerm (incompressible.py:148-165).
+template <class T>
+void isph_source(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* adv, const T* mass, const T* rho, T dt,
+                 T rho0, T* B) {
+  const int64_t n = c.nq;
+  for (int64_t i = 0; i < n; ++i) {
+    const T aix = adv ? adv[2 * i] : T(0), aiy = adv ? adv[2 * i + 1] : T(0);
+    const T vix = vel[2 * i] + aix * dt, viy = vel[2 * i + 1] + aiy * dt;
+    T acc = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      const int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T ajx = adv ? adv[2 * j] : T(0), ajy = adv ? adv[2 * j + 1] : T(0);
+      const T vjx = vel[2 * j] + ajx * dt, vjy = vel[2 * j + 1] + ajy * dt;
+      const T V = mass[j] / rho[i];
+      const T kx = V * g.gx, ky = V * g.gy;
+      acc += (vjx - vix) * kx + (vjy - viy) * ky;
+    }
+    B[i] = rho0 * acc;
+  }
+  const T meanf = (T)(tree_sum(n, [&](int64_t i) { return (double)B[i]; }) / (double)n);
+  for (int64_t i = 0; i < n; ++i) B[i] = B[i] - meanf;
+}
+
+// Closing passes of the ISPH step (incompressible.py:398-460).
+template <class T>
+void isph_close(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, const T* vel_in, const T* adv,
+                const T* pstar, const T* pss, T dt, T rho0, T* vprime, T* shift, T* pos, T* vel) {
+  const int64_t n = c.nq;
+  const T neg_dt2 = (T)(-((double)dt * (double)dt));
+  for (int64_t i = 0; i < n; ++i) {
+    const T psi = pstar[i], pqi = pss[i], rhoi = rho[i];
+    const T iri = T(1) / (rhoi * rhoi);
+    T gsx = 0, gsy = 0, gyx = 0, gyy = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      const int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T mj = mass[j], rhoj = rho[j];
+      const T V = mj / rhoj;
+      const T kx = V * g.gx, ky = V * g.gy;
+      const T s = psi + pstar[j];
+      gsx += s * kx;
+      gsy += s * ky;
+      const T irj = T(1) / (rhoj * rhoj);
+      const T a = pqi * iri + pss[j] * irj;
+      gyx += a * (mj * g.gx);
+      gyy += a * (mj * g.gy);
+    }
+    const T ax0 = adv ? adv[2 * i] : T(0), ay0 = adv ? adv[2 * i + 1] : T(0);
+    const T vsx = vel_in[2 * i] + ax0 * dt, vsy = vel_in[2 * i + 1] + ay0 * dt;
+    const T ax = (-gsx) / rho0, ay = (-gsy) / rho0;
+    vprime[2 * i] = vsx + ax * dt;
+    vprime[2 * i + 1] = vsy + ay * dt;
+    shift[2 * i] = (neg_dt2 * (rhoi * gyx)) / rho0;
+    shift[2 * i + 1] = (neg_dt2 * (rhoi * gyy)) / rho0;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    const T vix = vprime[2 * i], viy = vprime[2 * i + 1];
+    T g00 = 0, g01 = 0, g10 = 0, g11 = 0;
+    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
+      const int32_t j = c.nbr[k];
+      G<T> g = pg(c, i, j, d);
+      const T V = mass[j] / rho[j];
+      const T kx = V * g.gx, ky = V * g.gy;
+      const T dx = vix - vprime[2 * j], dy = viy - vprime[2 * j + 1];
+      g00 += dx * kx;
+      g01 += dx * ky;
+      g10 += dy * kx;
+      g11 += dy * ky;
+    }
+    const T sx = shift[2 * i], sy = shift[2 * i + 1];
+    vel[2 * i] = vix + (g00 * sx + g01 * sy);
+    vel[2 * i + 1] = viy + (g10 * sx + g11 * sy);
+    pos[2 * i] = (pos[2 * i] + vix * dt) + sx;
+    pos[2 * i + 1] = (pos[2 * i + 1] + viy * dt) + sy;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,23 +1284,24 @@ int orc_dfsph_solve_f32(const orc_domain* dom, int64_t nq, const int32_t* off, c
                         int32_t min_iters, int32_t max_iters, int32_t clampp, int32_t density_src, const float* vel,
                         const float* adv, const float* rho, const float* vol, const float* areas, const float* alpha,
                         const float* src0, float* p, float* acc, float* resid, int32_t* iters, double* err,
-                        double* err0) {
+                        double* err0, int32_t isph) {
   const float* rpos = qpos;
   const float* rh = qh;
   return dfsph_solve<float>(ORC_CSR(float), DomT<float>(*dom), dt, rho0, omega, thr, min_iters, max_iters, clampp,
-                            density_src, vel, adv, rho, vol, areas, alpha, src0, p, acc, resid, iters, err, err0);
+                            density_src, vel, adv, rho, vol, areas, alpha, src0, p, acc, resid, iters, err, err0,
+                            isph);
 }
 int orc_dfsph_solve_f64(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
                         const double* qpos, const double* qh, double h0, int32_t mode, double dt, double rho0,
                         double omega, double thr, int32_t min_iters, int32_t max_iters, int32_t clampp,
                         int32_t density_src, const double* vel, const double* adv, const double* rho,
                         const double* vol, const double* areas, const double* alpha, const double* src0, double* p,
-                        double* acc, double* resid, int32_t* iters, double* err, double* err0) {
+                        double* acc, double* resid, int32_t* iters, double* err, double* err0, int32_t isph) {
   const double* rpos = qpos;
   const double* rh = qh;
   return dfsph_solve<double>(ORC_CSR(double), DomT<double>(*dom), dt, rho0, omega, thr, min_iters, max_iters,
                              clampp, density_src, vel, adv, rho, vol, areas, alpha, src0, p, acc, resid, iters, err,
-                             err0);
+                             err0, isph);
 }
 
 // Wendland C2 values for a list of (q, h): W and dW/dr (kernelFunctions/
@@ -1152,6 +1415,56 @@ ORC_RHS(double, f64)
   }
 ORC_MDBC(float, f32)
 ORC_MDBC(double, f64)
+
+// ISPH: rows of (off, nbr) over one particle set (qpos = rpos).
+int orc_isph_laplacian_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,
+                           const float* qpos, float h0, const float* mass, const float* rho, float c2dt, float* val,
+                           float* aii, float* pre) {
+  const float* rpos = qpos;
+  const float* qh = nullptr;
+  const float* rh = nullptr;
+  int32_t mode = 0;
+  isph_laplacian<float>(ORC_CSR(float), DomT<float>(*dom), mass, rho, c2dt, val, aii, pre);
+  return 0;
+}
+
+int orc_spmv_f32(int64_t nq, const int32_t* off, const int32_t* nbr, const float* val, const float* x, float* y) {
+  Csr<float> c = make_csr<float>(nq, off, nbr, nullptr, nullptr, nullptr, nullptr, 0.f, 0);
+  spmv_csr<float>(c, val, x, y);
+  return 0;
+}
+
+int orc_bicgstab_f32(int64_t nq, const int32_t* off, const int32_t* nbr, const float* val, const float* pre,
+                     const float* b, float* x, float x0_scale, double rtol, double atol, int32_t maxiter,
+                     int32_t* iters, double* rnorm, double* bnorm, int32_t fp64) {
+  Csr<float> c = make_csr<float>(nq, off, nbr, nullptr, nullptr, nullptr, nullptr, 0.f, 0);
+  if (fp64) return bicgstab<double>(c, val, pre, b, x, x0_scale, rtol, atol, maxiter, iters, rnorm, bnorm);
+  return bicgstab<float>(c, val, pre, b, x, x0_scale, rtol, atol, maxiter, iters, rnorm, bnorm);
+}
+
+int orc_isph_source_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
+                        float h0, const float* vel, const float* adv, const float* mass, const float* rho, float dt,
+                        float rho0, float* B) {
+  const float* rpos = qpos;
+  const float* qh = nullptr;
+  const float* rh = nullptr;
+  int32_t mode = 0;
+  isph_source<float>(ORC_CSR(float), DomT<float>(*dom), vel, adv, mass, rho, dt, rho0, B);
+  return 0;
+}
+
+int orc_isph_close_f32(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr, const float* qpos,
+                       float h0, const float* mass, const float* rho, const float* vel_in, const float* adv,
+                       const float* pstar, const float* pss, float dt, float rho0, float* vprime, float* shift,
+                       float* pos, float* vel) {
+  const float* rpos = qpos;
+  const float* qh = nullptr;
+  const float* rh = nullptr;
+  int32_t mode = 0;
+  isph_close<float>(ORC_CSR(float), DomT<float>(*dom), mass, rho, vel_in, adv, pstar, pss, dt, rho0, vprime, shift,
+                    pos, vel);
+  return 0;
+}
 
 // The 2x2 pseudo-inverse on its own ([n,4] row-major in, L [n,4], ev [n,2]).
 int orc_pinv2x2_f32(const float* M, int64_t n, float* L, float* ev) {

@@ -198,7 +198,11 @@ def dfsph_prepare(dom, off, nbr, pos, vel, areas, vol, rho, dt, rho0, visc: Visc
 
 
 def dfsph_solve(dom, off, nbr, pos, h0, dt, rho0, omega, thr, min_iters, max_iters, clamp, density_src, vel, adv,
-                rho, vol, areas, alpha, src0, p0, dtype=np.float32):
+                rho, vol, areas, alpha, src0, p0, dtype=np.float32, isph=False):
+    """dfsphSolve's loop (v2/modules/dfsph.py:52-109); with `isph` the
+    relaxed Jacobi of solveIncompressible_relaxedJacobi
+    (v2/modules/incompressible.py:308-364): 'difference' pressure gradient,
+    error mean(|r|) (alpha, omega, clamp and warm start from the caller)."""
     n = pos.shape[0]
     cv = f32 if dtype == np.float32 else f64
     hc = C.c_float if dtype == np.float32 else C.c_double
@@ -212,7 +216,7 @@ def dfsph_solve(dom, off, nbr, pos, h0, dt, rho0, omega, thr, min_iters, max_ite
     fn(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(pos), None, hc(h0), C.c_int32(0), hc(dt), hc(rho0),
        hc(omega), hc(thr), C.c_int32(min_iters), C.c_int32(max_iters), C.c_int32(1 if clamp else 0),
        C.c_int32(1 if density_src else 0), _p(vel), _p(adv), _p(rho), _p(vol), _p(areas), _p(alpha), _p(src0),
-       _p(p), _p(acc), _p(res), C.byref(it), C.byref(err), C.byref(err0))
+       _p(p), _p(acc), _p(res), C.byref(it), C.byref(err), C.byref(err0), C.c_int32(1 if isph else 0))
     return p, acc, res, it.value, err.value, err0.value
 
 
@@ -612,3 +616,116 @@ def rk4_step(st: DeltaState, dom, h, prm: DeltaParams, dt: float, reorder=True, 
     dr = avg(*[k[2] for k in ks])
     new = DeltaState(x0 + dx * dtf, u0 + du * dtf, r0 + dr * dtf, st.mass, st.uid)
     return new, du
+
+
+# ---------------------------------------------------------------------------
+# ISPH (v2/modules/incompressible.py, v2/sparse.py)
+# ---------------------------------------------------------------------------
+def isph_laplacian(dom, off, nbr, pos, h0, mass, rho, dt):
+    """computeLaplacian (incompressible.py:102-133): per-entry values in CSR
+    order (the self entry holds -a_ii), a_ii, preconditioner -1/a_ii."""
+    n = pos.shape[0]
+    val = np.zeros(int(off[-1]), np.float32)
+    aii = np.zeros(n, np.float32)
+    pre = np.zeros(n, np.float32)
+    c2dt = float(np.float32(2.0 * dt))
+    lib().orc_isph_laplacian_f32(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(f32(pos)), C.c_float(h0),
+                                 _p(f32(mass)), _p(f32(rho)), C.c_float(c2dt), _p(val), _p(aii), _p(pre))
+    return val, aii, pre
+
+
+def spmv(off, nbr, val, x):
+    n = len(off) - 1
+    y = np.zeros(n, np.float32)
+    lib().orc_spmv_f32(C.c_int64(n), _p(off), _p(nbr), _p(f32(val)), _p(f32(x)), _p(y))
+    return y
+
+
+def bicgstab(off, nbr, val, pre, b, x0, x0_scale=1.0, rtol=1e-3, atol=0.0, maxiter=512, fp64=False):
+    """BiCGStab of v2/sparse.py:141-230 in the GPU's arithmetic (float32
+    vectors, or float64 with `fp64`; float64 dots in the GPU's reduction
+    order).  Returns (x, iterations, status, |r|, |b|); status 0 |r| < atol,
+    1 |s| < atol, -10 / -11 breakdowns, 2 iteration limit."""
+    n = len(off) - 1
+    x = f32(x0).copy()
+    it = C.c_int32()
+    rn, bn = C.c_double(), C.c_double()
+    st = lib().orc_bicgstab_f32(C.c_int64(n), _p(off), _p(nbr), _p(f32(val)), _p(f32(pre)), _p(f32(b)), _p(x),
+                                C.c_float(x0_scale), C.c_double(rtol), C.c_double(atol), C.c_int32(maxiter),
+                                C.byref(it), C.byref(rn), C.byref(bn), C.c_int32(1 if fp64 else 0))
+    return x, it.value, st, rn.value, bn.value
+
+
+def isph_source(dom, off, nbr, pos, h0, vel, adv, mass, rho, dt, rho0):
+    """computeDivergenceSourceTerm (incompressible.py:148-165), mean removed."""
+    n = pos.shape[0]
+    B = np.zeros(n, np.float32)
+    lib().orc_isph_source_f32(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(f32(pos)), C.c_float(h0),
+                              _p(f32(vel)), _p(None if adv is None else f32(adv)), _p(f32(mass)), _p(f32(rho)),
+                              C.c_float(dt), C.c_float(rho0), _p(B))
+    return B
+
+
+def isph_close(dom, off, nbr, wpos, h0, mass, rho, vel, adv, pstar, pss, dt, rho0, pos):
+    """Closing passes of the ISPH step (incompressible.py:398-460); returns
+    (v', shift, new positions, new velocities)."""
+    n = wpos.shape[0]
+    vp = np.zeros((n, 2), np.float32)
+    sh = np.zeros((n, 2), np.float32)
+    p_out = f32(pos).copy()
+    v_out = np.zeros((n, 2), np.float32)
+    lib().orc_isph_close_f32(C.byref(dom), C.c_int64(n), _p(off), _p(nbr), _p(f32(wpos)), C.c_float(h0),
+                             _p(f32(mass)), _p(f32(rho)), _p(f32(vel)), _p(None if adv is None else f32(adv)),
+                             _p(f32(pstar)), _p(f32(pss)), C.c_float(dt), C.c_float(rho0), _p(vp), _p(sh), _p(p_out),
+                             _p(v_out))
+    return vp, sh, p_out, v_out
+
+
+@dataclass
+class IsphState:
+    pos: np.ndarray
+    vel: np.ndarray
+    mass: np.ndarray
+    areas: np.ndarray
+    uid: np.ndarray
+    p_star: np.ndarray | None = None
+    p_ss: np.ndarray | None = None
+
+
+def isph_step(st: IsphState, dom: Domain, h: float, dt: float, rho0: float = 1.0, visc: Visc | None = Visc(),
+              gravity=None, rtol=1e-3, maxiter=512, fp64=False, omega=0.5, thr=5e-4, min_iters=2, max_iters=256):
+    """One ISPH step (v2/modules/incompressible.py:366-460) in float32,
+    sorted into cell order first as the GPU solver keeps its state.
+    Returns (state, info)."""
+    g = grid_build(dom, st.pos, h)
+    si = g.sort_index
+    take = lambda a: None if a is None else a[si]  # noqa: E731
+    st = IsphState(st.pos[si], st.vel[si], st.mass[si], st.areas[si], st.uid[si], take(st.p_star), take(st.p_ss))
+    g = grid_build(dom, st.pos, h)
+    _, off, nbr_s = neighbors_cell(dom, g, st.pos, h0=h)
+    nbr = g.sort_index[nbr_s].astype(np.int32)
+    wpos = np.empty_like(g.sorted_pos)
+    wpos[g.sort_index] = g.sorted_pos
+    wpos = f32(wpos)
+    n = st.pos.shape[0]
+    F = np.float32
+    rho, vol = density(dom, off, nbr, wpos, wpos, st.mass, st.mass, h0=h)
+    ad, an, ds, vs = dfsph_prepare(dom, off, nbr, wpos, st.vel, st.areas, vol, rho, dt, rho0, visc, h)
+    if visc is None:
+        vs = np.zeros_like(vs)
+    adv = (np.asarray(gravity, np.float32)[None, :] + vs) if gravity is not None else vs.copy()
+    lap, aii, pre = isph_laplacian(dom, off, nbr, wpos, h, st.mass, rho, dt)
+    B = isph_source(dom, off, nbr, wpos, h, st.vel, adv, st.mass, rho, dt, rho0)
+    x0 = st.p_star if st.p_star is not None else np.zeros(n, np.float32)
+    p_star, it_b, status, rn, bn = bicgstab(off, nbr, lap, pre, B, x0, 0.5 if st.p_star is not None else 1.0,
+                                            rtol, 0.0, maxiter, fp64)
+    alpha = (an / F(dt)).astype(np.float32)
+    p0 = (F(0.5) * f32(st.p_ss)) if st.p_ss is not None else np.zeros(n, np.float32)
+    p_ss, _, res, it_j, err_j, _ = dfsph_solve(dom, off, nbr, wpos, h, dt, rho0, omega, thr, min_iters, max_iters,
+                                               False, True, st.vel, adv, rho, vol, st.areas, alpha, None, p0,
+                                               isph=True)
+    vp, sh, pos, vel = isph_close(dom, off, nbr, wpos, h, st.mass, rho, st.vel, adv, p_star, p_ss, dt, rho0, st.pos)
+    new = IsphState(pos, vel, st.mass, st.areas, st.uid, p_star, p_ss)
+    info = dict(rho=rho, lap=lap, B=B, p_star=p_star, bicg_iters=it_b, bicg_status=status, bicg_residual=rn, p_ss=p_ss,
+                jacobi_iters=it_j, jacobi_error=err_j, vprime=vp, shift=sh, off=off, nbr=nbr)
+    return new, info
