@@ -129,23 +129,29 @@ __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* __restrict__ a, in
   if (i < n) a[i] = v;
 }
 
-// Wrapped position and global cell of every particle; provisional slot by atomics.
+// Wrapped position and global cell of every particle; provisional slot by
+// atomics (the in-cell sort fixes the final order).
 __global__ void __launch_bounds__(kBlock) k_bkeys(int n, const float2* __restrict__ x,
                                                   const int32_t* __restrict__ dom_of, const Dom* __restrict__ doms,
                                                   const BGrid* __restrict__ grids, float2* __restrict__ wq,
                                                   int32_t* __restrict__ keys, int32_t* __restrict__ slot,
                                                   int32_t* __restrict__ cell_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = dom_of[i];
-  const Dom d = doms[b];
-  const BGrid g = grids[b];
-  const float2 p = x[i];
-  const float2 w = make_float2(wrap_coord(p.x, d, 0), wrap_coord(p.y, d, 1));
-  wq[i] = w;
-  const int key = g.cell_off + bcell(w.y, d, 1, g) * g.nc[0] + bcell(w.x, d, 0, g);
-  keys[i] = key;
-  slot[i] = atomicAdd(&cell_count[key], 1);
+  const bool valid = i < n;
+  int key = 0;
+  if (valid) {
+    const int b = dom_of[i];
+    const Dom d = doms[b];
+    const BGrid g = grids[b];
+    const float2 p = x[i];
+    const float2 w = make_float2(wrap_coord(p.x, d, 0), wrap_coord(p.y, d, 1));
+    wq[i] = w;
+    key = g.cell_off + bcell(w.y, d, 1, g) * g.nc[0] + bcell(w.x, d, 0, g);
+    keys[i] = key;
+  }
+  // one atomic per distinct cell of the wave (the state is nearly cell-ordered)
+  const int sl = wave_slot(cell_count, key, valid);
+  if (valid) slot[i] = sl;
 }
 
 __global__ void __launch_bounds__(kBlock) k_bscatter(int n, const int32_t* __restrict__ keys,

@@ -190,6 +190,26 @@ __device__ __forceinline__ RowRef row_ref(const PairsDev& P, int i) {
   return r;
 }
 
+// slot = atomicAdd(&count[key], 1) for every valid lane of the wave, with one
+// atomic per distinct key: lanes sharing a key take consecutive slots in lane
+// order.  All 64 lanes must call it (invalid lanes pass valid = false).
+__device__ __forceinline__ int wave_slot(int32_t* __restrict__ count, int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long remaining = __ballot(valid);
+  int slot = 0;
+  while (remaining) {
+    const int leader = __ffsll((long long)remaining) - 1;
+    const int lkey = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(valid && key == lkey);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lkey], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (valid && key == lkey) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    remaining &= ~same;
+  }
+  return slot;
+}
+
 // body(j, load(j)) for j = b .. e-1 in order; the loads of U consecutive j
 // are issued before the first body of the group, so U gathers are in
 // flight at once.  The order of the bodies (and of any sums they form) is

@@ -31,13 +31,17 @@ __global__ void __launch_bounds__(kBlock) k_cell_keys(const float2* __restrict__
                                                       int32_t* __restrict__ keys, int32_t* __restrict__ slot,
                                                       int32_t* __restrict__ cell_count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float2 p = pos[i];
-  float wx = wrap_coord(p.x, d, 0), wy = wrap_coord(p.y, d, 1);
-  int cx = cell_coord(wx, d, 0, g), cy = cell_coord(wy, d, 1, g);
-  int key = cy * g.nc[0] + cx;
-  keys[i] = key;
-  slot[i] = atomicAdd(&cell_count[key], 1);
+  const bool valid = i < n;
+  int key = 0;
+  if (valid) {
+    float2 p = pos[i];
+    float wx = wrap_coord(p.x, d, 0), wy = wrap_coord(p.y, d, 1);
+    int cx = cell_coord(wx, d, 0, g), cy = cell_coord(wy, d, 1, g);
+    key = cy * g.nc[0] + cx;
+    keys[i] = key;
+  }
+  const int sl = wave_slot(cell_count, key, valid);
+  if (valid) slot[i] = sl;
 }
 
 __global__ void __launch_bounds__(kBlock) k_cell_scatter(int n, const int32_t* __restrict__ keys,
