@@ -238,6 +238,21 @@ SIGNATURES = {
     "dsph_op_sph_density": (C.c_int, [V, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_sph_interpolation": (C.c_int, [V, V, V, I32, I64, V, V, I64, I32, V, V, V, V]),
     "dsph_op_scatter_add": (C.c_int, [V, I32, I64, I32, V, V, V, V]),
+    "dsph_op_viscosity": (C.c_int, [V, V, V, V, V, V, I64, V, V, V, V, V, I64, I32, V, V, F32, F32, F32, F32, I32,
+                                    V, V]),
+    "dsph_op_mdbc": (C.c_int, [V, V, I64, V, V, I32, V, V, V, V, V, V, I64, V, V, F32, V, V, V]),
+    "dsph_op_neighborhood_count": (C.c_int, [V, V, I32, V, V, I64, V, I64, V, V, V, V, V, V, V]),
+    "dsph_op_neighborhood_fill": (C.c_int, [V, V, I32, V, V, I64, V, I64, V, V, V, V, V, V, I64, V, V, V, V, V,
+                                            V]),
+    "dsph_pairs_filter_count": (C.c_int, [P(Domain), I32, V, V, I32, V, V, I64, V, I64, V, V, V, V]),
+    "dsph_pairs_filter_fill": (C.c_int, [P(Domain), I32, V, V, I32, V, V, I64, V, I64, V, V, V, I64, V, V, V, V,
+                                         V, V, V, V]),
+    "dsph_verlet_check_workspace": (SZ, []),
+    "dsph_verlet_check": (C.c_int, [V, V, I64, V, I64, P(C.c_float), V, SZ, V]),
+    "dsph_hash_count_neighbors": (C.c_int, [V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V, I32, V,
+                                            V]),
+    "dsph_hash_build_neighbors": (C.c_int, [V, I64, V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V,
+                                            I32, V, V, V]),
 }
 
 _lib = None

@@ -117,11 +117,10 @@ inline T* at(void* ws, size_t off) {
 }
 
 __device__ __forceinline__ int bcell(float x, const Dom& d, int k, const BGrid& g) {
-  float f = floorf(__fdiv_rn(x - d.min[k], g.hcell));
-  int c = (int)f;
-  if (f < 0.0f) c = 0;
-  if (c > g.nc[k] - 1) c = g.nc[k] - 1;
-  return c;
+  // clamped in float before the conversion (NaN and out-of-range
+  // coordinates land in an edge cell)
+  const float f = fminf(fmaxf(floorf(__fdiv_rn(x - d.min[k], g.hcell)), 0.0f), (float)(g.nc[k] - 1));
+  return (int)f;
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* __restrict__ a, int64_t n, int32_t v) {

@@ -250,6 +250,16 @@ __device__ __forceinline__ void scan_row(const int32_t* __restrict__ nbr, const 
   }
 }
 
+// CSR segment [b, e) of row i (offsets off, lengths nn) clipped to the
+// pair arrays [0, np): inconsistent input cannot index out of bounds.
+__device__ __forceinline__ void seg_bounds(const int32_t* off, const int32_t* nn, int i, int64_t np, int64_t& b,
+                                           int64_t& e) {
+  b = off[i];
+  e = b + (int64_t)nn[i];
+  if (b < 0) b = 0;
+  if (e > np) e = np;
+}
+
 __device__ __forceinline__ float h_of(const float* arr, int idx, float h0) {
   return arr ? arr[idx] : h0;
 }

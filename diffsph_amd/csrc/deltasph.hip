@@ -21,6 +21,7 @@
 // tensor expressions, and the CPU oracle evaluates the same expressions.
 #include "common.h"
 #include "deltasph_impl.h"
+#include "mdbc_op.h"
 #include "pmath.h"
 
 namespace dsph {
@@ -447,21 +448,6 @@ __global__ void __launch_bounds__(kBlock) k_ghosts(const float2* __restrict__ bp
   gpos[i] = make_float2(x.x + md * nn.x, x.y + md * nn.y);
 }
 
-// mDBC wall density (mDBCDensity, v2/modules/mDBC.py:50-136): per ghost g
-// over its fluid neighbors j (V_j = m_j / rho_j, x_gj = x_j - x_g):
-//   Shepard  rho_S = sum m_j W / (sum V_j W + 1e-7)
-//   A = [[sum V W, sum V W x_gj^T], [sum V gradW, sum V x_gj (x) gradW]^T-layout as the
-//        reference fills A_g], b = [sum m_j W, sum m_j gradW]
-//   rho_b = rho0 (no neighbors), rho_S (1..5 neighbors), else
-//   c = pinv(A) b (pinv only for > 4 neighbors) and rho_b = c0 + (x_b - x_g) . c[1:].
-__device__ __forceinline__ float det3(float m11, float m12, float m13, float m21, float m22, float m23, float m31,
-                                      float m32, float m33) {
-  const float a = m11 * (m22 * m33 - m23 * m32);
-  const float b = m12 * (m21 * m33 - m23 * m31);
-  const float c = m13 * (m21 * m32 - m22 * m31);
-  return a - b + c;
-}
-
 // The reference's C++ mDBC op (mDBCDensityKernel, v2/cppSrc/mDBC.h:129-244):
 // x_gj = -(q h) xhat, b = sum m_j [W, gradW], Cramer's rule with
 // det + 1e-7, Shepard density for <= 5 neighbors, clamped at rho0.
@@ -470,45 +456,29 @@ __device__ float mdbc_op_row(const PairsDev& G, int i, const float* __restrict__
                              float* __restrict__ shepard) {
   const float2 xg = G.qpos[i];
   const float hi = h_of(G.qh, i, G.h0);
-  float nom = 0.f, den = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-  float m00 = 0.f, m01 = 0.f, m02 = 0.f, m10 = 0.f, m11 = 0.f, m12 = 0.f, m20 = 0.f, m21 = 0.f, m22 = 0.f;
+  MdbcOpAcc acc;
   const RowRef rr = row_ref(G, i);
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = G.nbr[k];
     const float h = pick_h(G.mode, hi, h_of(G.rh, j, G.h0));
     const Geom g = pair_geom(xg, G.rpos[j], h, G.dom);
-    const float m_j = mass[j], rho_j = rho[j];
-    const float V = m_j / rho_j;
-    const float xx = ((-g.q) * g.h) * g.ex, xy = ((-g.q) * g.h) * g.ey;
-    nom += m_j * g.w;
-    den += V * g.w;
-    const float vw = V * g.w, vgx = V * g.gx, vgy = V * g.gy;
-    m00 += vw;
-    m10 += vgx;
-    m20 += vgy;
-    m01 += vw * xx;
-    m11 += vgx * xx;
-    m21 += vgx * xy;
-    m02 += vw * xy;
-    m12 += vgy * xx;
-    m22 += vgy * xy;
-    bx += m_j * g.w;
-    by += m_j * g.gx;
-    bz += m_j * g.gy;
+    acc.add(mass[j], rho[j], g.w, g.gx, g.gy, ((-g.q) * g.h) * g.ex, ((-g.q) * g.h) * g.ey);
   }
-  const float shep = nom / (den + 1e-7f);
-  if (shepard) shepard[i] = shep;
-  if (rr.cnt <= 5) return shep < rho0 ? rho0 : shep;
-  const float d0 = det3(m00, m01, m02, m10, m11, m12, m20, m21, m22) + 1e-7f;
-  const float rx = det3(bx, m01, m02, by, m11, m12, bz, m21, m22) / d0;
-  const float ry = det3(m00, bx, m02, m10, by, m12, m20, bz, m22) / d0;
-  const float rz = det3(m00, m01, bx, m10, m11, by, m20, m21, bz) / d0;
   const float2 xb = bpos[i];
-  const float px = xb.x - xg.x, py = xb.y - xg.y;
-  const float out = rx + (px * ry + py * rz);
-  return out < rho0 ? rho0 : out;
+  float shep;
+  const float out = acc.finish(rr.cnt, xb.x - xg.x, xb.y - xg.y, rho0, &shep);
+  if (shepard) shepard[i] = shep;
+  return out;
 }
 
+// mDBC wall density (mDBCDensity, v2/modules/mDBC.py:50-136): per ghost g
+// over its fluid neighbors j (V_j = m_j / rho_j, x_gj = x_j - x_g):
+//   Shepard  rho_S = sum m_j W / (sum V_j W + 1e-7)
+//   A = [[sum V W, sum V W x_gj^T], [sum V gradW, sum V x_gj (x) gradW]^T-layout as the
+//        reference fills A_g], b = [sum m_j W, sum m_j gradW]
+//   rho_b = rho0 (no neighbors), rho_S (1..5 neighbors), else
+//   c = pinv(A) b (pinv only for > 4 neighbors) and rho_b = c0 + (x_b - x_g) . c[1:].
+// variant 1 is the C++ op above.
 __global__ void __launch_bounds__(kBlock) k_mdbc(PairsDev G, const float* __restrict__ mass,
                                                  const float* __restrict__ rho, const float2* __restrict__ bpos,
                                                  float rho0, float support, int variant, float* __restrict__ rho_b,

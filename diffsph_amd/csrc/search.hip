@@ -20,11 +20,10 @@ struct GridDev {
 };
 
 __device__ __forceinline__ int cell_coord(float x, const Dom& d, int k, const GridDev& g) {
-  float f = floorf(__fdiv_rn(x - d.min[k], g.hcell));
-  int c = (int)f;
-  if (f < 0.0f) c = 0;
-  if (c > g.nc[k] - 1) c = g.nc[k] - 1;
-  return c;
+  // clamped in float before the conversion: NaN (fmaxf returns the other
+  // operand) and out-of-range coordinates land in an edge cell
+  const float f = fminf(fmaxf(floorf(__fdiv_rn(x - d.min[k], g.hcell)), 0.0f), (float)(g.nc[k] - 1));
+  return (int)f;
 }
 
 __global__ void __launch_bounds__(kBlock) k_cell_keys(const float2* __restrict__ pos, int n, Dom d, GridDev g,

@@ -208,14 +208,6 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const float* __restrict__ sr
 // The caller's CSR may be inconsistent: each segment is clipped to
 // [0, num_pairs) and neighbor indices outside [0, num_ref) are skipped, so a
 // bad argument can never make the kernel read out of bounds.
-__device__ __forceinline__ void seg_bounds(const int32_t* off, const int32_t* nn, int i, int64_t np, int64_t& b,
-                                           int64_t& e) {
-  b = off[i];
-  e = b + (int64_t)nn[i];
-  if (b < 0) b = 0;
-  if (e > np) e = np;
-}
-
 __global__ void __launch_bounds__(kBlock) k_op_density(const float* __restrict__ mj, const int64_t* __restrict__ jj,
                                                        const float* __restrict__ W, int n, int64_t nref, int64_t np,
                                                        const int32_t* __restrict__ nn, const int32_t* __restrict__ off,

@@ -21,6 +21,7 @@ from __future__ import annotations
 import ctypes as C
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -247,42 +248,205 @@ def export_pairs(nl: NeighborList, q_sort_index=None, q_rank=None, r_sort_index=
     return res
 
 
+def filter_pairs(domain: _lib.Domain, mode, qpos_wrapped: torch.Tensor, qh: torch.Tensor | None,
+                 rpos_wrapped: torch.Tensor, rh: torch.Tensor | None, indices_j: torch.Tensor,
+                 num_neighbors: torch.Tensor, neighbor_offsets: torch.Tensor, kernels: bool = True) -> dict:
+    """computeNeighborhood (v2/modules/neighborhood.py:433-460) on a candidate
+    CSR (a Verlet list): keeps the entries with |mod(x_i - x_j)| / h_ij <= 1
+    and returns their per-pair arrays (i, j int64; distances = r / h_ij,
+    vectors = d / (r + 1e-7), supports = h_ij; kernels / gradients of the
+    Wendland C2 kernel as neighborSearch computes them, :585-589) with the
+    CSR counts and offsets of the kept entries.  One count pass, a scan (one
+    host read of the total, as the reference's tensor sizes need) and one
+    fill pass, each a wave per row."""
+    m = MODES[mode] if isinstance(mode, str) else int(mode)
+    _check_f32(qpos_wrapped, "queryPositions", (2,))
+    _check_f32(rpos_wrapped, "referencePositions", (2,))
+    n = qpos_wrapped.shape[0]
+    dev = qpos_wrapped.device
+    st = _lib.stream_handle(dev)
+    L = lib()
+    jj = indices_j.to(torch.int64).contiguous()
+    nn = num_neighbors.to(torch.int32).contiguous()
+    off = neighbor_offsets.to(torch.int32).contiguous()
+    args = (C.byref(domain), m, ptr(qpos_wrapped), ptr(qh), n, ptr(rpos_wrapped), ptr(rh), rpos_wrapped.shape[0],
+            ptr(jj), jj.numel(), ptr(nn), ptr(off))
+    counts = torch.zeros(n, dtype=torch.int32, device=dev)
+    check(L.dsph_pairs_filter_count(*args, ptr(counts), st), "pairs_filter_count")
+    offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    ws = WORKSPACE.get(L.dsph_scan_workspace(n), dev)
+    total = C.c_int64(0)
+    check(L.dsph_scan_exclusive(ptr(counts), n, ptr(offsets), C.byref(total), ptr(ws), ws.numel(), st), "scan")
+    P = int(total.value)
+    f = dict(dtype=torch.float32, device=dev)
+    res = {"i": torch.empty(P, dtype=torch.int64, device=dev), "j": torch.empty(P, dtype=torch.int64, device=dev),
+           "distances": torch.empty(P, **f), "vectors": torch.empty(P, 2, **f), "supports": torch.empty(P, **f)}
+    if kernels:
+        res["kernels"] = torch.empty(P, **f)
+        res["gradients"] = torch.empty(P, 2, **f)
+    check(L.dsph_pairs_filter_fill(*args, ptr(offsets), P, ptr(res["i"]), ptr(res["j"]), ptr(res["distances"]),
+                                   ptr(res["vectors"]), ptr(res["supports"]), ptr(res.get("kernels")),
+                                   ptr(res.get("gradients")), st), "pairs_filter_fill")
+    res["numNeighbors"] = counts
+    res["neighborOffsets"] = offsets[:-1]
+    return res
+
+
+def max_displacement(prior: torch.Tensor, now: torch.Tensor, supports: torch.Tensor) -> tuple[float, float]:
+    """(max_k |now_k - prior_k|, min supports) of the Verlet reuse tests
+    (v2/modules/neighborhood.py:470-471, 511-512); one host read."""
+    _check_f32(prior, "prior positions", (2,))
+    _check_f32(now, "positions", (2,))
+    sup = supports.to(torch.float32).contiguous()
+    L = lib()
+    dev = now.device
+    ws = WORKSPACE.get(L.dsph_verlet_check_workspace(), dev)
+    out = (C.c_float * 2)()
+    check(L.dsph_verlet_check(ptr(prior), ptr(now), now.shape[0], ptr(sup), sup.numel(), out, ptr(ws), ws.numel(),
+                              _lib.stream_handle(dev)), "verlet_check")
+    return float(out[0]), float(out[1])
+
+
+def _moved(max_dist: float, min_support: float, verlet_scale: float) -> bool:
+    # maxDistance * 2 > (verletScale - 1) * minSupport in float32 tensors
+    f = np.float32
+    return bool(f(max_dist) * f(2) > f(verlet_scale - 1) * f(min_support))
+
+
+def buildDataStructure(positions, supports, hMax, minExtent, maxExtent, periodic, numParticles, verbose=False):
+    """torchCompactRadius.buildDataStructure as the reference calls it
+    (v2/modules/neighborhood.py:403: supports already scaled by verletScale):
+    the reference set sorted into the dense cell grid, with the reference's
+    dict keys (referencePositions, referenceSupports, sortIndex,
+    sortedPositions, sortedSupports)."""
+    pos = positions.detach().to(torch.float32).contiguous()
+    sup = supports.detach().to(torch.float32).contiguous()
+    domain = _lib.make_domain(_as_list(minExtent), _as_list(maxExtent), _as_list(periodic))
+    hsearch = max(float(hMax), float(sup.max().item()) if sup.numel() else float(hMax))
+    grid = CellGrid.build(pos, sup, domain, hsearch)
+    si = grid.sort_index.long()
+    return {"referencePositions": pos, "referenceSupports": sup, "sortIndex": si, "sortedPositions": pos[si],
+            "sortedSupports": grid.sorted_h, "numParticles": int(numParticles), "_grid": grid, "_domain": domain,
+            "_hsearch": hsearch}
+
+
+def _candidates(queryPositions, querySupports, ds, mode="scatter"):
+    """(i, j, counts, offsets[:-1]) of the query set against the data
+    structure's reference set; see neighborSearchExisting."""
+    grid = ds["_grid"]
+    if grid is None:
+        grid = CellGrid.build(ds["_positions"], ds["_supports"], ds["_domain"],
+                              max(ds["_hsearch"], float(ds["_supports"].max().item())))
+        ds["_grid"] = grid
+    qpos = queryPositions.detach().to(torch.float32).contiguous()
+    qh = querySupports.detach().to(torch.float32).contiguous() if querySupports is not None else None
+    nl = search(grid, qpos, qh, mode, layout="csr")
+    ex = export_pairs(nl, None, None, grid.sort_index, want=("i", "j"))
+    return ex["i"], ex["j"], nl.counts, ex["offsets"][:-1]
+
+
+def neighborSearchExisting(queryPositions, querySupports, datastructure, mode="scatter", searchRange=1,
+                           algorithm="cpp"):
+    """torchCompactRadius.neighborSearchExisting (v2/modules/neighborhood.py:526):
+    (i, j) int64 of the query set against the data structure's reference set
+    (rows in query order, j original reference indices), kept where
+    |mod(x_i - x_j)| / h <= 1 with h the reference's (scatter), the query's
+    (gather) or their mean (symmetric) support.  A data structure whose
+    positions were updated since it was built is re-sorted first."""
+    i, j, _, _ = _candidates(queryPositions, querySupports, datastructure, mode)
+    return i, j
+
+
+def updateDataStructure(referenceState, config, priorDatastructure, verbose=False):
+    """v2/modules/neighborhood.py:462-478: rebuild when the reference set
+    changed size or moved by more than (verletScale - 1) min(h) / 2, else keep
+    the sort and take the current positions."""
+    vs = _verlet_scale(config)
+    ds = priorDatastructure
+    pos = referenceState["positions"]
+    if ds["referencePositions"].shape != pos.shape:
+        return _build_ds(referenceState, config), True
+    dmax, hmin = max_displacement(ds["referencePositions"], pos.detach().to(torch.float32).contiguous(),
+                                  ds["referenceSupports"])
+    if _moved(dmax, hmin, vs):
+        return _build_ds(referenceState, config), True
+    si = ds["sortIndex"]
+    ds["sortedPositions"] = pos[si]
+    ds["sortedSupports"] = referenceState["supports"][si]
+    ds["_positions"] = pos.detach().to(torch.float32).contiguous()
+    ds["_supports"] = (referenceState["supports"] * vs).detach().to(torch.float32).contiguous()
+    ds["_grid"] = None
+    return ds, False
+
+
+def _verlet_scale(config) -> float:
+    return float(config.get("neighborhood", {}).get("verletScale", 1.5))
+
+
+def _build_ds(state, config):
+    """getNeighborSearchDataStructure (v2/modules/neighborhood.py:402-403)."""
+    d = config["domain"]
+    sup = state["supports"]
+    return buildDataStructure(state["positions"], sup * _verlet_scale(config), sup.max(), d["minExtent"],
+                              d["maxExtent"], d.get("periodicity", d.get("periodic", False)),
+                              state["positions"].shape[0])
+
+
 def neighborSearch(queryState, referenceState, config, computeKernels=True, priorState=None,
                    neighborDatastructure=None, verbose=False):
     """Drop-in for v2/modules/neighborhood.py:491-593 (scatter mode, kernel
-    values from the Wendland C2 kernel).  Returns ``(datastructure,
-    neighborDict)``; the datastructure is a :class:`CellGrid` of the
-    reference set.  The CSR pair set is recomputed on every call (the
-    reference's Verlet list only saves candidate generation, the filtered
-    result is the same)."""
-    domain = _domain_from_config(config)
-    rpos = referenceState["positions"]
-    rh = referenceState["supports"].contiguous()
-    qpos = queryState["positions"].contiguous()
-    qh = queryState["supports"].contiguous()
-    hmax = float(rh.max().item())
-    grid = CellGrid.build(rpos.contiguous(), rh, domain, hmax)
-    same = queryState is referenceState
-    if same:
-        nl = search(grid, grid.sorted_pos, grid.sorted_h, "scatter", qpos_wrapped=True)
-        ex = export_pairs(nl, grid.sort_index, grid.rank, grid.sort_index)
+    values from the Wendland C2 kernel), with the reference's Verlet reuse:
+    the candidate list (support x verletScale, 'fullIndices') is rebuilt
+    when there is no prior state, the query set changed size, or it moved by
+    more than (verletScale - 1) min(h) / 2 (or the data structure was
+    rebuilt); otherwise the prior candidates are re-filtered at the current
+    positions.  Returns ``(datastructure, neighborDict)`` with the reference's
+    keys."""
+    vs = _verlet_scale(config)
+    if neighborDatastructure is None:
+        neighborDatastructure = _build_ds(referenceState, config)
+        dirty = True
     else:
-        nl = search(grid, qpos, qh, "scatter")
-        ex = export_pairs(nl, None, None, grid.sort_index)
-    counts = (ex["offsets"][1:] - ex["offsets"][:-1]).to(torch.int32)
+        neighborDatastructure, dirty = updateDataStructure(referenceState, config, neighborDatastructure, verbose)
+    qpos = queryState["positions"]
+    if priorState is not None:
+        if priorState["initialPositions"][0].shape != qpos.shape:
+            dirty = True
+        else:
+            dmax, hmin = max_displacement(priorState["initialPositions"][0].detach().to(torch.float32).contiguous(),
+                                          qpos.detach().to(torch.float32).contiguous(), queryState["supports"])
+            if _moved(dmax, hmin, vs):
+                dirty = True
+    else:
+        dirty = True
+    if dirty:
+        fi, fj, full_num, full_off = _candidates(qpos, queryState["supports"], neighborDatastructure, "scatter")
+        full = (fi, fj)
+        initial = (queryState["positions"], referenceState["positions"])
+    else:
+        full = priorState["fullIndices"]
+        full_num = priorState["fullNumNeighbors"]
+        full_off = priorState["fullNeighborOffsets"]
+        initial = priorState["initialPositions"]
+    domain = _domain_from_config(config)
+    pos_x = wrap_positions(qpos.detach().to(torch.float32).contiguous(), domain)
+    pos_y = wrap_positions(referenceState["positions"].detach().to(torch.float32).contiguous(), domain)
+    h_i = queryState["supports"].detach().to(torch.float32).contiguous()
+    h_j = referenceState["supports"].detach().to(torch.float32).contiguous()
+    fr = filter_pairs(domain, "scatter", pos_x, h_i, pos_y, h_j, full[1], full_num, full_off, kernels=computeKernels)
     nd = {
-        "indices": (ex["i"], ex["j"]),
-        "fullIndices": (ex["i"], ex["j"]),
-        "distances": ex["distances"],
-        "vectors": ex["vectors"],
-        "supports": ex["supports"],
-        "initialPositions": (qpos, rpos),
-        "numNeighbors": counts,
-        "neighborOffsets": ex["offsets"][:-1].contiguous(),
-        "fullNumNeighbors": counts,
-        "fullNeighborOffsets": ex["offsets"][:-1].contiguous(),
+        "indices": (fr["i"], fr["j"]),
+        "fullIndices": full,
+        "distances": fr["distances"],
+        "vectors": fr["vectors"],
+        "supports": fr["supports"],
+        "initialPositions": initial,
+        "numNeighbors": fr["numNeighbors"],
+        "neighborOffsets": fr["neighborOffsets"],
+        "fullNumNeighbors": full_num,
+        "fullNeighborOffsets": full_off,
     }
     if computeKernels:
-        nd["kernels"] = ex["kernels"]
-        nd["gradients"] = ex["gradients"]
-    return grid, nd
+        nd["kernels"] = fr["kernels"]
+        nd["gradients"] = fr["gradients"]
+    return neighborDatastructure, nd

@@ -761,6 +761,154 @@ int dsph_op_scatter_add(const float* input, int32_t width, int64_t num_pairs,
                         const int32_t* neighbor_offset, float* out,
                         void* stream);
 
+/* viscosityKernel (partiBench/viscosityKernel.cpp:5-21, per-particle
+ * viscosityFn partiBench/viscosityKernel.h:98-145): for row i over its CSR
+ * segment (h = supports[k], r = distances[k] h, xhat = vectors[k]):
+ *   pi = ((v_i - v_j) . xhat) / ((r + eps) + h^2)  (0 where the dot >= 0
+ *   and pi_switch is set), V = m_j / (rho_i + rho_j),
+ *   out_i = (alpha h_i cs rho0 / rho_i) sum V pi gradients[k].
+ * A-side arrays (velocities_i [.,2], densities_i, supports_i) have at least
+ * num_particles rows; B-side arrays (velocities_j, masses_j, densities_j)
+ * num_ref rows; out is [num_particles, 2]. */
+int dsph_op_viscosity(const float* velocities_i, const float* velocities_j,
+                      const float* masses_j, const float* densities_i,
+                      const float* densities_j, const float* supports_i,
+                      int64_t num_ref, const int64_t* indices_j,
+                      const float* supports, const float* gradients,
+                      const float* distances, const float* vectors,
+                      int64_t num_pairs, int32_t num_particles,
+                      const int32_t* num_neighbors,
+                      const int32_t* neighbor_offset, float cs, float rho0,
+                      float alpha, float eps, int32_t pi_switch, float* out,
+                      void* stream);
+
+/* mDBC (v2/cppSrc/mDBC.cpp:3-147, mDBCDensityKernel v2/cppSrc/mDBC.h:129-244)
+ * on the caller's per-pair arrays: boundary particle b (row b of the CSR)
+ * over its fluid neighbors j with x_bj = -(distances[k] supports[k])
+ * vectors[k]; Shepard density sum m W / (sum V W + 1e-7); with more than 5
+ * entries in the row, Cramer's rule on the 3x3 moment system (det + 1e-7)
+ * extrapolated by (positions_boundary[b] - positions_ghost[b]); clamped below
+ * at rho0.  masses_j / densities_j have num_ref entries; the position arrays
+ * and both outputs num_boundary rows. */
+int dsph_op_mdbc(const float* masses_j, const float* densities_j,
+                 int64_t num_ref, const float* positions_boundary,
+                 const float* positions_ghost, int32_t num_boundary,
+                 const int64_t* indices_j, const float* kernels,
+                 const float* gradients, const float* distances,
+                 const float* vectors, const float* supports,
+                 int64_t num_pairs, const int32_t* num_neighbors,
+                 const int32_t* neighbor_offset, float rho0,
+                 float* mdbc_density, float* shepard_density, void* stream);
+
+/* computeNeighborhood, C++ form (v2/cppSrc/computeNeighborhood.cpp:5-94,
+ * countingKernel / updateKernel v2/cppSrc/computeNeighborhood.h:115-187):
+ * keeps the CSR entries of row i whose periodic offset d (per axis, in
+ * double: ((p-q + L/2) - floor((p-q + L/2)/L) L) - L/2 with L = max - min in
+ * float) has |d|^2 <= h_j^2.  min_domain, max_domain ([2] float) and
+ * periodicity ([2] bool) are DEVICE pointers, as the reference reads them.
+ * Count: counts[i] (i < num_particles) = kept entries of row i.
+ * Fill: entry n of row i goes to new_offsets[i] + n (entries at or beyond
+ * `capacity` are dropped) with out_i = i, out_j = j, rij = |d| / h_j,
+ * xij = d / (|d| + 1e-6 h_i), hij = h_j. */
+int dsph_op_neighborhood_count(const float* positions_i, const float* supports_i,
+                               int32_t num_particles, const float* positions_j,
+                               const float* supports_j, int64_t num_ref,
+                               const int64_t* indices_j, int64_t num_pairs,
+                               const int32_t* num_neighbors,
+                               const int32_t* neighbor_offset,
+                               const float* min_domain, const float* max_domain,
+                               const uint8_t* periodicity, int32_t* counts,
+                               void* stream);
+int dsph_op_neighborhood_fill(const float* positions_i, const float* supports_i,
+                              int32_t num_particles, const float* positions_j,
+                              const float* supports_j, int64_t num_ref,
+                              const int64_t* indices_j, int64_t num_pairs,
+                              const int32_t* num_neighbors,
+                              const int32_t* neighbor_offset,
+                              const float* min_domain, const float* max_domain,
+                              const uint8_t* periodicity,
+                              const int32_t* new_offsets, int64_t capacity,
+                              int64_t* out_i, int64_t* out_j, float* out_rij,
+                              float* out_xij, float* out_hij, void* stream);
+
+/* computeNeighborhood, Python form (v2/modules/neighborhood.py:433-460) with
+ * the kernel values of neighborSearch (:585-589): the pair filter of the
+ * Verlet path.  Positions are the WRAPPED positions (getPeriodicPosition);
+ * the offset is the float32 min-image of v2/math.py:4-6 over `domain`,
+ * h_ij by `mode` (DSPH_MODE_*), kept when |d| / h_ij <= 1; outputs rij =
+ * |d| / h_ij, xij = d / (|d| + 1e-7), hij, and, where not NULL, the Wendland
+ * C2 kernels [P] and gradients [P,2].  Rows, clipping and capacity as in
+ * dsph_op_neighborhood_*. */
+int dsph_pairs_filter_count(const dsph_domain* domain, int32_t mode,
+                            const float* positions_i, const float* supports_i,
+                            int32_t num_particles, const float* positions_j,
+                            const float* supports_j, int64_t num_ref,
+                            const int64_t* indices_j, int64_t num_pairs,
+                            const int32_t* num_neighbors,
+                            const int32_t* neighbor_offset, int32_t* counts,
+                            void* stream);
+int dsph_pairs_filter_fill(const dsph_domain* domain, int32_t mode,
+                           const float* positions_i, const float* supports_i,
+                           int32_t num_particles, const float* positions_j,
+                           const float* supports_j, int64_t num_ref,
+                           const int64_t* indices_j, int64_t num_pairs,
+                           const int32_t* num_neighbors,
+                           const int32_t* neighbor_offset,
+                           const int32_t* new_offsets, int64_t capacity,
+                           int64_t* out_i, int64_t* out_j, float* out_rij,
+                           float* out_xij, float* out_hij, float* out_kernels,
+                           float* out_gradients, void* stream);
+
+/* Verlet reuse test inputs (v2/modules/neighborhood.py:462-478, 505-517):
+ * out_host[0] = max_k |pos_now[k] - pos_prior[k]| over n particles,
+ * out_host[1] = min of supports[0..num_supports).  Synchronises the stream
+ * (the caller decides 2 out[0] > (verletScale - 1) out[1] on the host, as
+ * the reference does).  Workspace: dsph_verlet_check_workspace(). */
+size_t dsph_verlet_check_workspace(void);
+int dsph_verlet_check(const float* pos_prior, const float* pos_now, int64_t n,
+                      const float* supports, int64_t num_supports,
+                      float* out_host, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
+/* countNeighbors / buildNeighborList (neighborhood.cpp:65-227, 253-435;
+ * device code neighborhood.h:100-325) over the caller's compact hash map:
+ * hash_table [hash_map_length, 2] int32 (begin, length into cell_table or
+ * -1), cell_table [num_cell_rows, 3] int64 (linear cell index, begin,
+ * length into the sorted arrays), num_cells [2] int32, min/max domain [2]
+ * float, periodicity [2] bool -- all DEVICE arrays.  For query i the cell
+ * floor((x_i - min) / h_cell) and its (2 search_range + 1)^2 neighbors
+ * (x offset outer, y inner; periodic axes wrapped) are looked up; sorted
+ * particle j is a neighbor when |d| < h_j (scatter), < h_i (gather) or
+ * < (h_i + h_j) / 2 (symmetric), |d| = sqrt of the float sum of the
+ * double per-axis offsets.  The bucket of a cell is the non-negative
+ * remainder of the int32 hash (73856093 cx + 19349663 cy) by
+ * hash_map_length (the bucket the table builder files the cell under).
+ * Count writes counts[num_query]; build writes neighbor i / j (int32, j in
+ * sorted order) from offsets[i], dropping entries at or beyond capacity.
+ * query_supports may be NULL in scatter mode, sorted_supports in gather
+ * mode. */
+int dsph_hash_count_neighbors(const float* query_positions,
+                              const float* query_supports, int32_t num_query,
+                              int32_t search_range, const float* sorted_positions,
+                              const float* sorted_supports, int32_t num_sorted,
+                              const int32_t* hash_table, int32_t hash_map_length,
+                              const int32_t* num_cells, const int64_t* cell_table,
+                              int64_t num_cell_rows, float h_cell,
+                              const float* min_domain, const float* max_domain,
+                              const uint8_t* periodicity, int32_t mode,
+                              int32_t* counts, void* stream);
+int dsph_hash_build_neighbors(const int32_t* offsets, int64_t capacity,
+                              const float* query_positions,
+                              const float* query_supports, int32_t num_query,
+                              int32_t search_range, const float* sorted_positions,
+                              const float* sorted_supports, int32_t num_sorted,
+                              const int32_t* hash_table, int32_t hash_map_length,
+                              const int32_t* num_cells, const int64_t* cell_table,
+                              int64_t num_cell_rows, float h_cell,
+                              const float* min_domain, const float* max_domain,
+                              const uint8_t* periodicity, int32_t mode,
+                              int32_t* out_i, int32_t* out_j, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,11 +149,10 @@ bool keep_f32(const float* qi, float hi, const float* rj, float hj, const DomT<f
 }
 
 int cell_of(float x, const DomT<float>& d, int k, int nc, float hcell) {
+  // clamped before the conversion (NaN -> cell 0), as the GPU search does
   float f = std::floor((x - d.min[k]) / hcell);
-  int c = (int)f;
-  if (f < 0.0f) c = 0;
-  if (c > nc - 1) c = nc - 1;
-  return c;
+  f = std::fmin(std::fmax(f, 0.0f), (float)(nc - 1));
+  return (int)f;
 }
 
 int nthreads_override = 0;
@@ -1476,6 +1475,236 @@ int orc_pinv2x2_f64(const double* M, int64_t n, double* L, double* ev) {
     pinv2<double>(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3], L + 4 * i, ev + 2 * i);
   return 0;
 }
+// ---------------------------------------------------------------------------
+// Drop-in ops on the reference's per-pair arrays, as the reference's CPU
+// branches run them: one loop over rows, each row's CSR segment in order.
+// ---------------------------------------------------------------------------
+
+// viscosityFn, partiBench/viscosityKernel.h:98-145 (CPU branch
+// partiBench/viscosityKernel.cpp:71-77).
+int orc_op_viscosity(const float* vel_i, const float* vel_j, const float* m_j, const float* rho_i, const float* rho_j,
+                     const float* h_i, const int64_t* idx_j, const float* supports, const float* grads,
+                     const float* dists, const float* vecs, int32_t n, const int32_t* nn, const int32_t* off, float cs,
+                     float rho0, float alpha, float eps, int32_t pi_switch, float* out) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    float res_x = 0.f, res_y = 0.f;
+    const float vel_x = vel_i[2 * i], vel_y = vel_i[2 * i + 1];
+    const float dens_i = rho_i[i];
+    for (int32_t t = 0; t < nn[i]; ++t) {
+      const int64_t k = (int64_t)off[i] + t;
+      const int32_t j = (int32_t)idx_j[k];
+      const float hh = supports[k];
+      const float v_x = vel_x - vel_j[2 * j];
+      const float v_y = vel_y - vel_j[2 * j + 1];
+      const float x_x = vecs[2 * k];
+      const float x_y = vecs[2 * k + 1];
+      const float r = dists[k] * hh;
+      const float vr = v_x * x_x + v_y * x_y;
+      float pij = vr / (r + eps + hh * hh);
+      if (pi_switch) pij = vr < 0 ? pij : 0;
+      const float Vj = m_j[j] / (dens_i + rho_j[j]);
+      res_x += Vj * pij * grads[2 * k];
+      res_y += Vj * pij * grads[2 * k + 1];
+    }
+    const float factor = alpha * h_i[i] * cs * rho0 / dens_i;
+    out[2 * i] = res_x * factor;
+    out[2 * i + 1] = res_y * factor;
+  }
+  return 0;
+}
+
+// mDBCDensityKernel, v2/cppSrc/mDBC.h:129-244 (CPU branch v2/cppSrc/mDBC.cpp:
+// 74-80): moment sums in the reference's fill order, Cramer's rule.
+static float det3_ref(float m11, float m12, float m13, float m21, float m22, float m23, float m31, float m32,
+                      float m33) {
+  float a = m11 * (m22 * m33 - m23 * m32);
+  float b = m12 * (m21 * m33 - m23 * m31);
+  float c = m13 * (m21 * m32 - m22 * m31);
+  return a - b + c;
+}
+int orc_op_mdbc(const float* m_j, const float* rho_j, const float* pos_b, const float* pos_g, int32_t nb,
+                const int64_t* idx_j, const float* W, const float* grads, const float* dists, const float* vecs,
+                const float* supports, const int32_t* nn, const int32_t* off, float rho0, float* out_mdbc,
+                float* out_shepard) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nb; ++i) {
+    float sn = 0.f, sd = 0.f;
+    float A[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    float b[3] = {0.f, 0.f, 0.f};
+    const int32_t cnt = nn[i];
+    for (int32_t t = 0; t < cnt; ++t) {
+      const int64_t k = (int64_t)off[i] + t;
+      const int32_t j = (int32_t)idx_j[k];
+      const float mj = m_j[j], rj = rho_j[j], w = W[k];
+      const float gx = grads[2 * k], gy = grads[2 * k + 1];
+      const float r = dists[k], hh = supports[k];
+      const float xx = -r * hh * vecs[2 * k], xy = -r * hh * vecs[2 * k + 1];
+      sn += mj * w;
+      sd += mj / rj * w;
+      A[0][0] += mj / rj * w;
+      A[1][0] += mj / rj * gx;
+      A[2][0] += mj / rj * gy;
+      A[0][1] += mj / rj * w * xx;
+      A[1][1] += mj / rj * gx * xx;
+      A[2][1] += mj / rj * gx * xy;
+      A[0][2] += mj / rj * w * xy;
+      A[1][2] += mj / rj * gy * xx;
+      A[2][2] += mj / rj * gy * xy;
+      b[0] += mj * w;
+      b[1] += mj * gx;
+      b[2] += mj * gy;
+    }
+    const float shep = sn / (sd + 1e-7f);
+    out_shepard[i] = shep;
+    if (cnt <= 5) {
+      out_mdbc[i] = std::max(shep, rho0);
+      continue;
+    }
+    const float d0 = det3_ref(A[0][0], A[0][1], A[0][2], A[1][0], A[1][1], A[1][2], A[2][0], A[2][1], A[2][2]);
+    const float c0 = det3_ref(b[0], A[0][1], A[0][2], b[1], A[1][1], A[1][2], b[2], A[2][1], A[2][2]) / (d0 + 1e-7f);
+    const float c1 = det3_ref(A[0][0], b[0], A[0][2], A[1][0], b[1], A[1][2], A[2][0], b[2], A[2][2]) / (d0 + 1e-7f);
+    const float c2 = det3_ref(A[0][0], A[0][1], b[0], A[1][0], A[1][1], b[1], A[2][0], A[2][1], b[2]) / (d0 + 1e-7f);
+    const float rx = pos_b[2 * i] - pos_g[2 * i];
+    const float ry = pos_b[2 * i + 1] - pos_g[2 * i + 1];
+    out_mdbc[i] = std::max(c0 + (rx * c1 + ry * c2), rho0);
+  }
+  return 0;
+}
+
+// moduloOp, v2/cppSrc/computeNeighborhood.h:110-113: float p - q, the rest in
+// double.
+static double modulo_ref(float p, float q, float h) {
+  return ((p - q + h / 2.0) - std::floor((p - q + h / 2.0) / h) * h) - h / 2.0;
+}
+
+// countingKernel / updateKernel, v2/cppSrc/computeNeighborhood.h:115-187.
+// With out_i == NULL only counts[i] is written; otherwise entries go to
+// new_off[i] + n.
+int orc_op_neighborhood(const float* pos_i, const float* h_i, int32_t n, const float* pos_j, const float* h_j,
+                        const int64_t* idx_j, const int32_t* nn, const int32_t* off, const float* dmin,
+                        const float* dmax, const uint8_t* per, int32_t* counts, const int32_t* new_off,
+                        int64_t* out_i, int64_t* out_j, float* out_rij, float* out_xij, float* out_hij) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const float xi = pos_i[2 * i], yi = pos_i[2 * i + 1], hi = h_i[i];
+    int32_t counter = 0;
+    for (int32_t t = 0; t < nn[i]; ++t) {
+      const int32_t j = (int32_t)idx_j[(int64_t)off[i] + t];
+      const float xj = pos_j[2 * j], yj = pos_j[2 * j + 1], hj = h_j[j];
+      auto dx = per[0] ? modulo_ref(xi, xj, dmax[0] - dmin[0]) : xi - xj;
+      auto dy = per[1] ? modulo_ref(yi, yj, dmax[1] - dmin[1]) : yi - yj;
+      auto sum = dx * dx + dy * dy;
+      if (!(sum <= hj * hj)) continue;
+      if (out_i) {
+        const int64_t k = (int64_t)new_off[i] + counter;
+        out_i[k] = i;
+        out_j[k] = j;
+        auto dist = std::sqrt(sum);
+        out_rij[k] = dist / hj;
+        out_xij[2 * k] = dx / (dist + 1e-6f * hi);
+        out_xij[2 * k + 1] = dy / (dist + 1e-6f * hi);
+        out_hij[k] = hj;
+      }
+      counter++;
+    }
+    if (counts) counts[i] = counter;
+  }
+  return 0;
+}
+
+// computeNeighborhood (v2/modules/neighborhood.py:433-460) on a candidate
+// CSR, float32, with the kernel values of neighborSearch (:585-589).  Same
+// count / fill convention as orc_op_neighborhood; positions are wrapped.
+int orc_pairs_filter(const orc_domain* dom, int32_t mode, const float* pos_i, const float* h_i, int32_t n,
+                     const float* pos_j, const float* h_j, const int64_t* idx_j, const int32_t* nn,
+                     const int32_t* off, int32_t* counts, const int32_t* new_off, int64_t* out_i, int64_t* out_j,
+                     float* out_rij, float* out_xij, float* out_hij, float* out_w, float* out_gw) {
+  const DomT<float> d(*dom);
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    int32_t counter = 0;
+    const float hi = h_i ? h_i[i] : 0.f;
+    for (int32_t t = 0; t < nn[i]; ++t) {
+      const int64_t j = idx_j[(int64_t)off[i] + t];
+      const float hj = h_j ? h_j[j] : 0.f;
+      const G<float> g = geom<float>(pos_i + 2 * i, pos_j + 2 * j, pick_h<float>(mode, hi, hj), d);
+      if (!(g.q <= 1.0f)) continue;
+      if (out_i) {
+        const int64_t k = (int64_t)new_off[i] + counter;
+        out_i[k] = i;
+        out_j[k] = j;
+        out_rij[k] = g.q;
+        out_xij[2 * k] = g.ex;
+        out_xij[2 * k + 1] = g.ey;
+        out_hij[k] = g.h;
+        out_w[k] = g.w;
+        out_gw[2 * k] = g.gx;
+        out_gw[2 * k + 1] = g.gy;
+      }
+      counter++;
+    }
+    if (counts) counts[i] = counter;
+  }
+  return 0;
+}
+
+// countNeighborsForParticle / buildNeighborhood, neighborhood.h:100-325, with
+// the cell offsets of neighborhood.cpp:150-163 passed in (noff rows of 2).
+// The hash bucket is the non-negative remainder (see the HIP twin).
+int orc_hash_neighbors(const float* qpos, const float* qh, int32_t nq, const int32_t* offsets, int32_t noff,
+                       const float* spos, const float* sh, const int32_t* table, int32_t hlen, const int32_t* ncell,
+                       const int64_t* cells, float hcell, const float* dmin, const float* dmax, const uint8_t* per,
+                       int32_t mode, int32_t* counts, const int32_t* list_off, int32_t* out_i, int32_t* out_j) {
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nq; ++i) {
+    int32_t cell[2];
+    for (int dd = 0; dd < 2; ++dd)
+      cell[dd] = std::fmin(std::fmax(std::floor((qpos[2 * i + dd] - dmin[dd]) / hcell), -1e9f), 1e9f);
+    int32_t counter = 0;
+    for (int32_t c = 0; c < noff; ++c) {
+      int32_t oc[2];
+      for (int dd = 0; dd < 2; ++dd) {
+        const int32_t v = cell[dd] + offsets[2 * c + dd];
+        oc[dd] = per[dd] ? (v >= 0 ? v % ncell[dd] : ((v % ncell[dd]) + ncell[dd]) % ncell[dd]) : v;
+      }
+      const int32_t lin = (int32_t)((uint32_t)oc[0] + (uint32_t)oc[1] * (uint32_t)ncell[0]);
+      const int32_t hsh = (int32_t)((uint32_t)oc[0] * 73856093u + (uint32_t)oc[1] * 19349663u);
+      const int32_t bucket = ((hsh % hlen) + hlen) % hlen;
+      const int32_t hb = table[2 * bucket], hl = table[2 * bucket + 1];
+      if (hb == -1) continue;
+      for (int32_t t = hb; t < hb + hl; ++t) {
+        if (cells[3 * t] != lin) continue;
+        const int64_t cb = cells[3 * t + 1], ce = cb + cells[3 * t + 2];
+        for (int64_t j = cb; j < ce; ++j) {
+          float sum = 0.f;
+          for (int dd = 0; dd < 2; ++dd) {
+            const float a = qpos[2 * i + dd], b = spos[2 * j + dd];
+            auto diff = per[dd] ? modulo_ref(a, b, dmax[dd] - dmin[dd]) : a - b;
+            sum += diff * diff;
+          }
+          const float dist = std::sqrt(sum);
+          bool keep = mode == 0 ? dist < sh[j] : (mode == 1 ? dist < qh[i] : dist < (qh[i] + sh[j]) / 2.f);
+          if (!keep) continue;
+          if (out_i) {
+            out_i[list_off[i] + counter] = i;
+            out_j[list_off[i] + counter] = (int32_t)j;
+          }
+          counter++;
+        }
+        break;
+      }
+    }
+    if (counts) counts[i] = counter;
+  }
+  return 0;
+}
+
 // The portable float32 atan2 and sin/cos.
 int orc_atan2_pm(const float* y, const float* x, int64_t n, float* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = dsph_pm::atan2f_pm(y[i], x[i]);

@@ -729,3 +729,147 @@ def isph_step(st: IsphState, dom: Domain, h: float, dt: float, rho0: float = 1.0
     info = dict(rho=rho, lap=lap, B=B, p_star=p_star, bicg_iters=it_b, bicg_status=status, bicg_residual=rn, p_ss=p_ss,
                 jacobi_iters=it_j, jacobi_error=err_j, vprime=vp, shift=sh, off=off, nbr=nbr)
     return new, info
+
+
+# ---------------------------------------------------------------------------
+# Drop-in ops on the reference's per-pair arrays and the compact hash map
+# ---------------------------------------------------------------------------
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _u8(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=bool).astype(np.uint8))
+
+
+def op_viscosity(vel_i, vel_j, m_j, rho_i, rho_j, h_i, idx_j, supports, grads, dists, vecs, nn, off, cs, rho0, alpha,
+                 eps, pi_switch):
+    """viscosityFn (partiBench/viscosityKernel.h:98-145) over CSR rows."""
+    n = len(nn)
+    out = np.zeros((n, 2), np.float32)
+    lib().orc_op_viscosity(_p(f32(vel_i)), _p(f32(vel_j)), _p(f32(m_j)), _p(f32(rho_i)), _p(f32(rho_j)), _p(f32(h_i)),
+                           _p(_i64(idx_j)), _p(f32(supports)), _p(f32(grads)), _p(f32(dists)), _p(f32(vecs)),
+                           C.c_int32(n), _p(_i32(nn)), _p(_i32(off)), C.c_float(cs), C.c_float(rho0),
+                           C.c_float(alpha), C.c_float(eps), C.c_int32(1 if pi_switch else 0), _p(out))
+    return out
+
+
+def op_mdbc(m_j, rho_j, pos_b, pos_g, idx_j, W, grads, dists, vecs, supports, nn, off, rho0):
+    """mDBCDensityKernel (v2/cppSrc/mDBC.h:129-244); returns (mDBC, Shepard)."""
+    nb = len(nn)
+    out = np.zeros(nb, np.float32)
+    shep = np.zeros(nb, np.float32)
+    lib().orc_op_mdbc(_p(f32(m_j)), _p(f32(rho_j)), _p(f32(pos_b)), _p(f32(pos_g)), C.c_int32(nb), _p(_i64(idx_j)),
+                      _p(f32(W)), _p(f32(grads)), _p(f32(dists)), _p(f32(vecs)), _p(f32(supports)), _p(_i32(nn)),
+                      _p(_i32(off)), C.c_float(rho0), _p(out), _p(shep))
+    return out, shep
+
+
+def op_neighborhood(pos_i, h_i, pos_j, h_j, idx_j, nn, off, dmin, dmax, periodic):
+    """computeNeighborhood, C++ form (v2/cppSrc/computeNeighborhood.cpp:5-94):
+    returns [counts, offsets, i, j, rij, xij, hij] as the reference does."""
+    n = len(nn)
+    args = (_p(f32(pos_i)), _p(f32(h_i)), C.c_int32(n), _p(f32(pos_j)), _p(f32(h_j)), _p(_i64(idx_j)), _p(_i32(nn)),
+            _p(_i32(off)), _p(f32(dmin)), _p(f32(dmax)), _p(_u8(periodic)))
+    counts = np.zeros(n, np.int32)
+    lib().orc_op_neighborhood(*args, _p(counts), None, None, None, None, None, None)
+    new_off = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])[:-1].astype(np.int32)
+    P = int(counts.sum())
+    oi, oj = np.zeros(P, np.int64), np.zeros(P, np.int64)
+    rij, xij, hij = np.zeros(P, np.float32), np.zeros((P, 2), np.float32), np.zeros(P, np.float32)
+    lib().orc_op_neighborhood(*args, None, _p(new_off), _p(oi), _p(oj), _p(rij), _p(xij), _p(hij))
+    return [counts, new_off, oi, oj, rij, xij, hij]
+
+
+def pairs_filter(dom, mode, pos_i, h_i, pos_j, h_j, idx_j, nn, off):
+    """computeNeighborhood (v2/modules/neighborhood.py:433-460) on a candidate
+    CSR plus Wendland C2 kernels; returns (counts, offsets, i, j, rij, xij,
+    hij, W, gradW)."""
+    n = len(nn)
+    args = (C.byref(dom), C.c_int32(mode), _p(f32(pos_i)), _p(f32(h_i)), C.c_int32(n), _p(f32(pos_j)), _p(f32(h_j)),
+            _p(_i64(idx_j)), _p(_i32(nn)), _p(_i32(off)))
+    counts = np.zeros(n, np.int32)
+    lib().orc_pairs_filter(*args, _p(counts), None, None, None, None, None, None, None, None)
+    new_off = np.concatenate([[0], np.cumsum(counts, dtype=np.int64)])[:-1].astype(np.int32)
+    P = int(counts.sum())
+    oi, oj = np.zeros(P, np.int64), np.zeros(P, np.int64)
+    rij, xij, hij = np.zeros(P, np.float32), np.zeros((P, 2), np.float32), np.zeros(P, np.float32)
+    W, gW = np.zeros(P, np.float32), np.zeros((P, 2), np.float32)
+    lib().orc_pairs_filter(*args, None, _p(new_off), _p(oi), _p(oj), _p(rij), _p(xij), _p(hij), _p(W), _p(gW))
+    return counts, new_off, oi, oj, rij, xij, hij, W, gW
+
+
+def cell_offsets(search_range: int, dim: int = 2) -> np.ndarray:
+    """The cell offset table of neighborhood.cpp:150-163, generated by the
+    reference's own counter loop (row o, column dim-d-1)."""
+    rows = (1 + 2 * search_range) ** dim
+    off = np.zeros((rows, dim), np.int32)
+    for d in range(dim):
+        itr, ctr = -search_range, 0
+        for o in range(rows):
+            c = o % (1 + 2 * search_range) ** d
+            if c == 0 and ctr > 0:
+                itr += 1
+            if itr > search_range:
+                itr = -search_range
+            off[o, dim - d - 1] = itr
+            ctr += 1
+    return off
+
+
+def build_compact_hash_map(x, min_domain, max_domain, h_max, hash_map_length):
+    """buildCompactHashMap (neighborhoodDebug.ipynb cell 4, with
+    sortReferenceParticles / hashCellIndices of cell 3) in numpy int32 /
+    float32 arithmetic.  Returns dict(sortIndex, sortedPositions, hashTable
+    [H,2] int32, cellTable [C,3] int64, numCells [2] int32, hCell float,
+    qMin)."""
+    x = f32(x)
+    mn, mx = f32(min_domain), f32(max_domain)
+    ext = mx - mn
+    ncf = np.floor(ext / np.float32(h_max))
+    h_cell = np.float32(np.max(ext / ncf))                       # compute_h
+    num_cells = np.ceil(ext / h_cell).astype(np.int32)
+    idx = np.floor((x - mn) / h_cell).astype(np.int32)
+    lin = idx[:, 0] + num_cells[0] * idx[:, 1]
+    sort_index = np.argsort(lin, kind="stable")
+    sorted_lin = lin[sort_index]
+    sorted_pos = x[sort_index]
+    cell_ids, first, counts = np.unique(sorted_lin, return_index=True, return_counts=True)
+    cum = np.concatenate([[0], np.cumsum(counts)])[:-1]
+    sorted_idx = np.floor((sorted_pos - mn) / h_cell).astype(np.int32)
+    grid_idx = sorted_idx[cum]
+    cell_table = np.stack([cell_ids.astype(np.int64), cum.astype(np.int64), counts.astype(np.int64)], axis=1)
+    with np.errstate(over="ignore"):
+        hsh = (grid_idx[:, 0] * np.int32(73856093) + grid_idx[:, 1] * np.int32(19349663)).astype(np.int32)
+    hashed = np.mod(hsh, hash_map_length)                          # torch % : non-negative
+    order = np.argsort(hashed, kind="stable")
+    hm, hfirst, hcount = np.unique(hashed[order], return_index=True, return_counts=True)
+    table = np.full((hash_map_length, 2), -1, np.int32)
+    table[:, 1] = 0
+    table[hm, 0] = hfirst
+    table[hm, 1] = hcount
+    return dict(sortIndex=sort_index, sortedPositions=sorted_pos, hashTable=table,
+                cellTable=np.ascontiguousarray(cell_table[order]), numCells=num_cells, hCell=float(h_cell), qMin=mn)
+
+
+def hash_neighbors(qpos, qh, search_range, spos, sh, table, ncell, cells, hcell, dmin, dmax, periodic, mode):
+    """countNeighbors + buildNeighborList (neighborhood.cpp:65-435) in the
+    reference's CPU order; returns (counts, i, j) with j in sorted order."""
+    nq = qpos.shape[0]
+    offs = cell_offsets(search_range)
+    mode_i = {"scatter": 0, "gather": 1, "symmetric": 2}[mode]
+    args = (_p(f32(qpos)), _p(None if qh is None else f32(qh)), C.c_int32(nq), _p(offs), C.c_int32(offs.shape[0]),
+            _p(f32(spos)), _p(None if sh is None else f32(sh)), _p(_i32(table)), C.c_int32(table.shape[0]),
+            _p(_i32(ncell)), _p(_i64(cells)), C.c_float(hcell), _p(f32(dmin)), _p(f32(dmax)), _p(_u8(periodic)),
+            C.c_int32(mode_i))
+    counts = np.zeros(nq, np.int32)
+    lib().orc_hash_neighbors(*args, _p(counts), None, None, None)
+    list_off = np.concatenate([[0], np.cumsum(counts)])[:-1].astype(np.int32)
+    P = int(counts.sum())
+    oi, oj = np.zeros(P, np.int32), np.zeros(P, np.int32)
+    lib().orc_hash_neighbors(*args, None, _p(list_off), _p(oi), _p(oj))
+    return counts, oi, oj
