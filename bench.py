@@ -200,10 +200,12 @@ def main():
     # Algorithmic bytes (each per-particle array once per launch, each pair
     # entry once; gathered neighbor fields and list padding not counted):
     # sweep A: p 4 + rho 4 + count 4 + a(write) 8 = 20 B/particle;
-    #          j 4 + V_j gradW_ij 8 = 12 B/pair
-    # sweep B: a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 32 B/particle; 12 B/pair
-    bytes_a = 20 * n + 12 * P
-    bytes_b = 32 * n + 12 * P
+    #          j (2 as a 16-bit delta, else 4) + V_j gradW_ij 8 = 10 (12) B/pair
+    # sweep B: a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 32 B/particle; 10 (12) B/pair
+    idx16 = solver.nl.nbr16 is not None and int(solver.nl.nbr16_overflow.item()) == 0
+    per_pair = 10 if idx16 else 12
+    bytes_a = 20 * n + per_pair * P
+    bytes_b = 32 * n + per_pair * P
     gbs_a = bytes_a / t_a / 1e9
     gbs_b = bytes_b / t_b / 1e9
     log(f"sweep A (dsph_pressure_accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
@@ -230,6 +232,7 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(byt),
                 "traffic_over_algorithmic": None if ratio is None else round(ratio, 4),
+                "neighbor_index": "int16 delta" if idx16 else "int32",
                 "other_sweep": {"kernel": "k_dfsph_sweep_a" if kname == "k_dfsph_sweep_b" else "k_dfsph_sweep_b",
                                 "achieved": round(gbs_a if kname == "k_dfsph_sweep_b" else gbs_b, 1),
                                 "kernel_us": round((t_a if kname == "k_dfsph_sweep_b" else t_b) * 1e6, 2)}}

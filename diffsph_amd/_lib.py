@@ -40,6 +40,8 @@ class Pairs(C.Structure):
         ("domain", Domain),
         ("counts", C.c_void_p),
         ("slice_off", C.c_void_p),
+        ("nbr16", C.c_void_p),
+        ("nbr16_overflow", C.c_void_p),
     ]
 
 
@@ -156,6 +158,21 @@ class Batch(C.Structure):
     ]
 
 
+class ShiftParams(C.Structure):
+    _fields_ = [
+        ("dx", C.c_double),
+        ("support", C.c_double),
+        ("kernel_scale", C.c_float),
+        ("R", C.c_float),
+        ("n", C.c_float),
+        ("cfl", C.c_float),
+        ("cs", C.c_float),
+        ("threshold", C.c_float),
+        ("mach", C.c_float),
+        ("pad_", C.c_int32),
+    ]
+
+
 V = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
@@ -177,6 +194,7 @@ SIGNATURES = {
     "dsph_sell_layout": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
     "dsph_sell_to_csr": (C.c_int, [P(Pairs), V, V]),
     "dsph_wrap_positions": (C.c_int, [P(Domain), V, I64, V, V]),
+    "dsph_neighbor_compact16": (C.c_int, [P(Pairs), I32, V, I64, V, V]),
     "dsph_scan_workspace": (SZ, [I64]),
     "dsph_scan_exclusive": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
     "dsph_gather_f32": (C.c_int, [V, V, I64, I32, V, V]),
@@ -211,6 +229,8 @@ SIGNATURES = {
     "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, I32, V, V, V]),
     "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),
     "dsph_rk4_combine": (C.c_int, [I64, P(RK4Stages), V, V, V, F32, V, V, V, V]),
+    "dsph_shift_workspace": (SZ, [I64]),
+    "dsph_deltaplus_shift": (C.c_int, [P(Pairs), P(Boundary), P(ShiftParams), V, V, V, V, V, V, SZ, V]),
     "dsph_max_norm_workspace": (SZ, []),
     "dsph_max_norm2": (C.c_int, [V, I64, P(C.c_float), V, SZ, V]),
     "dsph_sum_squares": (C.c_int, [V, I64, P(C.c_double), V, SZ, V]),

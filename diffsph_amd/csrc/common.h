@@ -148,6 +148,8 @@ struct PairsDev {
   Dom dom;
   const int32_t* cnt;
   const int32_t* slice;
+  const int16_t* nbr16;  // optional 16-bit deltas (pressure sweeps)
+  const int32_t* ovf;    // nonzero: nbr16 invalid
 };
 
 inline PairsDev make_pairs(const dsph_pairs& p) {
@@ -165,6 +167,8 @@ inline PairsDev make_pairs(const dsph_pairs& p) {
   o.dom = make_dom(p.domain);
   o.cnt = p.counts;
   o.slice = p.slice_off;
+  o.nbr16 = p.nbr16;
+  o.ovf = p.nbr16_overflow;
   return o;
 }
 
@@ -262,6 +266,29 @@ __device__ __forceinline__ void seg_bounds(const int32_t* off, const int32_t* nn
 
 __device__ __forceinline__ float h_of(const float* arr, int idx, float h0) {
   return arr ? arr[idx] : h0;
+}
+
+// XCD-aware block order.  The dispatcher deals the workgroups of a launch
+// round-robin over the 8 XCDs (block b runs on XCD b % 8), each with its own
+// L2.  logical_block() renumbers them so that XCD x processes one contiguous
+// range of logical blocks: particles in cell order are spatially local, so
+// each XCD's L2 then holds the neighbor fields of its own 1/8 of the domain
+// instead of all of it.  A bijection onto [0, gridDim.x); results do not
+// depend on it.
+#ifndef DSPH_XCD_REMAP
+#define DSPH_XCD_REMAP 1
+#endif
+constexpr int kNumXcd = 8;
+
+__device__ __forceinline__ int logical_block() {
+#if DSPH_XCD_REMAP
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb / kNumXcd, r = nb % kNumXcd;
+  const int x = b % kNumXcd, k = b / kNumXcd;
+  return x * q + (x < r ? x : r) + k;
+#else
+  return blockIdx.x;
+#endif
 }
 
 inline int grid_for(int64_t n, int block = kBlock) {

@@ -542,6 +542,74 @@ static int check_same_set(const dsph_pairs* pairs) {
   return DSPH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// delta+ particle shifting (shifting.py:119-147, applied as shifting.py:
+// 997-1018 and runner.py:40-42).
+// ---------------------------------------------------------------------------
+struct ShiftArgs {
+  const float* mass;
+  const float* rho;
+  const float* vmax;  // device max |v| (when mach < 0)
+  float2* pos;
+  float2* shift_out;
+  float w0, R, nexp, cfl, kscale, thr, cs, mach;
+  int nint;  // integral exponent in [0, 16], else -1
+};
+
+// (W / W_0)^n: a double product chain rounded once to float for integral n.
+__device__ __forceinline__ float shift_pow(float k, const ShiftArgs& A) {
+  if (A.nint >= 0) {
+    double p = 1.0;
+    for (int t = 0; t < A.nint; ++t) p *= (double)k;
+    return (float)p;
+  }
+  return powf(k, A.nexp);
+}
+
+__device__ __forceinline__ void shift_term(const Geom& g, float m_j, float rho_i, float rho_j, const ShiftArgs& A,
+                                           float& sx, float& sy) {
+  const float kn = shift_pow(g.w / A.w0, A);
+  const float term = 1.0f + A.R * kn;
+  const float s = term * (m_j / (rho_i + rho_j));
+  sx += s * g.gx;
+  sy += s * g.gy;
+}
+
+__global__ void __launch_bounds__(kBlock) k_deltaplus_shift(PairsDev P, BoundaryDev B, ShiftArgs A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float rho_i = A.rho[i];
+  float sx = 0.f, sy = 0.f;
+  const RowRef rr = row_ref(P, i);
+  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+    const int j = P.nbr[k];
+    const Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, h_of(P.rh, j, P.h0)), P.dom);
+    shift_term(g, A.mass[j], rho_i, A.rho[j], A, sx, sy);
+  }
+  if (B.on) {
+    const RowRef rb = row_ref(B.P, i);
+    for (int n = 0, k = rb.base; n < rb.cnt; ++n, k += rb.step) {
+      const int b = B.P.nbr[k];
+      const Geom g = pair_geom(xi, B.P.rpos[b], pick_h(B.P.mode, hi, h_of(B.P.rh, b, B.P.h0)), B.P.dom);
+      shift_term(g, B.mass[b], rho_i, B.rho[b], A, sx, sy);
+    }
+  }
+  const float ma = A.mach >= 0.0f ? A.mach : *A.vmax / A.cs;
+  const float hk = (hi / A.kscale) * 2.0f;
+  const float sc = (-A.cfl * ma) * (hk * hk);
+  float ux = -(sc * sx), uy = -(sc * sy);
+  // torch.clamp(update, -thr, thr): NaN stays NaN
+  ux = ux < -A.thr ? -A.thr : (ux > A.thr ? A.thr : ux);
+  uy = uy < -A.thr ? -A.thr : (uy > A.thr ? A.thr : uy);
+  const float2 x = A.pos[i];
+  // solveShifting returns dx = (x - u) - x; the runner adds it
+  const float dx = (x.x - ux) - x.x, dy = (x.y - uy) - x.y;
+  A.pos[i] = make_float2(x.x + dx, x.y + dy);
+  if (A.shift_out) A.shift_out[i] = make_float2(dx, dy);
+}
+
 }  // namespace dsph
 
 using namespace dsph;
@@ -692,6 +760,58 @@ extern "C" int dsph_max_norm2(const float* v, int64_t n, float* out_host, void* 
   DSPH_CHECK_HIP(hipMemcpyAsync(out_host, out, sizeof(float), hipMemcpyDeviceToHost, st));
   DSPH_CHECK_HIP(hipStreamSynchronize(st));
   return DSPH_OK;
+}
+
+extern "C" size_t dsph_shift_workspace(int64_t) { return dsph_max_norm_workspace(); }
+
+extern "C" int dsph_deltaplus_shift(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                    const dsph_shift_params* prm, const float* mass, const float* rho,
+                                    const float* vel, float* pos, float* shift_out, void* ws, size_t ws_bytes,
+                                    void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm || pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  s = check_boundary(boundary, pairs);
+  if (s != DSPH_OK) return s;
+  if (!(prm->dx > 0.0) || !(prm->support > 0.0) || !(prm->kernel_scale > 0.f) || !(prm->cs > 0.f))
+    return DSPH_ERR_ARG;
+  const int64_t n = pairs->nq;
+  if (n == 0) return DSPH_OK;
+  if (!mass || !rho || !pos || (prm->mach < 0.f && !vel)) return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_shift_workspace(n)) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  // W_0 = kernel(tensor(dx / support), tensor(support)) in float32
+  const float q0 = (float)(prm->dx / prm->support);
+  const float h0 = (float)prm->support;
+  float t = 1.0f - q0;
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  const float t4 = ((t * t) * t) * t;
+  const float w0 = ((t4 * (1.0f + 4.0f * q0)) * kC2) / (h0 * h0);
+  ShiftArgs A;
+  A.mass = mass;
+  A.rho = rho;
+  A.pos = reinterpret_cast<float2*>(pos);
+  A.shift_out = reinterpret_cast<float2*>(shift_out);
+  A.w0 = w0;
+  A.R = prm->R;
+  A.nexp = prm->n;
+  A.nint = (prm->n >= 0.f && prm->n <= 16.f && prm->n == floorf(prm->n)) ? (int)prm->n : -1;
+  A.cfl = prm->cfl;
+  A.kscale = prm->kernel_scale;
+  A.thr = (float)((double)prm->threshold * prm->dx);
+  A.cs = prm->cs;
+  A.mach = prm->mach;
+  float* vmax = reinterpret_cast<float*>(static_cast<char*>(ws) + sizeof(double) * 1032);
+  A.vmax = vmax;
+  if (prm->mach < 0.f) {
+    const int nb = 1024;
+    float* part = static_cast<float*>(ws);
+    k_max_norm<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(vel), (int)n, part);
+    k_max_final<<<1, 64, 0, st>>>(part, nb, vmax);
+    DSPH_LAUNCH_CHECK();
+  }
+  k_deltaplus_shift<<<grid_for(n), kBlock, 0, st>>>(make_pairs(*pairs), make_boundary(boundary), A);
+  return launch_status();
 }
 
 extern "C" int dsph_sum_squares(const float* v, int64_t n, double* out_host, void* ws, size_t ws_bytes, void* stream) {

@@ -176,10 +176,68 @@ __device__ __forceinline__ float2 ld_stream(const float2* p) {
   return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
 }
 
+// Neighbor index of entry k of row i: the int32 list, or the 16-bit modular
+// delta j = (i + d) mod n of the compact copy.
+struct Idx32 {
+  const int32_t* a;
+  __device__ __forceinline__ int operator()(int k) const { return ld_stream(a + k); }
+};
+struct Idx16 {
+  const int16_t* a;
+  int i, n;
+  __device__ __forceinline__ int operator()(int k) const {
+    int j = i + (int)__builtin_nontemporal_load(a + k);
+    j += j < 0 ? n : 0;
+    j -= j >= n ? n : 0;
+    return j;
+  }
+};
+
+__device__ __forceinline__ bool use_idx16(const PairsDev& P) { return P.nbr16 && *P.ovf == 0; }
+
 // Entries per unrolled group: the group's index and coefficient loads are
 // issued together, then its gathers, then the additions in entry order (the
 // sum is accumulated exactly as the one-at-a-time loop does).
-constexpr int kUnroll = 4;
+#ifndef DSPH_SWEEP_UNROLL
+#define DSPH_SWEEP_UNROLL 4
+#endif
+constexpr int kUnroll = DSPH_SWEEP_UNROLL;
+
+// Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij;
+// with `diff` the ISPH form (incompressible.py:276-285, gradient
+// 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
+// `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
+// else evaluated on the fly.
+template <class Idx>
+__device__ __forceinline__ void sweep_a_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
+                                           const float* __restrict__ p, float pi, float sj, float& ax, float& ay) {
+  int n = 0, k = rr.base;
+  for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
+    int j[kUnroll];
+    float2 c[kUnroll];
+    float pj[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      j[u] = idx(k + u * rr.step);
+      c[u] = ld_stream(kv + k + u * rr.step);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) pj[u] = sj * p[j[u]];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float s = pi + pj[u];
+      ax += s * c[u].x;
+      ay += s * c[u].y;
+    }
+  }
+  for (; n < rr.cnt; ++n, k += rr.step) {
+    const int j = idx(k);
+    const float2 c = ld_stream(kv + k);
+    const float s = pi + sj * p[j];
+    ax += s * c.x;
+    ay += s * c.y;
+  }
+}
 
 // Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij;
 // with `diff` the ISPH form (incompressible.py:276-285, gradient
@@ -191,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
                                                           const float* __restrict__ rho, const float* __restrict__ p,
                                                           float2* __restrict__ acc, const SolveCtrl* ctrl, int diff) {
   if (ctrl && ctrl->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float pi = p[i];
   // -p_j turns the sum into the difference form; negation is exact
@@ -199,32 +257,10 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
   if (kv) {
-    int n = 0, k = rr.base;
-    for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
-      int j[kUnroll];
-      float2 c[kUnroll];
-      float pj[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        j[u] = ld_stream(P.nbr + k + u * rr.step);
-        c[u] = ld_stream(kv + k + u * rr.step);
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) pj[u] = sj * p[j[u]];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const float s = pi + pj[u];
-        ax += s * c[u].x;
-        ay += s * c[u].y;
-      }
-    }
-    for (; n < rr.cnt; ++n, k += rr.step) {
-      const int j = ld_stream(P.nbr + k);
-      const float2 c = ld_stream(kv + k);
-      const float s = pi + sj * p[j];
-      ax += s * c.x;
-      ay += s * c.y;
-    }
+    if (use_idx16(P))
+      sweep_a_kv(rr, Idx16{P.nbr16, i, P.nr}, kv, p, pi, sj, ax, ay);
+    else
+      sweep_a_kv(rr, Idx32{P.nbr}, kv, p, pi, sj, ax, ay);
   } else {
     const float2 xi = P.qpos[i];
     const float hi = h_of(P.qh, i, P.h0);
@@ -259,6 +295,34 @@ struct SweepBArgs {
   int isph;  // error metric |r| (incompressible.py:357) instead of max(r/rho0, -thr)
 };
 
+template <class Idx>
+__device__ __forceinline__ float sweep_b_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
+                                            const float2* __restrict__ acc, float2 ai) {
+  float sum = 0.f;
+  int n = 0, k = rr.base;
+  for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
+    int j[kUnroll];
+    float2 c[kUnroll];
+    float2 aj[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      j[u] = idx(k + u * rr.step);
+      c[u] = ld_stream(kv + k + u * rr.step);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) aj[u] = acc[j[u]];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) sum += (aj[u].x - ai.x) * c[u].x + (aj[u].y - ai.y) * c[u].y;
+  }
+  for (; n < rr.cnt; ++n, k += rr.step) {
+    const int j = idx(k);
+    const float2 c = ld_stream(kv + k);
+    const float2 aj = acc[j];
+    sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
+  }
+  return sum;
+}
+
 // Sweep B (dfsph.py:39-50): K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij,
 // p_i += omega (s_i - K_i) / alpha_i (clamped at 0, in place: a thread
 // touches only its own p_i), residual K_i - s_i, per-block float64 partial
@@ -266,34 +330,18 @@ struct SweepBArgs {
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs B) {
   __shared__ double sh[kBlock / 64];
   if (B.ctrl && B.ctrl->done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = logical_block();
+  int i = lb * blockDim.x + threadIdx.x;
   double err = 0.0;
   if (i < P.nq) {
     const float2 ai = B.acc[i];
     float sum = 0.f;
     const RowRef rr = row_ref(P, i);
     if (B.kv) {
-      int n = 0, k = rr.base;
-      for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
-        int j[kUnroll];
-        float2 c[kUnroll];
-        float2 aj[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          j[u] = ld_stream(P.nbr + k + u * rr.step);
-          c[u] = ld_stream(B.kv + k + u * rr.step);
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) aj[u] = B.acc[j[u]];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) sum += (aj[u].x - ai.x) * c[u].x + (aj[u].y - ai.y) * c[u].y;
-      }
-      for (; n < rr.cnt; ++n, k += rr.step) {
-        const int j = ld_stream(P.nbr + k);
-        const float2 c = ld_stream(B.kv + k);
-        const float2 aj = B.acc[j];
-        sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
-      }
+      if (use_idx16(P))
+        sum = sweep_b_kv(rr, Idx16{P.nbr16, i, P.nr}, B.kv, B.acc, ai);
+      else
+        sum = sweep_b_kv(rr, Idx32{P.nbr}, B.kv, B.acc, ai);
     } else {
       const float2 xi = P.qpos[i];
       const float hi = h_of(P.qh, i, P.h0);
@@ -319,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     }
   }
   double bs = block_sum_d(err, sh);
-  if (threadIdx.x == 0) B.partials[blockIdx.x] = bs;
+  if (threadIdx.x == 0) B.partials[lb] = bs;
 }
 
 // One block: fixed-order sum of the sweep-B partials, the reference's loop
@@ -499,6 +547,51 @@ SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const flo
 }  // namespace dsph
 
 using namespace dsph;
+
+// 16-bit modular neighbor deltas (dsph_neighbor_compact16): one thread per
+// row, entries in the list's own layout.
+__global__ void k_ovf_clear(int32_t* flag) {
+  if (threadIdx.x == 0) *flag = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact16(PairsDev P, int limit, int16_t* __restrict__ out,
+                                                      int64_t cap, int32_t* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const RowRef rr = row_ref(P, i);
+  const int n = P.nr, half = P.nr / 2;
+  bool bad = false;
+  for (int t = 0, k = rr.base; t < rr.cnt; ++t, k += rr.step) {
+    if (k < 0 || k >= cap) {
+      bad = true;
+      continue;
+    }
+    int d = P.nbr[k] - i;
+    if (d > half) d -= n;
+    if (d < -half) d += n;
+    if (d > limit || d < -limit) {
+      bad = true;
+      d = 0;
+    }
+    out[k] = (int16_t)d;
+  }
+  if (bad) *flag = 1;
+}
+
+extern "C" int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit, int16_t* out, int64_t capacity,
+                                       int32_t* overflow, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (limit < 0 || limit > 32767 || capacity < 0) return DSPH_ERR_ARG;
+  if (!overflow) return DSPH_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  k_ovf_clear<<<1, 64, 0, st>>>(overflow);
+  DSPH_LAUNCH_CHECK();
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!out || !pairs->nbr) return DSPH_ERR_ARG;
+  k_compact16<<<grid_for(pairs->nq), kBlock, 0, st>>>(make_pairs(*pairs), limit, out, capacity, overflow);
+  return launch_status();
+}
 
 extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, const float* mass, const float* areas,
                                   const float* vol, const float* rho, float dt, float rho0,

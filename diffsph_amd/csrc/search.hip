@@ -161,9 +161,20 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
 // Fill row i.  CSR (slice == NULL): entries at off[i] + n.  Sliced ELL:
 // entries at slice[i/64] + 64 n + i%64, the rest of the slice's columns
 // padded with -1.  Writes stay below `cap` and inside the row's own range.
+// In the sliced layout the first kFillStage entries of every row are staged
+// in LDS (a per-thread column: conflict-free) while the candidates are
+// scanned, then written column by column, so each store of the wave is one
+// contiguous 256-byte line instead of 64 scattered words; entries past
+// kFillStage are written as they are found.
+#ifndef DSPH_FILL_STAGE
+#define DSPH_FILL_STAGE 32
+#endif
+constexpr int kFillStage = DSPH_FILL_STAGE;
+
 __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ slice,
                                                           int32_t* __restrict__ nbr, int64_t cap) {
+  __shared__ int32_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   float2 qi = a.qpos[i];
@@ -180,19 +191,28 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     step = 1;
     len = (int64_t)off[i + 1] - off[i];
   }
+  const int64_t staged = slice ? (kFillStage < len ? kFillStage : len) : 0;
   int64_t n = 0;
   for_candidates(a, qi, [&](int b, int e) {
     scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
                   [&](int j, const Cand& r) {
                     if (keep_pair(qi, hi, r.x, r.h, a)) {
-                      const int64_t o = base + n * step;
-                      if (n < len && o < cap) nbr[o] = j;
+                      if (n < staged) {
+                        stage[n][threadIdx.x] = j;
+                      } else {
+                        const int64_t o = base + n * step;
+                        if (n < len && o < cap) nbr[o] = j;
+                      }
                       ++n;
                     }
                   });
   });
-  for (; n < len; ++n) {
-    const int64_t o = base + n * step;
+  for (int64_t t = 0; t < staged; ++t) {
+    const int64_t o = base + t * step;
+    if (o < cap) nbr[o] = t < n ? stage[t][threadIdx.x] : -1;
+  }
+  for (int64_t t = n > staged ? n : staged; t < len; ++t) {
+    const int64_t o = base + t * step;
     if (o < cap) nbr[o] = -1;
   }
 }

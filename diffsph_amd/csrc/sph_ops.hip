@@ -259,6 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_op_scatter_add(const float* __restri
 int check_pairs(const dsph_pairs* p) {
   if (!p || p->nq < 0 || p->nr < 0 || p->nq >= INT32_MAX || p->nr >= INT32_MAX) return DSPH_ERR_ARG;
   if (p->mode < 0 || p->mode > 2) return DSPH_ERR_ARG;
+  if (p->nbr16 && !p->nbr16_overflow) return DSPH_ERR_ARG;
   if (p->nq > 0 && (!p->qpos || (!p->slice_off && !p->offsets))) return DSPH_ERR_ARG;
   if (p->nq > 0 && p->slice_off && !p->counts) return DSPH_ERR_ARG;
   if (p->nr > 0 && !p->rpos) return DSPH_ERR_ARG;
@@ -284,7 +285,7 @@ extern "C" const char* dsph_status_string(int s) {
   }
 }
 
-extern "C" int dsph_abi_version(void) { return 1; }
+extern "C" int dsph_abi_version(void) { return 2; }
 
 extern "C" int dsph_density(const dsph_pairs* pairs, const float* rmass, const float* qmass, float* rho, float* vol,
                             void* stream) {

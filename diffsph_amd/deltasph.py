@@ -79,6 +79,15 @@ class DeltaSPHConfig:
     # cell side of the search grids = 1.001 * support * cell_scale (>= 1);
     # changes the order of the neighbor entries, not the neighbor sets
     cell_scale: float = 1.0
+    # delta+ particle shifting after every step (solveShifting with scheme
+    # 'deltaSPH', runner.py:40-42; shifting.py:735-765 defaults); off in the
+    # harness configurations, as in the reference's benchmark notebooks
+    shifting: bool = False
+    shift_R: float = 0.25
+    shift_n: float = 4.0
+    shift_cfl: float = 1.5
+    shift_compute_mach: bool = True
+    shift_threshold: float = 0.5
 
     @property
     def kernel_scale(self) -> float:
@@ -321,11 +330,42 @@ class DeltaSPHSolver:
             self.drhodt = torch.empty_like(self.rho)
         check(L_.dsph_rk4_combine(self.n, C.byref(stages), ptr(self.pos), ptr(self.vel), ptr(self.rho),
                                   C.c_float(dt), None, ptr(self.dudt), ptr(self.drhodt), st), "rk4_combine")
+        if self.cfg.shifting:
+            self.shift()
         self.time += dt
         self.step_count += 1
         if self.cfg.adaptive:
             self.dt = compute_timestep(self.cfg, self.dt, self.max_accel())
         return dt
+
+    def shift(self, shift_out: torch.Tensor | None = None):
+        """delta+ particle shifting of the current state (solveShifting,
+        v2/modules/shifting.py:880-1018 with scheme 'deltaSPH'): a fresh
+        neighbor search, fluid and wall neighbors, positions moved in place;
+        `shift_out` [n,2] receives the displacement."""
+        cfg = self.cfg
+        st = _lib.stream_handle(self.pos.device)
+        nl, pr, wpos, grid = self.neighbors(self.pos)
+        bnd, keep = None, None
+        if self.walls is not None:
+            W = self.walls
+            nl_b = search(W.grid, wpos, None, "scatter", qpos_wrapped=True)
+            pr_b = nl_b.pairs()
+            bnd = _lib.Boundary()
+            bnd.pairs = C.pointer(pr_b)
+            bnd.mass, bnd.rho, bnd.vel = ptr(W.mass), ptr(W.rho), ptr(W.vel)
+            bnd.visc_coeff, bnd.visc_eps, bnd.viscosity, bnd.pi_switch = 0.0, 0.0, -1, 0
+            keep = (nl_b, pr_b)
+        prm = _lib.ShiftParams()
+        prm.dx, prm.support, prm.kernel_scale = cfg.dx, cfg.support, cfg.kernel_scale
+        prm.R, prm.n, prm.cfl, prm.cs = cfg.shift_R, cfg.shift_n, cfg.shift_cfl, cfg.cs
+        prm.threshold = cfg.shift_threshold
+        prm.mach = -1.0 if cfg.shift_compute_mach else 0.1
+        ws = WORKSPACE.get(lib().dsph_shift_workspace(self.n), self.pos.device)
+        check(lib().dsph_deltaplus_shift(C.byref(pr), C.byref(bnd) if bnd is not None else None, C.byref(prm),
+                                         ptr(self.mass), ptr(self.rho), ptr(self.vel), ptr(self.pos), ptr(shift_out),
+                                         ptr(ws), ws.numel(), st), "deltaplus_shift")
+        self._keep_shift = (nl, wpos, keep)
 
     def max_accel(self) -> float:
         """The acceleration of the time-step condition (see accel_norm)."""

@@ -53,6 +53,11 @@ class DFSPHConfig:
     pi_switch: bool = False
     gravity: tuple | None = None
     check_every: int = 0
+    # the pressure sweeps read a 16-bit modular-delta copy of the neighbor
+    # list (falls back to the int32 list on the device when a delta does not
+    # fit); results are identical either way
+    compact_index: bool = True
+    compact_limit: int = 32767      # largest |delta| kept in 16 bits (smaller values force the fallback)
 
 
 @dataclass
@@ -101,7 +106,13 @@ class DFSPHSolver:
         self.nl = None
         self.step_count = 0
         self._last_iters = {}
-        self.hooks = {}  # optional: 'den_init' -> callable(solver) returning an initial pressure
+        # optional hooks: 'div_init' / 'den_init' -> callable(solver) returning
+        # an initial pressure; 'after_div' -> callable(solver); 'div_split' ->
+        # (k1, callable(solver, p) returning a replacement pressure after k1
+        # divergence iterations)
+        self.hooks = {}
+        self._nbr16 = None
+        self._nbr16_flag = None
 
     # -- helpers ---------------------------------------------------------
     def _permute(self, t, idx):
@@ -132,6 +143,11 @@ class DFSPHSolver:
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
             self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
+        if cfg.compact_index:
+            if self._nbr16 is None or self._nbr16.shape[0] < need:
+                self._nbr16 = torch.empty(need + need // 8, dtype=torch.int16, device=self.pos.device)
+                self._nbr16_flag = torch.zeros(1, dtype=torch.int32, device=self.pos.device)
+            self.nl.attach_compact16(self._nbr16, self._nbr16_flag, cfg.compact_limit)
         return self.nl
 
     def _visc_params(self):
@@ -147,21 +163,24 @@ class DFSPHSolver:
         vp.scheme = 0 if cfg.viscosity == "deltaSPH_inviscid" else 1
         return vp
 
-    def _solve(self, pairs, density: bool, p: torch.Tensor, accel: torch.Tensor, adv=None):
+    def _solve(self, pairs, density: bool, p: torch.Tensor, accel: torch.Tensor, adv=None, min_iters=None,
+               max_iters=None, check_every=None):
         cfg = self.cfg
         prm = _lib.DfsphParams()
         prm.dt = cfg.dt
         prm.rho0 = cfg.rho0
         prm.omega = cfg.omega_den if density else cfg.omega_div
         prm.error_threshold = (cfg.threshold_den if density else cfg.threshold_div) / cfg.rho0
-        prm.min_iters = cfg.min_iters
-        prm.max_iters = cfg.max_iters
+        prm.min_iters = cfg.min_iters if min_iters is None else min_iters
+        prm.max_iters = cfg.max_iters if max_iters is None else max_iters
         prm.clamp_pressure = 1 if cfg.clamp_pressure else 0
         prm.source_density = 1 if density else 0
         # Queue as many iterations as the previous solve of this kind needed
         # before the first host check (iterations past convergence are no-ops).
         prev = self._last_iters.get(density, 0)
-        prm.check_every = cfg.check_every if cfg.check_every > 0 else max(4, min(prev + 2, cfg.max_iters))
+        prm.check_every = cfg.check_every if cfg.check_every > 0 else max(4, min(prev + 2, prm.max_iters))
+        if check_every is not None:
+            prm.check_every = check_every
         stats = _lib.SolveStats()
         L = lib()
         ws = WORKSPACE.get(L.dsph_dfsph_workspace(self.n), p.device)
@@ -235,9 +254,25 @@ class DFSPHSolver:
             p_div = self.hooks["div_init"](self).contiguous()
         else:
             p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
-        s = self._solve(pairs, False, p_div, self.a_div, None)
-        out.div_iters, out.div_error = s.iterations, s.final_error
-        self._last_iters[False] = s.iterations
+        split = self.hooks.get("div_split")
+        if split is None:
+            s = self._solve(pairs, False, p_div, self.a_div, None)
+            iters = s.iterations
+        else:
+            # two-phase solve: k1 iterations, then the hook may replace the
+            # pressure (a learned correction), then Jacobi to the same
+            # stopping rule (at least one more iteration)
+            k1, fn = split
+            s = self._solve(pairs, False, p_div, self.a_div, None, max_iters=k1, check_every=k1)
+            iters = s.iterations
+            if s.iterations >= k1 and not s.converged and cfg.max_iters > k1:
+                p_div = fn(self, p_div).contiguous()
+                prev = self._last_iters.get(False, 0)
+                self._last_iters[False] = max(prev - k1, 0)
+                s = self._solve(pairs, False, p_div, self.a_div, None, min_iters=1, max_iters=cfg.max_iters - k1)
+                iters += s.iterations
+        out.div_iters, out.div_error = iters, s.final_error
+        self._last_iters[False] = iters
         self.p_div = p_div
         if "after_div" in self.hooks:
             self.hooks["after_div"](self)

@@ -297,3 +297,138 @@ def train_warm_start(make_solver, rounds: int = 3, steps: int = 24, sample: floa
         if log is not None:
             log(f"round {r}: {Xn.shape[0]} samples, normalized mse {net.evaluate(Xn, Tn):.4f}")
     return net
+
+
+# ---------------------------------------------------------------------------
+# Learned preconditioner: a residual correction inside the divergence solve
+# ---------------------------------------------------------------------------
+class CorrectionNet(WarmStartNet):
+    """Learned preconditioner of the DFSPH divergence solve.
+
+    After ``k1`` Jacobi iterations the remaining error is smooth (Jacobi
+    removes the rough part first); a second per-particle MLP of the same
+    shape and kernels maps neighbor-aggregated features of the current
+    pressure p_k, the current residual r_k and the previous step's converged
+    pressure to the remaining change p* - p_k, i.e. it applies a learned
+    approximate inverse of the pressure operator to the residual.  Features
+    (same kernel as the warm start): 0 p_k, 1 sum V W p_k, 2 r_k,
+    3 sum V W r_k, 4 p_prev, 5 sum V W p_prev, 6 rho/rho0 - 1, 7 alpha."""
+
+    def __init__(self, device, k1: int = 4, seed: int = 1):
+        super().__init__(device, seed=seed)
+        self.k1 = k1
+
+    def features_at(self, solver, p: torch.Tensor, normalized: bool = True) -> torch.Tensor:
+        n = solver.n
+        X = torch.empty(n, NIN, device=self.device)
+        prev = solver.p_div if solver.p_div is not None else torch.zeros(n, device=self.device)
+        mean = self.mean if normalized else torch.zeros(NIN, device=self.device)
+        istd = self.istd if normalized else torch.ones(NIN, device=self.device)
+        pr = solver.nl.pairs()
+        check(lib().dsph_ws_features(C.byref(pr), ptr(solver.vol), ptr(solver.rho), ptr(p), ptr(prev),
+                                     ptr(solver.residual), ptr(solver.alpha_div), C.c_float(solver.cfg.rho0),
+                                     ptr(mean), ptr(istd), ptr(X), _lib.stream_handle(self.device)), "ws_features")
+        return X
+
+    def corrected(self, solver, p: torch.Tensor) -> torch.Tensor:
+        y = self.forward(self.features_at(solver, p))
+        q = p + (y * self.t_std + self.t_mean)
+        return q.clamp_min_(0.0) if solver.cfg.clamp_pressure else q
+
+    def install(self, solver, judge_every: int = 8, probe_iters: int = 2):
+        """Apply the correction after k1 divergence iterations.  Plain
+        Jacobi is the judge: on judged steps (the first, every
+        `judge_every`-th) `probe_iters` iterations from the corrected and
+        from the uncorrected pressure are compared and the correction is
+        kept only if its error metric is not larger; the verdict holds until
+        the next judgement.  The solve always ends on the solver's own
+        stopping rule."""
+        state = {"k": 0, "use": True}
+
+        def fn(sv, p):
+            q = self.corrected(sv, p)
+            if judge_every > 0 and state["k"] % judge_every == 0:
+                e_c = sv.probe_guess(q, probe_iters)[0]
+                e_p = sv.probe_guess(p, probe_iters)[0]
+                state["use"] = math.isfinite(e_c) and e_c <= e_p
+                self.judge.judged += 1
+                self.judge.err_net, self.judge.err_plain = e_c, e_p
+            state["k"] += 1
+            if state["use"]:
+                self.judge.used_net += 1
+                return q
+            self.judge.used_plain += 1
+            return p
+
+        solver.hooks["div_split"] = (self.k1, fn)
+        return state
+
+
+def collect_correction(solver, steps: int, k1: int, sample: float = 0.1, seed: int = 0):
+    """Run `steps` DFSPH steps recording, for every divergence solve that is
+    not converged after k1 iterations, (raw features at k1, p* - p_k1)."""
+    feat = CorrectionNet(solver.pos.device, k1=k1)
+    Xs, Ts = [], []
+    g = torch.Generator(device=solver.pos.device).manual_seed(seed)
+    rec = {}
+
+    def fn(sv, p):
+        rec["X"] = feat.features_at(sv, p, normalized=False)
+        rec["p"] = p.clone()
+        return installed[1](sv, p) if installed is not None else p
+
+    def after_div(sv):
+        if "X" in rec:
+            n = sv.n
+            k = max(1, int(n * sample))
+            idx = torch.randperm(n, device=sv.pos.device, generator=g)[:k]
+            Xs.append(rec["X"][idx])
+            Ts.append((sv.p_div - rec["p"])[idx])
+        rec.clear()
+        if chained is not None:
+            chained(sv)
+
+    chained = solver.hooks.get("after_div")
+    installed = solver.hooks.get("div_split")
+    saved = dict(solver.hooks)
+    solver.hooks["div_split"] = (k1, fn)
+    solver.hooks["after_div"] = after_div
+    try:
+        for _ in range(steps):
+            solver.step()
+    finally:
+        solver.hooks.clear()
+        solver.hooks.update(saved)
+    if not Xs:
+        return None, None
+    return torch.cat(Xs), torch.cat(Ts)
+
+
+def train_correction(make_solver, warm: WarmStartNet | None = None, k1: int = 4, rounds: int = 2, steps: int = 24,
+                     sample: float = 0.1, train_steps: int = 400, lr: float = 2e-3, seed: int = 0,
+                     log=None) -> CorrectionNet:
+    """Train the residual-correction network on the solver's own solves
+    (warm-started by `warm` when given, as it will run).  Round 0 records
+    uncorrected solves; later rounds run with the current network installed
+    (on-policy) and add their samples."""
+    net = None
+    Xa, Ta = [], []
+    for r in range(rounds):
+        sv = make_solver(r)
+        if warm is not None:
+            warm.install(sv)
+        if net is not None:
+            net.install(sv, judge_every=0)
+        X, T = collect_correction(sv, steps, k1, sample, seed=seed + r)
+        if X is None:
+            break
+        if net is None:
+            net = CorrectionNet(sv.pos.device, k1=k1, seed=seed + 1)
+            net.fit_normalization(X, T)
+        Xa.append(X)
+        Ta.append(T)
+        Xn, Tn = net.normalize(torch.cat(Xa), torch.cat(Ta))
+        net.train(Xn, Tn, steps=train_steps, lr=lr, seed=seed + r)
+        if log is not None:
+            log(f"correction round {r}: {Xn.shape[0]} samples, normalized mse {net.evaluate(Xn, Tn):.4f}")
+    return net

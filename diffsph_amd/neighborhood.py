@@ -134,6 +134,18 @@ class NeighborList:
     mode: int
     slice_off: torch.Tensor | None = None
     entries: int = 0        # length of the layout (pairs + padding)
+    nbr16: torch.Tensor | None = None           # 16-bit modular deltas (pressure sweeps)
+    nbr16_overflow: torch.Tensor | None = None  # int32 device flag
+
+    def attach_compact16(self, buf16: torch.Tensor, flag: torch.Tensor, limit: int = 32767):
+        """Write the 16-bit delta copy of the list into `buf16` (int16, at
+        least `entries` long) and use it in the pressure sweeps; `flag`
+        (int32 [1]) is set on the device when a delta does not fit."""
+        self.nbr16, self.nbr16_overflow = None, None
+        pr = self.pairs()
+        check(lib().dsph_neighbor_compact16(C.byref(pr), limit, ptr(buf16), buf16.numel(), ptr(flag),
+                                            _lib.stream_handle(buf16.device)), "neighbor_compact16")
+        self.nbr16, self.nbr16_overflow = buf16, flag
 
     def pairs(self) -> _lib.Pairs:
         p = _lib.Pairs()
@@ -150,6 +162,8 @@ class NeighborList:
         p.domain = self.grid.domain
         p.counts = ptr(self.counts)
         p.slice_off = ptr(self.slice_off)
+        p.nbr16 = ptr(self.nbr16)
+        p.nbr16_overflow = ptr(self.nbr16_overflow)
         return p
 
     def csr_nbr(self) -> torch.Tensor:

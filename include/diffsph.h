@@ -86,7 +86,27 @@ typedef struct dsph_pairs {
   dsph_domain domain;
   const int32_t* counts;    /* [nq] row lengths (sliced layout) */
   const int32_t* slice_off; /* [ceil(nq/64)+1] or NULL (CSR)    */
+  /* Optional 16-bit copy of nbr for the DFSPH pressure sweeps (NULL: not
+   * used): entry k holds d with j = (i + d) mod nr, |d| <= 32767, written
+   * by dsph_neighbor_compact16; nbr16_overflow is its device flag (nonzero:
+   * a delta did not fit and the sweeps read nbr). */
+  const int16_t* nbr16;
+  const int32_t* nbr16_overflow;
 } dsph_pairs;
+
+/* 16-bit neighbor deltas for the pressure sweeps (halves their index
+ * stream): out[k] = (nbr[k] - i) reduced mod nr into [-nr/2, nr/2], for
+ * every entry of every row in the list's own layout (padding entries are
+ * not written; `capacity` is the length of `out`).  *overflow (device
+ * int32) is cleared, then set when some |d| > `limit` (at most 32767) or
+ * an entry lies beyond `capacity`; the sweeps then keep reading nbr.  Sorted in cell
+ * order, a neighbor of i lies within about (cells per grid row + 1) x
+ * (particles per cell) slots of i, or as far across the periodic seam, so
+ * the deltas fit for grids up to ~2000 cells per row at ~16 particles per
+ * cell.  No counterpart in the reference (its lists are int64 pairs). */
+int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit,
+                            int16_t* out, int64_t capacity,
+                            int32_t* overflow, void* stream);
 
 /* Sliced-ELL layout from row counts: slice_off[s] = 64 * sum over earlier
  * slices of their longest row; slice_off[nslices] = padded entry count,
@@ -510,6 +530,43 @@ typedef struct dsph_rk4_stages {
 int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u,
                      float* rho, float dt, float* dxdt_avg, float* dudt_avg,
                      float* drhodt_avg, void* stream);
+
+/* delta+ particle shifting (deltaPlusShifting, v2/modules/shifting.py:
+ * 119-147), applied as solveShifting does with scheme 'deltaSPH', one
+ * iteration and no free-surface treatment (shifting.py:880-1018) and added
+ * to the positions as runner.py:40-42 does.  Per fluid particle i over its
+ * fluid entries (and, with `boundary`, its wall entries: the merged state
+ * of solveShifting):
+ *   S_i = sum_j (1 + R (W_ij / W_0)^n) m_j / (rho_i + rho_j) gradW_ij,
+ *   W_0 = W(fl32(dx / support), support),
+ *   u_i = -((-CFL Ma) ((h_i / kernel_scale) 2)^2) S_i, each component
+ *         clamped to +-fl32(threshold dx),
+ *   Ma  = max |vel| / cs (or `mach` when >= 0, computeMach False: 0.1),
+ *   pos_i += (pos_i - u_i) - pos_i.
+ * Wall particles do not move.  shift_out [nq,2] (may be NULL) receives the
+ * added displacement.  `pos` is the state's (unwrapped) position array in
+ * row order; the pair geometry uses pairs->qpos / rpos.  (W/W_0)^n is the
+ * float32 rounding of a double product chain for integral n in [0, 16]
+ * (powf otherwise); the reference's torch.pow may differ by an ulp. */
+typedef struct dsph_shift_params {
+  double dx;           /* particle spacing (config particle.dx)            */
+  double support;      /* config particle.support                          */
+  float kernel_scale;
+  float R;             /* shifting.R, default 0.25                         */
+  float n;             /* shifting.n, default 4                            */
+  float cfl;           /* shifting.CFL, default 1.5                        */
+  float cs;
+  float threshold;     /* shifting.threshold, default 0.5                  */
+  float mach;          /* < 0: max |vel| / cs                              */
+  int32_t pad_;
+} dsph_shift_params;
+
+size_t dsph_shift_workspace(int64_t nq);
+int dsph_deltaplus_shift(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                         const dsph_shift_params* prm, const float* mass,
+                         const float* rho, const float* vel, float* pos,
+                         float* shift_out, void* workspace,
+                         size_t workspace_bytes, void* stream);
 
 /* max_i |v_i| of an [n,2] field (float32 norm).  Writes the host float
  * *out_host (synchronises the stream).  The workspace size serves both

@@ -1465,6 +1465,98 @@ int orc_isph_close_f32(const orc_domain* dom, int64_t nq, const int32_t* off, co
   return 0;
 }
 
+// deltaPlusShifting (v2/modules/shifting.py:119-147) applied as
+// solveShifting (shifting.py:880-1018, scheme 'deltaSPH', one iteration, no
+// free surface) and runner.py:40-42.  prm: dx, support, kernel_scale, R, n,
+// CFL, cs, threshold, mach (< 0: max |vel| / cs).  pos is updated in place
+// (row order), shift_out (may be NULL) receives the added displacement.
+}  // extern "C"
+
+namespace {
+template <class T>
+T shift_pow(T k, double n) {
+  if (n >= 0.0 && n <= 16.0 && n == std::floor(n)) {
+    double p = 1.0;
+    for (int t = 0; t < (int)n; ++t) p *= (double)k;
+    return (T)p;
+  }
+  return (T)std::pow(k, (T)n);
+}
+
+template <class T>
+void deltaplus_shift(const Csr<T>& c, const DomT<T>& d, const double* prm, const T* mass, const T* rho,
+                     const T* vel, const BSet<T>* B, T* pos, T* shift_out) {
+  const T q0 = (T)(prm[0] / prm[1]);
+  const T h0 = (T)prm[1];
+  const T w0 = kernelW<T>(q0, h0);
+  const T kscale = (T)prm[2], R = (T)prm[3], cfl = (T)prm[5], cs = (T)prm[6];
+  const T thr = (T)(prm[7] * prm[0]);
+  T ma = (T)prm[8];
+  if (prm[8] < 0.0) {
+    T m = T(0);
+    for (int64_t i = 0; i < c.nq; ++i) {
+      T a = vel[2 * i] * vel[2 * i];
+      T b = vel[2 * i + 1] * vel[2 * i + 1];
+      T s = std::sqrt(a + b);
+      m = s > m ? s : m;
+    }
+    ma = m / cs;
+  }
+  set_threads();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < c.nq; ++i) {
+    T sx = T(0), sy = T(0);
+    for (int32_t k = c.off[i]; k < c.off[i + 1]; ++k) {
+      const int32_t j = c.nbr[k];
+      const G<T> g = pg<T>(c, i, j, d);
+      const T term = T(1) + R * shift_pow<T>(g.w / w0, prm[4]);
+      const T sc = term * (mass[j] / (rho[i] + rho[j]));
+      sx += sc * g.gx;
+      sy += sc * g.gy;
+    }
+    if (B) {
+      for (int32_t k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
+        const int32_t b = B->c.nbr[k];
+        const G<T> g = pg<T>(B->c, i, b, d);
+        const T term = T(1) + R * shift_pow<T>(g.w / w0, prm[4]);
+        const T sc = term * (B->mass[b] / (rho[i] + B->rho[b]));
+        sx += sc * g.gx;
+        sy += sc * g.gy;
+      }
+    }
+    const T hi = c.qh ? c.qh[i] : c.h0;
+    const T hk = (hi / kscale) * T(2);
+    const T scale = (-cfl * ma) * (hk * hk);
+    T u[2] = {-(scale * sx), -(scale * sy)};
+    for (int k = 0; k < 2; ++k) {
+      u[k] = u[k] < -thr ? -thr : (u[k] > thr ? thr : u[k]);
+      const T x = pos[2 * i + k];
+      const T dxk = (x - u[k]) - x;
+      pos[2 * i + k] = x + dxk;
+      if (shift_out) shift_out[2 * i + k] = dxk;
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+
+#define ORC_SHIFT(T, SUF)                                                                                         \
+  int orc_deltaplus_shift_##SUF(const orc_domain* dom, int64_t nq, const int32_t* off, const int32_t* nbr,         \
+                                const T* qpos, T h0, const double* prm, const T* mass, const T* rho, const T* vel, \
+                                const orc_boundary* bnd, T* pos, T* shift_out) {                                  \
+    const T* rpos = qpos;                                                                                         \
+    const T* qh = nullptr;                                                                                        \
+    const T* rh = nullptr;                                                                                        \
+    int32_t mode = 0;                                                                                             \
+    BSet<T> B;                                                                                                    \
+    if (bnd) B = make_bset<T>(bnd, nq, qpos, qh, h0);                                                             \
+    deltaplus_shift<T>(ORC_CSR(T), DomT<T>(*dom), prm, mass, rho, vel, bnd ? &B : nullptr, pos, shift_out);      \
+    return 0;                                                                                                     \
+  }
+ORC_SHIFT(float, f32)
+ORC_SHIFT(double, f64)
+
 // The 2x2 pseudo-inverse on its own ([n,4] row-major in, L [n,4], ev [n,2]).
 int orc_pinv2x2_f32(const float* M, int64_t n, float* L, float* ev) {
   for (int64_t i = 0; i < n; ++i) pinv2<float>(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3], L + 4 * i, ev + 2 * i);

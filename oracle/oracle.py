@@ -873,3 +873,22 @@ def hash_neighbors(qpos, qh, search_range, spos, sh, table, ncell, cells, hcell,
     oi, oj = np.zeros(P, np.int32), np.zeros(P, np.int32)
     lib().orc_hash_neighbors(*args, None, _p(list_off), _p(oi), _p(oj))
     return counts, oi, oj
+
+
+def deltaplus_shift(dom, off, nbr, wpos, h0, mass, rho, vel, pos, dx, support, kernel_scale, R=0.25, n=4.0,
+                    cfl=1.5, cs=10.0, threshold=0.5, mach=-1.0, wall: WallSet | None = None, dtype=np.float32):
+    """deltaPlusShifting (v2/modules/shifting.py:119-147) as solveShifting
+    applies it (scheme 'deltaSPH', one iteration, no free surface) and the
+    runner adds it; returns (new positions, added displacement)."""
+    nq = wpos.shape[0]
+    cv = f32 if dtype == np.float32 else f64
+    hc = C.c_float if dtype == np.float32 else C.c_double
+    wpos, mass, rho, vel = cv(wpos), cv(mass), cv(rho), cv(vel)
+    p = cv(pos).copy()
+    sh = np.zeros((nq, 2), dtype)
+    prm = np.array([dx, support, kernel_scale, R, n, cfl, cs, threshold, mach], np.float64)
+    bs = wall.struct(dtype) if wall is not None else None
+    fn = lib().orc_deltaplus_shift_f32 if dtype == np.float32 else lib().orc_deltaplus_shift_f64
+    fn(C.byref(dom), C.c_int64(nq), _p(np.ascontiguousarray(off, np.int32)), _p(np.ascontiguousarray(nbr, np.int32)),
+       _p(wpos), hc(h0), _p(prm), _p(mass), _p(rho), _p(vel), C.byref(bs) if bs is not None else None, _p(p), _p(sh))
+    return p, sh

@@ -110,3 +110,41 @@ def test_dfsph_solver_limits_and_nan_guard():
     s2 = _solver(d, dt, min_iters=0)
     st2 = s2.step()
     assert st2.den_iters == 0 and st2.div_iters == 0
+
+
+def test_compact16_deltas_decode_and_overflow():
+    """The 16-bit modular deltas decode to the int32 list entry for entry;
+    a limit below the largest delta raises the device flag."""
+    d, dt = _setup(64, 2)
+    s = _solver(d, dt)
+    s.neighbor_step()
+    nl = s.nl
+    assert int(s._nbr16_flag.item()) == 0
+    cnt = nl.counts.cpu().numpy()
+    so = nl.slice_off.cpu().numpy()
+    nbr = nl.nbr.cpu().numpy()
+    d16 = s._nbr16.cpu().numpy().astype(np.int64)
+    n = s.n
+    dmax = 0
+    for i in range(n):
+        k = so[i // 64] + 64 * np.arange(cnt[i]) + i % 64
+        j = (i + d16[k]) % n
+        np.testing.assert_array_equal(j, nbr[k])
+        dmax = max(dmax, int(np.abs(d16[k]).max()))
+    assert 0 < dmax <= n // 2
+    nl.attach_compact16(s._nbr16, s._nbr16_flag, limit=dmax - 1)
+    assert int(s._nbr16_flag.item()) == 1
+
+
+def test_dfsph_compact_index_paths_identical():
+    """16-bit deltas, the int32 list, and the device-side fallback after an
+    overflow give the same trajectory bit for bit."""
+    d, dt = _setup(96, 4)
+    runs = [_solver(d, dt), _solver(d, dt, compact_index=False), _solver(d, dt, compact_limit=2)]
+    for k in range(4):
+        its = [r.step() for r in runs]
+        assert len({(st.div_iters, st.den_iters) for st in its}) == 1, k
+    assert int(runs[2]._nbr16_flag.item()) == 1
+    for r in runs[1:]:
+        assert torch.equal(runs[0].pos, r.pos) and torch.equal(runs[0].vel, r.vel)
+        assert torch.equal(runs[0].p_div, r.p_div) and torch.equal(runs[0].p_den, r.p_den)

@@ -137,3 +137,54 @@ def test_judge_falls_back_to_plain(broken):
         assert (sp.div_iters, sp.den_iters) == (sj.div_iters, sj.den_iters), k
     assert net.judge.used_net == 0 and net.judge.used_plain == 7
     assert torch.equal(plain.pos, judged.pos)
+
+
+def test_two_phase_divergence_solve_is_transparent():
+    """The split solve used by the learned preconditioner (k1 iterations,
+    hook, then Jacobi to the same stopping rule) with a hook that leaves the
+    pressure unchanged is the single solve, bit for bit."""
+    ns = 128
+    d = sampling.dfsph_random_flow(ns, seed=3, jitter_width=0.1)
+    dt = 3.0 * d["dx"] / 10.0
+    a, b = _solver(d, dt), _solver(d, dt)
+    b.hooks["div_split"] = (3, lambda sv, p: p)
+    for k in range(10):
+        sa, sb = a.step(), b.step()
+        assert (sa.div_iters, sa.den_iters) == (sb.div_iters, sb.den_iters), k
+    assert torch.equal(a.pos, b.pos) and torch.equal(a.vel, b.vel) and torch.equal(a.p_div, b.p_div)
+
+
+def test_learned_correction_is_judged(trained_512):
+    """The residual-correction network (learned preconditioner) runs under
+    the Jacobi judge: whenever it is applied its judged error is not above
+    the uncorrected one, every density solve meets its threshold, and the
+    divergence iterations are not above the warm-start-only run's."""
+    from diffsph_amd.learned import train_correction
+    warm, mk = trained_512
+    corr = train_correction(lambda r: mk(30 + r), warm=warm, k1=4, rounds=1, steps=12, train_steps=200,
+                            log=lambda s: print(s, flush=True))
+    ref, sv = mk(0), mk(0)
+    warm.install(ref)
+    warm.install(sv)
+    state = corr.install(sv, judge_every=1)
+    verdicts = []
+    orig = sv.hooks["div_split"][1]
+
+    def spy(s, p):
+        q = orig(s, p)
+        verdicts.append((state["use"], corr.judge.err_net, corr.judge.err_plain))
+        return q
+
+    sv.hooks["div_split"] = (corr.k1, spy)
+    it_r, it_c = [], []
+    for k in range(24):
+        sr, sc = ref.step(), sv.step()
+        assert sc.den_error <= 5e-4 + 1e-12
+        if k >= 4:
+            it_r.append(sr.div_iters)
+            it_c.append(sc.div_iters)
+    used = [v for v in verdicts if v[0]]
+    print(f"correction used {len(used)} / {len(verdicts)}; iterations warm {np.mean(it_r):.2f} "
+          f"warm+correction {np.mean(it_c):.2f}", flush=True)
+    assert all(e_c <= e_p for _, e_c, e_p in used)
+    assert np.mean(it_c) <= np.mean(it_r) + 1.5
