@@ -116,6 +116,11 @@ inline T* at(void* ws, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(ws) + off);
 }
 
+#ifndef DSPH_BFILL_STAGE
+#define DSPH_BFILL_STAGE 32
+#endif
+constexpr int kBFillStage = DSPH_BFILL_STAGE;
+
 __device__ __forceinline__ int bcell(float x, const Dom& d, int k, const BGrid& g) {
   // clamped in float before the conversion (NaN and out-of-range
   // coordinates land in an edge cell)
@@ -228,6 +233,10 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
   __shared__ int sh_max[kBlock / 64];
   __shared__ int sh_over[kBlock / 64];
   __shared__ int sh_sum[kBlock / 64];
+  // the first kBFillStage entries of a row are staged in LDS (a per-thread
+  // column) and written column by column after the scan: one contiguous
+  // line per store of the wave instead of scattered words
+  __shared__ int32_t stage[kBFillStage][kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int total = 0;
   if (i < n) {
@@ -244,7 +253,10 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
                       const float dx = mod_offset(qi.x - c.x.x, d, 0);
                       const float dy = mod_offset(qi.y - c.x.y, d, 1);
                       if (within_support(dist2_exact(dx, dy), h)) {
-                        if (total < width) nbr[base + (int64_t)total * 64] = c.idx;
+                        if (total < kBFillStage && total < width)
+                          stage[total][threadIdx.x] = c.idx;
+                        else if (total < width)
+                          nbr[base + (int64_t)total * 64] = c.idx;
                         ++total;
                       }
                     });
@@ -270,6 +282,8 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
         }
       }
     }
+    const int staged = total < kBFillStage ? total : kBFillStage;
+    for (int t = 0; t < staged && t < width; ++t) nbr[base + (int64_t)t * 64] = stage[t][threadIdx.x];
     cnt[i] = total < width ? total : width;
   }
   int m = total, over = total > width ? 1 : 0, sum = total;

@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
 // contiguous 256-byte line instead of 64 scattered words; entries past
 // kFillStage are written as they are found.
 #ifndef DSPH_FILL_STAGE
-#define DSPH_FILL_STAGE 32
+#define DSPH_FILL_STAGE 24
 #endif
 constexpr int kFillStage = DSPH_FILL_STAGE;
 
