@@ -144,30 +144,6 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
 // Source of a solve with advection adv (may be NULL = 0): div(v*) with
 // v* = v + dt adv; density source 1 - areas_i/V_i - dt div(v*)
 // (dfsph.py:57, 62-65).
-__global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float2* __restrict__ kv,
-                                                         const float2* __restrict__ vel,
-                                                         const float2* __restrict__ adv,
-                                                         const float* __restrict__ vol,
-                                                         const float* __restrict__ areas, float dt, int density,
-                                                         float* __restrict__ src) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nq) return;
-  const float2 xi = P.qpos[i];
-  const float hi = h_of(P.qh, i, P.h0);
-  const float2 a_i = adv ? adv[i] : make_float2(0.f, 0.f);
-  const float2 vi = make_float2(vel[i].x + dt * a_i.x, vel[i].y + dt * a_i.y);
-  float dv = 0.f;
-  const RowRef rr = row_ref(P, i);
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
-    const float2 c = kv_of(P, kv, k, j, xi, hi, vol);
-    const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
-    const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
-    dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
-  }
-  src[i] = density ? (1.0f - areas[i] / vol[i]) - dt * dv : dv;
-}
-
 // Streamed per-entry loads of the sweeps: the neighbor index and V_j gradW_ij
 // are read exactly once per launch, so they bypass the caches
 // (non-temporal) and leave L2 to the gathered neighbor fields.
@@ -194,6 +170,70 @@ struct Idx16 {
 };
 
 __device__ __forceinline__ bool use_idx16(const PairsDev& P) { return P.nbr16 && *P.ovf == 0; }
+
+// Divergence source (dfsph.py:30-33, 62-63) with the coefficients streamed:
+// dv_i = sum_j (v*_j - v*_i) . V_j gradW_ij, v* = v + dt adv, entries in row
+// order with kUnroll index / coefficient loads in flight.
+template <class Idx>
+__device__ __forceinline__ float source_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
+                                           const float2* __restrict__ vel, const float2* __restrict__ adv, float dt,
+                                           float2 vi) {
+  float dv = 0.f;
+  int n = 0, k = rr.base;
+  for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+    int j[4];
+    float2 c[4], vj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = idx(k + u * rr.step);
+      c[u] = ld_stream(kv + k + u * rr.step);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float2 aj = adv ? adv[j[u]] : make_float2(0.f, 0.f);
+      vj[u] = make_float2(vel[j[u]].x + dt * aj.x, vel[j[u]].y + dt * aj.y);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) dv += (vj[u].x - vi.x) * c[u].x + (vj[u].y - vi.y) * c[u].y;
+  }
+  for (; n < rr.cnt; ++n, k += rr.step) {
+    const int j = idx(k);
+    const float2 c = ld_stream(kv + k);
+    const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
+    const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
+    dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
+  }
+  return dv;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float2* __restrict__ kv,
+                                                         const float2* __restrict__ vel,
+                                                         const float2* __restrict__ adv,
+                                                         const float* __restrict__ vol,
+                                                         const float* __restrict__ areas, float dt, int density,
+                                                         float* __restrict__ src) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float hi = h_of(P.qh, i, P.h0);
+  const float2 a_i = adv ? adv[i] : make_float2(0.f, 0.f);
+  const float2 vi = make_float2(vel[i].x + dt * a_i.x, vel[i].y + dt * a_i.y);
+  float dv = 0.f;
+  const RowRef rr = row_ref(P, i);
+  if (kv) {
+    dv = use_idx16(P) ? source_kv(rr, Idx16{P.nbr16, i, P.nr}, kv, vel, adv, dt, vi)
+                      : source_kv(rr, Idx32{P.nbr}, kv, vel, adv, dt, vi);
+  } else {
+    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+      const int j = P.nbr[k];
+      const float2 c = kv_of(P, kv, k, j, xi, hi, vol);
+      const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
+      const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
+      dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
+    }
+  }
+  src[i] = density ? (1.0f - areas[i] / vol[i]) - dt * dv : dv;
+}
 
 // Entries per unrolled group: the group's index and coefficient loads are
 // issued together, then its gathers, then the additions in entry order (the

@@ -102,6 +102,7 @@ struct SearchArgs {
   int mode;
   int wrapped;
   int nq;
+  int nref;
 };
 
 // A candidate reference particle: wrapped position and support.
@@ -171,10 +172,30 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
 #endif
 constexpr int kFillStage = DSPH_FILL_STAGE;
 
+// With `d16` the 16-bit modular delta (j - i) mod nref of every entry is
+// written beside it (rows = reference slots); a delta beyond int16 sets *ovf.
+__device__ __forceinline__ void put_entry(int32_t* __restrict__ nbr, int16_t* __restrict__ d16, int64_t o, int j,
+                                          int i, int nref, bool& bad) {
+  nbr[o] = j;
+  if (d16) {
+    int d = j - i;
+    const int half = nref / 2;
+    if (d > half) d -= nref;
+    if (d < -half) d += nref;
+    if (d > 32767 || d < -32767) {
+      bad = true;
+      d = 0;
+    }
+    d16[o] = (int16_t)d;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ slice,
-                                                          int32_t* __restrict__ nbr, int64_t cap) {
+                                                          int32_t* __restrict__ nbr, int64_t cap,
+                                                          int16_t* __restrict__ d16, int32_t* __restrict__ ovf) {
   __shared__ int32_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
+  bool bad = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   float2 qi = a.qpos[i];
@@ -201,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
                         stage[n][threadIdx.x] = j;
                       } else {
                         const int64_t o = base + n * step;
-                        if (n < len && o < cap) nbr[o] = j;
+                        if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
                       }
                       ++n;
                     }
@@ -209,12 +230,21 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   });
   for (int64_t t = 0; t < staged; ++t) {
     const int64_t o = base + t * step;
-    if (o < cap) nbr[o] = t < n ? stage[t][threadIdx.x] : -1;
+    if (o >= cap) continue;
+    if (t < n)
+      put_entry(nbr, d16, o, stage[t][threadIdx.x], i, a.nref, bad);
+    else
+      nbr[o] = -1;
   }
   for (int64_t t = n > staged ? n : staged; t < len; ++t) {
     const int64_t o = base + t * step;
     if (o < cap) nbr[o] = -1;
   }
+  if (bad) *ovf = 1;
+}
+
+__global__ void k_fill_flag_clear(int32_t* flag) {
+  if (threadIdx.x == 0) *flag = 0;
 }
 
 // Longest row of every 64-row slice, times 64 (one wave per slice).
@@ -265,6 +295,7 @@ static int make_search(SearchArgs& a, const dsph_domain* domain, const dsph_grid
   a.rh = sorted_h;
   a.qpos = reinterpret_cast<const float2*>(qpos);
   a.qh = qh;
+  a.nref = (int)nref;
   a.h0 = h0;
   a.mode = mode;
   a.wrapped = wrapped != 0;
@@ -393,7 +424,29 @@ extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* gr
   if (nq == 0) return DSPH_OK;
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity);
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity, nullptr,
+                                                                   nullptr);
+  return launch_status();
+}
+
+extern "C" int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                                    const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos,
+                                    const float* qh, int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                    const int32_t* offsets, const int32_t* slice_off, int64_t layout_total,
+                                    int32_t* nbr, int16_t* nbr16, int64_t capacity, int32_t* overflow,
+                                    void* stream) {
+  SearchArgs a;
+  int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
+                      h_uniform, mode);
+  if (s != DSPH_OK) return s;
+  if (!overflow || (capacity > 0 && !nbr16)) return DSPH_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  k_fill_flag_clear<<<1, 64, 0, st>>>(overflow);
+  DSPH_LAUNCH_CHECK();
+  if (nq == 0) return DSPH_OK;
+  if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
+  if (layout_total > capacity) return DSPH_ERR_CAPACITY;
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow);
   return launch_status();
 }
 

@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr
-from .neighborhood import WORKSPACE, CellGrid, NeighborList, search
+from .neighborhood import WORKSPACE, CellGrid, Delta16Buffers, NeighborList, search
 
 
 @dataclass
@@ -111,8 +111,7 @@ class DFSPHSolver:
         # (k1, callable(solver, p) returning a replacement pressure after k1
         # divergence iterations)
         self.hooks = {}
-        self._nbr16 = None
-        self._nbr16_flag = None
+        self._d16 = Delta16Buffers()
 
     # -- helpers ---------------------------------------------------------
     def _permute(self, t, idx):
@@ -139,15 +138,15 @@ class DFSPHSolver:
         self.p_div = self._permute(self.p_div, si)
         self.p_den = self._permute(self.p_den, si)
         self.grid = grid
-        self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True)
+        fused = cfg.compact_index and cfg.compact_limit >= 32767
+        self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True,
+                         delta16=self._d16 if fused else None)
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
             self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
-        if cfg.compact_index:
-            if self._nbr16 is None or self._nbr16.shape[0] < need:
-                self._nbr16 = torch.empty(need + need // 8, dtype=torch.int16, device=self.pos.device)
-                self._nbr16_flag = torch.zeros(1, dtype=torch.int32, device=self.pos.device)
-            self.nl.attach_compact16(self._nbr16, self._nbr16_flag, cfg.compact_limit)
+        if cfg.compact_index and not fused:
+            buf, flag = self._d16.get(need, self.pos.device)
+            self.nl.attach_compact16(buf, flag, cfg.compact_limit)
         return self.nl
 
     def _visc_params(self):

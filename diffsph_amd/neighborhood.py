@@ -176,8 +176,24 @@ class NeighborList:
         return out[: self.num_pairs]
 
 
+class Delta16Buffers:
+    """Grow-only int16 buffer and int32 overflow flag for the 16-bit
+    neighbor deltas written by :func:`search` (``delta16=``)."""
+
+    def __init__(self):
+        self.buf = None
+        self.flag = None
+
+    def get(self, n: int, device):
+        if self.buf is None or self.buf.numel() < n:
+            self.buf = torch.empty(n + n // 8, dtype=torch.int16, device=device)
+        if self.flag is None:
+            self.flag = torch.zeros(1, dtype=torch.int32, device=device)
+        return self.buf, self.flag
+
+
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
-           qpos_wrapped: bool = False, layout: str = "sell") -> NeighborList:
+           qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None) -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
     Counts, scans (host reads of the pair and entry totals, as the
@@ -213,13 +229,21 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
         raise ValueError(f"unknown layout {layout}")
     cap = max(entries, 1)
     nbr = torch.empty(cap, dtype=torch.int32, device=dev)
-    check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(slice_off), entries, ptr(nbr), cap, st), "neighbor_fill")
+    buf16 = flag16 = None
+    if delta16 is not None:
+        # rows are the grid's own slots (qpos_wrapped): the deltas are written in the same pass
+        buf16, flag16 = delta16.get(cap, dev)
+        check(L.dsph_neighbor_fill16(*args, ptr(offsets), ptr(slice_off), entries, ptr(nbr), ptr(buf16), cap,
+                                     ptr(flag16), st), "neighbor_fill16")
+    else:
+        check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(slice_off), entries, ptr(nbr), cap, st),
+              "neighbor_fill")
     if qpos_wrapped:
         wq = qpos
     else:
         wq = wrap_positions(qpos, grid.domain)
     return NeighborList(counts, offsets, nbr[:max(entries, 1)] if layout == "sell" else nbr[:npairs], npairs, wq, qh,
-                        grid, m, slice_off, entries)
+                        grid, m, slice_off, entries, buf16, flag16)
 
 
 def wrap_positions(pos: torch.Tensor, domain: _lib.Domain) -> torch.Tensor:

@@ -181,6 +181,21 @@ int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
                        int64_t layout_total, int32_t* nbr, int64_t capacity,
                        void* stream);
 
+/* dsph_neighbor_fill that also writes the 16-bit modular deltas of
+ * dsph_neighbor_compact16 (d = j - i reduced mod nref; |d| <= 32767, else
+ * *overflow is set) in the same pass, for a list whose rows are the sorted
+ * reference slots themselves (a set searched against its own grid and not
+ * remapped afterwards).  *overflow is cleared first.  No counterpart in the
+ * reference. */
+int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* grid,
+                         const int32_t* cell_start, const float* sorted_pos,
+                         const float* sorted_h, int64_t nref,
+                         const float* qpos, const float* qh, int64_t nq,
+                         int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                         const int32_t* offsets, const int32_t* slice_off,
+                         int64_t layout_total, int32_t* nbr, int16_t* nbr16,
+                         int64_t capacity, int32_t* overflow, void* stream);
+
 /* out = getPeriodicPosition(pos) (v2/modules/neighborhood.py:407-430):
  * periodic axes x' = remainder(x - min, max - min) + min in float32. */
 int dsph_wrap_positions(const dsph_domain* domain, const float* pos, int64_t n,

@@ -113,27 +113,36 @@ def test_dfsph_solver_limits_and_nan_guard():
 
 
 def test_compact16_deltas_decode_and_overflow():
-    """The 16-bit modular deltas decode to the int32 list entry for entry;
-    a limit below the largest delta raises the device flag."""
+    """The 16-bit modular deltas written by the fill pass decode to the
+    int32 list entry for entry and equal the separate compaction pass; a
+    limit below the largest delta raises the device flag."""
     d, dt = _setup(64, 2)
     s = _solver(d, dt)
     s.neighbor_step()
     nl = s.nl
-    assert int(s._nbr16_flag.item()) == 0
+    assert nl.nbr16 is not None and int(nl.nbr16_overflow.item()) == 0
     cnt = nl.counts.cpu().numpy()
     so = nl.slice_off.cpu().numpy()
     nbr = nl.nbr.cpu().numpy()
-    d16 = s._nbr16.cpu().numpy().astype(np.int64)
+    d16 = nl.nbr16.cpu().numpy().astype(np.int64)
+    buf = torch.empty_like(nl.nbr16)
+    flag = torch.zeros(1, dtype=torch.int32, device=buf.device)
+    fused, fused_flag = nl.nbr16, nl.nbr16_overflow
+    nl.attach_compact16(buf, flag)
+    assert int(flag.item()) == 0
+    d16b = buf.cpu().numpy().astype(np.int64)
     n = s.n
     dmax = 0
     for i in range(n):
         k = so[i // 64] + 64 * np.arange(cnt[i]) + i % 64
         j = (i + d16[k]) % n
         np.testing.assert_array_equal(j, nbr[k])
+        np.testing.assert_array_equal(d16b[k], d16[k])
         dmax = max(dmax, int(np.abs(d16[k]).max()))
     assert 0 < dmax <= n // 2
-    nl.attach_compact16(s._nbr16, s._nbr16_flag, limit=dmax - 1)
-    assert int(s._nbr16_flag.item()) == 1
+    nl.attach_compact16(buf, flag, limit=dmax - 1)
+    assert int(flag.item()) == 1
+    assert int(fused_flag.item()) == 0 and fused is not buf
 
 
 def test_dfsph_compact_index_paths_identical():
@@ -144,7 +153,7 @@ def test_dfsph_compact_index_paths_identical():
     for k in range(4):
         its = [r.step() for r in runs]
         assert len({(st.div_iters, st.den_iters) for st in its}) == 1, k
-    assert int(runs[2]._nbr16_flag.item()) == 1
+    assert int(runs[2].nl.nbr16_overflow.item()) == 1
     for r in runs[1:]:
         assert torch.equal(runs[0].pos, r.pos) and torch.equal(runs[0].vel, r.vel)
         assert torch.equal(runs[0].p_div, r.p_div) and torch.equal(runs[0].p_den, r.p_den)
