@@ -454,6 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
                                                         const float* __restrict__ pnp,
                                                         const float* __restrict__ src,
                                                         const float* __restrict__ alpha, float rho0,
+                                                        const float* __restrict__ pair_w,
                                                         const float* __restrict__ mean,
                                                         const float* __restrict__ istd, float* __restrict__ X) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,6 +463,17 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
   const float hi = h_of(P.qh, i, P.h0);
   float s1 = 0.f, s3 = 0.f, s5 = 0.f;
   const RowRef rr = row_ref(P, i);
+  if (pair_w) {
+    // W_ij streamed from the density pass; same sums, same order
+    const bool i16 = use_idx16(P);
+    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+      const int j = i16 ? Idx16{P.nbr16, i, P.nr}(k) : Idx32{P.nbr}(k);
+      const float vw = vol[j] * __builtin_nontemporal_load(pair_w + k);
+      s1 += vw * pdp[j];
+      s3 += vw * src[j];
+      s5 += vw * pnp[j];
+    }
+  } else
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
@@ -857,7 +869,8 @@ extern "C" int dsph_integrate_semi_implicit(float* pos, float* vel, const float*
 
 extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p_div_prev,
                                 const float* p_den_prev, const float* div_src, const float* alpha_div, float rho0,
-                                const float* mean16, const float* istd16, float* X, void* stream) {
+                                const float* pair_w, const float* mean16, const float* istd16, float* X,
+                                void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
@@ -866,6 +879,6 @@ extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const
     return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   k_ws_features<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, vol, rho, p_div_prev, p_den_prev, div_src,
-                                                                  alpha_div, rho0, mean16, istd16, X);
+                                                                  alpha_div, rho0, pair_w, mean16, istd16, X);
   return launch_status();
 }

@@ -13,7 +13,7 @@ size_t scan_ws(int64_t n);
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __restrict__ rmass,
                                                     const float* __restrict__ qmass, float* __restrict__ rho,
-                                                    float* __restrict__ vol) {
+                                                    float* __restrict__ vol, float* __restrict__ w_out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -27,7 +27,9 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
     float dx = mod_offset(xi.x - xj.x, P.dom, 0), dy = mod_offset(xi.y - xj.y, P.dom, 1);
     float h = pick_h(P.mode, hi, hj);
     float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
-    acc += rmass[j] * wendland_w(q, h);
+    const float w = wendland_w(q, h);
+    if (w_out) w_out[k] = w;
+    acc += rmass[j] * w;
   }
   rho[i] = acc;
   if (vol) vol[i] = qmass[i] / acc;
@@ -294,7 +296,18 @@ extern "C" int dsph_density(const dsph_pairs* pairs, const float* rmass, const f
   if (pairs->nq == 0) return DSPH_OK;
   if (!rmass || !rho || (vol && !qmass)) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_density<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, rmass, qmass, rho, vol);
+  k_density<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, rmass, qmass, rho, vol, nullptr);
+  return launch_status();
+}
+
+extern "C" int dsph_density_w(const dsph_pairs* pairs, const float* rmass, const float* qmass, float* rho,
+                              float* vol, float* pair_w, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!rmass || !rho || (vol && !qmass) || !pair_w) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_density<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, rmass, qmass, rho, vol, pair_w);
   return launch_status();
 }
 

@@ -102,6 +102,8 @@ class DFSPHSolver:
         self.dudt = torch.empty(n, 2, **f)
         self.residual = torch.empty(n, **f)
         self.kgrad = None   # V_j gradW_ij per neighbor-list entry, [entries, 2]
+        self.wpair = None   # W_ij per neighbor-list entry (density pass), [entries]
+        self.keep_pair_w = False  # set by the learned warm start, which reads wpair
         self.grid = None
         self.nl = None
         self.step_count = 0
@@ -144,6 +146,7 @@ class DFSPHSolver:
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
             self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
+            self.wpair = torch.empty(need + need // 8, dtype=torch.float32, device=self.pos.device)
         if cfg.compact_index and not fused:
             buf, flag = self._d16.get(need, self.pos.device)
             self.nl.attach_compact16(buf, flag, cfg.compact_limit)
@@ -237,8 +240,13 @@ class DFSPHSolver:
         nl = self.neighbor_step()
         pairs = nl.pairs()
         out = StepStats(num_pairs=nl.num_pairs)
-        check(L.dsph_density(C.byref(pairs), ptr(self.mass), ptr(self.mass), ptr(self.rho), ptr(self.vol), st),
-              "density")
+        if self.keep_pair_w:
+            # W_ij per list entry is kept for the learned warm start's features
+            check(L.dsph_density_w(C.byref(pairs), ptr(self.mass), ptr(self.mass), ptr(self.rho), ptr(self.vol),
+                                   ptr(self.wpair), st), "density")
+        else:
+            check(L.dsph_density(C.byref(pairs), ptr(self.mass), ptr(self.mass), ptr(self.rho), ptr(self.vol),
+                                 st), "density")
         vp = self._visc_params()
         check(L.dsph_dfsph_prepare(C.byref(pairs), ptr(self.vel), ptr(self.mass), ptr(self.areas), ptr(self.vol),
                                    ptr(self.rho), C.c_float(cfg.dt), C.c_float(cfg.rho0),

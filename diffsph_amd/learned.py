@@ -35,6 +35,11 @@ from ._lib import check, lib, ptr
 from .neighborhood import WORKSPACE
 
 NFEAT = 8       # used feature columns
+
+
+def _pair_w(solver):
+    """The density pass's W_ij of the current list, when the solver keeps it."""
+    return solver.wpair if getattr(solver, "keep_pair_w", False) else None
 NIN = 16        # padded input width of the network
 
 
@@ -109,7 +114,8 @@ class WarmStartNet:
         pr = solver.nl.pairs()
         check(lib().dsph_ws_features(C.byref(pr), ptr(solver.vol), ptr(solver.rho), ptr(pdp), ptr(pnp),
                                      ptr(solver.div_src), ptr(solver.alpha_div), C.c_float(solver.cfg.rho0),
-                                     ptr(mean), ptr(istd), ptr(X), _lib.stream_handle(self.device)), "ws_features")
+                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X),
+                                     _lib.stream_handle(self.device)), "ws_features")
         return X
 
     def predict_pressure(self, solver) -> torch.Tensor:
@@ -227,6 +233,7 @@ class WarmStartNet:
 
         solver.hooks["div_init"] = div_init
         solver.hooks["after_div"] = after_div
+        solver.keep_pair_w = True
         return state
 
 
@@ -262,6 +269,7 @@ def collect(solver, steps: int, sample: float = 0.25, seed: int = 0, guess=None)
     saved = dict(solver.hooks)
     solver.hooks["div_init"] = div_init
     solver.hooks["after_div"] = after_div
+    solver.keep_pair_w = True
     try:
         for _ in range(steps):
             solver.step()
@@ -327,7 +335,8 @@ class CorrectionNet(WarmStartNet):
         pr = solver.nl.pairs()
         check(lib().dsph_ws_features(C.byref(pr), ptr(solver.vol), ptr(solver.rho), ptr(p), ptr(prev),
                                      ptr(solver.residual), ptr(solver.alpha_div), C.c_float(solver.cfg.rho0),
-                                     ptr(mean), ptr(istd), ptr(X), _lib.stream_handle(self.device)), "ws_features")
+                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X),
+                                     _lib.stream_handle(self.device)), "ws_features")
         return X
 
     def corrected(self, solver, p: torch.Tensor) -> torch.Tensor:
@@ -361,6 +370,7 @@ class CorrectionNet(WarmStartNet):
             return p
 
         solver.hooks["div_split"] = (self.k1, fn)
+        solver.keep_pair_w = True
         return state
 
 
@@ -393,6 +403,7 @@ def collect_correction(solver, steps: int, k1: int, sample: float = 0.1, seed: i
     saved = dict(solver.hooks)
     solver.hooks["div_split"] = (k1, fn)
     solver.hooks["after_div"] = after_div
+    solver.keep_pair_w = True
     try:
         for _ in range(steps):
             solver.step()

@@ -245,6 +245,14 @@ size_t dsph_pairs_export_workspace(int64_t nq);
 int dsph_density(const dsph_pairs* pairs, const float* rmass,
                  const float* qmass, float* rho, float* vol, void* stream);
 
+/* dsph_density that also stores W_ij of every list entry in the list's own
+ * layout (pair_w, as long as nbr), for later passes that need the kernel
+ * values (the learned warm start's features).  No counterpart in the
+ * reference. */
+int dsph_density_w(const dsph_pairs* pairs, const float* rmass,
+                   const float* qmass, float* rho, float* vol, float* pair_w,
+                   void* stream);
+
 /* Generic operator of v2/sphOps.py:253-282 over on-the-fly kernels:
  * op = 0 interpolate  sum_j m_j/rho_j q_j W_ij
  *      1 gradient     (mode: 0 naive, 1 difference, 2 summation, 3 symmetric)
@@ -388,12 +396,15 @@ int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
 /* Per-particle input features [n][16] (columns 8..15 zero):
  *   0 p_div_prev  1 sum_j V_j W_ij p_div_prev_j  2 s_div  3 sum V W s_div
  *   4 p_den_prev  5 sum_j V_j W_ij p_den_prev_j  6 rho/rho0-1  7 alpha_div
- * normalised as (f - mean16[c]) * istd16[c] (device arrays). */
+ * normalised as (f - mean16[c]) * istd16[c] (device arrays).  pair_w
+ * (may be NULL): the W_ij of dsph_density_w for this list, streamed instead
+ * of re-evaluating the pair geometry (same values). */
 int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
                      const float* rho, const float* p_div_prev,
                      const float* p_den_prev, const float* div_src,
-                     const float* alpha_div, float rho0, const float* mean16,
-                     const float* istd16, float* X, void* stream);
+                     const float* alpha_div, float rho0, const float* pair_w,
+                     const float* mean16, const float* istd16, float* X,
+                     void* stream);
 
 /* MLP 16 -> 64 -> 64 -> 64 -> 1 (ReLU), bf16 weights and layer inputs on
  * MFMA with fp32 accumulation.  Parameters are a flat fp32 array of

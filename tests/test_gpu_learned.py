@@ -188,3 +188,42 @@ def test_learned_correction_is_judged(trained_512):
           f"warm+correction {np.mean(it_c):.2f}", flush=True)
     assert all(e_c <= e_p for _, e_c, e_p in used)
     assert np.mean(it_c) <= np.mean(it_r) + 1.5
+
+
+def test_features_match_oracle_interpolation():
+    """The warm-start features: the kernel-weighted neighbor sums (columns
+    1, 3, 5) are SPH interpolations sum_j m_j / rho_j q_j W_ij of p_div_prev,
+    the divergence source and p_den_prev, bit for bit with the oracle's
+    interpolation on the same list; streaming W_ij from the density pass
+    gives the same bits as evaluating the pair geometry again."""
+    from oracle import oracle as orc
+    ns = 64
+    d = sampling.dfsph_random_flow(ns, seed=7, jitter_width=0.1)
+    dt = 3.0 * d["dx"] / 10.0
+    s = _solver(d, dt)
+    net = WarmStartNet("cuda:0", seed=0)
+    got = {}
+
+    def div_init(sv):
+        if sv.p_div is not None and "X" not in got:
+            got["X"] = net.features(sv, normalized=False).cpu().numpy()
+            sv.keep_pair_w = False
+            got["X_geom"] = net.features(sv, normalized=False).cpu().numpy()
+            sv.keep_pair_w = True
+            got.update(pos=sv.pos.cpu().numpy(), mass=sv.mass.cpu().numpy(), rho=sv.rho.cpu().numpy(),
+                       pdiv=sv.p_div.cpu().numpy(), pden=sv.p_den.cpu().numpy(), src=sv.div_src.cpu().numpy())
+        return (sv.cfg.warm_start * sv.p_div) if sv.p_div is not None else torch.zeros_like(sv.rho)
+
+    s.hooks["div_init"] = div_init
+    s.keep_pair_w = True
+    for _ in range(3):
+        s.step()
+    assert np.array_equal(got["X"], got["X_geom"])
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    off, nbr, wpos = orc.neighbors_state_order(dom, got["pos"], d["h"])
+    m, rho = got["mass"], got["rho"]
+    for col, q in ((1, got["pdiv"]), (3, got["src"]), (5, got["pden"])):
+        ref = orc.sph_operation(dom, off, nbr, wpos, wpos, 0, 0, m, rho, q, 1, qm=m, qr=rho, qq=q, h0=d["h"])
+        np.testing.assert_array_equal(got["X"][:, col], ref.reshape(-1), err_msg=f"feature column {col}")
+    np.testing.assert_array_equal(got["X"][:, 0], got["pdiv"])
+    np.testing.assert_array_equal(got["X"][:, 2], got["src"])
