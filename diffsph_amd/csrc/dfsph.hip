@@ -91,19 +91,33 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
   const float2 vi = A.vel[i];
   float k1x = 0.f, k1y = 0.f, k2 = 0.f, dv = 0.f, vsx = 0.f, vsy = 0.f;
   const RowRef rr = row_ref(P, i);
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
-    const float hj = h_of(P.rh, j, P.h0);
-    const float h = pick_h(P.mode, hi, hj);
-    Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
-    const float Vj = A.vol[j];
-    const float2 vj = A.vel[j];
+  // neighbor data of one entry, gathered four entries ahead (the sums run
+  // in entry order)
+  struct Nb {
+    float2 x, v;
+    float h, vol, area;
+  };
+  auto load = [&](int j) {
+    Nb d;
+    d.x = P.rpos[j];
+    d.v = A.vel[j];
+    d.h = h_of(P.rh, j, P.h0);
+    d.vol = A.vol[j];
+    d.area = A.areas[j];
+    return d;
+  };
+  int n = 0, k = rr.base;
+  auto body = [&](int k, const Nb& d) {
+    const float h = pick_h(P.mode, hi, d.h);
+    Geom g = pair_geom(xi, d.x, h, P.dom);
+    const float Vj = d.vol;
+    const float2 vj = d.v;
     const float kx = Vj * g.gx, ky = Vj * g.gy;
     if (kv_out) kv_out[k] = make_float2(kx, ky);
     // alpha terms (dfsph.py:13-20)
     k1x += kx;
     k1y += ky;
-    const float c2 = (Vj * Vj) / (A.areas[j] * A.rho0);
+    const float c2 = (Vj * Vj) / (d.area * A.rho0);
     k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
     // divergence 'difference' of v (sphOps.py:92-101)
     dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
@@ -118,7 +132,18 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
       vsx += f * g.gx;
       vsy += f * g.gy;
     }
+  };
+  for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+    int j[4];
+    Nb d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = P.nbr[k + u * rr.step];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d[u] = load(j[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) body(k + u * rr.step, d[u]);
   }
+  for (; n < rr.cnt; ++n, k += rr.step) body(k, load(P.nbr[k]));
   const float Vi = A.vol[i];
   const float mass_i = A.areas[i] * A.rho0;
   const float k1sq = k1x * k1x + k1y * k1y;
@@ -335,6 +360,24 @@ struct SweepBArgs {
   int isph;  // error metric |r| (incompressible.py:357) instead of max(r/rho0, -thr)
 };
 
+// The loop condition of dfsph.py:88 from the per-block partials, summed in a
+// fixed order by one block of kBlock threads.  (A last-block-ticket fusion
+// into sweep B needs device-scope fences, which on this part write back the
+// per-XCD L2s: 1.4-1.7x slower solves, so the reduction stays a kernel.)
+__device__ __forceinline__ void reduce_partials(const double* partials, int nb, int n, SolveCtrl* ctrl, double* sh) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) v += partials[k];
+  double tot = block_sum_d(v, sh);
+  if (threadIdx.x == 0) {
+    double mean = tot / (double)n;
+    ctrl->iter += 1;
+    ctrl->err = mean;
+    if (ctrl->iter == 1) ctrl->err0 = mean;
+    bool go = ctrl->iter < ctrl->max_iters && (ctrl->iter < ctrl->min_iters || mean > ctrl->thr);
+    ctrl->done = go ? 0 : 1;
+  }
+}
+
 template <class Idx>
 __device__ __forceinline__ float sweep_b_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
                                             const float2* __restrict__ acc, float2 ai) {
@@ -416,17 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_reduce(const double* __restric
                                                          SolveCtrl* ctrl) {
   __shared__ double sh[kBlock / 64];
   if (ctrl->done) return;
-  double v = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) v += partials[k];
-  double tot = block_sum_d(v, sh);
-  if (threadIdx.x == 0) {
-    double mean = tot / (double)n;
-    ctrl->iter += 1;
-    ctrl->err = mean;
-    if (ctrl->iter == 1) ctrl->err0 = mean;
-    bool go = ctrl->iter < ctrl->max_iters && (ctrl->iter < ctrl->min_iters || mean > ctrl->thr);
-    ctrl->done = go ? 0 : 1;
-  }
+  reduce_partials(partials, nb, n, ctrl, sh);
 }
 
 __global__ void k_ctrl_init(SolveCtrl* c, int min_iters, int max_iters, double thr) {
@@ -464,15 +497,43 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
   float s1 = 0.f, s3 = 0.f, s5 = 0.f;
   const RowRef rr = row_ref(P, i);
   if (pair_w) {
-    // W_ij streamed from the density pass; same sums, same order
-    const bool i16 = use_idx16(P);
-    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-      const int j = i16 ? Idx16{P.nbr16, i, P.nr}(k) : Idx32{P.nbr}(k);
-      const float vw = vol[j] * __builtin_nontemporal_load(pair_w + k);
+    // W_ij streamed from the density pass; same sums, same order, four
+    // entries' gathers in flight
+    auto body = [&](float w, int j) {
+      const float vw = vol[j] * w;
       s1 += vw * pdp[j];
       s3 += vw * src[j];
       s5 += vw * pnp[j];
+    };
+    const bool i16 = use_idx16(P);
+    int n = 0, k = rr.base;
+    for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+      int j[4];
+      float w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kk = k + u * rr.step;
+        j[u] = i16 ? Idx16{P.nbr16, i, P.nr}(kk) : Idx32{P.nbr}(kk);
+        w[u] = __builtin_nontemporal_load(pair_w + kk);
+      }
+      float v[4], a[4], b[4], c[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] = vol[j[u]];
+        a[u] = pdp[j[u]];
+        b[u] = src[j[u]];
+        c[u] = pnp[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float vw = v[u] * w[u];
+        s1 += vw * a[u];
+        s3 += vw * b[u];
+        s5 += vw * c[u];
+      }
     }
+    for (; n < rr.cnt; ++n, k += rr.step)
+      body(__builtin_nontemporal_load(pair_w + k), i16 ? Idx16{P.nbr16, i, P.nr}(k) : Idx32{P.nbr}(k));
   } else
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];

@@ -163,14 +163,17 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
 // entries at slice[i/64] + 64 n + i%64, the rest of the slice's columns
 // padded with -1.  Writes stay below `cap` and inside the row's own range.
 // In the sliced layout the first kFillStage entries of every row are staged
-// in LDS (a per-thread column: conflict-free) while the candidates are
+// in LDS as 16-bit offsets j - i (a per-thread column: the staging costs 2
+// bytes per entry, so a whole typical row fits) while the candidates are
 // scanned, then written column by column, so each store of the wave is one
-// contiguous 256-byte line instead of 64 scattered words; entries past
-// kFillStage are written as they are found.
+// contiguous line instead of 64 scattered words.  An entry whose offset does
+// not fit in 16 bits, and every entry past kFillStage, is written as it is
+// found.
 #ifndef DSPH_FILL_STAGE
-#define DSPH_FILL_STAGE 24
+#define DSPH_FILL_STAGE 48
 #endif
 constexpr int kFillStage = DSPH_FILL_STAGE;
+constexpr int16_t kStagedOut = -32768;  // slot already written to global memory
 
 // With `d16` the 16-bit modular delta (j - i) mod nref of every entry is
 // written beside it (rows = reference slots); a delta beyond int16 sets *ovf.
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
                                                           const int32_t* __restrict__ slice,
                                                           int32_t* __restrict__ nbr, int64_t cap,
                                                           int16_t* __restrict__ d16, int32_t* __restrict__ ovf) {
-  __shared__ int32_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
+  __shared__ int16_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
   bool bad = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
@@ -218,10 +221,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
                   [&](int j, const Cand& r) {
                     if (keep_pair(qi, hi, r.x, r.h, a)) {
-                      if (n < staged) {
-                        stage[n][threadIdx.x] = j;
+                      const int64_t o = base + n * step;
+                      const int dj = j - i;
+                      if (n < staged && dj > -32768 && dj <= 32767) {
+                        stage[n][threadIdx.x] = (int16_t)dj;
                       } else {
-                        const int64_t o = base + n * step;
+                        if (n < staged) stage[n][threadIdx.x] = kStagedOut;
                         if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
                       }
                       ++n;
@@ -231,10 +236,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   for (int64_t t = 0; t < staged; ++t) {
     const int64_t o = base + t * step;
     if (o >= cap) continue;
-    if (t < n)
-      put_entry(nbr, d16, o, stage[t][threadIdx.x], i, a.nref, bad);
-    else
+    if (t < n) {
+      const int16_t v = stage[t][threadIdx.x];
+      if (v != kStagedOut) put_entry(nbr, d16, o, i + v, i, a.nref, bad);
+    } else {
       nbr[o] = -1;
+    }
   }
   for (int64_t t = n > staged ? n : staged; t < len; ++t) {
     const int64_t o = base + t * step;

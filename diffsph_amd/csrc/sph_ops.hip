@@ -20,17 +20,32 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
   const float hi = h_of(P.qh, i, P.h0);
   float acc = 0.0f;
   const RowRef rr = row_ref(P, i);
-  for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-    const int j = P.nbr[k];
-    const float hj = h_of(P.rh, j, P.h0);
-    const float2 xj = P.rpos[j];
-    float dx = mod_offset(xi.x - xj.x, P.dom, 0), dy = mod_offset(xi.y - xj.y, P.dom, 1);
-    float h = pick_h(P.mode, hi, hj);
+  struct Nb {
+    float2 x;
+    float h, m;
+  };
+  auto load = [&](int j) { return Nb{P.rpos[j], h_of(P.rh, j, P.h0), rmass[j]}; };
+  auto body = [&](int k, const Nb& d) {
+    float dx = mod_offset(xi.x - d.x.x, P.dom, 0), dy = mod_offset(xi.y - d.x.y, P.dom, 1);
+    float h = pick_h(P.mode, hi, d.h);
     float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
     const float w = wendland_w(q, h);
     if (w_out) w_out[k] = w;
-    acc += rmass[j] * w;
+    acc += d.m * w;
+  };
+  // four entries' gathers in flight; the sum runs in entry order
+  int n = 0, k = rr.base;
+  for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+    int j[4];
+    Nb d[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = P.nbr[k + u * rr.step];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d[u] = load(j[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) body(k + u * rr.step, d[u]);
   }
+  for (; n < rr.cnt; ++n, k += rr.step) body(k, load(P.nbr[k]));
   rho[i] = acc;
   if (vol) vol[i] = qmass[i] / acc;
 }
