@@ -46,8 +46,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--train-rounds", type=int, default=5)
-    ap.add_argument("--train-steps", type=int, default=32, help="trajectory steps per training round")
+    ap.add_argument("--train-rounds", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=48, help="trajectory steps per training round")
     ap.add_argument("--no-ablation", action="store_true", help="skip the p_prev (warm start 1.0) run")
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)

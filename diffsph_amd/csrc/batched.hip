@@ -117,7 +117,7 @@ inline T* at(void* ws, size_t off) {
 }
 
 #ifndef DSPH_BFILL_STAGE
-#define DSPH_BFILL_STAGE 32
+#define DSPH_BFILL_STAGE 48
 #endif
 constexpr int kBFillStage = DSPH_BFILL_STAGE;
 
@@ -233,10 +233,11 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
   __shared__ int sh_max[kBlock / 64];
   __shared__ int sh_over[kBlock / 64];
   __shared__ int sh_sum[kBlock / 64];
-  // the first kBFillStage entries of a row are staged in LDS (a per-thread
-  // column) and written column by column after the scan: one contiguous
-  // line per store of the wave instead of scattered words
-  __shared__ int32_t stage[kBFillStage][kBlock];
+  // the first kBFillStage entries of a row are staged in LDS as 16-bit
+  // offsets idx - i (a per-thread column) and written column by column after
+  // the scan: one contiguous line per store of the wave instead of scattered
+  // words; an offset beyond 16 bits is written at once (slot marked)
+  __shared__ int16_t stage[kBFillStage][kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int total = 0;
   if (i < n) {
@@ -253,10 +254,13 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
                       const float dx = mod_offset(qi.x - c.x.x, d, 0);
                       const float dy = mod_offset(qi.y - c.x.y, d, 1);
                       if (within_support(dist2_exact(dx, dy), h)) {
-                        if (total < kBFillStage && total < width)
-                          stage[total][threadIdx.x] = c.idx;
-                        else if (total < width)
+                        const int di = c.idx - i;
+                        if (total < kBFillStage && total < width && di > -32768 && di <= 32767) {
+                          stage[total][threadIdx.x] = (int16_t)di;
+                        } else if (total < width) {
+                          if (total < kBFillStage) stage[total][threadIdx.x] = (int16_t)-32768;
                           nbr[base + (int64_t)total * 64] = c.idx;
+                        }
                         ++total;
                       }
                     });
@@ -283,7 +287,10 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
       }
     }
     const int staged = total < kBFillStage ? total : kBFillStage;
-    for (int t = 0; t < staged && t < width; ++t) nbr[base + (int64_t)t * 64] = stage[t][threadIdx.x];
+    for (int t = 0; t < staged && t < width; ++t) {
+      const int16_t v = stage[t][threadIdx.x];
+      if (v != (int16_t)-32768) nbr[base + (int64_t)t * 64] = i + v;
+    }
     cnt[i] = total < width ? total : width;
   }
   int m = total, over = total > width ? 1 : 0, sum = total;
