@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--train-rounds", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=48, help="trajectory steps per training round")
     ap.add_argument("--no-ablation", action="store_true", help="skip the p_prev (warm start 1.0) run")
+    ap.add_argument("--neighbor-width", type=int, default=67,
+                    help="row width of the one-pass neighbor list (0: count pass + two-pass list)")
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -102,7 +104,7 @@ def main():
     def make(seed, warm=0.75):
         dd = d if seed == args.seed else sampling.dfsph_random_flow(args.nside, seed=seed, jitter_width=0.1)
         return DFSPHSolver(T(dd["pos"]), T(dd["vel"]), T(dd["mass"]), T(dd["area"]),
-                           DFSPHConfig(dt=dt, support=d["h"], warm_start=warm))
+                           DFSPHConfig(dt=dt, support=d["h"], warm_start=warm, neighbor_width=args.neighbor_width))
 
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
     t0 = time.perf_counter()
@@ -280,7 +282,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"dfsph_random_flow_{args.nside}x{args.nside}", "particles": n,
                    "solver": "DFSPH, learned warm start (MLP 16-64-64-64-1, bf16 MFMA) with plain-Jacobi judge",
-                   "dt": dt, "pairs": P, "learned": rnd(learned), "plain": rnd(plain),
+                   "dt": dt, "pairs": P, "neighbor_width": args.neighbor_width, "learned": rnd(learned), "plain": rnd(plain),
                    "p_prev_warm_start": rnd(prev),
                    "speedup_vs_plain": round(learned["value"] / plain["value"], 3),
                    "train_seconds": round(t_train, 2),

@@ -193,10 +193,15 @@ __device__ __forceinline__ void put_entry(int32_t* __restrict__ nbr, int16_t* __
   }
 }
 
+// fixed_w > 0: the one-pass fixed-width layout (slice s at s * 64 * fixed_w,
+// no count pass): counts[i] = min(entries, fixed_w), *row_ovf set when a row
+// has more entries than fixed_w (the list is then incomplete).
 __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ slice,
                                                           int32_t* __restrict__ nbr, int64_t cap,
-                                                          int16_t* __restrict__ d16, int32_t* __restrict__ ovf) {
+                                                          int16_t* __restrict__ d16, int32_t* __restrict__ ovf,
+                                                          int fixed_w, int32_t* __restrict__ counts,
+                                                          int32_t* __restrict__ row_ovf) {
   __shared__ int16_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
   bool bad = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,7 +210,11 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
   float hi = h_of(a.qh, i, a.h0);
   int64_t base, step, len;
-  if (slice) {
+  if (fixed_w > 0) {
+    base = (int64_t)(i >> 6) * 64 * fixed_w + (i & 63);
+    step = 64;
+    len = fixed_w;
+  } else if (slice) {
     const int s = i >> 6;
     base = (int64_t)slice[s] + (i & 63);
     step = 64;
@@ -215,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     step = 1;
     len = (int64_t)off[i + 1] - off[i];
   }
-  const int64_t staged = slice ? (kFillStage < len ? kFillStage : len) : 0;
+  const int64_t staged = (slice || fixed_w > 0) ? (kFillStage < len ? kFillStage : len) : 0;
   int64_t n = 0;
   for_candidates(a, qi, [&](int b, int e) {
     scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
@@ -239,15 +248,32 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     if (t < n) {
       const int16_t v = stage[t][threadIdx.x];
       if (v != kStagedOut) put_entry(nbr, d16, o, i + v, i, a.nref, bad);
-    } else {
+    } else if (fixed_w == 0) {
       nbr[o] = -1;
     }
   }
-  for (int64_t t = n > staged ? n : staged; t < len; ++t) {
-    const int64_t o = base + t * step;
-    if (o < cap) nbr[o] = -1;
+  if (fixed_w > 0) {
+    // entries past the row's count are never read: no padding
+    counts[i] = n < len ? (int32_t)n : (int32_t)len;
+    if (n > len) *row_ovf = 1;
+  } else {
+    for (int64_t t = n > staged ? n : staged; t < len; ++t) {
+      const int64_t o = base + t * step;
+      if (o < cap) nbr[o] = -1;
+    }
   }
   if (bad) *ovf = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fixed_slices(int nslices, int width, int32_t* __restrict__ slice_off,
+                                                         int32_t* __restrict__ row_ovf,
+                                                         int32_t* __restrict__ d16_ovf) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) {
+    *row_ovf = 0;
+    if (d16_ovf) *d16_ovf = 0;
+  }
+  if (s <= nslices) slice_off[s] = s * 64 * width;
 }
 
 __global__ void k_fill_flag_clear(int32_t* flag) {
@@ -432,7 +458,7 @@ extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* gr
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
   k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity, nullptr,
-                                                                   nullptr);
+                                                                   nullptr, 0, nullptr, nullptr);
   return launch_status();
 }
 
@@ -453,7 +479,33 @@ extern "C" int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* 
   if (nq == 0) return DSPH_OK;
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow);
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow, 0, nullptr,
+                                                    nullptr);
+  return launch_status();
+}
+
+extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                                        const float* sorted_pos, const float* sorted_h, int64_t nref,
+                                        const float* qpos, const float* qh, int64_t nq, int32_t qpos_wrapped,
+                                        float h_uniform, int32_t mode, int32_t width, int32_t* counts,
+                                        int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16,
+                                        int32_t* delta_overflow, int32_t* row_overflow, void* stream) {
+  SearchArgs a;
+  int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
+                      h_uniform, mode);
+  if (s != DSPH_OK) return s;
+  if (width < 1 || width > 4096 || !row_overflow || !slice_off || (nbr16 && !delta_overflow)) return DSPH_ERR_ARG;
+  const int64_t nslices = (nq + 63) / 64;
+  if (nslices * 64 * (int64_t)width >= INT32_MAX) return DSPH_ERR_ARG;
+  if (capacity < nslices * 64 * (int64_t)width) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  k_fixed_slices<<<grid_for(nslices + 1), kBlock, 0, st>>>((int)nslices, width, slice_off, row_overflow,
+                                                          delta_overflow);
+  DSPH_LAUNCH_CHECK();
+  if (nq == 0) return DSPH_OK;
+  if (!counts || !nbr) return DSPH_ERR_ARG;
+  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, nullptr, nullptr, nbr, capacity, nbr16, delta_overflow, width,
+                                                    counts, row_overflow);
   return launch_status();
 }
 

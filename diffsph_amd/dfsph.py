@@ -58,6 +58,11 @@ class DFSPHConfig:
     # fit); results are identical either way
     compact_index: bool = True
     compact_limit: int = 32767      # largest |delta| kept in 16 bits (smaller values force the fallback)
+    # row width of the one-pass neighbor list (no count pass); a longer row
+    # falls back to the two-pass list and widens it.  0: always two-pass.
+    # Odd: a power-of-two slice stride (64 x 64 entries) costs the pressure
+    # sweeps 4-6 % at 1M particles, an odd one about 2 %
+    neighbor_width: int = 67
 
 
 @dataclass
@@ -114,6 +119,7 @@ class DFSPHSolver:
         # divergence iterations)
         self.hooks = {}
         self._d16 = Delta16Buffers()
+        self._nbr_width = max(int(cfg.neighbor_width), 0)
 
     # -- helpers ---------------------------------------------------------
     def _permute(self, t, idx):
@@ -141,8 +147,12 @@ class DFSPHSolver:
         self.p_den = self._permute(self.p_den, si)
         self.grid = grid
         fused = cfg.compact_index and cfg.compact_limit >= 32767
+        w = self._nbr_width
         self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True,
-                         delta16=self._d16 if fused else None)
+                         delta16=self._d16 if fused else None, layout="fixed" if w > 0 else "sell", width=w)
+        if w > 0 and self.nl.width == 0:
+            # a row outgrew the width: this step used the two-pass list
+            self._nbr_width = (int(self.nl.counts.max()) + 16) | 1
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
             self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)

@@ -136,6 +136,7 @@ class NeighborList:
     entries: int = 0        # length of the layout (pairs + padding)
     nbr16: torch.Tensor | None = None           # 16-bit modular deltas (pressure sweeps)
     nbr16_overflow: torch.Tensor | None = None  # int32 device flag
+    width: int = 0          # row width of the one-pass fixed-width layout (0: two-pass)
 
     def attach_compact16(self, buf16: torch.Tensor, flag: torch.Tensor, limit: int = 32767):
         """Write the 16-bit delta copy of the list into `buf16` (int16, at
@@ -193,13 +194,19 @@ class Delta16Buffers:
 
 
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
-           qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None) -> NeighborList:
+           qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None,
+           width: int = 67) -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
     Counts, scans (host reads of the pair and entry totals, as the
     reference's `.item()` does) and fills.  With ``qpos_wrapped`` the
     queries are the grid's own wrapped sorted positions.  ``layout`` is
-    "sell" (sliced ELL, the layout the pair kernels stream) or "csr"."""
+    "sell" (sliced ELL, the layout the pair kernels stream), "csr", or
+    "fixed": the sliced layout with every row `width` entries wide, filled
+    in one pass without the count pass (one host read of the overflow flag
+    and the pair total); a row longer than `width` makes it fall back to
+    "sell" (the returned list then has ``width == 0``).  The entries, their
+    order and the counts are the same in all three layouts."""
     _check_f32(qpos, "queryPositions", (2,))
     m = MODES[mode] if isinstance(mode, str) else int(mode)
     nq = qpos.shape[0]
@@ -210,6 +217,11 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     args = (C.byref(grid.domain), C.byref(grid.grid), ptr(grid.cell_start), ptr(grid.sorted_pos),
             ptr(grid.sorted_h), grid.n, ptr(qpos), ptr(qh), nq, 1 if qpos_wrapped else 0,
             C.c_float(grid.h_uniform), m)
+    if layout == "fixed":
+        nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st)
+        if nl is not None:
+            return nl
+        layout = "sell"
     check(L.dsph_neighbor_count(*args, ptr(counts), st), "neighbor_count")
     offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
     ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
@@ -244,6 +256,30 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
         wq = wrap_positions(qpos, grid.domain)
     return NeighborList(counts, offsets, nbr[:max(entries, 1)] if layout == "sell" else nbr[:npairs], npairs, wq, qh,
                         grid, m, slice_off, entries, buf16, flag16)
+
+
+def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st):
+    nq = qpos.shape[0]
+    dev = qpos.device
+    nsl = (nq + 63) // 64
+    cap = max(nsl * 64 * width, 1)
+    slice_off = torch.empty(nsl + 1, dtype=torch.int32, device=dev)
+    nbr = torch.empty(cap, dtype=torch.int32, device=dev)
+    info = torch.empty(2, dtype=torch.int32, device=dev)   # [row overflow, pair total]
+    buf16 = flag16 = None
+    if delta16 is not None:
+        buf16, flag16 = delta16.get(cap, dev)
+    check(L.dsph_neighbor_fill_fixed(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(buf16),
+                                     ptr(flag16), ptr(info), st), "neighbor_fill_fixed")
+    offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+    ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
+    check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), None, ptr(ws), ws.numel(), st), "scan")
+    info[1:2].copy_(offsets[nq:nq + 1])
+    ovf, npairs = (int(v) for v in info.cpu())
+    if ovf:
+        return None
+    wq = qpos if qpos_wrapped else wrap_positions(qpos, grid.domain)
+    return NeighborList(counts, offsets, nbr, npairs, wq, qh, grid, m, slice_off, cap, buf16, flag16, width)
 
 
 def wrap_positions(pos: torch.Tensor, domain: _lib.Domain) -> torch.Tensor:

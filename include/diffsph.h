@@ -187,6 +187,26 @@ int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
  * reference slots themselves (a set searched against its own grid and not
  * remapped afterwards).  *overflow is cleared first.  No counterpart in the
  * reference. */
+/* One-pass neighbor list in a fixed-width sliced layout (no count pass, no
+ * host read of the list size): slice s starts at s * 64 * width (slice_off
+ * [ceil(nq/64)+1] is written), entry n of row i at slice_off[i/64] + 64 n +
+ * i%64, counts[i] = the row's entry count (capped at width).  When a row
+ * has more than `width` entries *row_overflow is set and the list is
+ * incomplete (build the two-pass list instead).  Entries past a row's count
+ * are not written.  With nbr16 the 16-bit deltas of
+ * dsph_neighbor_compact16 are written in the same pass (rows = reference
+ * slots; *delta_overflow as there).  Both flags are cleared first.
+ * capacity >= ceil(nq/64) * 64 * width.  No counterpart in the reference. */
+int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid,
+                             const int32_t* cell_start, const float* sorted_pos,
+                             const float* sorted_h, int64_t nref,
+                             const float* qpos, const float* qh, int64_t nq,
+                             int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                             int32_t width, int32_t* counts, int32_t* slice_off,
+                             int32_t* nbr, int64_t capacity, int16_t* nbr16,
+                             int32_t* delta_overflow, int32_t* row_overflow,
+                             void* stream);
+
 int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* grid,
                          const int32_t* cell_start, const float* sorted_pos,
                          const float* sorted_h, int64_t nref,

@@ -146,14 +146,19 @@ def test_compact16_deltas_decode_and_overflow():
 
 
 def test_dfsph_compact_index_paths_identical():
-    """16-bit deltas, the int32 list, and the device-side fallback after an
-    overflow give the same trajectory bit for bit."""
+    """16-bit deltas, the int32 list, the device-side fallback after an
+    overflow, the two-pass neighbor list and the one-pass list's fallback
+    (then widened) give the same trajectory bit for bit."""
     d, dt = _setup(96, 4)
-    runs = [_solver(d, dt), _solver(d, dt, compact_index=False), _solver(d, dt, compact_limit=2)]
+    runs = [_solver(d, dt), _solver(d, dt, compact_index=False), _solver(d, dt, compact_limit=2),
+            _solver(d, dt, neighbor_width=0), _solver(d, dt, neighbor_width=8)]
     for k in range(4):
         its = [r.step() for r in runs]
         assert len({(st.div_iters, st.den_iters) for st in its}) == 1, k
+        assert len({st.num_pairs for st in its}) == 1, k
     assert int(runs[2].nl.nbr16_overflow.item()) == 1
+    assert runs[0].nl.width == 67 and runs[3].nl.width == 0
+    assert runs[4]._nbr_width > 8 and runs[4].nl.width == runs[4]._nbr_width
     for r in runs[1:]:
         assert torch.equal(runs[0].pos, r.pos) and torch.equal(runs[0].vel, r.vel)
         assert torch.equal(runs[0].p_div, r.p_div) and torch.equal(runs[0].p_den, r.p_den)

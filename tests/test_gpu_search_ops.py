@@ -50,6 +50,15 @@ def test_search_matches_oracle_bitexact(orc, nside, seed, jit):
     np.testing.assert_array_equal(nl.counts.cpu().numpy(), c_o)
     np.testing.assert_array_equal(nl.offsets.cpu().numpy(), off_o)
     np.testing.assert_array_equal(nl.csr_nbr().cpu().numpy(), nbr_o)
+    # one-pass fixed-width layout, and its fallback when a row is longer
+    for width, fits in [(67, True), (int(c_o.max()), True), (int(c_o.max()) - 1, False)]:
+        print(f"  fixed width {width}", flush=True)
+        nf = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True, layout="fixed", width=width)
+        assert nf.width == (width if fits else 0)
+        assert nf.num_pairs == int(off_o[-1])
+        np.testing.assert_array_equal(nf.counts.cpu().numpy(), c_o)
+        np.testing.assert_array_equal(nf.offsets.cpu().numpy(), off_o)
+        np.testing.assert_array_equal(nf.csr_nbr().cpu().numpy(), nbr_o)
 
 
 def test_search_matches_brute_force_sets_small(orc):
@@ -91,6 +100,8 @@ def test_search_edge_cases():
     g = CellGrid.build(pos, None, dom, 0.1, h_uniform=0.1)
     nl = search(g, pos, None, "scatter")
     assert nl.num_pairs == 0 and nl.offsets.cpu().tolist() == [0]
+    nl = search(g, pos, None, "scatter", layout="fixed")
+    assert nl.num_pairs == 0 and nl.offsets.cpu().tolist() == [0] and nl.width == 67
     # single particle, far outside the box (wrapped)
     pos = torch.tensor([[5.3, -7.9]], device=dev)
     g = CellGrid.build(pos, None, dom, 0.1, h_uniform=0.1)
@@ -101,6 +112,8 @@ def test_search_edge_cases():
     g = CellGrid.build(pos, None, dom, 0.1, h_uniform=0.1)
     nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True)
     assert nl.num_pairs == 500 * 500
+    nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True, layout="fixed")
+    assert nl.num_pairs == 500 * 500 and nl.width == 0
     # too few periodic cells is an error, not a fault
     with pytest.raises(RuntimeError):
         CellGrid.build(pos, None, dom, 0.9, h_uniform=0.9)
