@@ -62,6 +62,7 @@ def parse():
                     help="dfsph: 1M DFSPH plain vs learned (the headline metric); ensemble: 512 periodic "
                          "delta-SPH domains in one HIP graph; tgv: delta-SPH Taylor-Green vortex; dambreak: "
                          "delta-SPH dam break with mDBC walls; rollout: adjoint of a Taylor-Green rollout")
+    ap.add_argument("--shifting", action="store_true", help="dambreak: delta+ particle shifting after every step")
     ap.add_argument("--ndom", type=int, default=512, help="ensemble: number of domains")
     ap.add_argument("--cell-scale", type=float, default=None, help="ensemble: skip the autotuner, use this cell scale")
     ap.add_argument("--block", type=int, default=256, help="ensemble: workgroup size with --cell-scale")
@@ -492,9 +493,13 @@ def bench_dambreak(args):
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     pos, vel, mass, rho, cfg, walls = dam_break_state(args.nside, dev)
+    if args.shifting:
+        from dataclasses import replace
+        cfg = replace(cfg, shifting=True)
     n = pos.shape[0]
     s = DeltaSPHSolver(pos, vel, mass, rho, cfg, walls=walls)
-    log(f"dam break: {n} fluid, {s.walls.n} wall particles, h {cfg.support:.5g}, c_s {cfg.cs:.4g}")
+    log(f"dam break: {n} fluid, {s.walls.n} wall particles, h {cfg.support:.5g}, c_s {cfg.cs:.4g}, "
+        f"shifting {cfg.shifting}")
     for _ in range(args.warmup):
         s.step()
     torch.cuda.synchronize()
@@ -510,7 +515,8 @@ def bench_dambreak(args):
     log(f"dam break: {args.steps} steps {secs:.2f} s, {value:.4g} particle-updates/s; t = {s.time:.4f}, "
         f"front x = {front:.4f}")
     config = {"workload": f"deltasph_dam_break_dx_1_{args.nside}", "particles": n, "wall_particles": s.walls.n,
-              "simulated_time": s.time, "front_x": front, "dt_last": s.dt}
+              "simulated_time": s.time, "front_x": front, "dt_last": s.dt,
+              "particle_shifting": "delta+ after every step" if cfg.shifting else None}
     print(json.dumps(_line("particle-updates/s, delta-SPH dam break with mDBC walls", value, "particle-updates/s",
                            args, args.steps, secs, "f32", config, None, None)), flush=True)
 

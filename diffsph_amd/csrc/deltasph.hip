@@ -399,11 +399,18 @@ __global__ void __launch_bounds__(kBlock) k_max_norm(const float2* __restrict__ 
   }
 }
 
-__global__ void k_max_final(const float* part, int nb, float* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: lane l folds partials l, l+64, ... then a shuffle tree (the
+// maximum does not depend on the order).  Launched as <<<1, 64>>>.
+__global__ void __launch_bounds__(64) k_max_final(const float* part, int nb, float* out) {
   float m = 0.0f;
-  for (int b = 0; b < nb; ++b) m = part[b] > m ? part[b] : m;
-  *out = m;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nb; b += 64) m = part[b] > m ? part[b] : m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float t = __shfl_down(m, o, 64);
+    m = t > m ? t : m;
+  }
+  if (threadIdx.x == 0) *out = m;
 }
 
 // sum of squares of n floats in float64 (NaN entries skipped): per-block
@@ -427,11 +434,15 @@ __global__ void __launch_bounds__(kBlock) k_sumsq(const float* __restrict__ v, i
   }
 }
 
-__global__ void k_sum_final(const double* part, int nb, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave, a fixed order: lane l sums partials l, l+64, ... in turn, then
+// a shuffle tree.  Launched as <<<1, 64>>>.
+__global__ void __launch_bounds__(64) k_sum_final(const double* part, int nb, double* out) {
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[b];
-  *out = s;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < nb; b += 64) s += part[b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (threadIdx.x == 0) *out = s;
 }
 
 // Ghost positions of wall particles (buildBoundaryGhostParticles,

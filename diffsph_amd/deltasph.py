@@ -79,6 +79,10 @@ class DeltaSPHConfig:
     # cell side of the search grids = 1.001 * support * cell_scale (>= 1);
     # changes the order of the neighbor entries, not the neighbor sets
     cell_scale: float = 1.0
+    # row width of the one-pass neighbor lists (no count pass); a longer row
+    # falls back to the two-pass list for that search and widens the width
+    # of that list kind.  0: always two-pass
+    neighbor_width: int = 67
     # delta+ particle shifting after every step (solveShifting with scheme
     # 'deltaSPH', runner.py:40-42; shifting.py:735-765 defaults); off in the
     # harness configurations, as in the reference's benchmark notebooks
@@ -222,8 +226,19 @@ class DeltaSPHSolver:
         self._grad = torch.empty(n, 2, **f)
         self._k = [(torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f)) for _ in range(4)]
         self._stage = (torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f))
+        self._widths = {}           # list kind -> row width of the one-pass list
 
     # -- pieces ------------------------------------------------------------
+    def _search(self, kind: str, grid: CellGrid, q: torch.Tensor):
+        """Neighbor list of the wrapped queries `q` against `grid`: the
+        one-pass fixed-width list while its rows fit, else the two-pass one
+        (and the next search of this kind is wider)."""
+        w = self._widths.get(kind, max(int(self.cfg.neighbor_width), 0))
+        nl = search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed" if w > 0 else "sell", width=w)
+        if w > 0 and nl.width == 0:
+            self._widths[kind] = (int(nl.counts.max()) + 16) | 1
+        return nl
+
     def neighbors(self, pos: torch.Tensor):
         """Neighbor list of `pos` with rows in the state's order and entries
         mapped to state indices; returns (NeighborList, pairs descriptor,
@@ -231,7 +246,7 @@ class DeltaSPHSolver:
         cfg = self.cfg
         grid = CellGrid.build(pos, None, self.domain, cfg.support * cfg.cell_scale, h_uniform=cfg.support)
         wpos = wrap_positions(pos, self.domain)
-        nl = search(grid, wpos, None, "scatter", qpos_wrapped=True)
+        nl = self._search("fluid", grid, wpos)
         check(lib().dsph_remap_neighbors(ptr(nl.nbr), nl.nbr.numel(), ptr(grid.sort_index), self.n,
                                          _lib.stream_handle(pos.device)), "remap_neighbors")
         pr = nl.pairs()
@@ -245,9 +260,9 @@ class DeltaSPHSolver:
         Returns (boundary descriptor, objects to keep alive)."""
         W = self.walls
         st = _lib.stream_handle(wpos.device)
-        nl_b = search(W.grid, wpos, None, "scatter", qpos_wrapped=True)
+        nl_b = self._search("wall", W.grid, wpos)
         pr_b = nl_b.pairs()
-        nl_g = search(grid, W.ghost, None, "scatter", qpos_wrapped=True)
+        nl_g = self._search("ghost", grid, W.ghost)
         check(lib().dsph_remap_neighbors(ptr(nl_g.nbr), nl_g.nbr.numel(), ptr(grid.sort_index), self.n, st),
               "remap_neighbors")
         pr_g = nl_g.pairs()
@@ -349,7 +364,7 @@ class DeltaSPHSolver:
         bnd, keep = None, None
         if self.walls is not None:
             W = self.walls
-            nl_b = search(W.grid, wpos, None, "scatter", qpos_wrapped=True)
+            nl_b = self._search("wall", W.grid, wpos)
             pr_b = nl_b.pairs()
             bnd = _lib.Boundary()
             bnd.pairs = C.pointer(pr_b)

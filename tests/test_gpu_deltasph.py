@@ -217,6 +217,28 @@ def test_dam_break_rk4_matches_oracle_bitexact():
     print(f"dt {dts}", flush=True)
 
 
+def test_neighbor_list_layouts_identical():
+    """Dam break at dx = 1/32, perturbed, with shifting, 3 RK4 steps: the
+    one-pass lists (default width), the two-pass lists, and a width that
+    every list kind outgrows at first (then widened) give the same
+    trajectory bit for bit."""
+    from dataclasses import replace
+    runs = []
+    for w in (67, 0, 4):
+        pos, vel, mass, rho, cfg, walls = _dam(32)
+        cfg = replace(cfg, neighbor_width=w, shifting=True)
+        s = DeltaSPHSolver(pos, vel, mass, rho, cfg, walls=walls)
+        dts = [s.step() for _ in range(3)]
+        runs.append((s, dts))
+    s0, d0 = runs[0]
+    assert s0._widths == {}
+    assert {"fluid", "ghost"} <= set(runs[2][0]._widths)
+    for s, d in runs[1:]:
+        assert d == d0 and s.last_pairs == s0.last_pairs
+        for a in ("pos", "vel", "rho", "uid"):
+            assert torch.equal(getattr(s, a), getattr(s0, a)), a
+
+
 def test_dam_break_config_properties():
     """The dam-break benchmark (131072 fluid + 9252 wall particles), all
     2000 RK4 steps: the state stays finite, the fluid stays inside the box,
