@@ -202,13 +202,14 @@ def main():
     P = solver.nl.num_pairs
     # Algorithmic bytes (each per-particle array once per launch, each pair
     # entry once; gathered neighbor fields and list padding not counted):
-    # sweep A: p 4 + rho 4 + count 4 + a(write) 8 = 20 B/particle;
-    #          j (2 as a 16-bit delta, else 4) + V_j gradW_ij 8 = 10 (12) B/pair
-    # sweep B: a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 32 B/particle; 10 (12) B/pair
+    # sweep A: x 8 + p 4 + rho 4 + count 4 + a(write) 8 = 28 B/particle;
+    #          j (2 as a 16-bit delta, else 4) + s_ij 4 = 6 (8) B/pair
+    # sweep B: x 8 + a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 40 B/particle; 6 (8) B/pair
+    # (the coefficient V_j gradW_ij is d_ij * s_ij, d_ij from the positions)
     idx16 = solver.nl.nbr16 is not None and int(solver.nl.nbr16_overflow.item()) == 0
-    per_pair = 10 if idx16 else 12
-    bytes_a = 20 * n + per_pair * P
-    bytes_b = 32 * n + per_pair * P
+    per_pair = 6 if idx16 else 8
+    bytes_a = 28 * n + per_pair * P
+    bytes_b = 40 * n + per_pair * P
     gbs_a = bytes_a / t_a / 1e9
     gbs_b = bytes_b / t_b / 1e9
     log(f"sweep A (dsph_pressure_accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "

@@ -115,6 +115,7 @@ struct Geom {
   float r, q, h;
   float ex, ey;   // xhat
   float w;        // W_ij
+  float dw;       // dW/dr
   float gx, gy;   // grad_i W_ij
 };
 
@@ -129,11 +130,18 @@ __device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const D
   g.ex = __fdiv_rn(g.dx, den);
   g.ey = __fdiv_rn(g.dy, den);
   g.w = wendland_w(g.q, h);
-  float dw = wendland_dw(g.q, h);
-  g.gx = g.ex * dw;
-  g.gy = g.ey * dw;
+  g.dw = wendland_dw(g.q, h);
+  g.gx = g.ex * g.dw;
+  g.gy = g.ey * g.dw;
   return g;
 }
+
+// The DFSPH pair coefficient V_j gradW_ij in the factored form
+// d_ij * s_ij with s_ij = V_j * (dW/dr / (r + 1e-7)): the per-entry scalar
+// s_ij is all the solver stores, d_ij is recomputed from the positions.
+// (Algebraically the reference's V_j * (d/(r+1e-7)) * dW/dr; it differs
+// from that product by a few ulp.)
+__device__ __forceinline__ float dfsph_t(const Geom& g) { return __fdiv_rn(g.dw, g.r + 1e-7f); }
 
 struct PairsDev {
   int nq, nr;

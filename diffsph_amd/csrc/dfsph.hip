@@ -2,11 +2,14 @@
 // kernels with device-side loop control.
 //
 // The prepare sweep (once per step, positions fixed for the whole solve)
-// evaluates the geometry of every pair and stores kV_ij = V_j gradW_ij -- the
-// product the reference forms as (m_j/rho_j) * gradKernels
-// (v2/sphOps.py:50,93) -- in the neighbor list's layout.  Every later sweep
-// of both solves streams kV instead of recomputing sqrt/div per pair; the
-// stored value is bit-identical to the one the on-the-fly evaluation gives.
+// evaluates the geometry of every pair and stores the scalar
+// s_ij = V_j dW/dr / (r + 1e-7) in the neighbor list's layout; the
+// coefficient kV_ij = V_j gradW_ij -- the product the reference forms as
+// (m_j/rho_j) * gradKernels (v2/sphOps.py:50,93) -- is d_ij * s_ij, with the
+// min-image offset d_ij = x_i - x_j recomputed from the positions (a gather
+// that hits the caches).  Every later sweep of both solves streams s (4 B per
+// entry) instead of recomputing sqrt/div per pair or streaming the 8-byte
+// vector; the on-the-fly evaluation gives the same bits.
 //
 // Per iteration two pair sweeps:
 //   A: a_i = -(1/rho_i) sum_j (p_i + p_j) kV_ij
@@ -55,14 +58,22 @@ __device__ __forceinline__ double block_sum_d(double v, double* sh) {
   return t;
 }
 
-// V_j gradW_ij of entry k: stored (kv != NULL) or evaluated on the fly.
-__device__ __forceinline__ float2 kv_of(const PairsDev& P, const float2* __restrict__ kv, int k, int j, float2 xi,
+// V_j gradW_ij of entry k as d_ij * s_ij (dfsph_t): s_ij stored per entry
+// (kv != NULL) or evaluated on the fly.
+__device__ __forceinline__ float2 coef_of(float2 xi, float2 xj, float sv, const Dom& d) {
+  const float dx = mod_offset(xi.x - xj.x, d, 0);
+  const float dy = mod_offset(xi.y - xj.y, d, 1);
+  return make_float2(dx * sv, dy * sv);
+}
+
+__device__ __forceinline__ float2 kv_of(const PairsDev& P, const float* __restrict__ kv, int k, int j, float2 xi,
                                         float hi, const float* __restrict__ vol) {
-  if (kv) return kv[k];
+  const float2 xj = P.rpos[j];
+  if (kv) return coef_of(xi, xj, kv[k], P.dom);
   const float hj = h_of(P.rh, j, P.h0);
-  Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
-  const float Vj = vol[j];
-  return make_float2(Vj * g.gx, Vj * g.gy);
+  Geom g = pair_geom(xi, xj, pick_h(P.mode, hi, hj), P.dom);
+  const float sv = vol[j] * dfsph_t(g);
+  return make_float2(g.dx * sv, g.dy * sv);
 }
 
 // ---------------------------------------------------------------------------
@@ -83,7 +94,7 @@ struct PrepArgs {
 
 __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
                                                           float* __restrict__ alpha_den, float* __restrict__ div_src,
-                                                          float2* __restrict__ visc_acc, float2* __restrict__ kv_out) {
+                                                          float2* __restrict__ visc_acc, float* __restrict__ kv_out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -112,13 +123,17 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
     Geom g = pair_geom(xi, d.x, h, P.dom);
     const float Vj = d.vol;
     const float2 vj = d.v;
-    const float kx = Vj * g.gx, ky = Vj * g.gy;
-    if (kv_out) kv_out[k] = make_float2(kx, ky);
+    // gradW_ij = d_ij * t_ij, V_j gradW_ij = d_ij * s_ij (dfsph_t)
+    const float t = dfsph_t(g);
+    const float sv = Vj * t;
+    const float kx = g.dx * sv, ky = g.dy * sv;
+    if (kv_out) kv_out[k] = sv;
     // alpha terms (dfsph.py:13-20)
     k1x += kx;
     k1y += ky;
     const float c2 = (Vj * Vj) / (d.area * A.rho0);
-    k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
+    const float gx = g.dx * t, gy = g.dy * t;
+    k2 += c2 * (gx * gx + gy * gy);
     // divergence 'difference' of v (sphOps.py:92-101)
     dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
     if (A.visc_on) {
@@ -173,6 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
 // are read exactly once per launch, so they bypass the caches
 // (non-temporal) and leave L2 to the gathered neighbor fields.
 __device__ __forceinline__ int ld_stream(const int32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ float2 ld_stream(const float2* p) {
   return make_float2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
 }
@@ -200,30 +216,35 @@ __device__ __forceinline__ bool use_idx16(const PairsDev& P) { return P.nbr16 &&
 // dv_i = sum_j (v*_j - v*_i) . V_j gradW_ij, v* = v + dt adv, entries in row
 // order with kUnroll index / coefficient loads in flight.
 template <class Idx>
-__device__ __forceinline__ float source_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
-                                           const float2* __restrict__ vel, const float2* __restrict__ adv, float dt,
-                                           float2 vi) {
+__device__ __forceinline__ float source_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
+                                           const float* __restrict__ kv, const float2* __restrict__ vel,
+                                           const float2* __restrict__ adv, float dt, float2 vi) {
   float dv = 0.f;
   int n = 0, k = rr.base;
   for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
     int j[4];
-    float2 c[4], vj[4];
+    float sv[4];
+    float2 c[4], vj[4], xj[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       j[u] = idx(k + u * rr.step);
-      c[u] = ld_stream(kv + k + u * rr.step);
+      sv[u] = ld_stream(kv + k + u * rr.step);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
+      xj[u] = P.rpos[j[u]];
       const float2 aj = adv ? adv[j[u]] : make_float2(0.f, 0.f);
       vj[u] = make_float2(vel[j[u]].x + dt * aj.x, vel[j[u]].y + dt * aj.y);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) dv += (vj[u].x - vi.x) * c[u].x + (vj[u].y - vi.y) * c[u].y;
+    for (int u = 0; u < 4; ++u) {
+      c[u] = coef_of(xi, xj[u], sv[u], P.dom);
+      dv += (vj[u].x - vi.x) * c[u].x + (vj[u].y - vi.y) * c[u].y;
+    }
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = ld_stream(kv + k);
+    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
     const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
     const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
     dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
@@ -231,7 +252,7 @@ __device__ __forceinline__ float source_kv(const RowRef& rr, const Idx& idx, con
   return dv;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float2* __restrict__ kv,
+__global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float* __restrict__ kv,
                                                          const float2* __restrict__ vel,
                                                          const float2* __restrict__ adv,
                                                          const float* __restrict__ vol,
@@ -246,8 +267,8 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float
   float dv = 0.f;
   const RowRef rr = row_ref(P, i);
   if (kv) {
-    dv = use_idx16(P) ? source_kv(rr, Idx16{P.nbr16, i, P.nr}, kv, vel, adv, dt, vi)
-                      : source_kv(rr, Idx32{P.nbr}, kv, vel, adv, dt, vi);
+    dv = use_idx16(P) ? source_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, vel, adv, dt, vi)
+                      : source_kv(P, xi, rr, Idx32{P.nbr}, kv, vel, adv, dt, vi);
   } else {
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
@@ -274,30 +295,36 @@ constexpr int kUnroll = DSPH_SWEEP_UNROLL;
 // `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
 // else evaluated on the fly.
 template <class Idx>
-__device__ __forceinline__ void sweep_a_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
-                                           const float* __restrict__ p, float pi, float sj, float& ax, float& ay) {
+__device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
+                                           const float* __restrict__ kv, const float* __restrict__ p, float pi,
+                                           float sj, float& ax, float& ay) {
   int n = 0, k = rr.base;
   for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
     int j[kUnroll];
-    float2 c[kUnroll];
+    float sv[kUnroll];
+    float2 xj[kUnroll];
     float pj[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       j[u] = idx(k + u * rr.step);
-      c[u] = ld_stream(kv + k + u * rr.step);
+      sv[u] = ld_stream(kv + k + u * rr.step);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) pj[u] = sj * p[j[u]];
+    for (int u = 0; u < kUnroll; ++u) {
+      xj[u] = P.rpos[j[u]];
+      pj[u] = sj * p[j[u]];
+    }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
+      const float2 c = coef_of(xi, xj[u], sv[u], P.dom);
       const float s = pi + pj[u];
-      ax += s * c[u].x;
-      ay += s * c[u].y;
+      ax += s * c.x;
+      ay += s * c.y;
     }
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = ld_stream(kv + k);
+    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
     const float s = pi + sj * p[j];
     ax += s * c.x;
     ay += s * c.y;
@@ -309,7 +336,7 @@ __device__ __forceinline__ void sweep_a_kv(const RowRef& rr, const Idx& idx, con
 // 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
 // `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
 // else evaluated on the fly.
-__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float2* __restrict__ kv,
+__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ kv,
                                                           const float* __restrict__ vol,
                                                           const float* __restrict__ rho, const float* __restrict__ p,
                                                           float2* __restrict__ acc, const SolveCtrl* ctrl, int diff) {
@@ -321,13 +348,13 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
   const float sj = diff ? -1.0f : 1.0f;
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
+  const float2 xi = P.qpos[i];
   if (kv) {
     if (use_idx16(P))
-      sweep_a_kv(rr, Idx16{P.nbr16, i, P.nr}, kv, p, pi, sj, ax, ay);
+      sweep_a_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, p, pi, sj, ax, ay);
     else
-      sweep_a_kv(rr, Idx32{P.nbr}, kv, p, pi, sj, ax, ay);
+      sweep_a_kv(P, xi, rr, Idx32{P.nbr}, kv, p, pi, sj, ax, ay);
   } else {
-    const float2 xi = P.qpos[i];
     const float hi = h_of(P.qh, i, P.h0);
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
@@ -342,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
 }
 
 struct SweepBArgs {
-  const float2* kv;
+  const float* kv;
   const float* vol;
   const float* src;
   const float* alpha;
@@ -365,8 +392,19 @@ struct SweepBArgs {
 // into sweep B needs device-scope fences, which on this part write back the
 // per-XCD L2s: 1.4-1.7x slower solves, so the reduction stays a kernel.)
 __device__ __forceinline__ void reduce_partials(const double* partials, int nb, int n, SolveCtrl* ctrl, double* sh) {
+  // Thread t adds partials t, t + blockDim, ... in that order; eight loads
+  // are issued before their adds so that their latencies overlap.
+  constexpr int kR = 8;
   double v = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) v += partials[k];
+  int k = threadIdx.x;
+  for (; k + (kR - 1) * (int)blockDim.x < nb; k += kR * (int)blockDim.x) {
+    double t[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) t[u] = partials[k + u * (int)blockDim.x];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) v += t[u];
+  }
+  for (; k < nb; k += blockDim.x) v += partials[k];
   double tot = block_sum_d(v, sh);
   if (threadIdx.x == 0) {
     double mean = tot / (double)n;
@@ -379,27 +417,34 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int nb, 
 }
 
 template <class Idx>
-__device__ __forceinline__ float sweep_b_kv(const RowRef& rr, const Idx& idx, const float2* __restrict__ kv,
-                                            const float2* __restrict__ acc, float2 ai) {
+__device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
+                                            const float* __restrict__ kv, const float2* __restrict__ acc, float2 ai) {
   float sum = 0.f;
   int n = 0, k = rr.base;
   for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
     int j[kUnroll];
-    float2 c[kUnroll];
+    float sv[kUnroll];
+    float2 xj[kUnroll];
     float2 aj[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       j[u] = idx(k + u * rr.step);
-      c[u] = ld_stream(kv + k + u * rr.step);
+      sv[u] = ld_stream(kv + k + u * rr.step);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) aj[u] = acc[j[u]];
+    for (int u = 0; u < kUnroll; ++u) {
+      xj[u] = P.rpos[j[u]];
+      aj[u] = acc[j[u]];
+    }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) sum += (aj[u].x - ai.x) * c[u].x + (aj[u].y - ai.y) * c[u].y;
+    for (int u = 0; u < kUnroll; ++u) {
+      const float2 c = coef_of(xi, xj[u], sv[u], P.dom);
+      sum += (aj[u].x - ai.x) * c.x + (aj[u].y - ai.y) * c.y;
+    }
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = ld_stream(kv + k);
+    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
     const float2 aj = acc[j];
     sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
   }
@@ -422,9 +467,9 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     const RowRef rr = row_ref(P, i);
     if (B.kv) {
       if (use_idx16(P))
-        sum = sweep_b_kv(rr, Idx16{P.nbr16, i, P.nr}, B.kv, B.acc, ai);
+        sum = sweep_b_kv(P, P.qpos[i], rr, Idx16{P.nbr16, i, P.nr}, B.kv, B.acc, ai);
       else
-        sum = sweep_b_kv(rr, Idx32{P.nbr}, B.kv, B.acc, ai);
+        sum = sweep_b_kv(P, P.qpos[i], rr, Idx32{P.nbr}, B.kv, B.acc, ai);
     } else {
       const float2 xi = P.qpos[i];
       const float hi = h_of(P.qh, i, P.h0);
@@ -638,7 +683,7 @@ SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const flo
                         const float* alpha, float* p, float* residual, const float* accel, double* partials,
                         SolveCtrl* ctrl, int64_t n) {
   SweepBArgs B;
-  B.kv = reinterpret_cast<const float2*>(kv);
+  B.kv = kv;
   B.vol = vol;
   B.src = src;
   B.alpha = alpha;
@@ -735,7 +780,7 @@ extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, con
   A.pi_switch = visc ? visc->pi_switch : 0;
   A.visc_factor = visc ? (float)(2.0 * (2 + 2) * (double)visc->alpha * (double)visc->rho0) : 0.f;
   k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, A, alpha_div, alpha_den, div_src, reinterpret_cast<float2*>(visc_acc), reinterpret_cast<float2*>(kgrad));
+      P, A, alpha_div, alpha_den, div_src, reinterpret_cast<float2*>(visc_acc), kgrad);
   return launch_status();
 }
 
@@ -773,7 +818,7 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
   double* partials = reinterpret_cast<double*>(w);
   w += al256((size_t)nb * sizeof(double));
   float* src = reinterpret_cast<float*>(w);
-  const float2* kv = reinterpret_cast<const float2*>(kgrad);
+  const float* kv = kgrad;
 
   // Source term.
   if (prm->source_density || adv) {
@@ -832,7 +877,7 @@ extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, 
   if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, reinterpret_cast<const float2*>(kgrad), vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0);
+      P, kgrad, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0);
   return launch_status();
 }
 
@@ -869,7 +914,7 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
   w += al256(sizeof(SolveCtrl));
   double* partials = reinterpret_cast<double*>(w);
-  const float2* kv = reinterpret_cast<const float2*>(kgrad);
+  const float* kv = kgrad;
   // Fixed count: the loop condition can never stop it early.
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, iterations, iterations, 0.0);
   SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);

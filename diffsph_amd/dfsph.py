@@ -155,7 +155,7 @@ class DFSPHSolver:
             self._nbr_width = (int(self.nl.counts.max()) + 16) | 1
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
-            self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
+            self.kgrad = torch.empty(need + need // 8, dtype=torch.float32, device=self.pos.device)
             self.wpair = torch.empty(need + need // 8, dtype=torch.float32, device=self.pos.device)
         if cfg.compact_index and not fused:
             buf, flag = self._d16.get(need, self.pos.device)

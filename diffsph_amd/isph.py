@@ -121,7 +121,7 @@ class ISPHSolver:
         self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True)
         need = max(self.nl.entries, 1)
         if self.kgrad is None or self.kgrad.shape[0] < need:
-            self.kgrad = torch.empty(need + need // 8, 2, dtype=torch.float32, device=self.pos.device)
+            self.kgrad = torch.empty(need + need // 8, dtype=torch.float32, device=self.pos.device)
             self.lap = torch.empty(need + need // 8, dtype=torch.float32, device=self.pos.device)
         return self.nl
 

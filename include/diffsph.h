@@ -334,10 +334,12 @@ typedef struct dsph_visc_params {
   int32_t scheme; /* 0 deltaSPH_inviscid, 1 deltaSPH_viscid (alpha = nu) */
 } dsph_visc_params;
 
-/* kgrad (optional, NULL = not stored): V_j gradW_ij of every list entry,
- * [entries][2] in the list's layout (CSR or sliced), bit-identical to the
- * on-the-fly value; the solve sweeps stream it instead of re-evaluating the
- * pair geometry. */
+/* kgrad (optional, NULL = not stored): the scalar s_ij = V_j dW/dr /
+ * (r + 1e-7) of every list entry, [entries] floats in the list's layout
+ * (CSR or sliced).  The DFSPH coefficient V_j gradW_ij is d_ij * s_ij with
+ * d_ij = x_i - x_j (min image), recomputed from the positions by every
+ * sweep that streams kgrad; bit-identical to the on-the-fly value.  (ABI
+ * version 3; version 2 stored the 8-byte vector.) */
 int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel,
                        const float* mass, const float* areas, const float* vol,
                        const float* rho, float dt, float rho0,

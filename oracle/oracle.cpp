@@ -110,7 +110,7 @@ T kernelDW(T q, T h) { return (((T(-20) * q) * cpow<T>(T(1) - q, 3)) * C2<T>()) 
 
 template <class T>
 struct G {
-  T dx, dy, r, q, h, ex, ey, w, gx, gy;
+  T dx, dy, r, q, h, ex, ey, w, dw, gx, gy;
 };
 
 // Pair geometry with the reference's conventions
@@ -129,10 +129,19 @@ G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
   g.ex = g.dx / (g.r + (T)1e-7);
   g.ey = g.dy / (g.r + (T)1e-7);
   g.w = kernelW<T>(g.q, h);
-  T dw = kernelDW<T>(g.q, h);
-  g.gx = g.ex * dw;
-  g.gy = g.ey * dw;
+  g.dw = kernelDW<T>(g.q, h);
+  g.gx = g.ex * g.dw;
+  g.gy = g.ey * g.dw;
   return g;
+}
+
+// The DFSPH pair coefficient V_j gradW_ij in the factored form the GPU
+// solver stores: t = dW/dr / (r + 1e-7), gradW = d * t, s = V_j * t,
+// V_j gradW = d * s (algebraically the reference's V_j * xhat * dW/dr,
+// v2/modules/dfsph.py:13-20, 35-50; a few ulp apart).
+template <class T>
+T dfsph_t(const G<T>& g) {
+  return g.dw / (g.r + (T)1e-7);
 }
 
 // The neighbor predicate in float32: r = sqrt(dx*dx + dy*dy) with separately
@@ -425,11 +434,14 @@ void dfsph_prepare(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* are
       int32_t j = c.nbr[k];
       G<T> g = pg(c, i, j, d);
       T Vj = vol[j];
-      k1x += Vj * g.gx;
-      k1y += Vj * g.gy;
+      const T t = dfsph_t(g);
+      const T sv = Vj * t;
+      T kx = g.dx * sv, ky = g.dy * sv;
+      k1x += kx;
+      k1y += ky;
       T c2 = (Vj * Vj) / (areas[j] * rho0);
-      k2 += c2 * (g.gx * g.gx + g.gy * g.gy);
-      T kx = Vj * g.gx, ky = Vj * g.gy;
+      const T gx = g.dx * t, gy = g.dy * t;
+      k2 += c2 * (gx * gx + gy * gy);
       dv += (vel[2 * j] - vel[2 * i]) * kx + (vel[2 * j + 1] - vel[2 * i + 1]) * ky;
       if (vp.on) {
         T vx = vel[2 * i] - vel[2 * j], vy = vel[2 * i + 1] - vel[2 * j + 1];
@@ -474,8 +486,9 @@ void sweep_a(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* rho, cons
       int32_t j = c.nbr[k];
       G<T> g = pg(c, i, j, d);
       T s = isph ? p[i] - p[j] : p[i] + p[j];
-      ax += s * (vol[j] * g.gx);
-      ay += s * (vol[j] * g.gy);
+      const T sv = vol[j] * dfsph_t(g);
+      ax += s * (g.dx * sv);
+      ay += s * (g.dy * sv);
     }
     acc[2 * i] = -ax / rho[i];
     acc[2 * i + 1] = -ay / rho[i];
@@ -496,7 +509,8 @@ double sweep_b(const Csr<T>& c, const DomT<T>& d, const T* vol, const T* acc, co
     for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
       int32_t j = c.nbr[k];
       G<T> g = pg(c, i, j, d);
-      sum += (acc[2 * j] - acc[2 * i]) * (vol[j] * g.gx) + (acc[2 * j + 1] - acc[2 * i + 1]) * (vol[j] * g.gy);
+      const T sv = vol[j] * dfsph_t(g);
+      sum += (acc[2 * j] - acc[2 * i]) * (g.dx * sv) + (acc[2 * j + 1] - acc[2 * i + 1]) * (g.dy * sv);
     }
     T K = -dtf * sum;
     T s = src[i];
@@ -525,7 +539,8 @@ void source(const Csr<T>& c, const DomT<T>& d, const T* vel, const T* adv, const
       G<T> g = pg(c, i, j, d);
       T bx = adv ? adv[2 * j] : T(0), by = adv ? adv[2 * j + 1] : T(0);
       T vjx = vel[2 * j] + dt * bx, vjy = vel[2 * j + 1] + dt * by;
-      T kx = vol[j] * g.gx, ky = vol[j] * g.gy;
+      const T sv = vol[j] * dfsph_t(g);
+      T kx = g.dx * sv, ky = g.dy * sv;
       dv += (vjx - vix) * kx + (vjy - viy) * ky;
     }
     src[i] = density_src ? (T(1) - areas[i] / vol[i]) - dt * dv : dv;
