@@ -166,17 +166,23 @@ def main():
     pairs = solver.nl.pairs()
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
+    # the solve loop's form of the sweeps: (x, y, p) / (x, y, a) records
     p = solver.p_den.clone()
-    acc = torch.empty(n, 2, device=dev)
+    xp = torch.empty(n, 4, device=dev)
+    xa = torch.empty(n, 4, device=dev)
+    _lib.check(L.dsph_pack_pressure(C.byref(pairs), _lib.ptr(p), _lib.ptr(xp), sh))
     reps = args.kernel_reps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def sweep_a():
+        _lib.check(L.dsph_pressure_accel_rec(C.byref(pairs), _lib.ptr(solver.kgrad), _lib.ptr(solver.rho),
+                                             _lib.ptr(xp), _lib.ptr(xa), sh))
+
     for _ in range(3):
-        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
-                                         _lib.ptr(solver.rho), _lib.ptr(p), _lib.ptr(acc), sh))
+        sweep_a()
     e0.record(stream)
     for _ in range(reps):
-        _lib.check(L.dsph_pressure_accel(C.byref(pairs), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
-                                         _lib.ptr(solver.rho), _lib.ptr(p), _lib.ptr(acc), sh))
+        sweep_a()
     e1.record(stream)
     e1.synchronize()
     t_a = e0.elapsed_time(e1) / reps * 1e-3
@@ -187,9 +193,9 @@ def main():
     parts = torch.empty((n + 255) // 256, dtype=torch.float64, device=dev)
 
     def sweep_b():
-        _lib.check(L.dsph_pressure_update(C.byref(pairs), C.byref(prm), _lib.ptr(solver.kgrad), _lib.ptr(solver.vol),
-                                          _lib.ptr(solver.div_src), _lib.ptr(solver.alpha_den), _lib.ptr(acc),
-                                          _lib.ptr(p), _lib.ptr(res), _lib.ptr(parts), sh))
+        _lib.check(L.dsph_pressure_update_rec(C.byref(pairs), C.byref(prm), _lib.ptr(solver.kgrad),
+                                              _lib.ptr(solver.div_src), _lib.ptr(solver.alpha_den), _lib.ptr(xa),
+                                              _lib.ptr(p), _lib.ptr(xp), _lib.ptr(res), _lib.ptr(parts), sh))
 
     for _ in range(3):
         sweep_b()
@@ -202,18 +208,19 @@ def main():
     P = solver.nl.num_pairs
     # Algorithmic bytes (each per-particle array once per launch, each pair
     # entry once; gathered neighbor fields and list padding not counted):
-    # sweep A: x 8 + p 4 + rho 4 + count 4 + a(write) 8 = 28 B/particle;
+    # sweep A: (x,y,p) record 16 + own x 8 + rho 4 + count 4 + (x,y,a) record(write) 16 = 48 B/particle;
     #          j (2 as a 16-bit delta, else 4) + s_ij 4 = 6 (8) B/pair
-    # sweep B: x 8 + a 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w) = 40 B/particle; 6 (8) B/pair
-    # (the coefficient V_j gradW_ij is d_ij * s_ij, d_ij from the positions)
+    # sweep B: (x,y,a) 16 + own x 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w)
+    #          + (x,y,p)(write) 16 = 64 B/particle; 6 (8) B/pair
+    # (the coefficient V_j gradW_ij is d_ij * s_ij, d_ij from the gathered records)
     idx16 = solver.nl.nbr16 is not None and int(solver.nl.nbr16_overflow.item()) == 0
     per_pair = 6 if idx16 else 8
-    bytes_a = 28 * n + per_pair * P
-    bytes_b = 40 * n + per_pair * P
+    bytes_a = 48 * n + per_pair * P
+    bytes_b = 64 * n + per_pair * P
     gbs_a = bytes_a / t_a / 1e9
     gbs_b = bytes_b / t_b / 1e9
-    log(f"sweep A (dsph_pressure_accel): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
-        f"sweep B (dsph_pressure_update): {t_b * 1e6:.1f} us, {bytes_b / 1e6:.1f} MB -> {gbs_b:.0f} GB/s; "
+    log(f"sweep A (dsph_pressure_accel_rec): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
+        f"sweep B (dsph_pressure_update_rec): {t_b * 1e6:.1f} us, {bytes_b / 1e6:.1f} MB -> {gbs_b:.0f} GB/s; "
         f"pairs {P} ({P / n:.1f}/particle)")
     if t_b >= t_a:
         kname, ach, tb, byt = "k_dfsph_sweep_b", gbs_b, t_b, bytes_b

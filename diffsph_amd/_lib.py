@@ -213,6 +213,9 @@ SIGNATURES = {
                                    V]),
     "dsph_pressure_accel": (C.c_int, [P(Pairs), V, V, V, V, V, V]),
     "dsph_pressure_update": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V]),
+    "dsph_pack_pressure": (C.c_int, [P(Pairs), V, V, V]),
+    "dsph_pressure_accel_rec": (C.c_int, [P(Pairs), V, V, V, V, V]),
+    "dsph_pressure_update_rec": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V]),
     "dsph_dfsph_iterate": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, I32, P(C.c_double), V, SZ,
                                      V]),
     "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),

@@ -294,10 +294,45 @@ constexpr int kUnroll = DSPH_SWEEP_UNROLL;
 // 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
 // `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
 // else evaluated on the fly.
-template <class Idx>
+// Neighbor fields gathered by the sweeps: separate arrays, or one 16-byte
+// record per particle (x, y, p) / (x, y, a) so that a pair costs one gather.
+struct GatherXP {   // sweep A: position and pressure
+  const float2* x;
+  const float* p;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float& pj) const {
+    xj = x[j];
+    pj = p[j];
+  }
+};
+struct GatherXP4 {
+  const float4* r;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float& pj) const {
+    const float4 v = r[j];
+    xj = make_float2(v.x, v.y);
+    pj = v.z;
+  }
+};
+struct GatherXA {   // sweep B: position and pressure acceleration
+  const float2* x;
+  const float2* a;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float2& aj) const {
+    xj = x[j];
+    aj = a[j];
+  }
+};
+struct GatherXA4 {
+  const float4* r;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float2& aj) const {
+    const float4 v = r[j];
+    xj = make_float2(v.x, v.y);
+    aj = make_float2(v.z, v.w);
+  }
+};
+
+template <class Idx, class Gat>
 __device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
-                                           const float* __restrict__ kv, const float* __restrict__ p, float pi,
-                                           float sj, float& ax, float& ay) {
+                                           const float* __restrict__ kv, const Gat& gat, float pi, float sj,
+                                           float& ax, float& ay) {
   int n = 0, k = rr.base;
   for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
     int j[kUnroll];
@@ -311,8 +346,9 @@ __device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const R
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      xj[u] = P.rpos[j[u]];
-      pj[u] = sj * p[j[u]];
+      float q;
+      gat(j[u], xj[u], q);
+      pj[u] = sj * q;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -324,8 +360,11 @@ __device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const R
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
-    const float s = pi + sj * p[j];
+    float2 xj;
+    float q;
+    gat(j, xj, q);
+    const float2 c = coef_of(xi, xj, ld_stream(kv + k), P.dom);
+    const float s = pi + sj * q;
     ax += s * c.x;
     ay += s * c.y;
   }
@@ -339,21 +378,35 @@ __device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const R
 __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ kv,
                                                           const float* __restrict__ vol,
                                                           const float* __restrict__ rho, const float* __restrict__ p,
-                                                          float2* __restrict__ acc, const SolveCtrl* ctrl, int diff) {
+                                                          float2* __restrict__ acc, const SolveCtrl* ctrl, int diff,
+                                                          const float4* __restrict__ xp, float4* __restrict__ xa) {
+  // xp (with kv): the (x, y, p) records replace p and the positions in the
+  // gathers; xa: the result is written as the (x, y, a) record instead of acc.
   if (ctrl && ctrl->done) return;
   int i = logical_block() * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
-  const float pi = p[i];
+  const float4 own = xp ? xp[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float pi = xp ? own.z : p[i];
   // -p_j turns the sum into the difference form; negation is exact
   const float sj = diff ? -1.0f : 1.0f;
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
   const float2 xi = P.qpos[i];
   if (kv) {
-    if (use_idx16(P))
-      sweep_a_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, p, pi, sj, ax, ay);
-    else
-      sweep_a_kv(P, xi, rr, Idx32{P.nbr}, kv, p, pi, sj, ax, ay);
+    const bool i16 = use_idx16(P);
+    if (xp) {
+      const GatherXP4 g{xp};
+      if (i16)
+        sweep_a_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, g, pi, sj, ax, ay);
+      else
+        sweep_a_kv(P, xi, rr, Idx32{P.nbr}, kv, g, pi, sj, ax, ay);
+    } else {
+      const GatherXP g{P.rpos, p};
+      if (i16)
+        sweep_a_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, g, pi, sj, ax, ay);
+      else
+        sweep_a_kv(P, xi, rr, Idx32{P.nbr}, kv, g, pi, sj, ax, ay);
+    }
   } else {
     const float hi = h_of(P.qh, i, P.h0);
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
@@ -365,7 +418,13 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
     }
   }
   const float r = rho[i];
-  acc[i] = make_float2(-ax / r, -ay / r);
+  const float2 a = make_float2(-ax / r, -ay / r);
+  if (xa) {
+    const float2 xr = xp ? make_float2(own.x, own.y) : P.rpos[i];
+    xa[i] = make_float4(xr.x, xr.y, a.x, a.y);
+  } else {
+    acc[i] = a;
+  }
 }
 
 struct SweepBArgs {
@@ -385,6 +444,8 @@ struct SweepBArgs {
   int clamp;
   int n;
   int isph;  // error metric |r| (incompressible.py:357) instead of max(r/rho0, -thr)
+  const float4* xa;  // (x, y, a) records in place of acc (with kv)
+  float4* xp;        // the updated (x, y, p) records are written here too
 };
 
 // The loop condition of dfsph.py:88 from the per-block partials, summed in a
@@ -416,9 +477,9 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int nb, 
   }
 }
 
-template <class Idx>
+template <class Idx, class Gat>
 __device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
-                                            const float* __restrict__ kv, const float2* __restrict__ acc, float2 ai) {
+                                            const float* __restrict__ kv, const Gat& gat, float2 ai) {
   float sum = 0.f;
   int n = 0, k = rr.base;
   for (; n + kUnroll <= rr.cnt; n += kUnroll, k += kUnroll * rr.step) {
@@ -432,10 +493,7 @@ __device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const 
       sv[u] = ld_stream(kv + k + u * rr.step);
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      xj[u] = P.rpos[j[u]];
-      aj[u] = acc[j[u]];
-    }
+    for (int u = 0; u < kUnroll; ++u) gat(j[u], xj[u], aj[u]);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const float2 c = coef_of(xi, xj[u], sv[u], P.dom);
@@ -444,8 +502,9 @@ __device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const 
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
-    const float2 aj = acc[j];
+    float2 xj, aj;
+    gat(j, xj, aj);
+    const float2 c = coef_of(xi, xj, ld_stream(kv + k), P.dom);
     sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
   }
   return sum;
@@ -462,14 +521,29 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
   int i = lb * blockDim.x + threadIdx.x;
   double err = 0.0;
   if (i < P.nq) {
-    const float2 ai = B.acc[i];
+    float2 ai, xr;
+    if (B.xa) {
+      const float4 own = B.xa[i];
+      ai = make_float2(own.z, own.w);
+      xr = make_float2(own.x, own.y);
+    } else {
+      ai = B.acc[i];
+      xr = B.xp ? P.rpos[i] : make_float2(0.f, 0.f);
+    }
     float sum = 0.f;
     const RowRef rr = row_ref(P, i);
     if (B.kv) {
-      if (use_idx16(P))
-        sum = sweep_b_kv(P, P.qpos[i], rr, Idx16{P.nbr16, i, P.nr}, B.kv, B.acc, ai);
-      else
-        sum = sweep_b_kv(P, P.qpos[i], rr, Idx32{P.nbr}, B.kv, B.acc, ai);
+      const bool i16 = use_idx16(P);
+      const float2 xi = P.qpos[i];
+      if (B.xa) {
+        const GatherXA4 g{B.xa};
+        sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
+                  : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
+      } else {
+        const GatherXA g{P.rpos, B.acc};
+        sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
+                  : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
+      }
     } else {
       const float2 xi = P.qpos[i];
       const float hi = h_of(P.qh, i, P.h0);
@@ -486,6 +560,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
     float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
     if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
     B.p[i] = pn;
+    if (B.xp) B.xp[i] = make_float4(xr.x, xr.y, pn, 0.f);
     if (B.residual) B.residual[i] = res;
     if (B.isph) {
       err = (double)fabsf(res);
@@ -679,6 +754,41 @@ __global__ void __launch_bounds__(kBlock) k_integrate_sie(float2* __restrict__ p
 
 inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
+// (x, y, p) records of the solve loop: x from the reference positions.
+__global__ void __launch_bounds__(kBlock) k_pack_xp(const float2* __restrict__ rpos, const float* __restrict__ p, int n,
+                                                    float4* __restrict__ xp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 x = rpos[i];
+  xp[i] = make_float4(x.x, x.y, p[i], 0.f);
+}
+
+// Solve workspace: control block, block partials, source, then the two
+// record arrays.
+struct SolveWs {
+  SolveCtrl* ctrl;
+  double* partials;
+  float* src;
+  float4* xp;
+  float4* xa;
+};
+SolveWs solve_ws(void* ws, int64_t n) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  const size_t nb = (size_t)grid_for(n > 0 ? n : 1);
+  char* w = static_cast<char*>(ws);
+  SolveWs r;
+  r.ctrl = reinterpret_cast<SolveCtrl*>(w);
+  w += al256(sizeof(SolveCtrl));
+  r.partials = reinterpret_cast<double*>(w);
+  w += al256(nb * sizeof(double));
+  r.src = reinterpret_cast<float*>(w);
+  w += al256(sizeof(float) * 2 * nn);
+  r.xp = reinterpret_cast<float4*>(w);
+  w += al256(sizeof(float4) * nn);
+  r.xa = reinterpret_cast<float4*>(w);
+  return r;
+}
+
 SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const float* vol, const float* src,
                         const float* alpha, float* p, float* residual, const float* accel, double* partials,
                         SolveCtrl* ctrl, int64_t n) {
@@ -698,6 +808,8 @@ SweepBArgs make_sweep_b(const dsph_dfsph_params* prm, const float* kv, const flo
   B.neg_thr = -prm->error_threshold;
   B.clamp = prm->clamp_pressure;
   B.n = (int)n;
+  B.xa = nullptr;
+  B.xp = nullptr;
   B.isph = prm->scheme == 1;
   return B;
 }
@@ -785,8 +897,10 @@ extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, con
 }
 
 extern "C" size_t dsph_dfsph_workspace(int64_t n) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
   size_t nb = (size_t)grid_for(n > 0 ? n : 1);
-  return al256(sizeof(SolveCtrl)) + al256(nb * sizeof(double)) + al256(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
+  return al256(sizeof(SolveCtrl)) + al256(nb * sizeof(double)) + al256(sizeof(float) * 2 * nn) +
+         2 * al256(sizeof(float4) * nn);
 }
 
 extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
@@ -812,13 +926,14 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
   hipStream_t st = as_stream(stream);
   PairsDev P = make_pairs(*pairs);
   const int nb = grid_for(n);
-  char* w = static_cast<char*>(ws);
-  SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
-  w += al256(sizeof(SolveCtrl));
-  double* partials = reinterpret_cast<double*>(w);
-  w += al256((size_t)nb * sizeof(double));
-  float* src = reinterpret_cast<float*>(w);
+  const SolveWs W = solve_ws(ws, n);
+  SolveCtrl* ctrl = W.ctrl;
+  double* partials = W.partials;
+  float* src = W.src;
   const float* kv = kgrad;
+  // with stored coefficients the loop gathers 16-byte records
+  float4* xp = kv ? W.xp : nullptr;
+  float4* xa = kv ? W.xa : nullptr;
 
   // Source term.
   if (prm->source_density || adv) {
@@ -832,6 +947,12 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, prm->min_iters, prm->max_iters, (double)prm->error_threshold);
   DSPH_LAUNCH_CHECK();
   SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);
+  B.xa = xa;
+  B.xp = xp;
+  if (xp) {
+    k_pack_xp<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(P.rpos), p, (int)n, xp);
+    DSPH_LAUNCH_CHECK();
+  }
 
   const int chunk = prm->check_every > 0 ? prm->check_every : 8;
   SolveCtrl host;
@@ -842,7 +963,7 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     if (launched + k > prm->max_iters) k = prm->max_iters - launched;
     for (int t = 0; t < k; ++t) {
       k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl,
-                                             B.isph);
+                                             B.isph, xp, xa);
       k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
       k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
     }
@@ -857,7 +978,8 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     DSPH_CHECK_HIP(hipStreamSynchronize(st));
   }
   // Final pressure acceleration (dfsph.py:105).
-  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, B.isph);
+  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, B.isph, xp,
+                                         nullptr);
   DSPH_LAUNCH_CHECK();
   if (stats) {
     stats->iterations = host.iter;
@@ -877,7 +999,7 @@ extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, 
   if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, kgrad, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0);
+      P, kgrad, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0, nullptr, nullptr);
   return launch_status();
 }
 
@@ -896,6 +1018,49 @@ extern "C" int dsph_pressure_update(const dsph_pairs* pairs, const dsph_dfsph_pa
   return launch_status();
 }
 
+extern "C" int dsph_pack_pressure(const dsph_pairs* pairs, const float* p, float* xp, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!p || !xp) return DSPH_ERR_ARG;
+  const int n = (int)pairs->nq;
+  k_pack_xp<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(reinterpret_cast<const float2*>(pairs->rpos), p, n,
+                                                           reinterpret_cast<float4*>(xp));
+  return launch_status();
+}
+
+extern "C" int dsph_pressure_accel_rec(const dsph_pairs* pairs, const float* kgrad, const float* rho, const float* xp,
+                                       float* xa, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!kgrad || !rho || !xp || !xa) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+      P, kgrad, nullptr, rho, nullptr, nullptr, nullptr, 0, reinterpret_cast<const float4*>(xp),
+      reinterpret_cast<float4*>(xa));
+  return launch_status();
+}
+
+extern "C" int dsph_pressure_update_rec(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
+                                        const float* src, const float* alpha, const float* xa, float* p, float* xp,
+                                        float* residual, double* block_partials, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (!prm || pairs->nq != pairs->nr) return DSPH_ERR_ARG;
+  const int64_t n = pairs->nq;
+  if (n == 0) return DSPH_OK;
+  if (!kgrad || !src || !alpha || !xa || !p || !xp || !block_partials) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  SweepBArgs B = make_sweep_b(prm, kgrad, nullptr, src, alpha, p, residual, nullptr, block_partials, nullptr, n);
+  B.xa = reinterpret_cast<const float4*>(xa);
+  B.xp = reinterpret_cast<float4*>(xp);
+  k_dfsph_sweep_b<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(P, B);
+  return launch_status();
+}
+
 extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm, const float* kgrad,
                                   const float* rho, const float* vol, const float* alpha, const float* src, float* p,
                                   float* accel, float* residual, int32_t iterations, double* final_error, void* ws,
@@ -910,16 +1075,21 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   hipStream_t st = as_stream(stream);
   PairsDev P = make_pairs(*pairs);
   const int nb = grid_for(n);
-  char* w = static_cast<char*>(ws);
-  SolveCtrl* ctrl = reinterpret_cast<SolveCtrl*>(w);
-  w += al256(sizeof(SolveCtrl));
-  double* partials = reinterpret_cast<double*>(w);
+  const SolveWs W = solve_ws(ws, n);
+  SolveCtrl* ctrl = W.ctrl;
+  double* partials = W.partials;
   const float* kv = kgrad;
+  float4* xp = kv ? W.xp : nullptr;
+  float4* xa = kv ? W.xa : nullptr;
   // Fixed count: the loop condition can never stop it early.
   k_ctrl_init<<<1, 1, 0, st>>>(ctrl, iterations, iterations, 0.0);
   SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, partials, ctrl, n);
+  B.xa = xa;
+  B.xp = xp;
+  if (xp) k_pack_xp<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(P.rpos), p, (int)n, xp);
   for (int t = 0; t < iterations; ++t) {
-    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl, B.isph);
+    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl, B.isph, xp,
+                                           xa);
     k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
     k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
   }

@@ -356,7 +356,9 @@ size_t dsph_dfsph_workspace(int64_t n);
  * computed here as 1 - areas/vol - dt*div(v*)).  p: initial pressure on
  * entry (warm start), solution on exit.  accel: pressure acceleration of the
  * final p.  Convergence is decided on the device (fp64 mean residual);
- * the host is synchronised once every `check_every` iterations. */
+ * the host is synchronised once every `check_every` iterations.  With kgrad
+ * the loop keeps (x, y, p) and (x, y, a) records in the workspace
+ * (dsph_pressure_accel_rec / dsph_pressure_update_rec). */
 int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
                      const float* kgrad, const float* vel, const float* adv, const float* rho,
                      const float* vol, const float* areas, const float* alpha,
@@ -383,10 +385,29 @@ int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad,
                         const float* rho, const float* p, float* accel,
                         void* stream);
 
+/* The two sweeps of the solve loop in the form dsph_dfsph_solve runs them
+ * when kgrad is stored: the gathered neighbor fields come as one 16-byte
+ * record per particle, xp = (x, y, p, 0) and xa = (x, y, a_x, a_y) [n][4]
+ * (x = the reference positions), so each pair costs one gather.
+ * dsph_pack_pressure builds xp from p; dsph_pressure_accel_rec computes the
+ * accelerations of dsph_pressure_accel into xa; dsph_pressure_update_rec
+ * is dsph_pressure_update reading xa and writing p and xp.  Same values as
+ * the unpacked entry points.  No counterpart in the reference (its arrays
+ * are separate tensors). */
+int dsph_pack_pressure(const dsph_pairs* pairs, const float* p, float* xp, void* stream);
+int dsph_pressure_accel_rec(const dsph_pairs* pairs, const float* kgrad, const float* rho,
+                            const float* xp, float* xa, void* stream);
+int dsph_pressure_update_rec(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
+                             const float* kgrad, const float* src, const float* alpha,
+                             const float* xa, float* p, float* xp, float* residual,
+                             double* block_partials, void* stream);
+
 /* Exactly `iterations` Jacobi iterations (sweep A + sweep B each, no
  * convergence test, no final acceleration) from the given p and source
  * (updatePressure, v2/modules/dfsph.py:39-50).  final_error (host, may be
- * NULL -> no synchronisation) receives the error metric of the last one. */
+ * NULL -> no synchronisation) receives the error metric of the last one.
+ * accel is scratch: with kgrad the iterations keep the accelerations in the
+ * workspace and accel is left unchanged. */
 int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
                        const float* kgrad, const float* rho, const float* vol, const float* alpha,
                        const float* src, float* p, float* accel,

@@ -145,6 +145,48 @@ def test_compact16_deltas_decode_and_overflow():
     assert int(fused_flag.item()) == 0 and fused is not buf
 
 
+def test_record_sweeps_match_separate_arrays():
+    """The solve loop's sweeps on (x, y, p) / (x, y, a) records give the
+    accelerations, pressures and residuals of the separate-array entry
+    points bit for bit, and the records carry the positions."""
+    import ctypes as C
+    from diffsph_amd import _lib
+    d, dt = _setup(64, 2)
+    s = _solver(d, dt)
+    s.step()
+    L = _lib.lib()
+    pr = s.nl.pairs()
+    dev = s.pos.device
+    st = _lib.stream_handle(dev)
+    n = s.n
+    p0 = torch.rand(n, device=dev)
+    acc = torch.empty(n, 2, device=dev)
+    _lib.check(L.dsph_pressure_accel(C.byref(pr), _lib.ptr(s.kgrad), _lib.ptr(s.vol), _lib.ptr(s.rho),
+                                     _lib.ptr(p0), _lib.ptr(acc), st))
+    xp = torch.empty(n, 4, device=dev)
+    xa = torch.empty(n, 4, device=dev)
+    _lib.check(L.dsph_pack_pressure(C.byref(pr), _lib.ptr(p0), _lib.ptr(xp), st))
+    _lib.check(L.dsph_pressure_accel_rec(C.byref(pr), _lib.ptr(s.kgrad), _lib.ptr(s.rho), _lib.ptr(xp),
+                                         _lib.ptr(xa), st))
+    assert torch.equal(xa[:, 2:], acc) and torch.equal(xa[:, :2], s.grid.sorted_pos)
+    prm = _lib.DfsphParams()
+    prm.dt, prm.rho0, prm.omega, prm.error_threshold = dt, 1.0, 0.3, 5e-4
+    prm.min_iters, prm.max_iters, prm.clamp_pressure, prm.source_density = 1, 1, 1, 0
+    nb = (n + 255) // 256
+    p1, p2 = p0.clone(), p0.clone()
+    r1, r2 = torch.empty(n, device=dev), torch.empty(n, device=dev)
+    b1 = torch.empty(nb, dtype=torch.float64, device=dev)
+    b2 = torch.empty(nb, dtype=torch.float64, device=dev)
+    _lib.check(L.dsph_pressure_update(C.byref(pr), C.byref(prm), _lib.ptr(s.kgrad), _lib.ptr(s.vol),
+                                      _lib.ptr(s.div_src), _lib.ptr(s.alpha_div), _lib.ptr(acc), _lib.ptr(p1),
+                                      _lib.ptr(r1), _lib.ptr(b1), st))
+    _lib.check(L.dsph_pressure_update_rec(C.byref(pr), C.byref(prm), _lib.ptr(s.kgrad), _lib.ptr(s.div_src),
+                                          _lib.ptr(s.alpha_div), _lib.ptr(xa), _lib.ptr(p2), _lib.ptr(xp),
+                                          _lib.ptr(r2), _lib.ptr(b2), st))
+    assert torch.equal(p1, p2) and torch.equal(r1, r2) and torch.equal(b1, b2)
+    assert torch.equal(xp[:, 2], p2) and torch.equal(xp[:, :2], s.grid.sorted_pos)
+
+
 def test_dfsph_compact_index_paths_identical():
     """16-bit deltas, the int32 list, the device-side fallback after an
     overflow, the two-pass neighbor list and the one-pass list's fallback
