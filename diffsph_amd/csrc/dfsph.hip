@@ -609,7 +609,10 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
                                                         const float* __restrict__ alpha, float rho0,
                                                         const float* __restrict__ pair_w,
                                                         const float* __restrict__ mean,
-                                                        const float* __restrict__ istd, float* __restrict__ X) {
+                                                        const float* __restrict__ istd, float* __restrict__ X,
+                                                        const float4* __restrict__ rec) {
+  // rec (with pair_w): (V, p_div_prev, s, p_den_prev) per particle, so that
+  // an entry costs one gather instead of four
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -637,12 +640,23 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
         w[u] = __builtin_nontemporal_load(pair_w + kk);
       }
       float v[4], a[4], b[4], c[4];
+      if (rec) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        v[u] = vol[j[u]];
-        a[u] = pdp[j[u]];
-        b[u] = src[j[u]];
-        c[u] = pnp[j[u]];
+        for (int u = 0; u < 4; ++u) {
+          const float4 r = rec[j[u]];
+          v[u] = r.x;
+          a[u] = r.y;
+          b[u] = r.z;
+          c[u] = r.w;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v[u] = vol[j[u]];
+          a[u] = pdp[j[u]];
+          b[u] = src[j[u]];
+          c[u] = pnp[j[u]];
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -1143,10 +1157,17 @@ extern "C" int dsph_integrate_semi_implicit(float* pos, float* vel, const float*
   return launch_status();
 }
 
+__global__ void __launch_bounds__(kBlock) k_pack_features(int n, const float* __restrict__ vol,
+                                                          const float* __restrict__ pdp, const float* __restrict__ src,
+                                                          const float* __restrict__ pnp, float4* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rec[i] = make_float4(vol[i], pdp[i], src[i], pnp[i]);
+}
+
 extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p_div_prev,
                                 const float* p_den_prev, const float* div_src, const float* alpha_div, float rho0,
                                 const float* pair_w, const float* mean16, const float* istd16, float* X,
-                                void* stream) {
+                                float* records, void* stream) {
   int s = check_pairs(pairs);
   if (s != DSPH_OK) return s;
   if (pairs->nq != pairs->nr) return DSPH_ERR_ARG;
@@ -1154,7 +1175,13 @@ extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const
   if (!vol || !rho || !p_div_prev || !p_den_prev || !div_src || !alpha_div || !mean16 || !istd16 || !X)
     return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_ws_features<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, vol, rho, p_div_prev, p_den_prev, div_src,
-                                                                  alpha_div, rho0, pair_w, mean16, istd16, X);
+  hipStream_t st = as_stream(stream);
+  float4* rec = (records && pair_w) ? reinterpret_cast<float4*>(records) : nullptr;
+  if (rec) {
+    k_pack_features<<<grid_for(P.nq), kBlock, 0, st>>>(P.nq, vol, p_div_prev, div_src, p_den_prev, rec);
+    DSPH_LAUNCH_CHECK();
+  }
+  k_ws_features<<<grid_for(P.nq), kBlock, 0, st>>>(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0,
+                                                   pair_w, mean16, istd16, X, rec);
   return launch_status();
 }

@@ -40,6 +40,20 @@ NFEAT = 8       # used feature columns
 def _pair_w(solver):
     """The density pass's W_ij of the current list, when the solver keeps it."""
     return solver.wpair if getattr(solver, "keep_pair_w", False) else None
+
+
+def _records(solver):
+    """Grow-only [n][4] scratch for the packed neighbor fields of the feature
+    pass (used with the kept W_ij)."""
+    if _pair_w(solver) is None:
+        return None
+    r = getattr(solver, "_feat_rec", None)
+    if r is None or r.shape[0] < solver.n:
+        r = torch.empty(solver.n, 4, dtype=torch.float32, device=solver.pos.device)
+        solver._feat_rec = r
+    return r
+
+
 NIN = 16        # padded input width of the network
 
 
@@ -114,7 +128,7 @@ class WarmStartNet:
         pr = solver.nl.pairs()
         check(lib().dsph_ws_features(C.byref(pr), ptr(solver.vol), ptr(solver.rho), ptr(pdp), ptr(pnp),
                                      ptr(solver.div_src), ptr(solver.alpha_div), C.c_float(solver.cfg.rho0),
-                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X),
+                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X), ptr(_records(solver)),
                                      _lib.stream_handle(self.device)), "ws_features")
         return X
 
@@ -335,7 +349,7 @@ class CorrectionNet(WarmStartNet):
         pr = solver.nl.pairs()
         check(lib().dsph_ws_features(C.byref(pr), ptr(solver.vol), ptr(solver.rho), ptr(p), ptr(prev),
                                      ptr(solver.residual), ptr(solver.alpha_div), C.c_float(solver.cfg.rho0),
-                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X),
+                                     ptr(_pair_w(solver)), ptr(mean), ptr(istd), ptr(X), ptr(_records(solver)),
                                      _lib.stream_handle(self.device)), "ws_features")
         return X
 

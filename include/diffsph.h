@@ -441,13 +441,15 @@ int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
  *   4 p_den_prev  5 sum_j V_j W_ij p_den_prev_j  6 rho/rho0-1  7 alpha_div
  * normalised as (f - mean16[c]) * istd16[c] (device arrays).  pair_w
  * (may be NULL): the W_ij of dsph_density_w for this list, streamed instead
- * of re-evaluating the pair geometry (same values). */
+ * of re-evaluating the pair geometry (same values).  records (may be NULL,
+ * used with pair_w): scratch [n][4] floats; the neighbor fields are packed
+ * there first so that an entry costs one gather (same values). */
 int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
                      const float* rho, const float* p_div_prev,
                      const float* p_den_prev, const float* div_src,
                      const float* alpha_div, float rho0, const float* pair_w,
                      const float* mean16, const float* istd16, float* X,
-                     void* stream);
+                     float* records, void* stream);
 
 /* MLP 16 -> 64 -> 64 -> 64 -> 1 (ReLU), bf16 weights and layer inputs on
  * MFMA with fp32 accumulation.  Parameters are a flat fp32 array of
