@@ -215,10 +215,41 @@ __device__ __forceinline__ bool use_idx16(const PairsDev& P) { return P.nbr16 &&
 // Divergence source (dfsph.py:30-33, 62-63) with the coefficients streamed:
 // dv_i = sum_j (v*_j - v*_i) . V_j gradW_ij, v* = v + dt adv, entries in row
 // order with kUnroll index / coefficient loads in flight.
-template <class Idx>
+// Neighbor position and v* = v + dt adv of entry j: separate arrays, or
+// one (x, y, v*) record per particle.
+struct GatherXV {
+  const float2* x;
+  const float2* vel;
+  const float2* adv;
+  float dt;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float2& vj) const {
+    xj = x[j];
+    const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
+    vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
+  }
+};
+struct GatherXV4 {
+  const float4* r;
+  __device__ __forceinline__ void operator()(int j, float2& xj, float2& vj) const {
+    const float4 v = r[j];
+    xj = make_float2(v.x, v.y);
+    vj = make_float2(v.z, v.w);
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_pack_xv(const float2* __restrict__ rpos, const float2* __restrict__ vel,
+                                                    const float2* __restrict__ adv, float dt, int n,
+                                                    float4* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 x = rpos[i];
+  const float2 a = adv ? adv[i] : make_float2(0.f, 0.f);
+  rec[i] = make_float4(x.x, x.y, vel[i].x + dt * a.x, vel[i].y + dt * a.y);
+}
+
+template <class Idx, class Gat>
 __device__ __forceinline__ float source_kv(const PairsDev& P, float2 xi, const RowRef& rr, const Idx& idx,
-                                           const float* __restrict__ kv, const float2* __restrict__ vel,
-                                           const float2* __restrict__ adv, float dt, float2 vi) {
+                                           const float* __restrict__ kv, const Gat& gat, float2 vi) {
   float dv = 0.f;
   int n = 0, k = rr.base;
   for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
@@ -231,11 +262,7 @@ __device__ __forceinline__ float source_kv(const PairsDev& P, float2 xi, const R
       sv[u] = ld_stream(kv + k + u * rr.step);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      xj[u] = P.rpos[j[u]];
-      const float2 aj = adv ? adv[j[u]] : make_float2(0.f, 0.f);
-      vj[u] = make_float2(vel[j[u]].x + dt * aj.x, vel[j[u]].y + dt * aj.y);
-    }
+    for (int u = 0; u < 4; ++u) gat(j[u], xj[u], vj[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       c[u] = coef_of(xi, xj[u], sv[u], P.dom);
@@ -244,9 +271,9 @@ __device__ __forceinline__ float source_kv(const PairsDev& P, float2 xi, const R
   }
   for (; n < rr.cnt; ++n, k += rr.step) {
     const int j = idx(k);
-    const float2 c = coef_of(xi, P.rpos[j], ld_stream(kv + k), P.dom);
-    const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
-    const float2 vj = make_float2(vel[j].x + dt * aj.x, vel[j].y + dt * aj.y);
+    float2 xj, vj;
+    gat(j, xj, vj);
+    const float2 c = coef_of(xi, xj, ld_stream(kv + k), P.dom);
     dv += (vj.x - vi.x) * c.x + (vj.y - vi.y) * c.y;
   }
   return dv;
@@ -257,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float
                                                          const float2* __restrict__ adv,
                                                          const float* __restrict__ vol,
                                                          const float* __restrict__ areas, float dt, int density,
-                                                         float* __restrict__ src) {
+                                                         float* __restrict__ src, const float4* __restrict__ rec) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -267,8 +294,16 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_source(PairsDev P, const float
   float dv = 0.f;
   const RowRef rr = row_ref(P, i);
   if (kv) {
-    dv = use_idx16(P) ? source_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, vel, adv, dt, vi)
-                      : source_kv(P, xi, rr, Idx32{P.nbr}, kv, vel, adv, dt, vi);
+    const bool i16 = use_idx16(P);
+    if (rec) {
+      const GatherXV4 g{rec};
+      dv = i16 ? source_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, g, vi)
+               : source_kv(P, xi, rr, Idx32{P.nbr}, kv, g, vi);
+    } else {
+      const GatherXV g{P.rpos, vel, adv, dt};
+      dv = i16 ? source_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, g, vi)
+               : source_kv(P, xi, rr, Idx32{P.nbr}, kv, g, vi);
+    }
   } else {
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
@@ -951,9 +986,17 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
 
   // Source term.
   if (prm->source_density || adv) {
+    // with stored coefficients the (x, y, v*) records go in the xa slot,
+    // which the loop fills only later
+    float4* xv = kv ? W.xa : nullptr;
+    if (xv) {
+      k_pack_xv<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(P.rpos), reinterpret_cast<const float2*>(vel),
+                                        reinterpret_cast<const float2*>(adv), prm->dt, (int)n, xv);
+      DSPH_LAUNCH_CHECK();
+    }
     k_dfsph_source<<<nb, kBlock, 0, st>>>(P, kv, reinterpret_cast<const float2*>(vel),
                                           reinterpret_cast<const float2*>(adv), vol, areas, prm->dt,
-                                          prm->source_density, src);
+                                          prm->source_density, src, xv);
   } else {
     DSPH_CHECK_HIP(hipMemcpyAsync(src, src0, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
   }
