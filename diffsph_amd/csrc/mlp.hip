@@ -177,16 +177,87 @@ __device__ __forceinline__ void mlp_forward_tile(const float* __restrict__ X, in
 
 // Inference: y[m] for m < n.  Grid-stride over 64-sample block tiles; every
 // wave of a block runs the same number of tiles (block-wide LDS barriers).
+// The weight fragments and biases of the lane are loaded once into
+// registers and reused for every tile (the same MFMA operations in the same
+// order as mlp_forward_tile, so the same bits).
 __global__ void __launch_bounds__(kBlock) k_mlp_forward(const float* __restrict__ X, int n,
                                                         const float* __restrict__ prm,
                                                         const short4v* __restrict__ pk, float* __restrict__ Y) {
   __shared__ short lds[4][16 * kLdsStride];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  short4v w1[4], w2[16], w3[16];
+  float b1[4], b2[4], b3[4], w4[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    w1[nt] = pk[kPkW1 + nt * 64 + l];
+    b1[nt] = prm[kOffB1 + 16 * nt + c];
+    b2[nt] = prm[kOffB2 + 16 * nt + c];
+    b3[nt] = prm[kOffB3 + 16 * nt + c];
+    w4[nt] = bf2f(f2bf(prm[kOffW4 + 16 * nt + c]));
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    w2[t] = pk[kPkW2 + t * 64 + l];
+    w3[t] = pk[kPkW3 + t * 64 + l];
+  }
+  const float b4 = prm[kOffB4];
   const int ntiles = (n + 63) / 64;
   for (int bt = blockIdx.x; bt < ntiles; bt += gridDim.x) {
     const int m0 = bt * 64 + w * 16;
     MlpTile T;
-    mlp_forward_tile(X, n, m0, prm, pk, lds[w], T);
+    short4v xa;
+    {
+      const int m = m0 + c;
+      if (m < n) {
+        const float4 v = *reinterpret_cast<const float4*>(X + (int64_t)m * kFin + 4 * q);
+        xa[0] = f2bf(v.x);
+        xa[1] = f2bf(v.y);
+        xa[2] = f2bf(v.z);
+        xa[3] = f2bf(v.w);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xa[j] = f2bf(0.0f);
+      }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      float4v acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma16(xa, w1[nt], acc);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float zz = acc[j] + b1[nt];
+        T.h[0][nt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
+      }
+    }
+#pragma unroll
+    for (int L = 1; L < 3; ++L) {
+      short4v a[4];
+      c_to_a(T.h[L - 1], a, lds[w]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        float4v acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(a[kt], L == 1 ? w2[kt * 4 + nt] : w3[kt * 4 + nt], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float zz = acc[j] + (L == 1 ? b2[nt] : b3[nt]);
+          T.h[L][nt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
+        }
+      }
+    }
+    float sy[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sy[j] += bf2f(T.h[2][nt][j]) * w4[nt];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float v = sy[j];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+      T.y[j] = v + b4;
+    }
     if (c == 0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -463,6 +534,7 @@ extern "C" int dsph_mlp_forward(const float* X, int64_t n, const float* params, 
   if (n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
   if (n == 0) return DSPH_OK;
   if (!X || !params || !packed || !Y) return DSPH_ERR_ARG;
+  if (reinterpret_cast<uintptr_t>(X) % 16 != 0) return DSPH_ERR_ARG;  // rows are read as float4
   k_mlp_forward<<<mlp_grid(n), kBlock, 0, as_stream(stream)>>>(X, (int)n, params,
                                                                static_cast<const short4v*>(packed), Y);
   return launch_status();

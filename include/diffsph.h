@@ -456,7 +456,8 @@ int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
  * dsph_mlp_num_params() values: W1[16][64] b1[64] W2[64][64] b2[64]
  * W3[64][64] b3[64] w4[64] b4[1].  dsph_mlp_pack converts them to the
  * bf16 MFMA operand layout (dsph_mlp_packed_bytes() bytes) used by
- * forward and training. */
+ * forward and training.  dsph_mlp_forward: X [n][16] fp32, 16-byte aligned
+ * (rows are read as float4; a misaligned X returns DSPH_ERR_ARG). */
 int dsph_mlp_num_params(void);
 int dsph_mlp_num_inputs(void);
 size_t dsph_mlp_packed_bytes(void);

@@ -48,6 +48,10 @@ def test_host_only_entry_points_run_without_gpu():
     assert L.dsph_grid_dims(C.byref(dom), C.c_float(-1.0), C.byref(g)) == 1
     assert L.dsph_neighbor_count(None, None, None, None, None, 0, None, None, 0, 0, 0.1, 0, None, None) == 1
     assert L.dsph_scan_workspace(10_000_000) >= 4 * (10_000_000 // 2048)
+    # the MLP forward reads input rows as float4: a misaligned X is rejected
+    # on the host before any launch (the pointers are never dereferenced)
+    assert L.dsph_mlp_forward(C.c_void_p(0x1004), 16, C.c_void_p(0x2000), C.c_void_p(0x3000),
+                              C.c_void_p(0x4000), None) == 1
 
 
 def test_grid_dims_match_oracle(orc):
