@@ -243,6 +243,98 @@ def rollout_grad(steps=4):
     torch.set_default_dtype(torch.float32)
 
 
+OUT_OPS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_sphops_16.npz")
+
+
+def sph_operations_and_shifting():
+    """Every mode of the reference's generic operator (v2/sphOps.py:253-282:
+    interpolate, gradient naive/difference/summation/symmetric, divergence
+    and divergenceConsistent in the same four modes) on scalar and vector
+    fields with non-uniform masses and densities, and deltaPlusShifting
+    (v2/modules/shifting.py:119-147) with and without the Mach number from
+    the velocities, on a jittered periodic 16x16 set."""
+    sys.path.insert(0, REF)
+    ops = importlib.import_module("diffSPH.v2.sphOps")
+    M = load()
+    shifting_src = os.path.join(REF, "diffSPH", "v2", "modules", "shifting.py")
+    # shifting.py imports v2.modules.neighborhood (torchCompactRadius,
+    # unavailable) and surfaceDetection at module level; only
+    # deltaPlusShifting is needed, so its source lines are executed alone,
+    # with the names it uses.
+    src = open(shifting_src).read().splitlines()
+    a = next(k for k, l in enumerate(src) if l.startswith("def deltaPlusShifting"))
+    b = next(k for k in range(a + 1, len(src)) if src[k].startswith("def "))
+    ns_ = {"torch": torch, "scatter_sum": M["math"].scatter_sum,
+           "record_function": torch.profiler.record_function}
+    exec(compile("\n".join(src[a:b]), shifting_src, "exec"), ns_)
+    deltaPlusShifting = ns_["deltaPlusShifting"]
+
+    rng = np.random.default_rng(23)
+    ns = 16
+    dx = 2.0 / ns
+    xs = np.linspace(-1 + dx / 2, 1 - dx / 2, ns, dtype=np.float32)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+    pos = (pos + rng.uniform(-0.2 * dx, 0.2 * dx, pos.shape)).astype(np.float32)
+    n = pos.shape[0]
+    h = np.float32(np.sqrt(50.0 * dx * dx / np.pi))
+    mass = (dx * dx * (1.0 + 0.2 * rng.uniform(-1, 1, n))).astype(np.float32)
+    rho = (1.0 + 0.05 * rng.standard_normal(n)).astype(np.float32)
+    qs = (np.sin(2 * pos[:, 0]) + pos[:, 1] ** 2 + 0.1 * rng.standard_normal(n)).astype(np.float32)
+    qv = np.stack([np.cos(pos[:, 1]) + 0.1 * rng.standard_normal(n), pos[:, 0] * pos[:, 1]], 1).astype(np.float32)
+    vel = (0.5 * rng.standard_normal((n, 2))).astype(np.float32)
+
+    T = torch.from_numpy
+    p = T(pos)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i_all, j_all = T(ii.ravel()), T(jj.ravel())
+    xij = p[i_all] - p[j_all]
+    xij = torch.stack([M["math"].mod(xij[:, k], -1.0, 1.0) for k in range(2)], -1)
+    rij = torch.linalg.norm(xij, dim=-1)
+    xij = xij / (rij + 1e-7).view(-1, 1)
+    hv = torch.full((n,), float(h))
+    rij = rij / hv[j_all]
+    keep = rij <= 1.0
+    i, j = i_all[keep], j_all[keep]
+    rij, xij, hij = rij[keep], xij[keep], hv[j]
+    W = M["W2"].kernel(rij, hij, 2)
+    gW = M["W2"].kernelGradient(rij, xij, hij, 2)
+    m_, r_ = T(mass), T(rho)
+    S, V = (T(qs), T(qs)), (T(qv), T(qv))
+    out = {}
+
+    def op(q, operation, gm="symmetric", dm="div"):
+        return ops.sphOperation((m_, m_), (r_, r_), q, (i, j), W, gW, rij, xij, hij, n, operation=operation,
+                                gradientMode=gm, divergenceMode=dm)
+
+    out["interp_scalar"] = op(S, "interpolate")
+    out["interp_vector"] = op(V, "interpolate")
+    for gm in ("naive", "difference", "summation", "symmetric"):
+        out[f"grad_{gm}_scalar"] = op(S, "gradient", gm)
+        out[f"grad_{gm}_vector"] = op(V, "gradient", gm)
+        out[f"div_{gm}"] = op(V, "divergence", gm)
+        out[f"divc_{gm}"] = op(V, "divergenceConsistent", gm)
+    st = {"masses": m_, "densities": r_, "numParticles": n}
+    nd = {"indices": (i, j), "kernels": W, "gradients": gW, "distances": rij, "vectors": xij, "supports": hij}
+    out["states_density"] = ops.sphOperationStates(st, st, None, nd, operation="density")
+    out["states_grad_difference"] = ops.sphOperationStates(st, st, S, nd, operation="gradient",
+                                                           gradientMode="difference")
+
+    ks = float(h / (2 * dx))
+    for mach in (True, False):
+        cfg = {"kernel": {"function": M["W2"], "kernelScale": ks},
+               "particle": {"dx": dx, "support": float(h)}, "domain": {"dim": 2},
+               "shifting": {"R": 0.25, "n": 4, "CFL": 1.5, "computeMach": mach}, "fluid": {"cs": 10.0}}
+        ps = {"neighborhood": {"indices": (i, j), "kernels": W, "gradients": gW}, "masses": m_, "densities": r_,
+              "velocities": T(vel), "supports": hv, "positions": p}
+        out[f"deltaplus_mach{int(mach)}"] = deltaPlusShifting(ps, cfg)
+    np_out = {k: v.detach().numpy().astype(np.float32) for k, v in out.items()}
+    np.savez(OUT_OPS, pos=pos, mass=mass, rho=rho, qs=qs, qv=qv, vel=vel, h=np.float32(h), dx=np.float32(dx),
+             kernel_scale=np.float32(ks), i=i.numpy().astype(np.int32), j=j.numpy().astype(np.int32), **np_out)
+    print("wrote", OUT_OPS, {k: v.shape for k, v in np_out.items()})
+
+
 if __name__ == "__main__":
     main()
     rollout_grad()
+    sph_operations_and_shifting()
