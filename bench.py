@@ -41,6 +41,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _baseline_threads() -> int:
+    """Threads of the CPU baseline: the host cores this process may use (its
+    CPU affinity), capped by OMP_NUM_THREADS when set (the GPU box gives each
+    GPU a share of its cores that way)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.strip().isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def _host_info(threads: int) -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    flags = None
+    mk = os.path.join(ROOT, "oracle", "Makefile")
+    if os.path.exists(mk):
+        for line in open(mk):
+            if line.startswith("CXXFLAGS"):
+                flags = "g++ " + line.split("=", 1)[1].strip()
+    return {"nproc": os.cpu_count(), "affinity_cores": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model, "compiler": flags, "threads_used": threads}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,7 +92,7 @@ def parse():
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (reference kernel for PMC byte counters)")
@@ -253,7 +291,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as orc
-        threads = min(os.cpu_count() or 1, 16)
+        threads = _baseline_threads()
         orc.set_threads(threads)
         dom = orc.make_domain((-1, -1), (1, 1), (True, True))
         ost = orc.DfsphState(snap["pos"], snap["vel"], snap["mass"], snap["area"], snap["uid"].astype(np.int32),
@@ -266,12 +304,35 @@ def main():
         c1 = time.perf_counter()
         cval = n * args.cpu_steps / (c1 - c0)
         same = its[0] == plain["first_step_iters_div"]
+        log(f"cpu baseline: {cval:.4g} particle-updates/s ({(c1 - c0):.1f} s, {threads} threads, div its {its})")
+        # one thread, on a bounded sample of the same workload family: the
+        # first 3 steps of the 256^2 case, also timed on all threads
+        small = sampling.dfsph_random_flow(256, seed=args.seed, jitter_width=0.1)
+        ns_ = small["pos"].shape[0]
+        rates = {}
+        for th in (1, threads):
+            orc.set_threads(th)
+            sst = orc.DfsphState(small["pos"].copy(), small["vel"].copy(), small["mass"].copy(), small["area"].copy(),
+                                 np.arange(ns_, dtype=np.int32))
+            s0 = time.perf_counter()
+            sits = []
+            for _ in range(3):
+                sst, sinfo = orc.dfsph_step(sst, dom, small["h"], 3.0 * small["dx"] / 10.0)
+                sits.append(int(sinfo["it_div"]))
+            rates[th] = ns_ * 3 / (time.perf_counter() - s0)
+        orc.set_threads(threads)
+        log(f"cpu 256^2 sample: 1 thread {rates[1]:.4g}, {threads} threads {rates[threads]:.4g} particle-updates/s "
+            f"(div its {sits})")
         cpu = {"value": round(cval, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} plain DFSPH step(s) of the {n}-particle workload from the GPU plain "
+               "sample": f"{args.cpu_steps} plain DFSPH steps of the {n}-particle workload from the GPU plain "
                          f"solver's state at its first timed step (step {args.warmup}), C++/OpenMP oracle, "
                          f"{threads} threads; div iterations {its} (GPU plain mean over the timed steps "
-                         f"{plain['iters_div_mean']:.1f}; first timed step matches the GPU: {same})"}
-        log(f"cpu baseline: {cval:.4g} particle-updates/s ({(c1 - c0):.1f} s, {threads} threads, div its {its})")
+                         f"{plain['iters_div_mean']:.1f}; first timed step matches the GPU: {same})",
+               "seconds": round(c1 - c0, 2),
+               "one_thread": {"value": round(rates[1], 1), "all_threads_same_sample": round(rates[threads], 1),
+                              "sample": f"first 3 plain DFSPH steps of the 256x256 workload ({ns_} particles, "
+                                        f"div iterations {sits}), 1 thread and {threads} threads"},
+               **_host_info(threads)}
 
     def rnd(r):
         return None if r is None else {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
@@ -364,7 +425,7 @@ def _oracle_deltasph_rate(states, cfg, steps):
     (domain, pos, vel, rho, mass) samples; returns (particle-updates/s,
     threads, particles)."""
     from oracle import oracle as orc
-    threads = min(os.cpu_count() or 1, 16)
+    threads = _baseline_threads()
     orc.set_threads(threads)
     prm = orc.DeltaParams(rho0=cfg.rho0, cs=cfg.cs, delta=cfg.delta, kernel_scale=cfg.kernel_scale,
                           viscosity=1 if cfg.viscosity == "deltaSPH_viscid" else 0,

@@ -13,6 +13,7 @@ namespace dsph {
 
 int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws(int64_t n);
+int scan_read_total(const void* ws, int64_t* total_host, hipStream_t st);
 
 struct GridDev {
   int nc[2];
@@ -291,7 +292,10 @@ __global__ void __launch_bounds__(kBlock) k_slice_len(const int32_t* __restrict_
     v = t > v ? t : v;
   }
   const int nslices = (nq + 63) >> 6;
-  if ((threadIdx.x & 63) == 0 && (i >> 6) < nslices) slen[i >> 6] = v * 64;
+  // 64 * (longest row) as int32; a row too long for that saturates, which
+  // the scan then reports as an overflow of the int32 slice offsets
+  if ((threadIdx.x & 63) == 0 && (i >> 6) < nslices)
+    slen[i >> 6] = v > (INT32_MAX >> 6) ? INT32_MAX : (v < 0 ? -1 : v * 64);
 }
 
 inline GridDev make_grid(const dsph_grid& g) {
@@ -473,6 +477,8 @@ extern "C" int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* 
                       h_uniform, mode);
   if (s != DSPH_OK) return s;
   if (!overflow || (capacity > 0 && !nbr16)) return DSPH_ERR_ARG;
+  // the deltas are j - i modulo n: row i must be reference slot i
+  if (nq != nref || !qpos_wrapped) return DSPH_ERR_ARG;
   hipStream_t st = as_stream(stream);
   k_fill_flag_clear<<<1, 64, 0, st>>>(overflow);
   DSPH_LAUNCH_CHECK();
@@ -495,6 +501,8 @@ extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_gr
                       h_uniform, mode);
   if (s != DSPH_OK) return s;
   if (width < 1 || width > 4096 || !row_overflow || !slice_off || (nbr16 && !delta_overflow)) return DSPH_ERR_ARG;
+  // 16-bit deltas are j - i modulo n: row i must be reference slot i
+  if (nbr16 && (nq != nref || !qpos_wrapped)) return DSPH_ERR_ARG;
   const int64_t nslices = (nq + 63) / 64;
   if (nslices * 64 * (int64_t)width >= INT32_MAX) return DSPH_ERR_ARG;
   if (capacity < nslices * 64 * (int64_t)width) return DSPH_ERR_CAPACITY;
@@ -528,12 +536,7 @@ extern "C" int dsph_sell_layout(const int32_t* counts, int64_t nq, int32_t* slic
   }
   int s = scan_exclusive(slen, ns, slice_off, scan_w, scan_ws(ns), st);
   if (s != DSPH_OK) return s;
-  if (total_host) {
-    int32_t t = 0;
-    DSPH_CHECK_HIP(hipMemcpyAsync(&t, slice_off + ns, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    DSPH_CHECK_HIP(hipStreamSynchronize(st));
-    *total_host = t;
-  }
+  if (total_host) return scan_read_total(scan_w, total_host, st);
   return DSPH_OK;
 }
 

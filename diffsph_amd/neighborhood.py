@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import DsphError, check, lib, ptr
 
 MODES = {"scatter": 0, "gather": 1, "symmetric": 2}
 
@@ -207,6 +207,9 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     and the pair total); a row longer than `width` makes it fall back to
     "sell" (the returned list then has ``width == 0``).  The entries, their
     order and the counts are the same in all three layouts."""
+    if delta16 is not None and not qpos_wrapped:
+        raise DsphError("search: 16-bit neighbor deltas need the grid's own sorted positions as queries "
+                        "(qpos_wrapped=True)")
     _check_f32(qpos, "queryPositions", (2,))
     m = MODES[mode] if isinstance(mode, str) else int(mode)
     nq = qpos.shape[0]

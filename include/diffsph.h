@@ -110,7 +110,10 @@ int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit,
 
 /* Sliced-ELL layout from row counts: slice_off[s] = 64 * sum over earlier
  * slices of their longest row; slice_off[nslices] = padded entry count,
- * also copied to *total_host when not NULL (synchronises the stream). */
+ * also copied to *total_host when not NULL (synchronises the stream).
+ * With total_host, a padded entry count beyond INT32_MAX returns
+ * DSPH_ERR_DIM (the int32 slice offsets would wrap); without it the caller
+ * guarantees the count fits. */
 size_t dsph_sell_workspace(int64_t nq);
 int dsph_sell_layout(const int32_t* counts, int64_t nq, int32_t* slice_off,
                      int64_t* total_host, void* workspace,
@@ -181,12 +184,6 @@ int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
                        int64_t layout_total, int32_t* nbr, int64_t capacity,
                        void* stream);
 
-/* dsph_neighbor_fill that also writes the 16-bit modular deltas of
- * dsph_neighbor_compact16 (d = j - i reduced mod nref; |d| <= 32767, else
- * *overflow is set) in the same pass, for a list whose rows are the sorted
- * reference slots themselves (a set searched against its own grid and not
- * remapped afterwards).  *overflow is cleared first.  No counterpart in the
- * reference. */
 /* One-pass neighbor list in a fixed-width sliced layout (no count pass, no
  * host read of the list size): slice s starts at s * 64 * width (slice_off
  * [ceil(nq/64)+1] is written), entry n of row i at slice_off[i/64] + 64 n +
@@ -195,7 +192,8 @@ int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* grid,
  * incomplete (build the two-pass list instead).  Entries past a row's count
  * are not written.  With nbr16 the 16-bit deltas of
  * dsph_neighbor_compact16 are written in the same pass (rows = reference
- * slots; *delta_overflow as there).  Both flags are cleared first.
+ * slots: nq == nref and qpos_wrapped != 0, else DSPH_ERR_ARG;
+ * *delta_overflow as there).  Both flags are cleared first.
  * capacity >= ceil(nq/64) * 64 * width.  No counterpart in the reference. */
 int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid,
                              const int32_t* cell_start, const float* sorted_pos,
@@ -207,6 +205,13 @@ int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid,
                              int32_t* delta_overflow, int32_t* row_overflow,
                              void* stream);
 
+/* dsph_neighbor_fill that also writes the 16-bit modular deltas of
+ * dsph_neighbor_compact16 (d = j - i reduced mod nref; |d| <= 32767, else
+ * *overflow is set) in the same pass, for a list whose rows are the sorted
+ * reference slots themselves (a set searched against its own grid and not
+ * remapped afterwards).  *overflow is cleared first.  Requires nq == nref and
+ * qpos_wrapped != 0 (the queries are the grid's own sorted positions),
+ * else DSPH_ERR_ARG.  No counterpart in the reference. */
 int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* grid,
                          const int32_t* cell_start, const float* sorted_pos,
                          const float* sorted_h, int64_t nref,
@@ -225,7 +230,11 @@ size_t dsph_scan_workspace(int64_t n);
 /* out[0..n] = exclusive prefix sum of in[0..n) (out[n] = total).  If
  * total_host != NULL the total is also copied to host memory (this
  * synchronises the stream, as the reference's `.item()` at
- * v2/cppSrc/computeNeighborhood.cpp:58 does). */
+ * v2/cppSrc/computeNeighborhood.cpp:58 does).  Partial sums are carried in
+ * int64: with total_host, a total beyond INT32_MAX (or a negative input)
+ * returns DSPH_ERR_DIM instead of wrapped offsets; without it the caller
+ * guarantees the total fits (as the fixed-width list does by its size
+ * check). */
 int dsph_scan_exclusive(const int32_t* in, int64_t n, int32_t* out,
                         int64_t* total_host, void* workspace,
                         size_t workspace_bytes, void* stream);

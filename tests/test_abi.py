@@ -91,3 +91,34 @@ def test_batch_workspace_host_checks():
                              C.byref(b)) == 1
     assert L.dsph_batch_init(doms, counts, 2, C.c_float(h), C.c_float(0.5 * h), 64, 256, C.c_void_p(256), ws,
                              C.byref(b)) == 1
+
+
+def test_delta16_requires_rows_equal_reference_slots():
+    """16-bit neighbor deltas decode j = i + d (mod n): a query set other
+    than the grid's own sorted particles is rejected on the host before any
+    launch (the fake pointers are never dereferenced)."""
+    from diffsph_amd import _lib
+    L = _lib.lib()
+    dom = _lib.make_domain((-1, -1), (1, 1), (True, True))
+    g = _lib.Grid()
+    assert L.dsph_grid_dims(C.byref(dom), C.c_float(0.1), C.byref(g)) == 0
+    P = C.c_void_p
+    common = (C.byref(dom), C.byref(g), P(0x1000), P(0x2000), None)
+    # nq != nref
+    assert L.dsph_neighbor_fill16(*common, 100, P(0x3000), None, 99, 1, C.c_float(0.1), 0, P(0x4000), None, 1000,
+                                  P(0x5000), P(0x6000), 1000, P(0x7000), None) == 1
+    # queries not the grid's wrapped positions
+    assert L.dsph_neighbor_fill16(*common, 100, P(0x3000), None, 100, 0, C.c_float(0.1), 0, P(0x4000), None, 1000,
+                                  P(0x5000), P(0x6000), 1000, P(0x7000), None) == 1
+    for nq, wrapped in [(99, 1), (100, 0)]:
+        assert L.dsph_neighbor_fill_fixed(*common, 100, P(0x3000), None, nq, wrapped, C.c_float(0.1), 0, 67,
+                                          P(0x4000), P(0x5000), P(0x6000), 2 * 64 * 67, P(0x8000), P(0x9000),
+                                          P(0xa000), None) == 1
+
+
+def test_search_rejects_delta16_for_foreign_queries():
+    import torch
+    from diffsph_amd import neighborhood
+    from diffsph_amd._lib import DsphError
+    with pytest.raises(DsphError, match="16-bit"):
+        neighborhood.search(None, torch.zeros((4, 2)), None, delta16=object(), qpos_wrapped=False)

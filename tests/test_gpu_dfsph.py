@@ -63,6 +63,36 @@ def test_dfsph_steps_match_oracle(orc, nside, seed, steps):
         assert abs(stats.den_error - info["err_den"]) <= 1e-6 * max(1e-12, abs(info["err_den"])) + 1e-9
 
 
+def test_dfsph_1m_steps_match_oracle(orc):
+    """The 1M-particle headline workload (1024^2, seed 0) through the HIP
+    path against the oracle: the first step (p = 0 warm start) and the
+    second (0.75 p_prev warm start; the divergence solve runs to maxIters
+    there), with identical pair counts, iteration counts and cell order and
+    every field within 1e-5 relative."""
+    d, dt = _setup(1024, 0)
+    print("case 1024^2 seed 0, 2 steps", flush=True)
+    s = _solver(d, dt)
+    dom = orc.make_domain((-1, -1), (1, 1), (True, True))
+    n = d["pos"].shape[0]
+    ost = orc.DfsphState(d["pos"].copy(), d["vel"].copy(), d["mass"].copy(), d["area"].copy(),
+                         np.arange(n, dtype=np.int32))
+    for k in range(2):
+        stats = s.step()
+        ost, info = orc.dfsph_step(ost, dom, d["h"], dt)
+        print(f"step {k}: pairs {stats.num_pairs}/{info['num_pairs']} div it {stats.div_iters}/{info['it_div']} "
+              f"den it {stats.den_iters}/{info['it_den']}", flush=True)
+        np.testing.assert_array_equal(s.uid.cpu().numpy(), info["uid"])
+        assert stats.num_pairs == info["num_pairs"]
+        assert stats.div_iters == info["it_div"] and stats.den_iters == info["it_den"]
+        for name, gpu, ref in [("rho", s.rho, info["rho"]), ("alpha_den", s.alpha_den, info["alpha_den"]),
+                               ("visc", s.visc, info["visc"]), ("p_div", s.p_div, info["p_div"]),
+                               ("p_den", s.p_den, info["p_den"]), ("a_den", s.a_den, info["a_den"]),
+                               ("vel", s.vel, ost.vel), ("pos", s.pos, ost.pos)]:
+            e = _rel(gpu.cpu().numpy(), ref)
+            print(f"  {name}: rel err vs oracle {e:.2e}", flush=True)
+            assert e <= RTOL, (k, name, e)
+
+
 def test_dfsph_float64_error_reported(orc):
     """Error of the float32 GPU step against a float64 run of the same ops."""
     d, dt = _setup(64, 3)
@@ -82,20 +112,27 @@ def test_dfsph_float64_error_reported(orc):
 
 
 def test_dfsph_1m_step_properties():
-    """The 1M-particle workload: the density solve converges; the divergence
-    solve (relaxed Jacobi, as the reference) may stop at maxIters on this
-    resolution, but never diverges."""
+    """The 1M-particle workload over 12 steps: the density solve converges
+    every step; the divergence solve (relaxed Jacobi, as the reference) may
+    stop at maxIters on the first steps but never diverges.  After the
+    start-up transient the mean |rho/rho0 - 1| of the summation density is
+    within the headline's 0.1 %.  The largest deviation is an expansion
+    (rho < rho0) left by the jittered lattice, which the clamped pressure
+    (p >= 0) cannot correct; compression stays within 0.5 %."""
     d, dt = _setup(1024, 0)
     s = _solver(d, dt)
-    for k in range(2):
+    for k in range(12):
         st = s.step()
         print(f"1M step {k}: div it {st.div_iters} err {st.div_error:.3e}; den it {st.den_iters} "
               f"err {st.den_error:.3e}; pairs {st.num_pairs}", flush=True)
         assert st.den_iters < 256 and st.den_error <= 5e-4
         assert st.div_iters <= 256 and np.isfinite(st.div_error) and st.div_error < 1.0
     mean_e, max_e = s.density_error()
-    print(f"density error mean {mean_e:.3e} max {max_e:.3e}", flush=True)
-    assert mean_e < 1e-2
+    e = (s.rho - 1.0)
+    print(f"density error mean {mean_e:.3e} max {max_e:.3e}; compression max {float(e.max()):.3e}, "
+          f"expansion max {float(-e.min()):.3e}", flush=True)
+    assert mean_e <= 1e-3
+    assert float(e.max()) <= 5e-3 and max_e <= 5e-2
     # UID permutation is a bijection
     assert torch.equal(torch.sort(s.uid.long()).values, torch.arange(s.n, device=s.uid.device))
 

@@ -310,3 +310,45 @@ def test_large_search_properties():
     k2 = torch.sort(cols * g.n + rows).values
     assert torch.equal(k1, k2)
     del off
+
+
+def test_search_1m_matches_oracle_bitexact(orc):
+    """The 1M-particle workload (1024^2 jittered lattice on [-1,1]^2, h =
+    3.99 dx): grid, counts, offsets and every list entry bit-exact in the
+    two-pass and in the one-pass fixed-width layout, and the 16-bit modular
+    deltas (deltas in the thousands, across the periodic seam) decode to the
+    int32 list."""
+    from diffsph_amd.neighborhood import Delta16Buffers
+    print("case 1024^2 seed 0", flush=True)
+    d, per = _case(1024, 0, 0.1)
+    dom_o = orc.make_domain((-1, -1), (1, 1), per)
+    g_o = orc.grid_build(dom_o, d["pos"], d["h"])
+    c_o, off_o, nbr_o = orc.neighbors_cell(dom_o, g_o, g_o.sorted_pos, h0=d["h"])
+    dev = torch.device("cuda:0")
+    dom = _lib.make_domain((-1, -1), (1, 1), per)
+    g = CellGrid.build(torch.from_numpy(d["pos"]).to(dev), None, dom, d["h"], h_uniform=d["h"])
+    np.testing.assert_array_equal(g.cell_start.cpu().numpy(), g_o.cell_start)
+    np.testing.assert_array_equal(g.sort_index.cpu().numpy(), g_o.sort_index)
+    np.testing.assert_array_equal(g.sorted_pos.cpu().numpy(), g_o.sorted_pos)
+    n = d["pos"].shape[0]
+    for layout, width in [("sell", 0), ("fixed", 67)]:
+        nl = search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True, layout=layout, width=width,
+                    delta16=Delta16Buffers())
+        print(f"  layout {layout}: pairs {nl.num_pairs} width {nl.width}", flush=True)
+        assert nl.num_pairs == int(off_o[-1]) and nl.width == width
+        np.testing.assert_array_equal(nl.counts.cpu().numpy(), c_o)
+        np.testing.assert_array_equal(nl.offsets.cpu().numpy(), off_o)
+        np.testing.assert_array_equal(nl.csr_nbr().cpu().numpy(), nbr_o)
+        # 16-bit deltas: j = (i + d) mod n for every stored entry
+        assert int(nl.nbr16_overflow.cpu()[0]) == 0
+        full = nl.nbr.cpu().numpy().astype(np.int64)
+        d16 = nl.nbr16[:full.shape[0]].cpu().numpy().astype(np.int64)
+        so = nl.slice_off.cpu().numpy().astype(np.int64)
+        rows = np.arange(n)
+        col = np.arange(int(c_o.max()))
+        idx = so[rows // 64][:, None] + 64 * col[None, :] + (rows % 64)[:, None]
+        valid = col[None, :] < c_o[:, None]
+        i_ = np.broadcast_to(rows[:, None], idx.shape)[valid]
+        e = idx[valid]
+        assert np.array_equal((i_ + d16[e]) % n, full[e])
+        assert np.abs(d16[e]).max() > 1000
