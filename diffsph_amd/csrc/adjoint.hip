@@ -49,12 +49,15 @@ __device__ __forceinline__ PGeo pgeo(float dx, float dy, float h) {
   g.dy = dy;
   g.r = sqrt_rn(dist2_exact(dx, dy));
   g.e = g.r + 1e-7f;
-  g.ex = dx / g.e;
-  g.ey = dy / g.e;
-  g.q = g.r / h;
+  // the forward kernels' reciprocal form (pair_geom)
+  const KConst k = kconst(h);
+  const float ri = 1.0f / g.e;
+  g.ex = dx * ri;
+  g.ey = dy * ri;
+  g.q = g.r * k.ih;
   g.rr = g.q * h;
   const float t = clamp01(1.0f - g.q);
-  g.w1 = wendland_dw(g.q, h);
+  g.w1 = wendland_dw(g.q, k);
   // d w1 / dr = C/h^4 (-20) (1-q)^2 (1-4q)
   g.w1p = g.q < 1.0f ? (((-20.0f * (t * t)) * (1.0f - 4.0f * g.q)) * kC2) / (((h * h) * h) * h) : 0.0f;
   g.gx = g.ex * g.w1;

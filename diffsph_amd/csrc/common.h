@@ -90,17 +90,31 @@ __device__ __forceinline__ float clamp01(float t) { return fminf(fmaxf(t, 0.0f),
 // is compiled with -ffp-contract=off and IEEE division/sqrt, so every value
 // is reproduced bit for bit by a CPU implementation of the same expressions.
 
-// W(q,h) = (1-q)^4 (1+4q) C / h^2, cpow(1-q,4) = ((t*t)*t)*t
-__device__ __forceinline__ float wendland_w(float q, float h) {
+// Kernel constants of a support h: 1/h, C/h^2 (for W) and C/h^3 (for
+// dW/dr).  Pair loops over a uniform support evaluate them once per row
+// (KSel) instead of dividing by h, h^2 and h^3 for every pair.
+struct KConst {
+  float ih, cw, cd;
+};
+__device__ __forceinline__ KConst kconst(float h) {
+  KConst k;
+  k.ih = 1.0f / h;
+  k.cw = kC2 / (h * h);
+  k.cd = kC2 / ((h * h) * h);
+  return k;
+}
+
+// W(q) = (1-q)^4 (1+4q) C/h^2, cpow(1-q,4) = ((t*t)*t)*t
+__device__ __forceinline__ float wendland_w(float q, const KConst& k) {
   float t = clamp01(1.0f - q);
   float t4 = ((t * t) * t) * t;
-  return ((t4 * (1.0f + 4.0f * q)) * kC2) / (h * h);
+  return (t4 * (1.0f + 4.0f * q)) * k.cw;
 }
-// dk/dq C / h^3 with dk/dq = -20 q (1-q)^3; grad W = xhat * this.
-__device__ __forceinline__ float wendland_dw(float q, float h) {
+// dk/dq C/h^3 with dk/dq = -20 q (1-q)^3; grad W = xhat * this.
+__device__ __forceinline__ float wendland_dw(float q, const KConst& k) {
   float t = clamp01(1.0f - q);
   float t3 = (t * t) * t;
-  return (((-20.0f * q) * t3) * kC2) / ((h * h) * h);
+  return ((-20.0f * q) * t3) * k.cd;
 }
 
 __device__ __forceinline__ float pick_h(int mode, float hi, float hj) {
@@ -109,17 +123,43 @@ __device__ __forceinline__ float pick_h(int mode, float hi, float hj) {
 
 // Geometry of one pair (query i, reference j) evaluated on the fly, with the
 // reference's conventions: d = x_i - x_j (min-image), r = |d|,
-// xhat = d/(r+1e-7), q = r/h_ij (v2/modules/neighborhood.py:445-451).
+// xhat = d/(r+1e-7), q = r/h_ij (v2/modules/neighborhood.py:445-451), in
+// reciprocal form: q = r * (1/h), xhat = d * (1/(r+1e-7)), one division per
+// pair (a few ulp from the quotients the reference forms).
 struct Geom {
   float dx, dy;   // x_i - x_j
   float r, q, h;
+  float ri;       // 1 / (r + 1e-7)
   float ex, ey;   // xhat
   float w;        // W_ij
   float dw;       // dW/dr
   float gx, gy;   // grad_i W_ij
 };
 
+__device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const KConst& k, const Dom& d) {
+  Geom g;
+  g.dx = mod_offset(pi.x - pj.x, d, 0);
+  g.dy = mod_offset(pi.y - pj.y, d, 1);
+  g.r = sqrt_rn(dist2_exact(g.dx, g.dy));
+  g.h = h;
+  g.q = g.r * k.ih;
+  g.ri = 1.0f / (g.r + 1e-7f);
+  g.ex = g.dx * g.ri;
+  g.ey = g.dy * g.ri;
+  g.w = wendland_w(g.q, k);
+  g.dw = wendland_dw(g.q, k);
+  g.gx = g.ex * g.dw;
+  g.gy = g.ey * g.dw;
+  return g;
+}
 __device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const Dom& d) {
+  return pair_geom(pi, pj, h, kconst(h), d);
+}
+
+// The same geometry with the reference's own quotients (q = r/h, xhat =
+// d/(r+1e-7), W and dW/dr divided by h^2 and h^3): the per-pair arrays
+// handed to reference callers (neighborDict exports) use this form.
+__device__ __forceinline__ Geom pair_geom_exact(float2 pi, float2 pj, float h, const Dom& d) {
   Geom g;
   g.dx = mod_offset(pi.x - pj.x, d, 0);
   g.dy = mod_offset(pi.y - pj.y, d, 1);
@@ -127,21 +167,32 @@ __device__ __forceinline__ Geom pair_geom(float2 pi, float2 pj, float h, const D
   g.h = h;
   g.q = __fdiv_rn(g.r, h);
   const float den = g.r + 1e-7f;
+  g.ri = 1.0f / den;
   g.ex = __fdiv_rn(g.dx, den);
   g.ey = __fdiv_rn(g.dy, den);
-  g.w = wendland_w(g.q, h);
-  g.dw = wendland_dw(g.q, h);
+  const float t = clamp01(1.0f - g.q);
+  const float t3 = (t * t) * t;
+  g.w = ((t3 * t * (1.0f + 4.0f * g.q)) * kC2) / (h * h);
+  g.dw = (((-20.0f * g.q) * t3) * kC2) / ((h * h) * h);
   g.gx = g.ex * g.dw;
   g.gy = g.ey * g.dw;
   return g;
 }
+
+// Kernel constants per pair: hoisted when both supports are the uniform h0
+// (the branch is uniform across the wave), else evaluated for the pair's h.
+struct KSel {
+  KConst k0;
+  bool uniform;
+  __device__ __forceinline__ KConst operator()(float h) const { return uniform ? k0 : kconst(h); }
+};
 
 // The DFSPH pair coefficient V_j gradW_ij in the factored form
 // d_ij * s_ij with s_ij = V_j * (dW/dr / (r + 1e-7)): the per-entry scalar
 // s_ij is all the solver stores, d_ij is recomputed from the positions.
 // (Algebraically the reference's V_j * (d/(r+1e-7)) * dW/dr; it differs
 // from that product by a few ulp.)
-__device__ __forceinline__ float dfsph_t(const Geom& g) { return __fdiv_rn(g.dw, g.r + 1e-7f); }
+__device__ __forceinline__ float dfsph_t(const Geom& g) { return g.dw * g.ri; }
 
 struct PairsDev {
   int nq, nr;
@@ -178,6 +229,14 @@ inline PairsDev make_pairs(const dsph_pairs& p) {
   o.nbr16 = p.nbr16;
   o.ovf = p.nbr16_overflow;
   return o;
+}
+
+// The kernel constants of a pair list (evaluated once per thread).
+__device__ __forceinline__ KSel ksel(const PairsDev& P) {
+  KSel s;
+  s.uniform = !P.qh && !P.rh;
+  s.k0 = kconst(P.h0);
+  return s;
 }
 
 // Where the entries of row i live: entry n is at base + n*step (sliced ELL:

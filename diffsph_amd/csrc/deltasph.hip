@@ -91,6 +91,8 @@ struct DomainPerParticle {
 // the identity test uses the fluid row length (the fluid numNeighbors), and
 // the wall term of the density gradient is the plain 'difference' gradient
 // sum_b V_b (rho_i - rho_b) gradW_ib (densityGradient, deltaPlus.py:262-263).
+// One pass over the row: L_i does not depend on j, so the renormalized
+// gradient is L_i applied once to s_i = sum_j V_j (rho_i - rho_j) gradW_ij.
 template <class DF>
 __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, BoundaryDev B, DF domain,
                                                                const float* __restrict__ mass,
@@ -103,8 +105,10 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   const Dom dm = domain(i);
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   const RowRef rr = row_ref(P, i);
-  float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f;
+  const float rho_i = rho[i];
+  float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f, sx = 0.f, sy = 0.f;
   auto ld = [&](int j) {
     NbrXMR d;
     d.x = P.rpos[j];
@@ -112,9 +116,9 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
     d.r = rho[j];
     return d;
   };
-  scan_row<4, false>(P.nbr, rr, ld, [&](int j, const NbrXMR& d) {
+  scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrXMR& d) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, d.x, h, dm);
+    const Geom g = pair_geom(xi, d.x, h, ks(h), dm);
     const float Vj = d.m / d.r;
     const float s = -(g.q * g.h);  // -(distances * supports)
     const float qx = s * g.ex, qy = s * g.ey;
@@ -123,7 +127,11 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
     m01 += qx * ky;
     m10 += qy * kx;
     m11 += qy * ky;
+    const float dr = rho_i - d.r;
+    sx += dr * kx;
+    sy += dr * ky;
   });
+  float bx = 0.f, by = 0.f;
   if (B.on) {
     const RowRef br = row_ref(B.P, i);
     float b00 = 0.f, b01 = 0.f, b10 = 0.f, b11 = 0.f;
@@ -139,6 +147,9 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
       b01 += qx * ky;
       b10 += qy * kx;
       b11 += qy * ky;
+      const float dr = rho_i - B.rho[j];
+      bx += dr * kx;
+      by += dr * ky;
     }
     m00 = m00 + b00;
     m01 = m01 + b01;
@@ -154,32 +165,9 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   const dsph_pm::Pinv2 Lm = dsph_pm::pinv2x2(m00, m01, m10, m11);
   if (L_out) L_out[i] = make_float4(Lm.l00, Lm.l01, Lm.l10, Lm.l11);
   if (ev_out) ev_out[i] = make_float2(Lm.ev0, Lm.ev1);
-  const float rho_i = rho[i];
-  float ax = 0.f, ay = 0.f;
-  scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrXMR& e) {
-    const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, e.x, h, dm);
-    const float Vj = e.m / e.r;
-    const float nx = Lm.l00 * g.gx + Lm.l01 * g.gy;
-    const float ny = Lm.l10 * g.gx + Lm.l11 * g.gy;
-    const float kx = Vj * nx, ky = Vj * ny;
-    const float d = rho_i - e.r;
-    ax += d * kx;
-    ay += d * ky;
-  });
+  float ax = Lm.l00 * sx + Lm.l01 * sy;
+  float ay = Lm.l10 * sx + Lm.l11 * sy;
   if (B.on) {
-    const RowRef br = row_ref(B.P, i);
-    float bx = 0.f, by = 0.f;
-    for (int n = 0, k = br.base; n < br.cnt; ++n, k += br.step) {
-      const int j = B.P.nbr[k];
-      const float h = pick_h(B.P.mode, hi, h_of(B.P.rh, j, B.P.h0));
-      const Geom g = pair_geom(xi, B.P.rpos[j], h, B.P.dom);
-      const float Vj = B.mass[j] / B.rho[j];
-      const float kx = Vj * g.gx, ky = Vj * g.gy;
-      const float d = rho_i - B.rho[j];
-      bx += d * kx;
-      by += d * ky;
-    }
     ax = ax + bx;
     ay = ay + by;
   }
@@ -205,6 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, 
   const float rho_i = A.rho[i];
   const float2 gri = A.diffusion ? A.grad_rho[i] : make_float2(0.f, 0.f);
   const float p_i = eos_p(rho_i, A.rho0, A.eos_c);
+  const KSel ks = ksel(P);
   float accD = 0.f, accVx = 0.f, accVy = 0.f, accC = 0.f, accPx = 0.f, accPy = 0.f;
   const RowRef rr = row_ref(P, i);
   auto ld = [&](int j) {
@@ -218,7 +207,7 @@ __global__ void __launch_bounds__(kBlock) k_dsph_rhs(PairsDev P, BoundaryDev B, 
   };
   scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrRhs& e) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
-    const Geom g = pair_geom(xi, e.x, h, dm);
+    const Geom g = pair_geom(xi, e.x, h, ks(h), dm);
     const float2 uj = e.u;
     const float rho_j = e.r;
     const float m_j = e.m;

@@ -99,6 +99,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   const float2 vi = A.vel[i];
   float k1x = 0.f, k1y = 0.f, k2 = 0.f, dv = 0.f, vsx = 0.f, vsy = 0.f;
   const RowRef rr = row_ref(P, i);
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
   int n = 0, k = rr.base;
   auto body = [&](int k, const Nb& d) {
     const float h = pick_h(P.mode, hi, d.h);
-    Geom g = pair_geom(xi, d.x, h, P.dom);
+    Geom g = pair_geom(xi, d.x, h, ks(h), P.dom);
     const float Vj = d.vol;
     const float2 vj = d.v;
     // gradW_ij = d_ij * t_ij, V_j gradW_ij = d_ij * s_ij (dfsph_t)
@@ -652,6 +653,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   float s1 = 0.f, s3 = 0.f, s5 = 0.f;
   const RowRef rr = row_ref(P, i);
   if (pair_w) {
@@ -710,8 +712,9 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
     const float h = pick_h(P.mode, hi, hj);
     const float2 xj = P.rpos[j];
     const float dx = mod_offset(xi.x - xj.x, P.dom, 0), dy = mod_offset(xi.y - xj.y, P.dom, 1);
-    const float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
-    const float vw = vol[j] * wendland_w(q, h);
+    const KConst kk = ks(h);
+    const float q = sqrt_rn(dist2_exact(dx, dy)) * kk.ih;
+    const float vw = vol[j] * wendland_w(q, kk);
     s1 += vw * pdp[j];
     s3 += vw * src[j];
     s5 += vw * pnp[j];

@@ -179,7 +179,7 @@ struct GeoPy {
   }
   __device__ bool eval(const Q& q, int64_t j, P& p) const {
     const float h = pick_h(mode, q.h, hj ? hj[j] : 0.f);
-    p = pair_geom(q.x, make_float2(pj[2 * j], pj[2 * j + 1]), h, dom);
+    p = pair_geom_exact(q.x, make_float2(pj[2 * j], pj[2 * j + 1]), h, dom);
     return p.q <= 1.0f;
   }
   __device__ void write(const Q&, int row, int64_t j, const P& p, const PairOut& o, int64_t k) const {

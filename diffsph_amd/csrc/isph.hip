@@ -68,12 +68,14 @@ __global__ void __launch_bounds__(kBlock) k_isph_laplacian(PairsDev P, const flo
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   const RowRef rr = row_ref(P, i);
   float a = 0.0f;
   int self_k = -1;
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
-    const Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, h_of(P.rh, j, P.h0)), P.dom);
+    const float hh = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], hh, ks(hh), P.dom);
     const float gn = sqrt_rn(dist2_exact(g.gx, g.gy));
     const float quot = g.q * g.h + 1e-7f * g.h;
     const float t = (c2dt * (mass[j] / rho[j])) * (gn / quot);
@@ -520,13 +522,15 @@ __global__ void __launch_bounds__(kBlock) k_isph_close1(PairsDev P, CloseArgs A)
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   const float psi = A.pstar[i], pqi = A.pss[i], rhoi = A.rho[i];
   const float iri = 1.0f / (rhoi * rhoi);
   float gsx = 0.f, gsy = 0.f, gyx = 0.f, gyy = 0.f;
   const RowRef rr = row_ref(P, i);
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
-    const Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, h_of(P.rh, j, P.h0)), P.dom);
+    const float hh = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], hh, ks(hh), P.dom);
     const float mj = A.mass[j], rhoj = A.rho[j];
     const float V = mj / rhoj;
     const float kx = V * g.gx, ky = V * g.gy;
@@ -556,12 +560,14 @@ __global__ void __launch_bounds__(kBlock) k_isph_close2(PairsDev P, const float*
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   const float2 vi = vprime[i];
   float g00 = 0.f, g01 = 0.f, g10 = 0.f, g11 = 0.f;
   const RowRef rr = row_ref(P, i);
   for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
     const int j = P.nbr[k];
-    const Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, h_of(P.rh, j, P.h0)), P.dom);
+    const float hh = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], hh, ks(hh), P.dom);
     const float V = mass[j] / rho[j];
     const float kx = V * g.gx, ky = V * g.gy;
     const float2 vj = vprime[j];
@@ -592,6 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_isph_source(PairsDev P, const float2
   if (i < P.nq) {
     const float2 xi = P.qpos[i];
     const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
     const float rhoi = rho[i];
     const float2 ai = adv ? adv[i] : make_float2(0.f, 0.f);
     const float vix = vel[i].x + ai.x * dt, viy = vel[i].y + ai.y * dt;
@@ -599,7 +606,8 @@ __global__ void __launch_bounds__(kBlock) k_isph_source(PairsDev P, const float2
     const RowRef rr = row_ref(P, i);
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
-      const Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, h_of(P.rh, j, P.h0)), P.dom);
+      const float hh = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
+    const Geom g = pair_geom(xi, P.rpos[j], hh, ks(hh), P.dom);
       const float2 aj = adv ? adv[j] : make_float2(0.f, 0.f);
       const float vjx = vel[j].x + aj.x * dt, vjy = vel[j].y + aj.y * dt;
       const float V = mass[j] / rhoi;

@@ -18,6 +18,7 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
+  const KSel ks = ksel(P);
   float acc = 0.0f;
   const RowRef rr = row_ref(P, i);
   struct Nb {
@@ -27,9 +28,9 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
   auto load = [&](int j) { return Nb{P.rpos[j], h_of(P.rh, j, P.h0), rmass[j]}; };
   auto body = [&](int k, const Nb& d) {
     float dx = mod_offset(xi.x - d.x.x, P.dom, 0), dy = mod_offset(xi.y - d.x.y, P.dom, 1);
-    float h = pick_h(P.mode, hi, d.h);
-    float q = __fdiv_rn(sqrt_rn(dist2_exact(dx, dy)), h);
-    const float w = wendland_w(q, h);
+    const KConst kk = ks(pick_h(P.mode, hi, d.h));
+    float q = sqrt_rn(dist2_exact(dx, dy)) * kk.ih;
+    const float w = wendland_w(q, kk);
     if (w_out) w_out[k] = w;
     acc += d.m * w;
   };
@@ -76,10 +77,12 @@ __global__ void __launch_bounds__(kBlock) k_sph_operation(PairsDev P, int op, in
 #pragma unroll
   for (int c = 0; c < W * 2; ++c) acc[c] = 0.0f;
   const RowRef row = row_ref(P, i);
+  const KSel ks = ksel(P);
   for (int n = 0, k = row.base; n < row.cnt; ++n, k += row.step) {
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
-    Geom g = pair_geom(xi, P.rpos[j], pick_h(P.mode, hi, hj), P.dom);
+    const float h = pick_h(P.mode, hi, hj);
+    Geom g = pair_geom(xi, P.rpos[j], h, ks(h), P.dom);
     float qj[W];
 #pragma unroll
     for (int c = 0; c < W; ++c) qj[c] = rq[(int64_t)j * W + c];
@@ -180,7 +183,8 @@ __global__ void __launch_bounds__(kBlock) k_export_pairs(PairsDev P, const int32
     const int j = P.nbr[k];
     const float hj = h_of(P.rh, j, P.h0);
     const float h = pick_h(P.mode, hi, hj);
-    Geom g = pair_geom(xi, P.rpos[j], h, P.dom);
+    // exported per-pair arrays carry the reference's own quotients
+    Geom g = pair_geom_exact(xi, P.rpos[j], h, P.dom);
     if (out_i) out_i[o] = q;
     if (out_j) out_j[o] = r_sort ? r_sort[j] : j;
     if (out_r) out_r[o] = g.q;

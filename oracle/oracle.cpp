@@ -104,18 +104,44 @@ T C2() { return (T)(7.0 / 3.14159265358979323846); }
 // k(q) C / h^2 and dk/dq C / h^3, evaluated left to right as written in
 // Wendland2.py:30-36 (cpow(x,p) = x*x*...*x).
 template <class T>
-T kernelW(T q, T h) { return ((cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q)) * C2<T>()) / (h * h); }
+T kernelW_exact(T q, T h) { return ((cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q)) * C2<T>()) / (h * h); }
 template <class T>
-T kernelDW(T q, T h) { return (((T(-20) * q) * cpow<T>(T(1) - q, 3)) * C2<T>()) / ((h * h) * h); }
+T kernelDW_exact(T q, T h) { return (((T(-20) * q) * cpow<T>(T(1) - q, 3)) * C2<T>()) / ((h * h) * h); }
 
+// The solvers' reciprocal form: the constants 1/h, C/h^2 and C/h^3 of a
+// support are evaluated once and multiplied in (the GPU kernels hoist them
+// out of their pair loops), a few ulp from the quotients above.
 template <class T>
-struct G {
-  T dx, dy, r, q, h, ex, ey, w, dw, gx, gy;
+struct KC {
+  T ih, cw, cd;
 };
+template <class T>
+KC<T> kconst(T h) {
+  KC<T> k;
+  k.ih = T(1) / h;
+  k.cw = C2<T>() / (h * h);
+  k.cd = C2<T>() / ((h * h) * h);
+  return k;
+}
+template <class T>
+T kernelW(T q, const KC<T>& k) { return (cpow<T>(T(1) - q, 4) * (T(1) + T(4) * q)) * k.cw; }
+template <class T>
+T kernelDW(T q, const KC<T>& k) { return ((T(-20) * q) * cpow<T>(T(1) - q, 3)) * k.cd; }
+template <class T>
+T kernelW(T q, T h) { return kernelW<T>(q, kconst<T>(h)); }
+template <class T>
+T kernelDW(T q, T h) { return kernelDW<T>(q, kconst<T>(h)); }
 
 // Pair geometry with the reference's conventions
 // (v2/modules/neighborhood.py:445-451, 587-591): d = x_i - x_j,
-// xhat = d / (r + 1e-7), q = r / h.
+// xhat = d / (r + 1e-7), q = r / h, in the solvers' reciprocal form
+// q = r * (1/h), xhat = d * (1/(r + 1e-7)) (one division per pair, as the
+// GPU evaluates it).
+template <class T>
+struct G {
+  T dx, dy, r, q, h, ri, ex, ey, w, dw, gx, gy;
+};
+
 template <class T>
 G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
   G<T> g;
@@ -125,11 +151,35 @@ G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
   T b = g.dy * g.dy;
   g.r = std::sqrt(a + b);
   g.h = h;
+  const KC<T> k = kconst<T>(h);
+  g.q = g.r * k.ih;
+  g.ri = T(1) / (g.r + (T)1e-7);
+  g.ex = g.dx * g.ri;
+  g.ey = g.dy * g.ri;
+  g.w = kernelW<T>(g.q, k);
+  g.dw = kernelDW<T>(g.q, k);
+  g.gx = g.ex * g.dw;
+  g.gy = g.ey * g.dw;
+  return g;
+}
+
+// The reference's own quotients, for the per-pair arrays handed to
+// reference callers (neighborDict exports).
+template <class T>
+G<T> geom_exact(const T* xi, const T* xj, T h, const DomT<T>& d) {
+  G<T> g;
+  g.dx = modoff<T>(xi[0] - xj[0], d, 0);
+  g.dy = modoff<T>(xi[1] - xj[1], d, 1);
+  T a = g.dx * g.dx;
+  T b = g.dy * g.dy;
+  g.r = std::sqrt(a + b);
+  g.h = h;
   g.q = g.r / h;
+  g.ri = T(1) / (g.r + (T)1e-7);
   g.ex = g.dx / (g.r + (T)1e-7);
   g.ey = g.dy / (g.r + (T)1e-7);
-  g.w = kernelW<T>(g.q, h);
-  g.dw = kernelDW<T>(g.q, h);
+  g.w = kernelW_exact<T>(g.q, h);
+  g.dw = kernelDW_exact<T>(g.q, h);
   g.gx = g.ex * g.dw;
   g.gy = g.ey * g.dw;
   return g;
@@ -141,7 +191,7 @@ G<T> geom(const T* xi, const T* xj, T h, const DomT<T>& d) {
 // v2/modules/dfsph.py:13-20, 35-50; a few ulp apart).
 template <class T>
 T dfsph_t(const G<T>& g) {
-  return g.dw / (g.r + (T)1e-7);
+  return g.dw * g.ri;
 }
 
 // The neighbor predicate in float32: r = sqrt(dx*dx + dy*dy) with separately
@@ -911,14 +961,19 @@ struct BSet {
 // (+ wall: 'fluidwBoundary', deltaPlus.py:241), identity for rows with < 4
 // fluid neighbors (deltaPlus.py:245), pinv2x2, then
 // renormalizedDensityGradient (densityDiffusion.py:7-26) + the plain wall
-// gradient (densityGradient, deltaPlus.py:262-263).
+// gradient (densityGradient, deltaPlus.py:262-263).  One pass over each row:
+// L_i is the same for every pair of row i, so
+// sum_j V_j (rho_i - rho_j) L_i gradW_ij = L_i sum_j V_j (rho_i - rho_j) gradW_ij
+// (the sums are accumulated with M and L applied once; a few ulp from the
+// reference's per-pair product).
 template <class T>
 void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const T* rho, T* Lout, T* evout,
                         T* grad_rho, const BSet<T>* B) {
   set_threads();
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < c.nq; ++i) {
-    T m00 = 0, m01 = 0, m10 = 0, m11 = 0;
+    T m00 = 0, m01 = 0, m10 = 0, m11 = 0, sx = 0, sy = 0;
+    const T rho_i = rho[i];
     for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
       int32_t j = c.nbr[k];
       G<T> g = pg(c, i, j, d);
@@ -930,7 +985,11 @@ void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const 
       m01 += qx * ky;
       m10 += qy * kx;
       m11 += qy * ky;
+      T dr = rho_i - rho[j];
+      sx += dr * kx;
+      sy += dr * ky;
     }
+    T bx = 0, by = 0;
     if (B) {
       T b00 = 0, b01 = 0, b10 = 0, b11 = 0;
       for (int k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
@@ -944,6 +1003,9 @@ void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const 
         b01 += qx * ky;
         b10 += qy * kx;
         b11 += qy * ky;
+        T dr = rho_i - B->rho[j];
+        bx += dr * kx;
+        by += dr * ky;
       }
       m00 = m00 + b00;
       m01 = m01 + b01;
@@ -964,29 +1026,9 @@ void dsph_normalization(const Csr<T>& c, const DomT<T>& d, const T* mass, const 
       evout[2 * i] = ev[0];
       evout[2 * i + 1] = ev[1];
     }
-    T ax = 0, ay = 0;
-    for (int k = c.off[i]; k < c.off[i + 1]; ++k) {
-      int32_t j = c.nbr[k];
-      G<T> g = pg(c, i, j, d);
-      T Vj = mass[j] / rho[j];
-      T nx = L[0] * g.gx + L[1] * g.gy;
-      T ny = L[2] * g.gx + L[3] * g.gy;
-      T kx = Vj * nx, ky = Vj * ny;
-      T dr = rho[i] - rho[j];
-      ax += dr * kx;
-      ay += dr * ky;
-    }
+    T ax = L[0] * sx + L[1] * sy;
+    T ay = L[2] * sx + L[3] * sy;
     if (B) {
-      T bx = 0, by = 0;
-      for (int k = B->c.off[i]; k < B->c.off[i + 1]; ++k) {
-        int32_t j = B->c.nbr[k];
-        G<T> g = pg(B->c, i, j, d);
-        T Vj = B->mass[j] / B->rho[j];
-        T kx = Vj * g.gx, ky = Vj * g.gy;
-        T dr = rho[i] - B->rho[j];
-        bx += dr * kx;
-        by += dr * ky;
-      }
       ax = ax + bx;
       ay = ay + by;
     }
@@ -1503,7 +1545,7 @@ void deltaplus_shift(const Csr<T>& c, const DomT<T>& d, const double* prm, const
                      const T* vel, const BSet<T>* B, T* pos, T* shift_out) {
   const T q0 = (T)(prm[0] / prm[1]);
   const T h0 = (T)prm[1];
-  const T w0 = kernelW<T>(q0, h0);
+  const T w0 = kernelW_exact<T>(q0, h0);
   const T kscale = (T)prm[2], R = (T)prm[3], cfl = (T)prm[5], cs = (T)prm[6];
   const T thr = (T)(prm[7] * prm[0]);
   T ma = (T)prm[8];
@@ -1739,7 +1781,7 @@ int orc_pairs_filter(const orc_domain* dom, int32_t mode, const float* pos_i, co
     for (int32_t t = 0; t < nn[i]; ++t) {
       const int64_t j = idx_j[(int64_t)off[i] + t];
       const float hj = h_j ? h_j[j] : 0.f;
-      const G<float> g = geom<float>(pos_i + 2 * i, pos_j + 2 * j, pick_h<float>(mode, hi, hj), d);
+      const G<float> g = geom_exact<float>(pos_i + 2 * i, pos_j + 2 * j, pick_h<float>(mode, hi, hj), d);
       if (!(g.q <= 1.0f)) continue;
       if (out_i) {
         const int64_t k = (int64_t)new_off[i] + counter;
