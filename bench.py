@@ -41,6 +41,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _rho_rule(plain, learned):
+    """What the metric's 'rho-err <= 0.1 %' means and whether it holds."""
+    return {
+        "rule": "mean |rho_i/rho0 - 1| of the summation density after every timed step <= 1e-3 "
+                "(the solver stops on the reference's mean residual rule, v2/modules/dfsph.py:96)",
+        "met_plain": plain["rho_err_mean_worst_step"] <= 1e-3,
+        "met_learned": learned["rho_err_mean_worst_step"] <= 1e-3,
+        "max_note": "the largest |rho/rho0 - 1| is an expansion (rho < rho0): the clamped pressure (p >= 0, "
+                    "dfsph.py:46-47) cannot remove it and tighter thresholds leave it in place, so max <= 1e-3 "
+                    "is not met by either solver; the compression side is listed beside it",
+    }
+
+
 def _baseline_threads() -> int:
     """Threads of the CPU baseline: the host cores this process may use (its
     CPU affinity), capped by OMP_NUM_THREADS when set (the GPU box gives each
@@ -87,6 +100,8 @@ def parse():
     ap.add_argument("--train-rounds", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=48, help="trajectory steps per training round")
     ap.add_argument("--no-ablation", action="store_true", help="skip the p_prev (warm start 1.0) run")
+    ap.add_argument("--tight", action="store_true",
+                    help="also run plain / learned at a 5x tighter density-solve threshold (1e-4)")
     ap.add_argument("--neighbor-width", type=int, default=67,
                     help="row width of the one-pass neighbor list (0: count pass + two-pass list)")
     ap.add_argument("--nside", type=int, default=1024)
@@ -140,10 +155,11 @@ def main():
     dt = 3.0 * d["dx"] / 10.0
     T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
 
-    def make(seed, warm=0.75):
+    def make(seed, warm=0.75, threshold_den=5e-4):
         dd = d if seed == args.seed else sampling.dfsph_random_flow(args.nside, seed=seed, jitter_width=0.1)
         return DFSPHSolver(T(dd["pos"]), T(dd["vel"]), T(dd["mass"]), T(dd["area"]),
-                           DFSPHConfig(dt=dt, support=d["h"], warm_start=warm, neighbor_width=args.neighbor_width))
+                           DFSPHConfig(dt=dt, support=d["h"], warm_start=warm, neighbor_width=args.neighbor_width,
+                                       threshold_den=threshold_den))
 
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
     t0 = time.perf_counter()
@@ -162,23 +178,46 @@ def main():
                             mass=solver.mass.cpu().numpy(), area=solver.areas.cpu().numpy(),
                             uid=solver.uid.cpu().numpy(), p_div=solver.p_div.cpu().numpy(),
                             p_den=solver.p_den.cpu().numpy())
+        # summation-density error after every timed step: rho of step k+1's
+        # start is the density of the positions step k produced.  The three
+        # statistics (mean |e|, max e, min e with e = rho/rho0 - 1) are
+        # reduced on the device without a host read inside the timed loop.
+        ev = torch.zeros(args.steps, 3, dtype=torch.float32, device=dev)
+
+        def record(k):
+            e = solver.rho - 1.0
+            lo, hi = torch.aminmax(e)
+            ev[k, 0] = e.abs().mean()
+            ev[k, 1] = hi
+            ev[k, 2] = lo
+
         torch.cuda.synchronize()
         it_div, it_den, errs = [], [], []
         c0 = time.perf_counter()
-        for _ in range(args.steps):
+        for k in range(args.steps):
             st = solver.step()
+            if k > 0:
+                record(k - 1)
             it_div.append(st.div_iters)
             it_den.append(st.den_iters)
             errs.append(st.den_error)
         torch.cuda.synchronize()
         secs = time.perf_counter() - c0
-        me, mx = solver.density_error()
+        solver.step()  # untimed: the density of the last timed step's positions
+        record(args.steps - 1)
+        e = ev.cpu().numpy().astype(np.float64)
         r = {"value": n * args.steps / secs, "ms_per_step": secs * 1e3 / args.steps,
-             "iters_div_mean": float(np.mean(it_div)), "iters_den_mean": float(np.mean(it_den)),
-             "den_residual_mean": float(np.mean(errs)), "rho_err_mean": me, "rho_err_max": mx,
+             "iters_div_mean": float(np.mean(it_div)), "iters_div_max": int(np.max(it_div)),
+             "iters_den_mean": float(np.mean(it_den)), "iters_den_max": int(np.max(it_den)),
+             "den_residual_mean": float(np.mean(errs)),
+             "rho_err_mean": float(e[:, 0].mean()), "rho_err_mean_worst_step": float(e[:, 0].max()),
+             "rho_err_max": float(max(e[:, 1].max(), -e[:, 2].min())),
+             "rho_err_max_compression": float(e[:, 1].max()), "rho_err_max_expansion": float(-e[:, 2].min()),
              "first_step_iters_div": it_div[0]}
         log(f"{label}: {r['ms_per_step']:.3f} ms/step, {r['value']:.4g} particle-updates/s; iters div "
-            f"{r['iters_div_mean']:.1f} den {r['iters_den_mean']:.1f}; rho err mean {me:.2e} max {mx:.2e}")
+            f"{r['iters_div_mean']:.1f} den {r['iters_den_mean']:.1f}; rho err mean {r['rho_err_mean']:.2e} "
+            f"(worst step {r['rho_err_mean_worst_step']:.2e}), max compression {r['rho_err_max_compression']:.2e}, "
+            f"max expansion {r['rho_err_max_expansion']:.2e}")
         return r
 
     snap = {}
@@ -190,6 +229,15 @@ def main():
     log(f"judge: {net.judge.judged} judged steps, net used {net.judge.used_net}, plain used {net.judge.used_plain}")
     value = learned["value"]
     secs = learned["ms_per_step"] * args.steps * 1e-3
+    # the same comparison with a 5x tighter density-solve threshold
+    tight = None
+    if args.tight:
+        tp = run("plain (threshold_den 1e-4)", make(args.seed, threshold_den=1e-4))
+        tsolver = make(args.seed, threshold_den=1e-4)
+        net.install(tsolver)
+        tl = run("learned (threshold_den 1e-4)", tsolver)
+        tight = {"threshold_den": 1e-4, "plain": tp, "learned": tl,
+                 "speedup_vs_plain": round(tl["value"] / tp["value"], 3)}
 
     if args.copy_calibration:
         src = torch.empty(64 << 20, device=dev)
@@ -355,6 +403,10 @@ def main():
                    "dt": dt, "pairs": P, "neighbor_width": args.neighbor_width, "learned": rnd(learned), "plain": rnd(plain),
                    "p_prev_warm_start": rnd(prev),
                    "speedup_vs_plain": round(learned["value"] / plain["value"], 3),
+                   "rho_err_rule": _rho_rule(plain, learned),
+                   "tight_density_threshold": None if tight is None else {
+                       "threshold_den": tight["threshold_den"], "plain": rnd(tight["plain"]),
+                       "learned": rnd(tight["learned"]), "speedup_vs_plain": tight["speedup_vs_plain"]},
                    "train_seconds": round(t_train, 2),
                    "judge": {"judged": net.judge.judged, "net": net.judge.used_net,
                              "plain": net.judge.used_plain}},
@@ -493,7 +545,7 @@ def bench_ensemble(args):
         rate, threads, parts = _oracle_deltasph_rate(states, cfg, 1)
         cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"one RK4 step of the first {len(states)} ensemble domains ({parts} particles), "
-                         f"C++/OpenMP oracle, {threads} threads"}
+                         f"C++/OpenMP oracle, {threads} threads", **_host_info(threads)}
         log(f"cpu baseline: {rate:.4g} particle-updates/s")
     config = {"workload": f"ensemble_{len(ens.counts)}_periodic_deltasph", "domains": len(ens.counts),
               "particles": n, "min_domain": min(ens.counts), "max_domain": max(ens.counts),
@@ -542,7 +594,7 @@ def bench_tgv(args):
               mass.cpu().numpy())], cfg, max(1, min(args.steps, 40)))
         cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"{parts // n} RK4 steps of the {n}-particle Taylor-Green state, C++/OpenMP oracle, "
-                         f"{threads} threads"}
+                         f"{threads} threads", **_host_info(threads)}
         log(f"cpu baseline: {rate:.4g} particle-updates/s")
     config = {"workload": f"deltasph_taylor_green_{args.nside}x{args.nside}", "particles": n,
               "steps_per_graph_launch": args.steps, "simulated_time": t, "energy_ratio": ratio,

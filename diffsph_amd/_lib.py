@@ -45,6 +45,22 @@ class Pairs(C.Structure):
     ]
 
 
+class PairArrays(C.Structure):
+    _fields_ = [
+        ("num_rows", C.c_int64),
+        ("num_ref", C.c_int64),
+        ("num_pairs", C.c_int64),
+        ("offsets", C.c_void_p),
+        ("j", C.c_void_p),
+        ("kernels", C.c_void_p),
+        ("gradients", C.c_void_p),
+        ("distances", C.c_void_p),
+        ("vectors", C.c_void_p),
+        ("supports", C.c_void_p),
+        ("laplacians", C.c_void_p),
+    ]
+
+
 class DfsphParams(C.Structure):
     _fields_ = [
         ("dt", C.c_float),
@@ -277,6 +293,18 @@ SIGNATURES = {
                                          V, V, V, V]),
     "dsph_verlet_check_workspace": (SZ, []),
     "dsph_verlet_check": (C.c_int, [V, V, I64, V, I64, P(C.c_float), V, SZ, V]),
+    "dsph_pp_offsets": (C.c_int, [V, I64, I32, V, V, V]),
+    "dsph_pp_operation": (C.c_int, [P(PairArrays), I32, I32, I32, V, V, V, V, V, V, V, I32, V, V]),
+    "dsph_pp_dfsph_alpha": (C.c_int, [P(PairArrays), V, V, V, V, F32, F32, V, V]),
+    "dsph_pp_dfsph_update_workspace": (SZ, [I64]),
+    "dsph_pp_dfsph_update": (C.c_int, [P(PairArrays), V, V, V, V, V, V, V, F32, F32, I32, F32, F32, V, V, V, V, SZ,
+                                       V]),
+    "dsph_ips_assemble": (C.c_int, [P(Pairs), V, V, V, I64, V]),
+    "dsph_ips_spmv": (C.c_int, [P(Pairs), V, I64, V, V, V]),
+    "dsph_vec_lincomb": (C.c_int, [I64, V, F32, V, F32, V, V, V]),
+    "dsph_vec_reduce_workspace": (SZ, [I64]),
+    "dsph_vec_dot": (C.c_int, [I64, V, V, V, V, SZ, V]),
+    "dsph_vec_max_norm2": (C.c_int, [I64, V, V, V, SZ, V]),
     "dsph_hash_count_neighbors": (C.c_int, [V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V, I32, V,
                                             V]),
     "dsph_hash_build_neighbors": (C.c_int, [V, I64, V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V,

@@ -244,6 +244,30 @@ class DFSPHSolver:
 
     # -- one step --------------------------------------------------------
     def step(self) -> StepStats:
+        """One DFSPH step: search, density, solves, semi-implicit Euler."""
+        out = self._forces()
+        cfg = self.cfg
+        g = None
+        if cfg.gravity is not None:
+            g = (C.c_float * 2)(*[float(v) for v in cfg.gravity])  # host pointer
+        check(lib().dsph_integrate_semi_implicit(ptr(self.pos), ptr(self.vel), ptr(self.a_den), ptr(self.visc),
+                                                 ptr(self.a_div), None, g, self.n, C.c_float(cfg.dt),
+                                                 ptr(self.dudt), _lib.stream_handle(self.pos.device)), "integrate")
+        self.step_count += 1
+        return out
+
+    def derivatives(self) -> StepStats:
+        """The step without its integration: fills every per-step field and
+        dudt = a_p + g + visc + a_div (v2/simulationSchemes/dfsph.py:209), in
+        the state's (cell) order; positions and velocities are unchanged."""
+        out = self._forces()
+        d = self.a_den
+        if self.cfg.gravity is not None:
+            d = d + torch.tensor(self.cfg.gravity, dtype=torch.float32, device=d.device)
+        torch.add(torch.add(d, self.visc), self.a_div, out=self.dudt)
+        return out
+
+    def _forces(self) -> StepStats:
         cfg = self.cfg
         L = lib()
         st = _lib.stream_handle(self.pos.device)
@@ -307,13 +331,6 @@ class DFSPHSolver:
         out.den_iters, out.den_error, out.den_error0 = s.iterations, s.final_error, s.initial_error
         self._last_iters[True] = s.iterations
         self.p_den = p_den
-        g = None
-        if cfg.gravity is not None:
-            g = (C.c_float * 2)(*[float(v) for v in cfg.gravity])  # host pointer
-        check(L.dsph_integrate_semi_implicit(ptr(self.pos), ptr(self.vel), ptr(self.a_den), ptr(self.visc),
-                                             ptr(self.a_div), None, g, self.n, C.c_float(cfg.dt),
-                                             ptr(self.dudt), st), "integrate")
-        self.step_count += 1
         return out
 
     def density_error(self):
@@ -328,3 +345,92 @@ class DFSPHSolver:
         """Positions, velocities and densities in input (UID) order."""
         order = torch.argsort(self.uid.long())
         return self.pos[order], self.vel[order], self.rho[order]
+
+
+# ---------------------------------------------------------------------------
+# dfsphSolve on a reference neighborDict (v2/modules/dfsph.py:52-109)
+# ---------------------------------------------------------------------------
+
+def dfsphSolve(stateA, stateB, neighborhood, config):
+    """Drop-in for the reference's ``dfsphSolve(stateA, stateB, neighborhood,
+    config)`` (v2/modules/dfsph.py:52-109): the same state keys, config keys,
+    warm start (0.75 of ``pressureIncompressible`` / ``pressureDivergence``),
+    stopping rule, clamping and side effects (``predictedVelocities``,
+    ``actualArea``, ``sourceTerm``, ``alpha``, ``pressureA``, ``pressureB``,
+    ``pressureAccel``, ``residual``, ``convergence_<sourceTerm>``); returns
+    the pressure acceleration.  It runs on the neighborDict's own per-pair
+    gradients: alpha, the source, the pressure acceleration and each Jacobi
+    update are HIP kernels over the pair arrays, and every iteration reads
+    the float64 convergence metric back, as the reference's ``.item()`` does
+    (the fused solver of :class:`DFSPHSolver` keeps the loop on the
+    device)."""
+    from .sphops import _pair_arrays, sphOperationStates
+    dfc = config["dfsph"]
+    solve_density = dfc["sourceTerm"] == "density"
+    dt = float(config["timestep"]["dt"])
+    rho0 = float(config["fluid"]["rho0"])
+    dt_pow = dt * dt if solve_density else dt
+    n = int(stateA["numParticles"])
+    nb = int(stateB["numParticles"])
+    vel = stateA["velocities"]
+    stateA["predictedAcceleration"] = torch.zeros_like(vel)
+    stateA["predictedVelocities"] = vel + dt * stateA["advection"]
+    stateA["actualArea"] = stateA["masses"] / stateA["densities"]
+    div = sphOperationStates(stateA, stateB, (stateA["predictedVelocities"], stateB["predictedVelocities"]),
+                             neighborhood, operation="divergence", gradientMode="difference")
+    if solve_density:
+        stateA["sourceTerm"] = 1 - stateA["areas"] / stateA["actualArea"] - dt * div
+    else:
+        stateA["sourceTerm"] = div
+    dev = vel.device
+    stateA["pressureB"] = torch.zeros(n, device=dev)
+    stateA["pressureA"] = torch.zeros(n, device=dev)
+    if "pressureIncompressible" in stateA and solve_density:
+        stateA["pressureB"] = 0.75 * stateA["pressureIncompressible"]
+        stateA["pressureA"] = 0.75 * stateA["pressureIncompressible"]
+    elif "pressureDivergence" in stateA and not solve_density:
+        stateA["pressureB"] = 0.75 * stateA["pressureDivergence"]
+        stateA["pressureA"] = 0.75 * stateA["pressureDivergence"]
+    pa, keep = _pair_arrays(neighborhood["indices"], n, nb, gradients=neighborhood["gradients"])
+    L = lib()
+    st = _lib.stream_handle(dev)
+    alpha = torch.empty(n, device=dev)
+    f32 = lambda t: t.contiguous().to(torch.float32)  # noqa: E731
+    actA, actB = f32(stateA["actualArea"]), f32(stateB["actualArea"])
+    check(L.dsph_pp_dfsph_alpha(C.byref(pa), ptr(actA), ptr(actB), ptr(f32(stateA["areas"])),
+                                ptr(f32(stateB["areas"])), C.c_float(dt_pow), C.c_float(rho0), ptr(alpha), st),
+          "dfsph alpha")
+    stateA["alpha"] = alpha
+    min_iters, max_iters = int(dfc["minIters"]), int(dfc["maxIters"])
+    thr = float(dfc["errorThreshold"])
+    omega = float(dfc["omega"])
+    clampp = 1 if dfc["clampPressure"] else 0
+    ws = WORKSPACE.get(L.dsph_pp_dfsph_update_workspace(n), dev)
+    err_dev = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    def pressure_accel():
+        g = sphOperationStates(stateA, stateB, (stateA["pressureB"], stateB["pressureB"]), neighborhood,
+                               operation="gradient", gradientMode="summation")
+        return -g / stateA["densities"].view(-1, 1)
+
+    errors = []
+    it = 0
+    error = 0.0
+    while it < max_iters and (it < min_iters or error > thr):
+        stateA["pressureAccel"] = pressure_accel()
+        stateA["pressureA"] = stateA["pressureB"].clone()
+        p_new = torch.empty(n, device=dev)
+        res = torch.empty(n, device=dev)
+        check(L.dsph_pp_dfsph_update(C.byref(pa), ptr(f32(stateB["masses"])), ptr(f32(stateB["densities"])),
+                                     ptr(f32(stateA["pressureAccel"])), ptr(f32(stateB["pressureAccel"])),
+                                     ptr(f32(stateA["sourceTerm"])), ptr(alpha), ptr(f32(stateA["pressureA"])),
+                                     C.c_float(dt_pow), C.c_float(omega), clampp, C.c_float(rho0),
+                                     C.c_float(thr), ptr(p_new), ptr(res), ptr(err_dev), ptr(ws), ws.numel(), st),
+              "dfsph update")
+        stateA["pressureB"], stateA["residual"] = p_new, res
+        error = float(err_dev.item())
+        errors.append(error)
+        it += 1
+    a_p = pressure_accel()
+    stateA[f"convergence_{dfc['sourceTerm']}"] = errors
+    return a_p

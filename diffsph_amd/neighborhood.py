@@ -527,3 +527,29 @@ def neighborSearch(queryState, referenceState, config, computeKernels=True, prio
         nd["kernels"] = fr["kernels"]
         nd["gradients"] = fr["gradients"]
     return neighborDatastructure, nd
+
+
+def searchNeighbors(state, config):
+    """v2/modules/neighborhood.py:352-399: the fluid-fluid search (with the
+    Verlet reuse of the prior neighborhood and data structure) into
+    state['fluid']['datastructure' / 'neighborhood' / 'numNeighbors'], and
+    with ``config['boundary']['active']`` the boundary-boundary,
+    boundary->fluid and fluid->boundary searches into the same keys as the
+    reference."""
+    f = state["fluid"]
+    f["datastructure"], f["neighborhood"] = neighborSearch(
+        f, f, config, computeKernels=True, priorState=f.get("neighborhood"),
+        neighborDatastructure=f.get("datastructure"), verbose=False)
+    f["numNeighbors"] = f["neighborhood"]["numNeighbors"]
+    if config["boundary"]["active"]:
+        b = state["boundary"]
+        b["datastructure"], b["neighborhood"] = neighborSearch(
+            b, b, config, computeKernels=True, priorState=b.get("neighborhood"),
+            neighborDatastructure=b.get("datastructure"), verbose=False)
+        b["numNeighbors"] = b["neighborhood"]["numNeighbors"]
+        _, state["fluidToBoundaryNeighborhood"] = neighborSearch(
+            b, f, config, True, state.get("fluidToBoundaryNeighborhood"), f["datastructure"], False)
+        _, state["boundaryToFluidNeighborhood"] = neighborSearch(
+            f, b, config, True, state.get("boundaryToFluidNeighborhood"), b["datastructure"], False)
+        state["boundaryToFluidNumNeighbors"] = state["boundaryToFluidNeighborhood"]["numNeighbors"]
+        state["fluidToBoundaryNumNeighbors"] = state["fluidToBoundaryNeighborhood"]["numNeighbors"]

@@ -440,6 +440,114 @@ int dsph_integrate_semi_implicit(float* pos, float* vel, const float* a0,
                                  void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Generic SPH operation and DFSPH module pieces on a reference            */
+/* neighborDict's per-pair arrays (v2/sphOps.py:253-303,                   */
+/* v2/modules/dfsph.py:7-109)                                              */
+/* ---------------------------------------------------------------------- */
+
+/* A row-sorted pair list as the reference's neighborDict holds it:
+ * pairs k of row r are offsets[r] .. offsets[r+1]-1; j[k] is the
+ * neighbor's index in set B (int64, 'indices'[1]); per-pair arrays
+ * 'kernels' W_ij [P], 'gradients' [P,2], 'distances' r_ij/h_ij [P],
+ * 'vectors' x_ij [P,2], 'supports' h_ij [P], optional 'laplacians' [P]
+ * (NULL where an operation does not read them). */
+typedef struct dsph_pair_arrays {
+  int64_t num_rows;         /* particles of set A                  */
+  int64_t num_ref;          /* particles of set B                  */
+  int64_t num_pairs;
+  const int32_t* offsets;   /* [num_rows+1]                        */
+  const int64_t* j;         /* [num_pairs]                         */
+  const float* kernels;
+  const float* gradients;
+  const float* distances;
+  const float* vectors;
+  const float* supports;
+  const float* laplacians;
+} dsph_pair_arrays;
+
+/* Row offsets of a pair list whose row indices `rows` (the reference's
+ * indices[0], int64) are sorted ascending, as every reference neighbor
+ * search returns them.  *unsorted (device int32) is cleared, then set when
+ * a row index decreases or lies outside [0, num_rows); the offsets are then
+ * not meaningful. */
+int dsph_pp_offsets(const int64_t* rows, int64_t num_pairs, int32_t num_rows,
+                    int32_t* offsets, int32_t* unsorted, void* stream);
+
+/* sphOperation (v2/sphOps.py:253-282) on per-pair arrays.
+ *   op:   0 density (sphDensityInterpolation, :19-29), 1 interpolate (:6-17),
+ *         2 gradient (:32-62), 3 divergence (:65-106),
+ *         4 divergenceConsistent (:108-149), 5 curl (:152-188),
+ *         6 laplacian (:191-250)
+ *   mode: gradientMode 0 naive, 1 difference, 2 summation, 3 symmetric;
+ *         for the Laplacian also 4 conserving, 5 divergenceFree, 6 dot
+ *         (anything else takes the Laplacian's default branch)
+ *   dot_mode: divergenceMode 'dot' (contract the first axis of a 2x2
+ *         tensor field) instead of 'div'.
+ * The quantity is per particle (q_a [num_rows][q_width] for i, q_b
+ * [num_ref][q_width] for j: the reference's (q_i, q_j) tuple; q_a may be
+ * NULL where the mode reads q_j only) or per pair (q_pairs [P][q_width]:
+ * the reference's single tensor).  q_width 1, 2 or 4 (a 2x2 tensor).
+ * Output per row: density/curl 1, interpolate q_width, gradient
+ * 2 q_width, divergence q_width/2, Laplacian q_width values. */
+int dsph_pp_operation(const dsph_pair_arrays* pairs, int32_t op, int32_t mode,
+                      int32_t dot_mode, const float* masses_a,
+                      const float* masses_b, const float* densities_a,
+                      const float* densities_b, const float* q_a,
+                      const float* q_b, const float* q_pairs, int32_t q_width,
+                      float* out, void* stream);
+
+/* computeAlpha (v2/modules/dfsph.py:7-27): actual areas A = m / rho of
+ * both sets, rest areas, dt_pow = dt^2 (density solve) or dt
+ * (divergence solve). */
+int dsph_pp_dfsph_alpha(const dsph_pair_arrays* pairs,
+                        const float* actual_area_a, const float* actual_area_b,
+                        const float* areas_a, const float* areas_b,
+                        float dt_pow, float rho0, float* alpha, void* stream);
+
+/* One updatePressure (v2/modules/dfsph.py:39-50) from the pressure
+ * accelerations of both sets, with the convergence metric of :96 written to
+ * *error_device (device float64: mean over rows of max(residual/rho0,
+ * -threshold), summed in a fixed order). */
+size_t dsph_pp_dfsph_update_workspace(int64_t num_rows);
+int dsph_pp_dfsph_update(const dsph_pair_arrays* pairs, const float* masses_b,
+                         const float* densities_b, const float* accel_a,
+                         const float* accel_b, const float* source,
+                         const float* alpha, const float* pressure_in,
+                         float dt_pow, float omega, int32_t clamp_pressure,
+                         float rho0, float threshold, float* pressure_out,
+                         float* residual, double* error_device,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Implicit particle shifting (computeShifting / solveShifting with scheme */
+/* 'IPS', v2/modules/shifting.py:192-284, 318-460, 463-600)                */
+/* ---------------------------------------------------------------------- */
+
+/* The IPS system on a same-set neighbor list: rhs[n][2] (interleaved) =
+ * sum_j A_j gradW_ij and, per list entry k (in the list's own layout, up
+ * to hessian_entries), hessian[k] = A_j H_ij as (h00, h01, h11, 0) with the
+ * Hessian2D of src/diffSPH/kernels.py:263-303 (Wendland C2).  The pair
+ * geometry uses the reference's quotients. */
+int dsph_ips_assemble(const dsph_pairs* pairs, const float* areas, float* rhs,
+                      float* hessian, int64_t hessian_entries, void* stream);
+/* y[i] = sum over row i of the entry blocks times x[j] (2-vectors per
+ * particle, interleaved), the matvec of bicgstab_shifting. */
+int dsph_ips_spmv(const dsph_pairs* pairs, const float* hessian,
+                  int64_t hessian_entries, const float* x, float* y,
+                  void* stream);
+/* out = x + a y + b z over n floats (y, z may be NULL), float32 in that
+ * order. */
+int dsph_vec_lincomb(int64_t n, const float* x, float a, const float* y,
+                     float b, const float* z, float* out, void* stream);
+/* *out_device = sum a_k b_k (float64, fixed-order reduction); the largest
+ * per-particle norm |(x_2i, x_2i+1)| of n particles. */
+size_t dsph_vec_reduce_workspace(int64_t n);
+int dsph_vec_dot(int64_t n, const float* a, const float* b, double* out_device,
+                 void* workspace, size_t workspace_bytes, void* stream);
+int dsph_vec_max_norm2(int64_t n, const float* x, double* out_device,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Learned warm start of the DFSPH divergence solve (no counterpart in the  */
 /* reference: its dfsphSolve warm-starts with 0.75 p_prev,                  */
 /* v2/modules/dfsph.py:70-75)                                               */

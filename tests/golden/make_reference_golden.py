@@ -314,6 +314,32 @@ def sph_operations_and_shifting():
         out[f"grad_{gm}_vector"] = op(V, "gradient", gm)
         out[f"div_{gm}"] = op(V, "divergence", gm)
         out[f"divc_{gm}"] = op(V, "divergenceConsistent", gm)
+    for gm in ("naive", "difference", "summation", "symmetric"):
+        out[f"curl_{gm}"] = op(V, "curl", gm)
+    # 2x2 tensor field: its gradient and both divergence contractions
+    Tq = torch.einsum("ni,nj->nij", T(qv), T(qv)) + 0.1 * torch.eye(2)[None]
+    TT = (Tq, Tq)
+    out["qt"] = Tq
+    out["grad_difference_tensor"] = op(TT, "gradient", "difference")
+    out["div_difference_tensor_div"] = op(TT, "divergence", "difference", "div")
+    out["div_difference_tensor_dot"] = op(TT, "divergence", "difference", "dot")
+    # Laplacian branches (v2/sphOps.py:191-250)
+    lapk = M["W2"].kernelLaplacian(rij, hij, 2)
+    out["laplacian_kernels"] = lapk
+
+    def lap(q, gm, lk=None):
+        return ops.sphOperation((m_, m_), (r_, r_), q, (i, j), W, gW, rij, xij, hij, n, operation="laplacian",
+                                gradientMode=gm, kernelLaplacians=lk)
+
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):  # the naive branch prints
+        out["lap_naive_scalar"] = lap(S, "naive", lapk)
+        out["lap_naive_vector"] = lap(V, "naive", lapk)
+    for gm in ("difference", "conserving", "divergenceFree", "dot"):
+        out[f"lap_{gm}_scalar"] = lap(S, gm)
+        out[f"lap_{gm}_vector"] = lap(V, gm)
+    out["direct_laplacian_difference"] = op(S, "directLaplacian", "difference")
     st = {"masses": m_, "densities": r_, "numParticles": n}
     nd = {"indices": (i, j), "kernels": W, "gradients": gW, "distances": rij, "vectors": xij, "supports": hij}
     out["states_density"] = ops.sphOperationStates(st, st, None, nd, operation="density")
@@ -334,7 +360,94 @@ def sph_operations_and_shifting():
     print("wrote", OUT_OPS, {k: v.shape for k, v in np_out.items()})
 
 
+OUT_IPS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_ips_16.npz")
+
+
+def _shifting_functions(M):
+    """The IPS functions of v2/modules/shifting.py executed from their source
+    lines (the module imports the unavailable torchCompactRadius and
+    surfaceDetection at top level)."""
+    sparse = importlib.import_module("diffSPH.v2.sparse")
+    src_path = os.path.join(REF, "diffSPH", "v2", "modules", "shifting.py")
+    src = open(src_path).read().splitlines()
+    ns_ = {"torch": torch, "scatter_sum": M["math"].scatter_sum, "record_function": torch.profiler.record_function,
+           "matvec_sparse_coo": sparse.matvec_sparse_coo, "make_id": sparse.make_id,
+           "_get_tensor_eps": sparse._get_tensor_eps, "_get_atol_rtol": sparse._get_atol_rtol,
+           "Optional": __import__("typing").Optional, "Tuple": __import__("typing").Tuple,
+           "Callable": __import__("typing").Callable}
+    for name in ("evalKernel", "deltaPlusShifting", "bicgstab_shifting", "getShiftingMatrices",
+                 "multiplySparseShifting", "computeShifting"):
+        a = next(k for k, l in enumerate(src) if l.startswith(f"def {name}("))
+        b = next((k for k in range(a + 1, len(src)) if src[k] and not src[k][0].isspace()
+                  and not src[k].startswith("#")), len(src))
+        exec(compile("\n".join(src[a:b]), src_path, "exec"), ns_)
+    return ns_
+
+
+def ips_shifting():
+    """Implicit particle shifting (computeShifting with scheme IPS,
+    v2/modules/shifting.py:192-284, 361-460; the BiCGStab of :318-...) on a
+    jittered periodic 16x16 set: the kernel Hessians, the assembled J and B,
+    and the shift for the reference's default solver threshold (-0.5: one
+    BiCGStab iteration) and for a positive one (the solve runs to rtol)."""
+    sys.path.insert(0, REF)
+    M = load()
+    kern = importlib.import_module("diffSPH.kernels")
+    fns = _shifting_functions(M)
+    rng = np.random.default_rng(31)
+    ns = 16
+    dx = 2.0 / ns
+    xs = np.linspace(-1 + dx / 2, 1 - dx / 2, ns, dtype=np.float32)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+    pos = (pos + rng.uniform(-0.2 * dx, 0.2 * dx, pos.shape)).astype(np.float32)
+    n = pos.shape[0]
+    h = np.float32(np.sqrt(50.0 * dx * dx / np.pi))
+    T = torch.from_numpy
+    p = T(pos)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i_all, j_all = T(ii.ravel()), T(jj.ravel())
+    xij = p[i_all] - p[j_all]
+    xij = torch.stack([M["math"].mod(xij[:, k], -1.0, 1.0) for k in range(2)], -1)
+    rij = torch.linalg.norm(xij, dim=-1)
+    xij = xij / (rij + 1e-7).view(-1, 1)
+    hv = torch.full((n,), float(h))
+    rij = rij / hv[j_all]
+    keep = rij <= 1.0
+    i, j = i_all[keep], j_all[keep]
+    rij, xij, hij = rij[keep], xij[keep], hv[j]
+    W = M["W2"].kernel(rij, hij, 2)
+    gW = M["W2"].kernelGradient(rij, xij, hij, 2)
+    area = torch.full((n,), dx * dx)
+    out = {}
+    wrapper = kern.KernelWrapper(M["W2"])
+    K, J, H = fns["evalKernel"](rij, xij, hij, wrapper, 2)
+    out["hessian"] = H
+    for label, thr in (("default", -0.5), ("solve", 1.0)):
+        cfg = {"shifting": {"freeSurface": False, "summationDensity": False, "initialization": "zero",
+                            "tol": 1e-4, "rtol": 1e-4, "maxSolveIter": 64, "preconditioner": "Jacobi",
+                            "verbose": False, "solverThreshold": thr, "solver": "BiCGStab_wJacobi",
+                            "R": 0.25, "n": 4, "CFL": 1.5, "computeMach": True},
+               "kernel": {"function": wrapper, "kernelScale": float(h / (2 * dx))}, "domain": {"dim": 2},
+               "particle": {"dx": dx, "support": float(h)}, "fluid": {"rho0": 1.0, "cs": 10.0}}
+        ps = {"numParticles": n, "positions": p, "areas": area, "masses": area * 1.0,
+              "densities": torch.ones(n), "supports": hv, "boundaryMarker": torch.zeros(n, dtype=torch.int32),
+              "neighborhood": {"indices": (i, j), "distances": rij, "vectors": xij, "supports": hij, "kernels": W,
+                               "gradients": gW}}
+        update, K_, J_, H_, B, conv, iters, residual = fns["computeShifting"](ps, cfg, computeRho=False)
+        out[f"{label}_update"] = update
+        out[f"{label}_J"] = J_
+        out[f"{label}_B"] = B
+        out[f"{label}_iters"] = torch.tensor(float(iters))
+        out[f"{label}_residual"] = torch.tensor(float(residual))
+    np_out = {k: v.detach().numpy().astype(np.float32) for k, v in out.items()}
+    np.savez(OUT_IPS, pos=pos, h=np.float32(h), dx=np.float32(dx), i=i.numpy().astype(np.int32),
+             j=j.numpy().astype(np.int32), **np_out)
+    print("wrote", OUT_IPS, {k: (v.shape, v.ravel()[:1]) for k, v in np_out.items()})
+
+
 if __name__ == "__main__":
     main()
     rollout_grad()
     sph_operations_and_shifting()
+    ips_shifting()
