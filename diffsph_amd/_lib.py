@@ -305,6 +305,10 @@ SIGNATURES = {
     "dsph_vec_reduce_workspace": (SZ, [I64]),
     "dsph_vec_dot": (C.c_int, [I64, V, V, V, V, SZ, V]),
     "dsph_vec_max_norm2": (C.c_int, [I64, V, V, V, SZ, V]),
+    "dsph_noalloc_scratch_bytes": (SZ, []),
+    "dsph_dfsph_solve_noalloc_max_particles": (I64, []),
+    "dsph_dfsph_solve_noalloc": (C.c_int, [P(Pairs), P(DfsphParams), V, V, V, V, V, V, V, V, V, V, V, P(SolveStats),
+                                           V]),
     "dsph_hash_count_neighbors": (C.c_int, [V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V, I32, V,
                                             V]),
     "dsph_hash_build_neighbors": (C.c_int, [V, I64, V, V, I32, I32, V, V, I32, V, I32, V, V, I64, F32, V, V, V,

@@ -375,6 +375,27 @@ int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params* prm,
                      float* residual, dsph_solve_stats* stats,
                      void* workspace, size_t workspace_bytes, void* stream);
 
+/* dsph_dfsph_solve without a workspace argument and without any device
+ * allocation.  No counterpart in the reference: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The scratch is one static device array of
+ * dsph_noalloc_scratch_bytes() (64 MiB) per GPU, held until the process
+ * ends; it fits solves of up to dsph_dfsph_solve_noalloc_max_particles()
+ * particles (1,676,400), larger ones return DSPH_ERR_CAPACITY before any
+ * launch.  Every call on a GPU shares that GPU's array: calls that overlap
+ * on one GPU (two streams or host threads) race.  Results equal
+ * dsph_dfsph_solve's bit for bit. */
+size_t dsph_noalloc_scratch_bytes(void);
+int64_t dsph_dfsph_solve_noalloc_max_particles(void);
+int dsph_dfsph_solve_noalloc(const dsph_pairs* pairs,
+                             const dsph_dfsph_params* prm, const float* kgrad,
+                             const float* vel, const float* adv,
+                             const float* rho, const float* vol,
+                             const float* areas, const float* alpha,
+                             const float* src0, float* p, float* accel,
+                             float* residual, dsph_solve_stats* stats,
+                             void* stream);
+
 /* One pressure update (updatePressure, v2/modules/dfsph.py:39-50):
  * K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij with dtf = dt^2 for the
  * density solve (prm->source_density) else dt, p_i += omega (s_i - K_i) /
