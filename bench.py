@@ -292,29 +292,35 @@ def main():
     e1.synchronize()
     t_b = e0.elapsed_time(e1) / reps * 1e-3
     P = solver.nl.num_pairs
-    # Algorithmic bytes (each per-particle array once per launch, each pair
-    # entry once; gathered neighbor fields and list padding not counted):
-    # sweep A: (x,y,p) record 16 + own x 8 + rho 4 + count 4 + (x,y,a) record(write) 16 = 48 B/particle;
-    #          j (2 as a 16-bit delta, else 4) + s_ij 4 = 6 (8) B/pair
-    # sweep B: (x,y,a) 16 + own x 8 + p 4(r)+4(w) + s 4 + alpha 4 + count 4 + residual 4(w)
-    #          + (x,y,p)(write) 16 = 64 B/particle; 6 (8) B/pair
-    # (the coefficient V_j gradW_ij is d_ij * s_ij, d_ij from the gathered records)
+    # Algorithmic bytes, the survey's model: SoA fp32 arrays once per launch,
+    # an int32 CSR index once per entry, gathered neighbor fields not counted,
+    # nothing else per pair (the coefficient is assumed recomputed):
+    #   pass 1 (a = -grad p / rho): x,y 8 + p 4 + rho 4 + offset 4 + a 8 (w) = 28 B/particle + 4 B/entry
+    #   pass 2 (K, p update, residual partial): x,y 8 + a 8 + p,rho,alpha,s 16 + offset 4 + p 4 (w)
+    #                                          = 40 B/particle + 4 B/entry
+    model_a = 28 * n + 4 * P
+    model_b = 40 * n + 4 * P
+    # The bytes this layout moves (same counting rules): sweep A (x,y,p) record 16 + own x 8 + rho 4 +
+    # count 4 + (x,y,a) record (w) 16 = 48 B/particle; sweep B (x,y,a) 16 + own x 8 + p 4 (r) + 4 (w) +
+    # s 4 + alpha 4 + count 4 + residual 4 (w) + (x,y,p) (w) 16 = 64 B/particle; per entry the index
+    # (2 as a 16-bit delta, else 4) and the stored coefficient s_ij 4.
     idx16 = solver.nl.nbr16 is not None and int(solver.nl.nbr16_overflow.item()) == 0
     per_pair = 6 if idx16 else 8
     bytes_a = 48 * n + per_pair * P
     bytes_b = 64 * n + per_pair * P
-    gbs_a = bytes_a / t_a / 1e9
-    gbs_b = bytes_b / t_b / 1e9
-    log(f"sweep A (dsph_pressure_accel_rec): {t_a * 1e6:.1f} us, {bytes_a / 1e6:.1f} MB -> {gbs_a:.0f} GB/s; "
-        f"sweep B (dsph_pressure_update_rec): {t_b * 1e6:.1f} us, {bytes_b / 1e6:.1f} MB -> {gbs_b:.0f} GB/s; "
+    log(f"sweep A (dsph_pressure_accel_rec): {t_a * 1e6:.1f} us, model {model_a / 1e6:.1f} MB -> "
+        f"{model_a / t_a / 1e9:.0f} GB/s (layout {bytes_a / 1e6:.1f} MB -> {bytes_a / t_a / 1e9:.0f} GB/s); "
+        f"sweep B (dsph_pressure_update_rec): {t_b * 1e6:.1f} us, model {model_b / 1e6:.1f} MB -> "
+        f"{model_b / t_b / 1e9:.0f} GB/s (layout {bytes_b / 1e6:.1f} MB -> {bytes_b / t_b / 1e9:.0f} GB/s); "
         f"pairs {P} ({P / n:.1f}/particle)")
     if t_b >= t_a:
-        kname, ach, tb, byt = "k_dfsph_sweep_b", gbs_b, t_b, bytes_b
+        kname, tb, mbyt, byt = "k_dfsph_sweep_b", t_b, model_b, bytes_b
     else:
-        kname, ach, tb, byt = "k_dfsph_sweep_a", gbs_a, t_a, bytes_a
+        kname, tb, mbyt, byt = "k_dfsph_sweep_a", t_a, model_a, bytes_a
+    ach = mbyt / tb / 1e9
     # HBM bytes per launch from the committed PMC profile of the same kernel
     # (FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy): its ratio to the
-    # algorithmic bytes of the profiled launch, applied to this launch.
+    # layout bytes of the profiled launch, applied to this launch.
     traffic, ratio = None, None
     pmc_path = os.path.join(ROOT, "profiles", "dfsph_sweeps_pmc.json")
     if os.path.exists(pmc_path):
@@ -325,14 +331,18 @@ def main():
         if prof_bytes and prof_alg:
             ratio = prof_bytes / prof_alg
             traffic = int(round(ratio * byt))
+    other = "k_dfsph_sweep_a" if kname == "k_dfsph_sweep_b" else "k_dfsph_sweep_b"
+    t_o, m_o = (t_a, model_a) if kname == "k_dfsph_sweep_b" else (t_b, model_b)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(byt),
-                "traffic_over_algorithmic": None if ratio is None else round(ratio, 4),
+                "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(mbyt),
+                "algorithmic_model": "28/40 B per particle + 4 B per entry (passes 1/2, int32 CSR index)",
+                "layout_bytes": int(byt), "layout_achieved": round(byt / tb / 1e9, 1),
+                "layout_frac": round(byt / tb / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic_over_layout": None if ratio is None else round(ratio, 4),
                 "neighbor_index": "int16 delta" if idx16 else "int32",
-                "other_sweep": {"kernel": "k_dfsph_sweep_a" if kname == "k_dfsph_sweep_b" else "k_dfsph_sweep_b",
-                                "achieved": round(gbs_a if kname == "k_dfsph_sweep_b" else gbs_b, 1),
-                                "kernel_us": round((t_a if kname == "k_dfsph_sweep_b" else t_b) * 1e6, 2)}}
+                "other_sweep": {"kernel": other, "achieved": round(m_o / t_o / 1e9, 1),
+                                "kernel_us": round(t_o * 1e6, 2)}}
 
     # ---- CPU baseline: the oracle on the host cores, continuing the plain
     # solver's trajectory from the state at the start of its timed window
@@ -445,7 +455,7 @@ def _batch_roofline(b, pairs_per_particle, reps):
     on the lists the last probe left) and their algorithmic bytes:
       rows (neighbor fill): query pos 8 + domain 4 + count write 4 B/particle, entry write 4 B/pair
       normalization:        pos 8 + mass 4 + rho 4 + count 4 + domain 4 + grad write 8 B/particle,
-                            entries read twice 8 B/pair
+                            entry 4 B/pair (one pass over the row)
       right-hand side:      pos 8 + vel 8 + rho 4 + mass 4 + grad 8 + count 4 + domain 4
                             + dudt 8 + drho/dt 4 B/particle, entry 4 B/pair
     (gathered neighbor fields are counted once per particle, not per pair)."""
@@ -457,7 +467,7 @@ def _batch_roofline(b, pairs_per_particle, reps):
     n = b.n
     P = pairs_per_particle * n
     out = {}
-    for pas, name, per_p, per_pair in ((0, "k_bfill", 16, 4), (1, "k_dsph_normalization", 32, 8),
+    for pas, name, per_p, per_pair in ((0, "k_bfill", 16, 4), (1, "k_dsph_normalization", 32, 4),
                                        (2, "k_dsph_rhs", 52, 4)):
         def fn(pas=pas):
             _lib.check(L.dsph_batch_replay_pass(C.byref(b.batch), C.byref(b.prm), pas, sh), "replay_pass")
@@ -470,6 +480,63 @@ def _batch_roofline(b, pairs_per_particle, reps):
     return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "kernel_us": r["us"],
             "algorithmic_bytes": r["bytes"], "passes": out}
+
+
+def _solver_roofline(s, reps, vjp=False):
+    """HIP-event timing of the single-domain delta-SPH solver's pair passes
+    on the neighbor lists of its current state, and their algorithmic bytes
+    (SoA fp32 arrays once per launch, 4 B per list entry, gathered neighbor
+    fields not counted):
+      k_dsph_normalization: pos 8 + mass 4 + rho 4 + offset 4 + grad write 8 = 28 B/particle + 4 B/entry
+      k_dsph_rhs:           pos 8 + vel 8 + rho 4 + mass 4 + grad 8 + offset 4 + dudt 8 + drho/dt 4
+                            = 48 B/particle + 4 B/entry
+      with walls, each pass + 4 B/particle (wall-list offset) + 4 B per wall entry;
+      adjoint (dsph_deltasph_rhs_vjp: k_vjp_lg 32, k_vjp_lsm 52, k_vjp_main 108 B/particle, 4 B/entry each)
+                            = 192 B/particle + 12 B/entry."""
+    import torch
+    from diffsph_amd import _lib
+    from diffsph_amd.neighborhood import WORKSPACE
+    L = _lib.lib()
+    stream = torch.cuda.current_stream(s.pos.device)
+    sh = stream.cuda_stream
+    n = s.n
+    nl, pr, wpos, grid = s.neighbors(s.pos)
+    bnd, keep, wall_entries = None, None, 0
+    if s.walls is not None:
+        bnd, keep = s.wall_pairs(wpos, grid, s.rho)
+        wall_entries = keep[0].num_pairs
+    bp = C.byref(bnd) if bnd is not None else None
+    P = nl.num_pairs
+    du, dr = torch.empty(n, 2, device=s.pos.device), torch.empty(n, device=s.pos.device)
+    walls_b = (4 * n + 4 * wall_entries) if s.walls is not None else 0
+    passes = {
+        "k_dsph_normalization": (lambda: _lib.check(L.dsph_deltasph_normalization(
+            C.byref(pr), bp, _lib.ptr(s.mass), _lib.ptr(s.rho), None, None, _lib.ptr(s._grad), sh)),
+            28 * n + 4 * P + walls_b),
+        "k_dsph_rhs": (lambda: _lib.check(L.dsph_deltasph_rhs(
+            C.byref(pr), bp, C.byref(s.prm), _lib.ptr(s.vel), _lib.ptr(s.mass), _lib.ptr(s.rho),
+            _lib.ptr(s._grad), _lib.ptr(du), _lib.ptr(dr), None, sh)), 48 * n + 4 * P + walls_b),
+    }
+    if vjp:
+        g = torch.Generator(device="cpu").manual_seed(0)
+        cot = [torch.randn(n, 2, generator=g).to(s.pos.device), torch.randn(n, 2, generator=g).to(s.pos.device),
+               torch.randn(n, generator=g).to(s.pos.device)]
+        ws = WORKSPACE.get(L.dsph_deltasph_vjp_workspace(n), s.pos.device)
+        gx, gu, gr = torch.empty_like(s.pos), torch.empty_like(s.vel), torch.empty_like(s.rho)
+        passes["k_vjp_lg+k_vjp_lsm+k_vjp_main"] = (lambda: _lib.check(L.dsph_deltasph_rhs_vjp(
+            C.byref(pr), C.byref(s.prm), _lib.ptr(s.vel), _lib.ptr(s.mass), _lib.ptr(s.rho), _lib.ptr(s._grad),
+            _lib.ptr(cot[0]), _lib.ptr(cot[1]), _lib.ptr(cot[2]), _lib.ptr(gx), _lib.ptr(gu), _lib.ptr(gr),
+            _lib.ptr(ws), ws.numel(), sh)), 192 * n + 12 * P)
+    out = {}
+    for name, (fn, byt) in passes.items():
+        t = _time_pass(fn, reps, stream)
+        out[name] = {"us": round(t * 1e6, 2), "bytes": int(byt), "gbs": round(byt / t / 1e9, 1)}
+        log(f"{name}: {t * 1e6:.1f} us, {byt / 1e6:.1f} MB algorithmic -> {byt / t / 1e9:.0f} GB/s")
+    dom = max(out, key=lambda k: out[k]["us"])
+    r = out[dom]
+    return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "kernel_us": r["us"],
+            "algorithmic_bytes": r["bytes"], "entries": P, "wall_entries": wall_entries, "passes": out}
 
 
 def _oracle_deltasph_rate(states, cfg, steps):
@@ -624,9 +691,12 @@ def bench_dambreak(args):
     for _ in range(args.warmup):
         s.step()
     torch.cuda.synchronize()
+    # state at the start of the timed window (the CPU baseline continues it)
+    snap = [t.cpu().numpy() for t in (s.pos, s.vel, s.rho, s.mass)]
+    dts = []
     c0 = time.perf_counter()
     for k in range(args.steps):
-        s.step()
+        dts.append(s.step())
         if (k + 1) % 500 == 0:
             log(f"step {k + 1}: t = {s.time:.5f}, dt = {s.dt:.3g}")
     torch.cuda.synchronize()
@@ -635,11 +705,42 @@ def bench_dambreak(args):
     front = float(s.pos[:, 0].max().item())
     log(f"dam break: {args.steps} steps {secs:.2f} s, {value:.4g} particle-updates/s; t = {s.time:.4f}, "
         f"front x = {front:.4f}")
+    roof = _solver_roofline(s, args.kernel_reps)
+    cpu = None
+    if not args.no_cpu_baseline and not cfg.shifting:
+        from oracle import oracle as orc
+        from diffsph_amd.deltasph import MDBC
+        threads = _baseline_threads()
+        orc.set_threads(threads)
+        dom = orc.make_domain(cfg.domain_min, cfg.domain_max, cfg.periodic)
+        ow = orc.make_walls(dom, walls["pos"].cpu().numpy(), walls["dist"].cpu().numpy(),
+                            walls["normal"].cpu().numpy(), float(walls["area"]), cfg.rho0, cfg.support,
+                            viscosity=0, visc_coeff=cfg.boundary_alpha, eps=cfg.boundary_eps,
+                            mdbc_variant=MDBC[cfg.mdbc])
+        st = orc.DeltaState(*snap, np.arange(n, dtype=np.int32))
+        k_cpu = max(1, min(args.cpu_steps, len(dts)))
+        c1 = time.perf_counter()
+        for k in range(k_cpu):
+            st, _ = orc.rk4_step(st, dom, cfg.support, _oracle_delta_params(cfg), dts[k], walls=ow)
+        rate = n * k_cpu / (time.perf_counter() - c1)
+        cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{k_cpu} RK4 steps with mDBC walls from the GPU state at the first timed step "
+                         f"(same time steps), C++/OpenMP oracle, {threads} threads", **_host_info(threads)}
+        log(f"cpu baseline: {rate:.4g} particle-updates/s")
     config = {"workload": f"deltasph_dam_break_dx_1_{args.nside}", "particles": n, "wall_particles": s.walls.n,
               "simulated_time": s.time, "front_x": front, "dt_last": s.dt,
               "particle_shifting": "delta+ after every step" if cfg.shifting else None}
     print(json.dumps(_line("particle-updates/s, delta-SPH dam break with mDBC walls", value, "particle-updates/s",
-                           args, args.steps, secs, "f32", config, None, None)), flush=True)
+                           args, args.steps, secs, "f32", config, roof, cpu)), flush=True)
+
+
+def _oracle_delta_params(cfg):
+    from oracle import oracle as orc
+    return orc.DeltaParams(rho0=cfg.rho0, cs=cfg.cs, delta=cfg.delta, kernel_scale=cfg.kernel_scale,
+                           viscosity={"deltaSPH_viscid": 1, "deltaSPH_inviscid": 0, None: -1}[cfg.viscosity],
+                           visc_coeff=cfg.nu if cfg.viscosity == "deltaSPH_viscid" else cfg.alpha, eps=cfg.eps,
+                           kappa=cfg.kappa, gravity=cfg.gravity or (0.0, 0.0), pi_switch=cfg.pi_switch,
+                           eos=0 if cfg.eos == "isoThermal" else 1, diffusion=cfg.diffusion)
 
 
 def bench_rollout(args):
@@ -673,10 +774,26 @@ def bench_rollout(args):
     value = n * steps / secs
     log(f"rollout {args.nside}^2 x {steps} steps: forward {np.mean(fwd):.3f} s, backward {np.mean(bwd):.3f} s, "
         f"{value:.4g} particle-steps/s (forward + adjoint)")
+    from diffsph_amd.deltasph import DeltaSPHSolver
+    sv = DeltaSPHSolver(pos, vel, mass, rho, cfg)
+    sv.step()
+    roof = _solver_roofline(sv, args.kernel_reps, vjp=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        # forward steps only: the adjoint has no CPU restatement, so this
+        # rate is an upper bound on a CPU forward + adjoint
+        rate, threads, parts = _oracle_deltasph_rate(
+            [((-1.0, -1.0), (1.0, 1.0), pos.cpu().numpy(), vel.cpu().numpy(), rho.cpu().numpy(),
+              mass.cpu().numpy())], cfg, max(1, args.cpu_steps))
+        cpu = {"value": round(rate, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{parts // n} forward RK4 steps of the {n}-particle Taylor-Green state (no adjoint: "
+                         f"an upper bound on the CPU's forward + adjoint rate), C++/OpenMP oracle, {threads} threads",
+               **_host_info(threads)}
+        log(f"cpu baseline (forward only): {rate:.4g} particle-steps/s")
     config = {"workload": f"deltasph_rollout_adjoint_{args.nside}x{args.nside}_64_steps_checkpoint_8",
               "particles": n, "forward_seconds": float(np.mean(fwd)), "backward_seconds": float(np.mean(bwd))}
     print(json.dumps(_line("particle-steps/s, forward + adjoint of a 64-step delta-SPH rollout", value,
-                           "particle-steps/s", args, steps, secs, "f32", config, None, None)), flush=True)
+                           "particle-steps/s", args, steps, secs, "f32", config, roof, cpu)), flush=True)
 
 
 if __name__ == "__main__":

@@ -22,6 +22,7 @@
 // reference's loop condition (no grid-wide fence or atomic inside B).  Every
 // kernel of the loop returns at once when `done` is set, so the host may
 // queue several iterations ahead and synchronise only once per chunk.
+
 #include "common.h"
 
 namespace dsph {
@@ -411,20 +412,28 @@ __device__ __forceinline__ void sweep_a_kv(const PairsDev& P, float2 xi, const R
 // 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
 // `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
 // else evaluated on the fly.
-__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ kv,
-                                                          const float* __restrict__ vol,
-                                                          const float* __restrict__ rho, const float* __restrict__ p,
-                                                          float2* __restrict__ acc, const SolveCtrl* ctrl, int diff,
-                                                          const float4* __restrict__ xp, float4* __restrict__ xa) {
-  // xp (with kv): the (x, y, p) records replace p and the positions in the
-  // gathers; xa: the result is written as the (x, y, a) record instead of acc.
-  if (ctrl && ctrl->done) return;
-  int i = logical_block() * blockDim.x + threadIdx.x;
-  if (i >= P.nq) return;
+struct SweepAArgs {
+  const float* kv;
+  const float* vol;
+  const float* rho;
+  const float* p;
+  float2* acc;
+  int diff;
+  const float4* xp;
+  float4* xa;
+};
+
+// One row of sweep A.  xp (with kv): the (x, y, p) records replace p and
+// the positions in the gathers; xa: the result is written as the (x, y, a)
+// record instead of acc.
+__device__ __forceinline__ void sweep_a_row(const PairsDev& P, const SweepAArgs& A, int i) {
+  const float* kv = A.kv;
+  const float* p = A.p;
+  const float4* xp = A.xp;
   const float4 own = xp ? xp[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float pi = xp ? own.z : p[i];
   // -p_j turns the sum into the difference form; negation is exact
-  const float sj = diff ? -1.0f : 1.0f;
+  const float sj = A.diff ? -1.0f : 1.0f;
   float ax = 0.f, ay = 0.f;
   const RowRef rr = row_ref(P, i);
   const float2 xi = P.qpos[i];
@@ -447,20 +456,37 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const floa
     const float hi = h_of(P.qh, i, P.h0);
     for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
       const int j = P.nbr[k];
-      const float2 c = kv_of(P, nullptr, k, j, xi, hi, vol);
+      const float2 c = kv_of(P, nullptr, k, j, xi, hi, A.vol);
       const float s = pi + sj * p[j];
       ax += s * c.x;
       ay += s * c.y;
     }
   }
-  const float r = rho[i];
+  const float r = A.rho[i];
   const float2 a = make_float2(-ax / r, -ay / r);
-  if (xa) {
+  if (A.xa) {
     const float2 xr = xp ? make_float2(own.x, own.y) : P.rpos[i];
-    xa[i] = make_float4(xr.x, xr.y, a.x, a.y);
+    A.xa[i] = make_float4(xr.x, xr.y, a.x, a.y);
   } else {
-    acc[i] = a;
+    A.acc[i] = a;
   }
+}
+
+// Sweep A (dfsph.py:35-37): a_i = -(1/rho_i) sum_j (p_i + p_j) V_j gradW_ij;
+// with `diff` the ISPH form (incompressible.py:276-285, gradient
+// 'difference'): a_i = -(1/rho_i) sum_j (p_i - p_j) V_j gradW_ij.
+// `ctrl` NULL -> unconditional.  With `kv` the coefficients are streamed,
+// else evaluated on the fly.
+__global__ void __launch_bounds__(kBlock) k_dfsph_sweep_a(PairsDev P, const float* __restrict__ kv,
+                                                          const float* __restrict__ vol,
+                                                          const float* __restrict__ rho, const float* __restrict__ p,
+                                                          float2* __restrict__ acc, const SolveCtrl* ctrl, int diff,
+                                                          const float4* __restrict__ xp, float4* __restrict__ xa) {
+  if (ctrl && ctrl->done) return;
+  int i = logical_block() * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  const SweepAArgs A{kv, vol, rho, p, acc, diff, xp, xa};
+  sweep_a_row(P, A, i);
 }
 
 struct SweepBArgs {
@@ -546,6 +572,54 @@ __device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const 
   return sum;
 }
 
+// One row of sweep B; returns the row's term of the error metric.
+__device__ __forceinline__ double sweep_b_row(const PairsDev& P, const SweepBArgs& B, int i) {
+  float2 ai, xr;
+  if (B.xa) {
+    const float4 own = B.xa[i];
+    ai = make_float2(own.z, own.w);
+    xr = make_float2(own.x, own.y);
+  } else {
+    ai = B.acc[i];
+    xr = B.xp ? P.rpos[i] : make_float2(0.f, 0.f);
+  }
+  float sum = 0.f;
+  const RowRef rr = row_ref(P, i);
+  if (B.kv) {
+    const bool i16 = use_idx16(P);
+    const float2 xi = P.qpos[i];
+    if (B.xa) {
+      const GatherXA4 g{B.xa};
+      sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
+                : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
+    } else {
+      const GatherXA g{P.rpos, B.acc};
+      sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
+                : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
+    }
+  } else {
+    const float2 xi = P.qpos[i];
+    const float hi = h_of(P.qh, i, P.h0);
+    for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
+      const int j = P.nbr[k];
+      const float2 c = kv_of(P, nullptr, k, j, xi, hi, B.vol);
+      const float2 aj = B.acc[j];
+      sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
+    }
+  }
+  const float K = -B.dtf * sum;
+  const float s = B.src[i];
+  const float res = K - s;
+  float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
+  if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
+  B.p[i] = pn;
+  if (B.xp) B.xp[i] = make_float4(xr.x, xr.y, pn, 0.f);
+  if (B.residual) B.residual[i] = res;
+  if (B.isph) return (double)fabsf(res);
+  const float rr0 = res / B.rho0;
+  return (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
+}
+
 // Sweep B (dfsph.py:39-50): K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij,
 // p_i += omega (s_i - K_i) / alpha_i (clamped at 0, in place: a thread
 // touches only its own p_i), residual K_i - s_i, per-block float64 partial
@@ -556,55 +630,7 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
   const int lb = logical_block();
   int i = lb * blockDim.x + threadIdx.x;
   double err = 0.0;
-  if (i < P.nq) {
-    float2 ai, xr;
-    if (B.xa) {
-      const float4 own = B.xa[i];
-      ai = make_float2(own.z, own.w);
-      xr = make_float2(own.x, own.y);
-    } else {
-      ai = B.acc[i];
-      xr = B.xp ? P.rpos[i] : make_float2(0.f, 0.f);
-    }
-    float sum = 0.f;
-    const RowRef rr = row_ref(P, i);
-    if (B.kv) {
-      const bool i16 = use_idx16(P);
-      const float2 xi = P.qpos[i];
-      if (B.xa) {
-        const GatherXA4 g{B.xa};
-        sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
-                  : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
-      } else {
-        const GatherXA g{P.rpos, B.acc};
-        sum = i16 ? sweep_b_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, B.kv, g, ai)
-                  : sweep_b_kv(P, xi, rr, Idx32{P.nbr}, B.kv, g, ai);
-      }
-    } else {
-      const float2 xi = P.qpos[i];
-      const float hi = h_of(P.qh, i, P.h0);
-      for (int n = 0, k = rr.base; n < rr.cnt; ++n, k += rr.step) {
-        const int j = P.nbr[k];
-        const float2 c = kv_of(P, nullptr, k, j, xi, hi, B.vol);
-        const float2 aj = B.acc[j];
-        sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
-      }
-    }
-    const float K = -B.dtf * sum;
-    const float s = B.src[i];
-    const float res = K - s;
-    float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
-    if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
-    B.p[i] = pn;
-    if (B.xp) B.xp[i] = make_float4(xr.x, xr.y, pn, 0.f);
-    if (B.residual) B.residual[i] = res;
-    if (B.isph) {
-      err = (double)fabsf(res);
-    } else {
-      const float rr0 = res / B.rho0;
-      err = (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
-    }
-  }
+  if (i < P.nq) err = sweep_b_row(P, B, i);
   double bs = block_sum_d(err, sh);
   if (threadIdx.x == 0) B.partials[lb] = bs;
 }

@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import DsphError, check, lib, ptr
 from .neighborhood import WORKSPACE, CellGrid, Delta16Buffers, NeighborList, search
 
 
@@ -63,6 +63,9 @@ class DFSPHConfig:
     # Odd: a power-of-two slice stride (64 x 64 entries) costs the pressure
     # sweeps 4-6 % at 1M particles, an odd one about 2 %
     neighbor_width: int = 67
+    # one cooperative (persistent) launch per pressure solve: disabled, see
+    # DFSPHSolver.__init__
+    persistent: bool = False
 
 
 @dataclass
@@ -79,6 +82,13 @@ class DFSPHSolver:
     """Owns the particle state (cell-ordered, on the GPU) and steps it."""
 
     def __init__(self, positions, velocities, masses, areas, cfg: DFSPHConfig, uid=None):
+        if cfg.persistent:
+            # A cooperative launch holding the whole Jacobi loop (grid-wide
+            # barriers, loop state on the device) matched the chunked solve
+            # bit for bit at 9k particles but did not return at 1M particles
+            # (1792 blocks, the occupancy the runtime reports); the cause was
+            # not found, so the path is not built.
+            raise DsphError("the persistent (cooperative-launch) solve is disabled")
         dev = positions.device
         if not positions.is_cuda:
             raise RuntimeError("DFSPHSolver needs GPU tensors")

@@ -446,8 +446,52 @@ def ips_shifting():
     print("wrote", OUT_IPS, {k: (v.shape, v.ravel()[:1]) for k, v in np_out.items()})
 
 
+OUT_NOISE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_noise.npz")
+
+
+def octave_noise():
+    """Perlin octave noise of v2/noiseFunctions (generateOctaveNoise with
+    kind 'perlin', generator.py:87-99, over perlinNoise2D, perlin.py): the
+    two functions of generator.py executed from their source lines (the
+    module imports numba, unavailable here, for its simplex kinds)."""
+    sys.path.insert(0, REF)
+    perlin = importlib.import_module("diffSPH.v2.noiseFunctions.perlin")
+    src_path = os.path.join(REF, "diffSPH", "v2", "noiseFunctions", "generator.py")
+    src = open(src_path).read().splitlines()
+    ns_ = {"torch": torch, "np": np, "perlinNoise1D": perlin.perlinNoise1D, "perlinNoise2D": perlin.perlinNoise2D,
+           "perlinNoise3D": perlin.perlinNoise3D, "interpolant": perlin.interpolant}
+    for name in ("generatePerlin", "generateOctaveNoise"):
+        a = next(k for k, l in enumerate(src) if l.startswith(f"def {name}("))
+        b = next((k for k in range(a + 1, len(src)) if src[k] and not src[k][0].isspace()), len(src))
+        exec(compile("\n".join(src[a:b]), src_path, "exec"), ns_)
+    out = {}
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):  # perlinNoise2D prints its arguments
+        cases = {"a": dict(n=64, octaves=3, lacunarity=2, persistence=0.5, baseFrequency=2, tileable=True,
+                           seed=1234, normalized=True),
+                 "b": dict(n=48, octaves=2, lacunarity=2, persistence=0.7, baseFrequency=3, tileable=False,
+                           seed=7, normalized=False),
+                 "c": dict(n=32, octaves=1, lacunarity=2, persistence=0.5, baseFrequency=4, tileable=True,
+                           seed=99, normalized=True)}
+        for key, kw in cases.items():
+            xx, yy, noise = ns_["generateOctaveNoise"](kw["n"], dim=2, octaves=kw["octaves"],
+                                                       lacunarity=kw["lacunarity"], persistence=kw["persistence"],
+                                                       baseFrequency=kw["baseFrequency"], tileable=kw["tileable"],
+                                                       kind="perlin", seed=kw["seed"], normalized=kw["normalized"])
+            out[f"{key}_noise"] = noise.numpy().astype(np.float32)
+            out[f"{key}_xx"] = xx.numpy().astype(np.float32)
+            out[f"{key}_yy"] = yy.numpy().astype(np.float32)
+            out[f"{key}_params"] = np.array([kw["n"], kw["octaves"], kw["lacunarity"], kw["persistence"],
+                                             kw["baseFrequency"], float(kw["tileable"]), kw["seed"],
+                                             float(kw["normalized"])], np.float64)
+    np.savez(OUT_NOISE, **out)
+    print("wrote", OUT_NOISE, {k: v.shape for k, v in out.items()})
+
+
 if __name__ == "__main__":
     main()
     rollout_grad()
     sph_operations_and_shifting()
     ips_shifting()
+    octave_noise()

@@ -122,3 +122,15 @@ def test_search_rejects_delta16_for_foreign_queries():
     from diffsph_amd._lib import DsphError
     with pytest.raises(DsphError, match="16-bit"):
         neighborhood.search(None, torch.zeros((4, 2)), None, delta16=object(), qpos_wrapped=False)
+
+
+def test_persistent_solve_flag_is_rejected():
+    """The cooperative-launch solve is not built; asking for it raises
+    before any GPU work."""
+    from diffsph_amd._lib import DsphError
+    from diffsph_amd.dfsph import DFSPHConfig, DFSPHSolver
+    import torch
+    x = torch.zeros(4, 2)
+    cfg = DFSPHConfig(dt=1e-3, support=0.1, persistent=True)
+    with pytest.raises(DsphError, match="persistent"):
+        DFSPHSolver(x, x, torch.ones(4), torch.ones(4), cfg)

@@ -241,3 +241,23 @@ def test_dfsph_compact_index_paths_identical():
     for r in runs[1:]:
         assert torch.equal(runs[0].pos, r.pos) and torch.equal(runs[0].vel, r.vel)
         assert torch.equal(runs[0].p_div, r.p_div) and torch.equal(runs[0].p_den, r.p_den)
+
+
+@pytest.mark.skip(reason="the persistent (cooperative-launch) solve is disabled: it matched the chunked solve "
+                         "bit for bit at 96x96 but did not return at 1024x1024; cause not found")
+@pytest.mark.parametrize("nside,steps", [(96, 3), (1024, 2)])
+def test_persistent_solve_matches_chunked(nside, steps):
+    """The solves as one cooperative launch each (loop state, residual and
+    stopping rule on the device) give the same iterations and bit-identical
+    fields as the chunked host-checked loop."""
+    d, dt = _setup(nside, 5)
+    print(f"case nside={nside}", flush=True)
+    a = _solver(d, dt)
+    b = _solver(d, dt, persistent=True)
+    for k in range(steps):
+        sa, sb = a.step(), b.step()
+        print(f"step {k}: div {sa.div_iters}/{sb.div_iters} den {sa.den_iters}/{sb.den_iters}", flush=True)
+        assert (sa.div_iters, sa.den_iters) == (sb.div_iters, sb.den_iters)
+        assert sa.den_error == sb.den_error and sa.div_error == sb.div_error
+        for name in ("p_div", "p_den", "a_div", "a_den", "vel", "pos", "rho"):
+            assert torch.equal(getattr(a, name), getattr(b, name)), name
