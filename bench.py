@@ -116,6 +116,8 @@ def parse():
                          "delta-SPH domains in one HIP graph; tgv: delta-SPH Taylor-Green vortex; dambreak: "
                          "delta-SPH dam break with mDBC walls; rollout: adjoint of a Taylor-Green rollout")
     ap.add_argument("--shifting", action="store_true", help="dambreak: delta+ particle shifting after every step")
+    ap.add_argument("--capture", type=int, default=50,
+                    help="dambreak: steps per captured graph launch (0: host-checked steps)")
     ap.add_argument("--ndom", type=int, default=512, help="ensemble: number of domains")
     ap.add_argument("--cell-scale", type=float, default=None, help="ensemble: skip the autotuner, use this cell scale")
     ap.add_argument("--block", type=int, default=256, help="ensemble: workgroup size with --cell-scale")
@@ -694,13 +696,47 @@ def bench_dambreak(args):
     # state at the start of the timed window (the CPU baseline continues it)
     snap = [t.cpu().numpy() for t in (s.pos, s.vel, s.rho, s.mass)]
     dts = []
-    c0 = time.perf_counter()
-    for k in range(args.steps):
-        dts.append(s.step())
-        if (k + 1) % 500 == 0:
-            log(f"step {k + 1}: t = {s.time:.5f}, dt = {s.dt:.3g}")
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - c0
+    eager = None
+    if args.capture > 0:
+        if args.steps % args.capture:
+            raise SystemExit("--steps must be a multiple of --capture")
+        # the host-checked steps on a copy, for comparison
+        s2 = DeltaSPHSolver(s.pos, s.vel, s.mass, s.rho, cfg, walls=walls)
+        s2.dt, s2.time, s2._widths = s.dt, s.time, dict(s._widths)
+        for _ in range(3):
+            s2.step()
+        torch.cuda.synchronize()
+        e0 = time.perf_counter()
+        ne = min(args.steps, 200)
+        for _ in range(ne):
+            s2.step()
+        torch.cuda.synchronize()
+        eager = (time.perf_counter() - e0) * 1e3 / ne
+        del s2
+        log(f"host-checked steps: {eager:.4f} ms/step")
+        c_cap = time.perf_counter()
+        cs = s.capture(args.capture)
+        torch.cuda.synchronize()
+        t_cap = time.perf_counter() - c_cap
+        c0 = time.perf_counter()
+        for k in range(args.steps // args.capture):
+            t_before = s.time
+            cs.launch()
+            dts.append(s.time - t_before)
+            if (k + 1) * args.capture % 500 == 0:
+                log(f"step {(k + 1) * args.capture}: t = {s.time:.5f}, dt = {s.dt:.3g}")
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - c0
+        log(f"captured: {args.capture} steps per launch, capture {t_cap:.2f} s, fallbacks {cs.fallbacks}")
+        dts = None
+    else:
+        c0 = time.perf_counter()
+        for k in range(args.steps):
+            dts.append(s.step())
+            if (k + 1) % 500 == 0:
+                log(f"step {k + 1}: t = {s.time:.5f}, dt = {s.dt:.3g}")
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - c0
     value = n * args.steps / secs
     front = float(s.pos[:, 0].max().item())
     log(f"dam break: {args.steps} steps {secs:.2f} s, {value:.4g} particle-updates/s; t = {s.time:.4f}, "
@@ -718,10 +754,12 @@ def bench_dambreak(args):
                             viscosity=0, visc_coeff=cfg.boundary_alpha, eps=cfg.boundary_eps,
                             mdbc_variant=MDBC[cfg.mdbc])
         st = orc.DeltaState(*snap, np.arange(n, dtype=np.int32))
-        k_cpu = max(1, min(args.cpu_steps, len(dts)))
+        k_cpu = max(1, args.cpu_steps)
+        dt_cpu = dts if dts else [cfg.dt if cfg.dt is not None else s.dt] * k_cpu
+        k_cpu = min(k_cpu, len(dt_cpu))
         c1 = time.perf_counter()
         for k in range(k_cpu):
-            st, _ = orc.rk4_step(st, dom, cfg.support, _oracle_delta_params(cfg), dts[k], walls=ow)
+            st, _ = orc.rk4_step(st, dom, cfg.support, _oracle_delta_params(cfg), dt_cpu[k], walls=ow)
         rate = n * k_cpu / (time.perf_counter() - c1)
         cpu = {"value": round(rate, 1), "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"{k_cpu} RK4 steps with mDBC walls from the GPU state at the first timed step "
@@ -729,7 +767,9 @@ def bench_dambreak(args):
         log(f"cpu baseline: {rate:.4g} particle-updates/s")
     config = {"workload": f"deltasph_dam_break_dx_1_{args.nside}", "particles": n, "wall_particles": s.walls.n,
               "simulated_time": s.time, "front_x": front, "dt_last": s.dt,
-              "particle_shifting": "delta+ after every step" if cfg.shifting else None}
+              "particle_shifting": "delta+ after every step" if cfg.shifting else None,
+              "steps_per_graph_launch": args.capture if args.capture > 0 else None,
+              "host_checked_ms_per_step": None if eager is None else round(eager, 4)}
     print(json.dumps(_line("particle-updates/s, delta-SPH dam break with mDBC walls", value, "particle-updates/s",
                            args, args.steps, secs, "f32", config, roof, cpu)), flush=True)
 

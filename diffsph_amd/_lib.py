@@ -253,6 +253,9 @@ SIGNATURES = {
     "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, I32, V, V, V]),
     "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),
     "dsph_rk4_combine": (C.c_int, [I64, P(RK4Stages), V, V, V, F32, V, V, V, V]),
+    "dsph_rk_update_dev": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, V, I32, V]),
+    "dsph_rk4_combine_dev": (C.c_int, [I64, P(RK4Stages), V, V, V, V, V, V, V, V]),
+    "dsph_timestep_dev": (C.c_int, [V, I64, P(TimestepParams), I32, V, V, V, SZ, V]),
     "dsph_shift_workspace": (SZ, [I64]),
     "dsph_deltaplus_shift": (C.c_int, [P(Pairs), P(Boundary), P(ShiftParams), V, V, V, V, V, V, SZ, V]),
     "dsph_max_norm_workspace": (SZ, []),

@@ -27,25 +27,14 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr
-from .deltasph import DeltaSPHConfig, compute_timestep, timestep_constants
+from .deltasph import DeltaSPHConfig, compute_timestep
+from .deltasph import timestep_params as _timestep_params
 
 
 def timestep_params(cfg: DeltaSPHConfig) -> _lib.TimestepParams:
-    dt_v, dt_c = timestep_constants(cfg)
-    tp = _lib.TimestepParams()
-    tp.support = cfg.support
-    tp.kernel_scale = cfg.kernel_scale
-    tp.dt_visc = float(dt_v)
-    tp.dt_acoustic = float(dt_c)
-    tp.max_dt = float(np.float32(cfg.max_dt))
-    tp.min_dt = float(np.float32(cfg.min_dt))
-    tp.adaptive = 1 if cfg.adaptive else 0
-    tp.viscosity_constraint = 1 if cfg.viscosity_constraint else 0
-    tp.acceleration_constraint = 1 if cfg.acceleration_constraint else 0
-    tp.acoustic_constraint = 1 if cfg.acoustic_constraint else 0
     if cfg.accel_norm != "global":
         raise ValueError("the batched step implements the reference's global acceleration norm only")
-    return tp
+    return _timestep_params(cfg)
 
 
 class HipGraph:

@@ -322,9 +322,15 @@ __global__ void __launch_bounds__(kBlock) k_rk_update(int n, const float* __rest
                                                       float* __restrict__ x, const float* __restrict__ u0,
                                                       const float* __restrict__ ku, float* __restrict__ u,
                                                       const float* __restrict__ r0, const float* __restrict__ kr,
-                                                      float* __restrict__ r, float c) {
+                                                      float* __restrict__ r, float c,
+                                                      const float* __restrict__ dt_dev, int half) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (dt_dev) {
+    // c = fl32(dt * 0.5) or dt, as the host forms it from the float32 step
+    const float t = dt_dev[0];
+    c = half ? t * 0.5f : t;
+  }
   if (x) {
     x[2 * i] = x0[2 * i] + kx[2 * i] * c;
     x[2 * i + 1] = x0[2 * i + 1] + kx[2 * i + 1] * c;
@@ -346,19 +352,21 @@ struct Rk4Args {
   float* state[3];       // x, u, rho (in/out)
   float* avg[3];         // optional averaged derivatives
   float dt;
+  const float* dt_dev;   // when set, the step is read from here instead of dt
   int n;
 };
 
 __global__ void __launch_bounds__(kBlock) k_rk4_combine(Rk4Args A) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
+  const float dt = A.dt_dev ? A.dt_dev[0] : A.dt;
   for (int f = 0; f < 3; ++f) {
     if (!A.state[f]) continue;
     const int w = f == 2 ? 1 : 2;
     for (int c = 0; c < w; ++c) {
       const int64_t o = (int64_t)i * w + c;
       const float d = rk4_avg(A.k[f][0][o], A.k[f][1][o], A.k[f][2][o], A.k[f][3][o]);
-      A.state[f][o] = A.state[f][o] + d * A.dt;
+      A.state[f][o] = A.state[f][o] + d * dt;
       if (A.avg[f]) A.avg[f][o] = d;
     }
   }
@@ -432,6 +440,27 @@ __global__ void __launch_bounds__(64) k_sum_final(const double* part, int nb, do
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   if (threadIdx.x == 0) *out = s;
+}
+
+// computeTimestep (v2/modules/timestep.py:8-39) on the device: the float32
+// formula of the host version from the acceleration norm left by the
+// reduction (global: the float32 rounding of sqrt of the float64 sum of
+// squares; max: the largest per-particle norm); time += the step used.
+__global__ void k_timestep_single(const double* __restrict__ sumsq, const float* __restrict__ maxn,
+                                  dsph_timestep_params tp, float* __restrict__ dt, double* __restrict__ time) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float used = dt[0];
+  if (time) time[0] = time[0] + (double)used;
+  if (!tp.adaptive) return;
+  const float amax = sumsq ? (float)sqrt(sumsq[0]) : maxn[0];
+  const float dt_a = (0.25f * sqrtf(tp.support / (amax + 1e-7f))) / tp.kernel_scale;
+  float nw = used;
+  if (tp.viscosity_constraint) nw = tp.dt_visc;
+  if (tp.acceleration_constraint) nw = dt_a < nw ? dt_a : nw;
+  if (tp.acoustic_constraint) nw = tp.dt_acoustic < nw ? tp.dt_acoustic : nw;
+  nw = tp.max_dt < nw ? tp.max_dt : nw;
+  nw = tp.min_dt > nw ? tp.min_dt : nw;
+  dt[0] = nw;
 }
 
 // Ghost positions of wall particles (buildBoundaryGhostParticles,
@@ -719,15 +748,27 @@ extern "C" int dsph_rk_update(int64_t n, const float* x0, const float* dxdt, flo
   if ((x && (!x0 || !dxdt)) || (u && (!u0 || !dudt)) || (rho && (!rho0 || !drhodt))) return DSPH_ERR_ARG;
   if (n == 0) return DSPH_OK;
   k_rk_update<<<grid_for(n), kBlock, 0, as_stream(stream)>>>((int)n, x0, dxdt, x, u0, dudt, u, rho0, drhodt, rho,
-                                                              c);
+                                                              c, nullptr, 0);
   return launch_status();
 }
 
-extern "C" int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u, float* rho, float dt,
-                                float* dxdt_avg, float* dudt_avg, float* drhodt_avg, void* stream) {
+extern "C" int dsph_rk_update_dev(int64_t n, const float* x0, const float* dxdt, float* x, const float* u0,
+                                  const float* dudt, float* u, const float* rho0, const float* drhodt, float* rho,
+                                  const float* dt, int32_t half, void* stream) {
+  if (n < 0 || n >= INT32_MAX || !dt) return DSPH_ERR_ARG;
+  if ((x && (!x0 || !dxdt)) || (u && (!u0 || !dudt)) || (rho && (!rho0 || !drhodt))) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  k_rk_update<<<grid_for(n), kBlock, 0, as_stream(stream)>>>((int)n, x0, dxdt, x, u0, dudt, u, rho0, drhodt, rho,
+                                                              0.0f, dt, half ? 1 : 0);
+  return launch_status();
+}
+
+static int rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u, float* rho, float dt,
+                       const float* dt_dev, float* dxdt_avg, float* dudt_avg, float* drhodt_avg, void* stream) {
   if (!k || n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
   if (n == 0) return DSPH_OK;
   Rk4Args A;
+  A.dt_dev = dt_dev;
   const float* const* src[3] = {k->dxdt, k->dudt, k->drhodt};
   float* st[3] = {x, u, rho};
   float* av[3] = {dxdt_avg, dudt_avg, drhodt_avg};
@@ -743,6 +784,18 @@ extern "C" int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, f
   A.n = (int)n;
   k_rk4_combine<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(A);
   return launch_status();
+}
+
+extern "C" int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u, float* rho, float dt,
+                                float* dxdt_avg, float* dudt_avg, float* drhodt_avg, void* stream) {
+  return rk4_combine(n, k, x, u, rho, dt, nullptr, dxdt_avg, dudt_avg, drhodt_avg, stream);
+}
+
+extern "C" int dsph_rk4_combine_dev(int64_t n, const dsph_rk4_stages* k, float* x, float* u, float* rho,
+                                    const float* dt, float* dxdt_avg, float* dudt_avg, float* drhodt_avg,
+                                    void* stream) {
+  if (!dt) return DSPH_ERR_ARG;
+  return rk4_combine(n, k, x, u, rho, 0.0f, dt, dxdt_avg, dudt_avg, drhodt_avg, stream);
 }
 
 extern "C" size_t dsph_max_norm_workspace(void) { return sizeof(double) * 1040; }
@@ -827,6 +880,34 @@ extern "C" int dsph_sum_squares(const float* v, int64_t n, double* out_host, voi
   DSPH_CHECK_HIP(hipMemcpyAsync(out_host, out, sizeof(double), hipMemcpyDeviceToHost, st));
   DSPH_CHECK_HIP(hipStreamSynchronize(st));
   return DSPH_OK;
+}
+
+extern "C" int dsph_timestep_dev(const float* dudt_avg, int64_t n, const dsph_timestep_params* tp,
+                                 int32_t accel_norm, float* dt, double* time, void* ws, size_t ws_bytes,
+                                 void* stream) {
+  if (!tp || !dt || n < 0 || n >= INT32_MAX || (n > 0 && !dudt_avg) || accel_norm < 0 || accel_norm > 1)
+    return DSPH_ERR_ARG;
+  if (!ws || ws_bytes < dsph_max_norm_workspace()) return DSPH_ERR_CAPACITY;
+  hipStream_t st = as_stream(stream);
+  const int nb = 1024;
+  if (accel_norm == 0) {
+    // the reduction tree of dsph_sum_squares over the 2n components
+    double* part = static_cast<double*>(ws);
+    double* out = part + nb;
+    k_sumsq<<<nb, kBlock, 0, st>>>(dudt_avg, 2 * n, part);
+    k_sum_final<<<1, 64, 0, st>>>(part, nb, out);
+    DSPH_LAUNCH_CHECK();
+    k_timestep_single<<<1, 64, 0, st>>>(out, nullptr, *tp, dt, time);
+  } else {
+    // the reduction of dsph_max_norm2
+    float* part = static_cast<float*>(ws);
+    float* out = part + nb;
+    k_max_norm<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(dudt_avg), (int)n, part);
+    k_max_final<<<1, 64, 0, st>>>(part, nb, out);
+    DSPH_LAUNCH_CHECK();
+    k_timestep_single<<<1, 64, 0, st>>>(nullptr, out, *tp, dt, time);
+  }
+  return launch_status();
 }
 
 extern "C" int dsph_boundary_ghosts(const float* bpos, const float* distances, const float* normals, int64_t nb,

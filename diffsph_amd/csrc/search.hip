@@ -277,6 +277,11 @@ __global__ void __launch_bounds__(kBlock) k_fixed_slices(int nslices, int width,
   if (s <= nslices) slice_off[s] = s * 64 * width;
 }
 
+__global__ void __launch_bounds__(kBlock) k_zero_i32(int32_t* __restrict__ a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
 __global__ void k_fill_flag_clear(int32_t* flag) {
   if (threadIdx.x == 0) *flag = 0;
 }
@@ -418,7 +423,8 @@ extern "C" int dsph_grid_build(const dsph_domain* domain, const dsph_grid* grid,
 
   Dom d = make_dom(*domain);
   GridDev g = make_grid(*grid);
-  DSPH_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)ncell * 4, st));
+  // a fill kernel rather than a memset node, so the call can be captured
+  k_zero_i32<<<grid_for(ncell > 0 ? ncell : 1), kBlock, 0, st>>>(count, ncell);
   if (n > 0) {
     k_cell_keys<<<grid_for(n), kBlock, 0, st>>>(reinterpret_cast<const float2*>(pos), (int)n, d, g, keys, slot, count);
     DSPH_LAUNCH_CHECK();

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import DsphError, check, lib, ptr
 from .neighborhood import WORKSPACE, CellGrid, search, wrap_positions
 
 VISCOSITY = {None: -1, "none": -1, "deltaSPH_inviscid": 0, "deltaSPH_viscid": 1}
@@ -132,6 +132,24 @@ def timestep_constants(cfg: DeltaSPHConfig):
     return dt_v, dt_c
 
 
+def timestep_params(cfg: DeltaSPHConfig) -> _lib.TimestepParams:
+    """The computeTimestep constants for the device form of the step
+    (dsph_timestep_dev, and per domain the batched step)."""
+    dt_v, dt_c = timestep_constants(cfg)
+    tp = _lib.TimestepParams()
+    tp.support = cfg.support
+    tp.kernel_scale = cfg.kernel_scale
+    tp.dt_visc = float(dt_v)
+    tp.dt_acoustic = float(dt_c)
+    tp.max_dt = float(np.float32(cfg.max_dt))
+    tp.min_dt = float(np.float32(cfg.min_dt))
+    tp.adaptive = 1 if cfg.adaptive else 0
+    tp.viscosity_constraint = 1 if cfg.viscosity_constraint else 0
+    tp.acceleration_constraint = 1 if cfg.acceleration_constraint else 0
+    tp.acoustic_constraint = 1 if cfg.acoustic_constraint else 0
+    return tp
+
+
 def compute_timestep(cfg: DeltaSPHConfig, dt_prev: float, max_accel: float | None) -> float:
     """computeTimestep (v2/modules/timestep.py:8-39) in float32 arithmetic."""
     F = np.float32
@@ -189,6 +207,55 @@ class RHS:
     num_pairs: int
 
 
+class CapturedSteps:
+    """`steps` RK4 steps of a :class:`DeltaSPHSolver` recorded in one graph
+    (:meth:`DeltaSPHSolver.capture`)."""
+
+    def __init__(self, solver, graph, steps, flag, widths, bufs):
+        self.solver = solver
+        self.graph = graph
+        self.steps = steps
+        self.flag = flag
+        self.widths = widths
+        self.bufs = bufs            # the state buffers the graph reads and writes
+        self.fallbacks = 0          # launches redone eagerly after a row overflow
+
+    @property
+    def valid(self) -> bool:
+        """False once the solver's list widths changed (recapture)."""
+        return self.solver._widths == self.widths
+
+    def launch(self):
+        """Advance the solver by `steps` steps; one host read at the end."""
+        s = self.solver
+        if not self.valid:
+            raise DsphError("captured steps are stale: the neighbor list widths changed; capture again")
+        # eager steps since the capture may have moved the state to other
+        # tensors and changed the clock: bring both back to the graph's buffers
+        for name, buf in zip(("pos", "vel", "mass", "rho", "uid"), self.bufs):
+            cur = getattr(s, name)
+            if cur.data_ptr() != buf.data_ptr():
+                buf.copy_(cur)
+                setattr(s, name, buf)
+        s._dt_dev.fill_(float(np.float32(s.dt)))
+        s._time_dev.fill_(float(s.time))
+        snap = s._snapshot()
+        self.graph.replay()
+        if int(self.flag.item()):
+            # a row did not fit its fixed width: redo the steps with the
+            # host-checked path from the state before the replay
+            s._restore(snap)
+            s.dt = float(snap[5].item())
+            s.time = float(snap[6].item())
+            for _ in range(self.steps):
+                s.step()
+            self.fallbacks += 1
+            return
+        s.dt = float(s._dt_dev.item())
+        s.time = float(s._time_dev.item())
+        s.step_count += self.steps
+
+
 class DeltaSPHSolver:
     """Owns the particle state (cell-ordered, on the GPU) and steps it with RK4."""
 
@@ -219,7 +286,7 @@ class DeltaSPHSolver:
         self.step_count = 0
         self.dudt = None            # RK4-averaged derivatives of the last step
         self.drhodt = None
-        self.last_pairs = 0
+        self._last_nl = None
         self.last_sort = None
         self.on_sorted = None       # optional callback(solver) after the per-step sort
         n = self.n
@@ -227,6 +294,12 @@ class DeltaSPHSolver:
         self._k = [(torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f)) for _ in range(4)]
         self._stage = (torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f))
         self._widths = {}           # list kind -> row width of the one-pass list
+        self._deferred = None       # int32 [1] device flag: searches check overflow later (captured steps)
+
+    @property
+    def last_pairs(self) -> int:
+        """Fluid pairs of the last right-hand side."""
+        return 0 if self._last_nl is None else self._last_nl.pair_count()
 
     # -- pieces ------------------------------------------------------------
     def _search(self, kind: str, grid: CellGrid, q: torch.Tensor):
@@ -234,6 +307,11 @@ class DeltaSPHSolver:
         one-pass fixed-width list while its rows fit, else the two-pass one
         (and the next search of this kind is wider)."""
         w = self._widths.get(kind, max(int(self.cfg.neighbor_width), 0))
+        if self._deferred is not None:
+            if w < 1:
+                raise DsphError("captured steps need one-pass neighbor lists (neighbor_width > 0)")
+            return search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed", width=w,
+                          deferred=self._deferred)
         nl = search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed" if w > 0 else "sell", width=w)
         if w > 0 and nl.width == 0:
             self._widths[kind] = (int(nl.counts.max()) + 16) | 1
@@ -329,8 +407,8 @@ class DeltaSPHSolver:
         for s, c in enumerate([half, half, dt, None]):
             kx, ku, kr = self._k[s]
             kx.copy_(u)                     # dx/dt of this stage = its velocity
-            res = self.rhs(x, u, r, ku, kr)
-            self.last_pairs = res.num_pairs
+            self.rhs(x, u, r, ku, kr)
+            self._last_nl = self._keep[0]
             if c is not None:
                 check(L_.dsph_rk_update(self.n, ptr(x0), ptr(kx), ptr(xs), ptr(u0), ptr(ku), ptr(us), ptr(r0),
                                         ptr(kr), ptr(rs), C.c_float(c), st), "rk_update")
@@ -352,6 +430,122 @@ class DeltaSPHSolver:
         if self.cfg.adaptive:
             self.dt = compute_timestep(self.cfg, self.dt, self.max_accel())
         return dt
+
+    # -- steps without host synchronisation (captured into a graph) ---------
+    def _device_clock(self):
+        """Device copies of the step and the time (created on first use from
+        the host values)."""
+        if getattr(self, "_dt_dev", None) is None:
+            dev = self.pos.device
+            self._dt_dev = torch.tensor([np.float32(self.dt)], dtype=torch.float32, device=dev)
+            self._time_dev = torch.tensor([self.time], dtype=torch.float64, device=dev)
+            self._tp = timestep_params(self.cfg)
+        return self._dt_dev, self._time_dev
+
+    def _step_device(self):
+        """One RK4 step as step() computes it, with the time step and the time
+        read and updated on the device (dsph_rk_update_dev,
+        dsph_rk4_combine_dev, dsph_timestep_dev) and the searches' overflow
+        flags OR-ed into self._deferred: nothing is read back."""
+        L_ = lib()
+        st = _lib.stream_handle(self.pos.device)
+        dt_dev, time_dev = self._device_clock()
+        self._sort()
+        x0, u0, r0 = self.pos, self.vel, self.rho
+        xs, us, rs = self._stage
+        x, u, r = x0, u0, r0
+        for k, half in enumerate((1, 1, 0, None)):
+            kx, ku, kr = self._k[k]
+            kx.copy_(u)
+            self.rhs(x, u, r, ku, kr)
+            self._last_nl = self._keep[0]
+            if half is not None:
+                check(L_.dsph_rk_update_dev(self.n, ptr(x0), ptr(kx), ptr(xs), ptr(u0), ptr(ku), ptr(us), ptr(r0),
+                                            ptr(kr), ptr(rs), ptr(dt_dev), half, st), "rk_update_dev")
+                x, u, r = xs, us, rs
+        stages = _lib.RK4Stages()
+        for k in range(4):
+            stages.dxdt[k] = ptr(self._k[k][0])
+            stages.dudt[k] = ptr(self._k[k][1])
+            stages.drhodt[k] = ptr(self._k[k][2])
+        check(L_.dsph_rk4_combine_dev(self.n, C.byref(stages), ptr(self.pos), ptr(self.vel), ptr(self.rho),
+                                      ptr(dt_dev), None, ptr(self.dudt), ptr(self.drhodt), st), "rk4_combine_dev")
+        if self.cfg.shifting:
+            self.shift()
+        ws = WORKSPACE.get(L_.dsph_max_norm_workspace(), self.pos.device)
+        check(L_.dsph_timestep_dev(ptr(self.dudt), self.n, C.byref(self._tp),
+                                   0 if self.cfg.accel_norm == "global" else 1, ptr(dt_dev), ptr(time_dev),
+                                   ptr(ws), ws.numel(), st), "timestep_dev")
+
+    def _state(self):
+        return (self.pos, self.vel, self.mass, self.rho, self.uid)
+
+    def _snapshot(self):
+        dt_dev, time_dev = self._device_clock()
+        return [t.clone() for t in self._state()] + [dt_dev.clone(), time_dev.clone()]
+
+    def _restore(self, snap):
+        for dst, src in zip(self._state() + (self._dt_dev, self._time_dev), snap):
+            dst.copy_(src)
+
+    def capture(self, steps: int) -> "CapturedSteps":
+        """Record `steps` RK4 steps into one HIP graph (torch.cuda.CUDAGraph
+        over the library's launches on the current stream).  Inside the graph
+        nothing is read back: the time step is formed on the device, and the
+        neighbor searches use the one-pass fixed-width lists with their
+        overflow flags collected on the device.  :meth:`CapturedSteps.launch`
+        replays the graph, reads that flag and the clock once, and when a
+        row did not fit redoes the steps with :meth:`step` (which widens the
+        lists) from a copy of the state taken before the replay.  The steps
+        are bit-identical to :meth:`step`.  The state keeps its buffers (the
+        graph writes the final state of its steps back into them)."""
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
+        if self.on_sorted is not None:
+            raise DsphError("capture: the per-step sort callback cannot run inside a graph")
+        for kind in ("fluid",) + (("wall", "ghost") if self.walls is not None else ()):
+            if self._widths.get(kind, int(self.cfg.neighbor_width)) < 1:
+                raise DsphError("capture: needs one-pass neighbor lists (neighbor_width > 0)")
+        dev = self.pos.device
+        if self.dudt is None:
+            self.dudt = torch.empty_like(self.vel)
+            self.drhodt = torch.empty_like(self.rho)
+        self._device_clock()
+        self._dt_dev.fill_(float(np.float32(self.dt)))
+        self._time_dev.fill_(float(self.time))
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        bufs = self._state()
+        # one step outside the graph on a copy of the state: sizes the
+        # workspaces the captured launches use (nothing grows inside)
+        snap = self._snapshot()
+        derivs = (self.dudt.clone(), self.drhodt.clone())
+        self._deferred = flag
+        try:
+            self._step_device()
+            torch.cuda.current_stream(dev).synchronize()
+            self.pos, self.vel, self.mass, self.rho, self.uid = bufs
+            self._restore(snap)
+            self.dudt.copy_(derivs[0])
+            self.drhodt.copy_(derivs[1])
+            graph = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(device=dev)
+            side.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.graph(graph, stream=side):
+                flag.zero_()
+                for _ in range(steps):
+                    self._step_device()
+                for dst, src in zip(bufs, self._state()):
+                    if dst.data_ptr() != src.data_ptr():
+                        dst.copy_(src)
+            torch.cuda.current_stream(dev).wait_stream(side)
+        finally:
+            self._deferred = None
+            self.pos, self.vel, self.mass, self.rho, self.uid = bufs
+        cs = CapturedSteps(self, graph, steps, flag, self._widths.copy(), bufs)
+        # the graph holds raw pointers into the shared scratch buffer: keep
+        # that tensor alive even if the buffer is regrown later
+        cs.scratch = WORKSPACE.get(0, dev)
+        return cs
 
     def shift(self, shift_out: torch.Tensor | None = None):
         """delta+ particle shifting of the current state (solveShifting,

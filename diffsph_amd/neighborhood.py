@@ -127,7 +127,7 @@ class NeighborList:
     counts: torch.Tensor    # int32 [nq]
     offsets: torch.Tensor   # int32 [nq+1]
     nbr: torch.Tensor       # int32 [entries]
-    num_pairs: int
+    num_pairs: int | None   # None until read (deferred overflow check)
     qpos: torch.Tensor      # wrapped query positions [nq,2] (row order)
     qh: torch.Tensor | None
     grid: CellGrid
@@ -147,6 +147,13 @@ class NeighborList:
         check(lib().dsph_neighbor_compact16(C.byref(pr), limit, ptr(buf16), buf16.numel(), ptr(flag),
                                             _lib.stream_handle(buf16.device)), "neighbor_compact16")
         self.nbr16, self.nbr16_overflow = buf16, flag
+
+    def pair_count(self) -> int:
+        """num_pairs, read from the offsets when the list was built with a
+        deferred overflow check (synchronises then)."""
+        if self.num_pairs is None:
+            self.num_pairs = int(self.offsets[-1].item())
+        return self.num_pairs
 
     def pairs(self) -> _lib.Pairs:
         p = _lib.Pairs()
@@ -195,7 +202,7 @@ class Delta16Buffers:
 
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
            qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None,
-           width: int = 67) -> NeighborList:
+           width: int = 67, deferred: torch.Tensor | None = None) -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
     Counts, scans (host reads of the pair and entry totals, as the
@@ -206,7 +213,14 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     in one pass without the count pass (one host read of the overflow flag
     and the pair total); a row longer than `width` makes it fall back to
     "sell" (the returned list then has ``width == 0``).  The entries, their
-    order and the counts are the same in all three layouts."""
+    order and the counts are the same in all three layouts.
+
+    ``deferred`` (int32 [1] device tensor, "fixed" layout only): no host
+    read at all -- the row-overflow flag is OR-ed into it on the device and
+    the list is returned as is (rows longer than `width` truncated), with
+    ``num_pairs`` None (read on demand by :meth:`NeighborList.pair_count`).
+    The caller checks the flag later and redoes the work when it is set;
+    this is what lets a whole step be captured into a graph."""
     if delta16 is not None and not qpos_wrapped:
         raise DsphError("search: 16-bit neighbor deltas need the grid's own sorted positions as queries "
                         "(qpos_wrapped=True)")
@@ -220,8 +234,10 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     args = (C.byref(grid.domain), C.byref(grid.grid), ptr(grid.cell_start), ptr(grid.sorted_pos),
             ptr(grid.sorted_h), grid.n, ptr(qpos), ptr(qh), nq, 1 if qpos_wrapped else 0,
             C.c_float(grid.h_uniform), m)
+    if deferred is not None and (layout != "fixed" or int(width) < 1):
+        raise DsphError("search: deferred overflow checks need the fixed-width layout")
     if layout == "fixed":
-        nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st)
+        nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st, deferred)
         if nl is not None:
             return nl
         layout = "sell"
@@ -261,7 +277,7 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
                         grid, m, slice_off, entries, buf16, flag16)
 
 
-def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st):
+def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st, deferred=None):
     nq = qpos.shape[0]
     dev = qpos.device
     nsl = (nq + 63) // 64
@@ -277,10 +293,14 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
     offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
     ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
     check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), None, ptr(ws), ws.numel(), st), "scan")
-    info[1:2].copy_(offsets[nq:nq + 1])
-    ovf, npairs = (int(v) for v in info.cpu())
-    if ovf:
-        return None
+    if deferred is not None:
+        torch.bitwise_or(deferred, info[:1], out=deferred)
+        npairs = None
+    else:
+        info[1:2].copy_(offsets[nq:nq + 1])
+        ovf, npairs = (int(v) for v in info.cpu())
+        if ovf:
+            return None
     wq = qpos if qpos_wrapped else wrap_positions(qpos, grid.domain)
     return NeighborList(counts, offsets, nbr, npairs, wq, qh, grid, m, slice_off, cap, buf16, flag16, width)
 

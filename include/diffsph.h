@@ -741,6 +741,19 @@ int dsph_rk4_combine(int64_t n, const dsph_rk4_stages* k, float* x, float* u,
                      float* rho, float dt, float* dxdt_avg, float* dudt_avg,
                      float* drhodt_avg, void* stream);
 
+/* dsph_rk_update / dsph_rk4_combine with the time step read from device
+ * memory (dt[0], float32) when the kernels run: c = fl32(dt * 0.5) with
+ * `half`, else dt, as the host forms them.  For steps whose time step is
+ * computed on the device (dsph_timestep_dev) and that are captured into a
+ * graph; results are bit-identical to the host-step forms. */
+int dsph_rk_update_dev(int64_t n, const float* x0, const float* dxdt, float* x,
+                       const float* u0, const float* dudt, float* u,
+                       const float* rho0, const float* drhodt, float* rho,
+                       const float* dt, int32_t half, void* stream);
+int dsph_rk4_combine_dev(int64_t n, const dsph_rk4_stages* k, float* x, float* u,
+                         float* rho, const float* dt, float* dxdt_avg,
+                         float* dudt_avg, float* drhodt_avg, void* stream);
+
 /* delta+ particle shifting (deltaPlusShifting, v2/modules/shifting.py:
  * 119-147), applied as solveShifting does with scheme 'deltaSPH', one
  * iteration and no free-surface treatment (shifting.py:880-1018) and added
@@ -895,6 +908,19 @@ typedef struct dsph_timestep_params {
   int32_t acceleration_constraint;
   int32_t acoustic_constraint;
 } dsph_timestep_params;
+
+/* computeTimestep (v2/modules/timestep.py:8-39) of one domain on the
+ * device, after an RK4 step: time[0] (float64, may be NULL) += dt[0], then,
+ * when tp->adaptive, dt[0] = the new step from the RK4-averaged dudt_avg
+ * [n,2].  accel_norm 0: |dudt| = sqrt of the float64 sum of squares of all
+ * 2n components (the reduction tree of dsph_sum_squares); 1: the largest
+ * per-particle norm (that of dsph_max_norm2).  The float32 formula and its
+ * order are those of the host computation: no host synchronisation, the
+ * result is bit-identical to reading the norm back and forming the step on
+ * the host.  ws: dsph_max_norm_workspace() bytes. */
+int dsph_timestep_dev(const float* dudt_avg, int64_t n, const dsph_timestep_params* tp,
+                      int32_t accel_norm, float* dt, double* time, void* ws,
+                      size_t ws_bytes, void* stream);
 
 /* A batch laid out in one device workspace by dsph_batch_init. */
 typedef struct dsph_batch {
