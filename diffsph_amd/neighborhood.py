@@ -125,7 +125,7 @@ class NeighborList:
     padded with -1) when ``slice_off`` is set, else CSR.  ``offsets`` is the
     CSR scan of ``counts`` in both cases."""
     counts: torch.Tensor    # int32 [nq]
-    offsets: torch.Tensor   # int32 [nq+1]
+    offsets: torch.Tensor | None  # int32 [nq+1] (None until built: see ensure_offsets)
     nbr: torch.Tensor       # int32 [entries]
     num_pairs: int | None   # None until read (deferred overflow check)
     qpos: torch.Tensor      # wrapped query positions [nq,2] (row order)
@@ -148,11 +148,24 @@ class NeighborList:
                                             _lib.stream_handle(buf16.device)), "neighbor_compact16")
         self.nbr16, self.nbr16_overflow = buf16, flag
 
+    def ensure_offsets(self) -> torch.Tensor:
+        """The CSR offsets (a list built with a deferred overflow check has
+        none until asked for)."""
+        if self.offsets is None:
+            nq = self.counts.shape[0]
+            dev = self.counts.device
+            off = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+            ws = WORKSPACE.get(lib().dsph_scan_workspace(nq), dev)
+            check(lib().dsph_scan_exclusive(ptr(self.counts), nq, ptr(off), None, ptr(ws), ws.numel(),
+                                            _lib.stream_handle(dev)), "scan")
+            self.offsets = off
+        return self.offsets
+
     def pair_count(self) -> int:
         """num_pairs, read from the offsets when the list was built with a
         deferred overflow check (synchronises then)."""
         if self.num_pairs is None:
-            self.num_pairs = int(self.offsets[-1].item())
+            self.num_pairs = int(self.ensure_offsets()[-1].item())
         return self.num_pairs
 
     def pairs(self) -> _lib.Pairs:
@@ -178,7 +191,8 @@ class NeighborList:
         """Neighbor indices in CSR order (rows back to back)."""
         if self.slice_off is None:
             return self.nbr
-        out = torch.empty(max(self.num_pairs, 1), dtype=torch.int32, device=self.nbr.device)
+        self.ensure_offsets()
+        out = torch.empty(max(self.pair_count(), 1), dtype=torch.int32, device=self.nbr.device)
         pr = self.pairs()
         check(lib().dsph_sell_to_csr(C.byref(pr), ptr(out), _lib.stream_handle(out.device)), "sell_to_csr")
         return out[: self.num_pairs]
@@ -290,13 +304,15 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
         buf16, flag16 = delta16.get(cap, dev)
     check(L.dsph_neighbor_fill_fixed(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(buf16),
                                      ptr(flag16), ptr(info), st), "neighbor_fill_fixed")
-    offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
-    ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
-    check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), None, ptr(ws), ws.numel(), st), "scan")
     if deferred is not None:
+        # the pair kernels read the sliced rows (counts + slice offsets): the
+        # CSR offsets are built only if asked for (NeighborList.ensure_offsets)
         torch.bitwise_or(deferred, info[:1], out=deferred)
-        npairs = None
+        offsets, npairs = None, None
     else:
+        offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
+        ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
+        check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), None, ptr(ws), ws.numel(), st), "scan")
         info[1:2].copy_(offsets[nq:nq + 1])
         ovf, npairs = (int(v) for v in info.cpu())
         if ovf:
@@ -318,8 +334,8 @@ def export_pairs(nl: NeighborList, q_sort_index=None, q_rank=None, r_sort_index=
                  want=("i", "j", "distances", "vectors", "supports", "kernels", "gradients")):
     """Per-pair arrays of the reference's neighborDict in original indices."""
     nq = nl.counts.shape[0]
-    dev = nl.offsets.device
-    P = nl.num_pairs
+    dev = nl.counts.device
+    P = nl.pair_count()
     out_off = torch.empty(nq + 1, dtype=torch.int32, device=dev)
     res = {}
 

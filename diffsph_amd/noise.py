@@ -109,14 +109,21 @@ def sampleNoise(noiseConfig):
 
 
 def sampleNoisyParticles(noiseConfig, config, sdfs=(), randomizeParticles=False):
-    """A periodic fluid block with a noise stream function: returns
-    (noiseState, None) with the reference's state keys.  SDF bodies and
-    randomized (shifted) particle sets are not built."""
+    """A fluid block with a noise stream function: returns (noiseState, None)
+    with the reference's state keys.  `sdfs`: list of {'sdf': fn, 'type':
+    ...}; the fluid particles inside the 'boundary' bodies are removed
+    (sampleBoundaryParticles, 'regular' sampling) and, as the reference
+    assigns them (sampling.py:651), 'normals' receives the union's sdf
+    values and 'distances' its unit gradients.  Randomized (shifted)
+    particle sets and the boundary ramp of an active boundary are not
+    built."""
     from . import sampling
     from .neighborhood import neighborSearch
     from .sphops import sphOperationStates
-    if sdfs or randomizeParticles:
-        raise NotImplementedError("SDF bodies and randomized particle sets are not built")
+    if randomizeParticles:
+        raise NotImplementedError("randomized particle sets are not built")
+    if config.get("boundary", {}).get("active", False):
+        raise NotImplementedError("the potential ramp of an active boundary (generateRamp) is not built")
     dev = torch.device(config.get("compute", {}).get("device", "cuda:0"))
     dom = config["domain"]
     mn = [float(v) for v in dom["minExtent"]]
@@ -139,7 +146,26 @@ def sampleNoisyParticles(noiseConfig, config, sdfs=(), randomizeParticles=False)
           "index": torch.arange(N, device=dev), "densities": torch.full((N,), rho0, device=dev),
           "velocities": torch.zeros(N, 2, device=dev), "accelerations": torch.zeros(N, 2, device=dev)}
     st["masses"] = st["areas"] * rho0
+    if len(sdfs) > 0:
+        st["distances"] = torch.full((N,), float("inf"), device=dev)
     st["potential"] = noise.flatten().to(dev)
+    bodies = [b["sdf"] for b in sdfs if b["type"] == "boundary"]
+    if bodies:
+        from . import sdf as sdfm
+        # the boundary lattice uses the fluid's spacing and support
+        bcfg = dict(config, particle=dict(config.get("particle", {}), dx=dx, support=h),
+                    compute=dict(config.get("compute", {}), device=dev))
+        bcfg["kernel"] = dict(config["kernel"])
+        bcfg["domain"] = dict(config["domain"], dim=2)
+        _, _, _, fmask = sdfm.sampleBoundaryParticles(st, bodies, bcfg, ["regular"] * len(bodies))
+        for k in list(st.keys()):
+            if isinstance(st[k], torch.Tensor):
+                st[k] = st[k][fmask]
+        st["numParticles"] = st["positions"].shape[0]
+        combined = bodies[0]
+        for b in bodies[1:]:
+            combined = sdfm.op_union(combined, b)
+        _, _, st["normals"], st["distances"] = sdfm.filterParticlesWithSDF(st["positions"], combined, h, 1e-4)
     _, nd = neighborSearch(st, st, config)
     st["neighborhood"] = nd
     g = sphOperationStates(st, st, (st["potential"], st["potential"]), nd, operation="gradient",

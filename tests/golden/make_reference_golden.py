@@ -489,9 +489,100 @@ def octave_noise():
     print("wrote", OUT_NOISE, {k: v.shape for k, v in out.items()})
 
 
+OUT_SDF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_sdf.npz")
+
+
+def sdf_fixtures():
+    """Signed distances of the reference's 2-D primitives
+    (v2/sdfFunctionality/implicitFunctions.py, vmapped as v2/sdf.py's getSDF
+    does), its operators (operators.py), filterParticlesWithSDF and the
+    'regular' boundary sampling of sampleBoundaryParticles
+    (v2/sampling.py:104-113, 55-74, 824-893; executed from their source
+    lines: sampling.py imports scikit-image, unavailable here).  Every call
+    gets its own copy of the points (some primitives modify them)."""
+    sys.path.insert(0, REF)
+    imf = importlib.import_module("diffSPH.v2.sdfFunctionality.implicitFunctions")
+    ops = importlib.import_module("diffSPH.v2.sdfFunctionality.operators")
+    fd = importlib.import_module("diffSPH.v2.finiteDifference")
+    rmath = importlib.import_module("diffSPH.v2.math")
+    src_path = os.path.join(REF, "diffSPH", "v2", "sampling.py")
+    src = open(src_path).read().splitlines()
+    ns_ = {"torch": torch, "np": np, "List": list, "centralDifferenceStencil": fd.centralDifferenceStencil,
+           "continuousGradient": fd.continuousGradient, "volumeToSupport": rmath.volumeToSupport,
+           "operatorDict": ops.operatorDict}
+    for name in ("filterParticlesWithSDF", "sampleRegular", "sampleSDF_regular", "sampleBoundaryParticles"):
+        a = next(k for k, l in enumerate(src) if l.startswith(f"def {name}("))
+        b = next((k for k in range(a + 1, len(src)) if src[k] and not src[k][0].isspace()), len(src))
+        exec(compile("\n".join(src[a:b]), src_path, "exec"), ns_)
+    g = torch.Generator().manual_seed(5)
+    pts = (torch.rand(400, 2, generator=g) * 3.0 - 1.5).to(torch.float32)
+    V = lambda f, k: torch.vmap(f, in_dims=(0, *([None] * k)), out_dims=0)  # noqa: E731
+    T = torch.tensor
+    prim = {
+        "circle": (V(imf.sdCircle, 1), (0.55,)),
+        "box": (V(imf.sdBox, 1), (T([0.6, 0.35]),)),
+        "roundedBox": (imf.sdRoundedBox, (T([0.6, 0.4]), T([0.1, 0.2, 0.05, 0.15]))),
+        "orientedBox": (V(imf.sdOrientedBox, 3), (T([-0.5, -0.2]), T([0.4, 0.3]), 0.2)),
+        "segment": (V(imf.sdSegment, 2), (T([-0.3, 0.1]), T([0.5, -0.4]))),
+        "rhombus": (V(imf.sdRhombus, 1), (T([0.7, 0.4]),)),
+        "trapezoid": (V(imf.sdTrapezoid, 3), (0.6, 0.3, 0.4)),
+        "parallelogram": (V(imf.sdParallelogram, 3), (0.5, 0.3, 0.2)),
+        "equilateralTriangle": (imf.sdEquilateralTriangle, (0.6,)),
+        "triangleIsosceles": (V(imf.sdTriangleIsosceles, 1), (T([0.4, 0.9]),)),
+        "triangle": (V(imf.sdTriangle, 3), (T([-0.5, -0.4]), T([0.6, -0.2]), T([0.1, 0.7]))),
+        "unevenCapsule": (V(imf.sdUnevenCapsule, 3), (0.3, 0.15, 0.6)),
+        "pentagon": (V(imf.sdPentagon, 1), (0.5,)),
+        "hexagon": (V(imf.sdHexagon, 1), (0.5,)),
+        "octogon": (V(imf.sdOctogon, 1), (0.5,)),
+        "polygon": (V(imf.sdPolygon, 1), (T([[-0.6, -0.5], [0.7, -0.4], [0.5, 0.6], [-0.1, 0.2], [-0.5, 0.5]]),)),
+    }
+    out = {"points": pts.numpy()}
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):  # some primitives print shapes
+        for name, (f, args) in prim.items():
+            out[f"prim_{name}"] = f(pts.clone(), *args).numpy().astype(np.float32)
+    circ = lambda p: V(imf.sdCircle, 1)(p, 0.55)  # noqa: E731
+    box = lambda p: V(imf.sdBox, 1)(p, T([0.6, 0.35]))  # noqa: E731
+    od = ops.operatorDict
+    opc = {
+        "union": od["union"](circ, box), "intersection": od["intersection"](circ, box),
+        "difference": od["difference"](circ, box), "smooth_union": od["smooth_union"](circ, box, 0.2),
+        "smooth_intersection": od["smooth_intersection"](circ, box, 0.2),
+        "smooth_difference": od["smooth_difference"](circ, box, 0.2), "twist": od["twist"](box, 0.7),
+        "bend": od["bend"](box, 0.6), "taper": od["taper"](box, 0.8), "shear": od["shear"](box, 0.3),
+        "scale": od["scale"](box, 1.5), "translate": od["translate"](circ, T([0.2, -0.1])),
+        "rotate": od["rotate"](box, 0.4), "mirror": od["mirror"](box, 0.1), "flip": od["flip"](box, 0.2),
+        "invert": od["invert"](circ), "shell": od["shell"](circ, 0.05), "round": od["round"](box, 0.05),
+        "blend": od["blend"](circ, box, 0.3), "select": od["select"](circ, box, 0.1),
+    }
+    for name, f in opc.items():
+        out[f"op_{name}"] = f(pts.clone()).numpy().astype(np.float32)
+    # lattice filtering and the regular boundary sampling of a circular body
+    dx = 2.0 / 32
+    lat, area = ns_["sampleRegular"](dx, 2, -1.0, 1.0, 50, False, None)
+    h = float(rmath.volumeToSupport(torch.tensor(area), 50.0, 2))
+    _, mask, vals, grads = ns_["filterParticlesWithSDF"](lat.clone(), od["invert"](box), h, 1e-4)
+    out.update(lattice=lat.numpy(), h=np.float32(h), filter_mask=mask.numpy(), filter_values=vals.numpy(),
+               filter_grads=grads.numpy())
+    config = {"particle": {"dx": torch.tensor(dx), "support": torch.tensor(h)},
+              "domain": {"dim": 2, "minExtent": torch.tensor([-1.0, -1.0]), "maxExtent": torch.tensor([1.0, 1.0])},
+              "kernel": {"targetNeighbors": 50, "function": None}, "simulation": {"correctArea": False},
+              "compute": {"device": "cpu", "dtype": torch.float32}, "fluid": {"rho0": 1.0}}
+    body = lambda p: V(imf.sdCircle, 1)(p, 0.3)  # noqa: E731
+    fluid = {"positions": lat.clone(), "supports": torch.full((lat.shape[0],), h)}
+    bp, bv, bid, fmask = ns_["sampleBoundaryParticles"](fluid, [body], config, ["regular"])
+    _, _, bvals, bgrads = ns_["filterParticlesWithSDF"](bp, od["invert"](body), h, 1e-4)
+    out.update(boundary_positions=bp.numpy(), boundary_volumes=bv.numpy(), boundary_ids=bid.numpy(),
+               fluid_mask=fmask.numpy(), boundary_distances=(-bvals).numpy(), boundary_normals=bgrads.numpy())
+    np.savez(OUT_SDF, **out)
+    print("wrote", OUT_SDF, {k: np.shape(v) for k, v in out.items()})
+
+
 if __name__ == "__main__":
     main()
     rollout_grad()
     sph_operations_and_shifting()
     ips_shifting()
     octave_noise()
+    sdf_fixtures()

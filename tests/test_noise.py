@@ -52,3 +52,27 @@ def test_noise_initial_condition_is_divergence_free():
     gu = sphOperationStates(st, st, (v, v), st["neighborhood"], operation="gradient", gradientMode="difference")
     print(f"mean |div u| {float(div.abs().mean()):.3e}, mean |grad u| {float(gu.abs().mean()):.3e}", flush=True)
     assert float(div.abs().mean()) < 0.05 * float(gu.abs().mean())
+
+
+@pytest.mark.gpu
+def test_noise_initial_condition_with_a_boundary_body():
+    """A circular 'boundary' body: the fluid particles inside it are removed
+    (the mask of sampleBoundaryParticles: sdf >= -1e-4 kept), 'normals'
+    holds the body's sdf values and 'distances' its unit gradients (the
+    reference's assignment), and the velocity is still scaled to u_mag."""
+    from diffsph_amd import sdf
+    from diffsph_amd.noise import sampleNoisyParticles
+    cfg = {"domain": {"minExtent": [-1.0, -1.0], "maxExtent": [1.0, 1.0], "periodicity": [True, True], "dim": 2},
+           "fluid": {"rho0": 1.0, "u_mag": 1.5}, "kernel": {"targetNeighbors": 50},
+           "neighborhood": {"verletScale": 1.0}, "compute": {"device": "cuda:0"}}
+    body = lambda p: sdf.sdCircle(p, 0.3)  # noqa: E731
+    st, _ = sampleNoisyParticles({"n": 64, "baseFrequency": 2, "seed": 3}, cfg, sdfs=[{"sdf": body,
+                                                                                        "type": "boundary"}])
+    lat, _ = sdf.sampleRegular(2.0 / 64, 2, [-1.0, -1.0], [1.0, 1.0], 50)
+    keep = (body(lat) >= -1e-4).numpy()
+    assert st["numParticles"] == int(keep.sum()) < 4096
+    np.testing.assert_array_equal(st["positions"].cpu().numpy(), lat.numpy()[keep])
+    np.testing.assert_allclose(st["normals"].cpu().numpy(), body(lat).numpy()[keep], atol=1e-6)
+    g = st["distances"].cpu().numpy()
+    np.testing.assert_allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-5)
+    assert float(torch.linalg.norm(st["velocities"], dim=1).max()) == pytest.approx(1.5, rel=1e-5)
