@@ -175,96 +175,112 @@ __device__ __forceinline__ void mlp_forward_tile(const float* __restrict__ X, in
   }
 }
 
-// Inference: y[m] for m < n.  Grid-stride over 64-sample block tiles; every
-// wave of a block runs the same number of tiles (block-wide LDS barriers).
-// The weight fragments and biases of the lane are loaded once into
-// registers and reused for every tile (the same MFMA operations in the same
-// order as mlp_forward_tile, so the same bits).
+// Inference on the transposed product, Z^T = W^T H^T: hidden units are the
+// M rows and the 16 samples of a wave tile the N columns.  The C layout of a
+// layer's output (lane (c, q) holds rows 4q..4q+3 of an M tile for sample c)
+// is then already the B layout of the next layer's K chunk, so activations
+// never leave registers: no LDS and no barriers.  Layer 1 (K = 16 inputs)
+// runs on v_mfma_f32_16x16x16_bf16; layers 2 and 3 (K = 64) on gfx950's
+// v_mfma_f32_16x16x32_bf16, two K chunks each.  A K chunk of 32 hidden units
+// takes rows 4q..4q+3 of two consecutive M tiles into lane group q's eight
+// K slots; the weight (A) operand uses the same permutation of K, so the
+// products pair up (the sum over K does not depend on its order in the
+// operands beyond fp32 rounding).  The weight fragments are the packed B
+// operands of the training kernels read as A operands (the two layouts are
+// mirror images): W1 tile nt is lane (c, q)'s W1[4q+j][16nt+c] =
+// W1^T[16nt+c][4q+j]; W2/W3 tiles (kt, nt) give the halves of a K chunk.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 cat8(short4v lo, short4v hi) {
+  short v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ float4v mfma32(bf16x8 a, bf16x8 b, float4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Grid-stride over 16-sample wave tiles (no block-wide synchronisation).
 __global__ void __launch_bounds__(kBlock) k_mlp_forward(const float* __restrict__ X, int n,
                                                         const float* __restrict__ prm,
                                                         const short4v* __restrict__ pk, float* __restrict__ Y) {
-  __shared__ short lds[4][16 * kLdsStride];
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  short4v w1[4], w2[16], w3[16];
-  float b1[4], b2[4], b3[4], w4[4];
+  const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  short4v w1[4];
+  bf16x8 w2[4][2], w3[4][2];
+  float b1[4][4], b2[4][4], b3[4][4], w4[4][4];
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    w1[nt] = pk[kPkW1 + nt * 64 + l];
-    b1[nt] = prm[kOffB1 + 16 * nt + c];
-    b2[nt] = prm[kOffB2 + 16 * nt + c];
-    b3[nt] = prm[kOffB3 + 16 * nt + c];
-    w4[nt] = bf2f(f2bf(prm[kOffW4 + 16 * nt + c]));
-  }
+  for (int mt = 0; mt < 4; ++mt) {
+    w1[mt] = pk[kPkW1 + mt * 64 + l];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    w2[t] = pk[kPkW2 + t * 64 + l];
-    w3[t] = pk[kPkW3 + t * 64 + l];
-  }
-  const float b4 = prm[kOffB4];
-  const int ntiles = (n + 63) / 64;
-  for (int bt = blockIdx.x; bt < ntiles; bt += gridDim.x) {
-    const int m0 = bt * 64 + w * 16;
-    MlpTile T;
-    short4v xa;
-    {
-      const int m = m0 + c;
-      if (m < n) {
-        const float4 v = *reinterpret_cast<const float4*>(X + (int64_t)m * kFin + 4 * q);
-        xa[0] = f2bf(v.x);
-        xa[1] = f2bf(v.y);
-        xa[2] = f2bf(v.z);
-        xa[3] = f2bf(v.w);
-      } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) xa[j] = f2bf(0.0f);
-      }
-    }
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      float4v acc = {0.f, 0.f, 0.f, 0.f};
-      acc = mfma16(xa, w1[nt], acc);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float zz = acc[j] + b1[nt];
-        T.h[0][nt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
-      }
-    }
-#pragma unroll
-    for (int L = 1; L < 3; ++L) {
-      short4v a[4];
-      c_to_a(T.h[L - 1], a, lds[w]);
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        float4v acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(a[kt], L == 1 ? w2[kt * 4 + nt] : w3[kt * 4 + nt], acc);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float zz = acc[j] + (L == 1 ? b2[nt] : b3[nt]);
-          T.h[L][nt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
-        }
-      }
-    }
-    float sy[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) sy[j] += bf2f(T.h[2][nt][j]) * w4[nt];
+    for (int kc = 0; kc < 2; ++kc) {
+      w2[mt][kc] = cat8(pk[kPkW2 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW2 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
+      w3[mt][kc] = cat8(pk[kPkW3 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW3 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      float v = sy[j];
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-      T.y[j] = v + b4;
+      const int r = 16 * mt + 4 * q + j;
+      b1[mt][j] = prm[kOffB1 + r];
+      b2[mt][j] = prm[kOffB2 + r];
+      b3[mt][j] = prm[kOffB3 + r];
+      w4[mt][j] = bf2f(f2bf(prm[kOffW4 + r]));
     }
-    if (c == 0) {
+  }
+  const float b4 = prm[kOffB4];
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int ntiles = (n + 15) / 16;
+  // the next tile's inputs are loaded while this one is computed
+  auto load = [&](int t) {
+    const int m = t * 16 + c;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < ntiles && m < n) v = *reinterpret_cast<const float4*>(X + (int64_t)m * kFin + 4 * q);
+    return v;
+  };
+  float4 xn = load(wave);
+  for (int t = wave; t < ntiles; t += nwaves) {
+    const int m = t * 16 + c;
+    const float4 xv = xn;
+    xn = load(t + nwaves);
+    short4v xb;
+    xb[0] = f2bf(xv.x);
+    xb[1] = f2bf(xv.y);
+    xb[2] = f2bf(xv.z);
+    xb[3] = f2bf(xv.w);
+    short4v h[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      float4v acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma16(w1[mt], xb, acc);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = m0 + 4 * q + j;
-        if (m < n) Y[m] = T.y[j];
+        const float zz = acc[j] + b1[mt][j];
+        h[mt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
       }
     }
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      const bf16x8 k0 = cat8(h[0], h[1]), k1 = cat8(h[2], h[3]);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        float4v acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma32(L == 0 ? w2[mt][0] : w3[mt][0], k0, acc);
+        acc = mfma32(L == 0 ? w2[mt][1] : w3[mt][1], k1, acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float zz = acc[j] + (L == 0 ? b2[mt][j] : b3[mt][j]);
+          h[mt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
+        }
+      }
+    }
+    // y = h3 . w4 + b4: the lane's 16 rows, then the four lane groups
+    float s = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s += bf2f(h[mt][j]) * w4[mt][j];
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    if (q == 0 && m < n) Y[m] = s + b4;
   }
 }
 
@@ -529,14 +545,26 @@ static int mlp_grid(int64_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Inference grid: four 16-sample wave tiles per 256-thread block pass, at
+// most 512 blocks (two per CU at the kernel's occupancy of two waves per
+// SIMD): every wave loads the ~40 KB of weight fragments once and then
+// strides over its tiles, so the weight reads stay small next to the input.
+static int mlp_forward_grid(int64_t n) {
+  const int64_t tiles = (n + 15) / 16;
+  int64_t g = (tiles + 3) / 4;
+  if (g < 1) g = 1;
+  if (g > 512) g = 512;
+  return (int)g;
+}
+
 extern "C" int dsph_mlp_forward(const float* X, int64_t n, const float* params, const void* packed, float* Y,
                                 void* stream) {
   if (n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
   if (n == 0) return DSPH_OK;
   if (!X || !params || !packed || !Y) return DSPH_ERR_ARG;
   if (reinterpret_cast<uintptr_t>(X) % 16 != 0) return DSPH_ERR_ARG;  // rows are read as float4
-  k_mlp_forward<<<mlp_grid(n), kBlock, 0, as_stream(stream)>>>(X, (int)n, params,
-                                                               static_cast<const short4v*>(packed), Y);
+  k_mlp_forward<<<mlp_forward_grid(n), kBlock, 0, as_stream(stream)>>>(X, (int)n, params,
+                                                                       static_cast<const short4v*>(packed), Y);
   return launch_status();
 }
 
