@@ -481,7 +481,9 @@ def _batch_roofline(b, pairs_per_particle, reps):
     r = out[dom]
     return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "kernel_us": r["us"],
-            "algorithmic_bytes": r["bytes"], "passes": out}
+            "algorithmic_bytes": r["bytes"], "passes": out,
+            "limiter": "normalization and right-hand side: VALU issue (96-99 % of it, SQ_INSTS_VALU); "
+                       "fill: scalar loop control and the list writes (profiles/ensemble_512_kernels.md)"}
 
 
 def _solver_roofline(s, reps, vjp=False):
@@ -538,7 +540,9 @@ def _solver_roofline(s, reps, vjp=False):
     r = out[dom]
     return {"bound": "hbm", "achieved": r["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "kernel_us": r["us"],
-            "algorithmic_bytes": r["bytes"], "entries": P, "wall_entries": wall_entries, "passes": out}
+            "algorithmic_bytes": r["bytes"], "entries": P, "wall_entries": wall_entries, "passes": out,
+            "limiter": "VALU-heavy pair kernels with two waves per SIMD at this size: gather latency "
+                       "(profiles/dambreak_131k_kernels.md)"}
 
 
 def _oracle_deltasph_rate(states, cfg, steps):

@@ -190,97 +190,136 @@ __device__ __forceinline__ void mlp_forward_tile(const float* __restrict__ X, in
 // mirror images): W1 tile nt is lane (c, q)'s W1[4q+j][16nt+c] =
 // W1^T[16nt+c][4q+j]; W2/W3 tiles (kt, nt) give the halves of a K chunk.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
 __device__ __forceinline__ bf16x8 cat8(short4v lo, short4v hi) {
-  short v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-  return __builtin_bit_cast(bf16x8, v);
+  return cat8(__builtin_bit_cast(bf16x4, lo), __builtin_bit_cast(bf16x4, hi));
+}
+
+// relu(z) rounded to bf16 (round to nearest even: the hardware conversion,
+// v_cvt_pk_bf16_f32, two values per instruction).  The bias is already in
+// z: the MFMA accumulator starts from it.  max(z, 0) is taken on the bit
+// pattern as a signed integer (one instruction; negative values and -0
+// become +0, positive values are kept).
+__device__ __forceinline__ float relu_bits(float z) {
+  return __int_as_float(max(__float_as_int(z), 0));
+}
+__device__ __forceinline__ bf16x4 relu_bf16(float4v acc) {
+  float2v lo, hi;
+  lo[0] = relu_bits(acc[0]);
+  lo[1] = relu_bits(acc[1]);
+  hi[0] = relu_bits(acc[2]);
+  hi[1] = relu_bits(acc[3]);
+  const bf16x2 l2 = __builtin_convertvector(lo, bf16x2), h2 = __builtin_convertvector(hi, bf16x2);
+  return __builtin_shufflevector(l2, h2, 0, 1, 2, 3);
 }
 
 __device__ __forceinline__ float4v mfma32(bf16x8 a, bf16x8 b, float4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Grid-stride over 16-sample wave tiles (no block-wide synchronisation).
+struct InferW {
+  short4v w1[4];
+  bf16x8 w2[4][2], w3[4][2];
+  bf16x8 w4[2];  // output row: w4 in row 0 of the A tile, zeros elsewhere
+  float4v b1[4], b2[4], b3[4];
+  float4v b4;
+};
+
+// One 16-sample tile from its inputs (lane (c, q): sample c, inputs
+// 4q..4q+3).  Every layer, the output one included, is an MFMA whose
+// accumulator starts from the bias; returns the output tile, whose row 0
+// (lane (c, 0), element 0) is y of sample c.
+__device__ __forceinline__ float4v infer_tile(const InferW& W, float4 xv) {
+  float2v x01, x23;
+  x01[0] = xv.x;
+  x01[1] = xv.y;
+  x23[0] = xv.z;
+  x23[1] = xv.w;
+  const bf16x4 xb = __builtin_shufflevector(__builtin_convertvector(x01, bf16x2),
+                                            __builtin_convertvector(x23, bf16x2), 0, 1, 2, 3);
+  const short4v xs = __builtin_bit_cast(short4v, xb);
+  bf16x4 h[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) h[mt] = relu_bf16(mfma16(W.w1[mt], xs, W.b1[mt]));
+#pragma unroll
+  for (int L = 0; L < 2; ++L) {
+    const bf16x8 k0 = cat8(h[0], h[1]), k1 = cat8(h[2], h[3]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      float4v acc = L == 0 ? W.b2[mt] : W.b3[mt];
+      acc = mfma32(L == 0 ? W.w2[mt][0] : W.w3[mt][0], k0, acc);
+      acc = mfma32(L == 0 ? W.w2[mt][1] : W.w3[mt][1], k1, acc);
+      h[mt] = relu_bf16(acc);
+    }
+  }
+  float4v y = mfma32(W.w4[0], cat8(h[0], h[1]), W.b4);
+  return mfma32(W.w4[1], cat8(h[2], h[3]), y);
+}
+
+// Grid-stride over 16-sample wave tiles, two tiles per pass so that their
+// independent MFMA chains overlap, the next pass's inputs in flight (no
+// block-wide synchronisation).
 __global__ void __launch_bounds__(kBlock) k_mlp_forward(const float* __restrict__ X, int n,
                                                         const float* __restrict__ prm,
                                                         const short4v* __restrict__ pk, float* __restrict__ Y) {
   const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
-  short4v w1[4];
-  bf16x8 w2[4][2], w3[4][2];
-  float b1[4][4], b2[4][4], b3[4][4], w4[4][4];
+  InferW W;
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
-    w1[mt] = pk[kPkW1 + mt * 64 + l];
+    W.w1[mt] = pk[kPkW1 + mt * 64 + l];
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) {
-      w2[mt][kc] = cat8(pk[kPkW2 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW2 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
-      w3[mt][kc] = cat8(pk[kPkW3 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW3 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
+      W.w2[mt][kc] = cat8(pk[kPkW2 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW2 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
+      W.w3[mt][kc] = cat8(pk[kPkW3 + ((2 * kc) * 4 + mt) * 64 + l], pk[kPkW3 + ((2 * kc + 1) * 4 + mt) * 64 + l]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = 16 * mt + 4 * q + j;
-      b1[mt][j] = prm[kOffB1 + r];
-      b2[mt][j] = prm[kOffB2 + r];
-      b3[mt][j] = prm[kOffB3 + r];
-      w4[mt][j] = bf2f(f2bf(prm[kOffW4 + r]));
+      W.b1[mt][j] = prm[kOffB1 + r];
+      W.b2[mt][j] = prm[kOffB2 + r];
+      W.b3[mt][j] = prm[kOffB3 + r];
     }
   }
-  const float b4 = prm[kOffB4];
+  // output A tile: row 0 (lanes c == 0) holds w4 at the K chunk's hidden
+  // units (32 kc + 4q + j, then 32 kc + 16 + 4q + j), the other rows zero
+#pragma unroll
+  for (int kc = 0; kc < 2; ++kc) {
+    short v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = 32 * kc + (j < 4 ? 4 * q + j : 16 + 4 * q + (j - 4));
+      v[j] = c == 0 ? f2bf(prm[kOffW4 + r]) : (short)0;
+    }
+    W.w4[kc] = __builtin_bit_cast(bf16x8, v);
+  }
+  W.b4 = float4v{prm[kOffB4], 0.f, 0.f, 0.f};
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int ntiles = (n + 15) / 16;
-  // the next tile's inputs are loaded while this one is computed
   auto load = [&](int t) {
     const int m = t * 16 + c;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (t < ntiles && m < n) v = *reinterpret_cast<const float4*>(X + (int64_t)m * kFin + 4 * q);
     return v;
   };
-  float4 xn = load(wave);
-  for (int t = wave; t < ntiles; t += nwaves) {
-    const int m = t * 16 + c;
-    const float4 xv = xn;
-    xn = load(t + nwaves);
-    short4v xb;
-    xb[0] = f2bf(xv.x);
-    xb[1] = f2bf(xv.y);
-    xb[2] = f2bf(xv.z);
-    xb[3] = f2bf(xv.w);
-    short4v h[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      float4v acc = {0.f, 0.f, 0.f, 0.f};
-      acc = mfma16(w1[mt], xb, acc);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float zz = acc[j] + b1[mt][j];
-        h[mt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
-      }
-    }
-#pragma unroll
-    for (int L = 0; L < 2; ++L) {
-      const bf16x8 k0 = cat8(h[0], h[1]), k1 = cat8(h[2], h[3]);
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        float4v acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma32(L == 0 ? w2[mt][0] : w3[mt][0], k0, acc);
-        acc = mfma32(L == 0 ? w2[mt][1] : w3[mt][1], k1, acc);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float zz = acc[j] + (L == 0 ? b2[mt][j] : b3[mt][j]);
-          h[mt][j] = f2bf(zz > 0.0f ? zz : 0.0f);
-        }
-      }
-    }
-    // y = h3 . w4 + b4: the lane's 16 rows, then the four lane groups
-    float s = 0.f;
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) s += bf2f(h[mt][j]) * w4[mt][j];
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
-    if (q == 0 && m < n) Y[m] = s + b4;
+  // the next pass's inputs are loaded while this pass computes
+  float4 n0 = load(wave), n1 = load(wave + nwaves);
+  for (int t = wave; t < ntiles; t += 2 * nwaves) {
+    const int t1 = t + nwaves;
+    const float4 x0 = n0, x1 = n1;
+    n0 = load(t + 2 * nwaves);
+    n1 = load(t1 + 2 * nwaves);
+    const float4v y0 = infer_tile(W, x0);
+    const float4v y1 = infer_tile(W, x1);
+    const int m0 = t * 16 + c, m1 = t1 * 16 + c;
+    if (q == 0 && m0 < n) Y[m0] = y0[0];
+    if (q == 0 && t1 < ntiles && m1 < n) Y[m1] = y1[0];
   }
 }
 
