@@ -513,10 +513,11 @@ def _solver_roofline(s, reps, vjp=False):
     P = nl.num_pairs
     du, dr = torch.empty(n, 2, device=s.pos.device), torch.empty(n, device=s.pos.device)
     walls_b = (4 * n + 4 * wall_entries) if s.walls is not None else 0
+    _lib.check(L.dsph_volume(n, _lib.ptr(s.mass), _lib.ptr(s.rho), _lib.ptr(s._vol), sh))
     passes = {
-        "k_dsph_normalization": (lambda: _lib.check(L.dsph_deltasph_normalization(
-            C.byref(pr), bp, _lib.ptr(s.mass), _lib.ptr(s.rho), None, None, _lib.ptr(s._grad), sh)),
-            28 * n + 4 * P + walls_b),
+        "k_dsph_normalization": (lambda: _lib.check(L.dsph_deltasph_normalization_v(
+            C.byref(pr), bp, _lib.ptr(s.mass), _lib.ptr(s.rho), _lib.ptr(s._vol), None, None, _lib.ptr(s._grad),
+            sh)), 28 * n + 4 * P + walls_b),
         "k_dsph_rhs": (lambda: _lib.check(L.dsph_deltasph_rhs(
             C.byref(pr), bp, C.byref(s.prm), _lib.ptr(s.vel), _lib.ptr(s.mass), _lib.ptr(s.rho),
             _lib.ptr(s._grad), _lib.ptr(du), _lib.ptr(dr), None, sh)), 48 * n + 4 * P + walls_b),

@@ -249,6 +249,8 @@ SIGNATURES = {
     "dsph_remap_neighbors": (C.c_int, [V, I64, V, I64, V]),
     "dsph_deltasph_normalization": (C.c_int, [P(Pairs), P(Boundary), V, V, V, V, V, V]),
     "dsph_deltasph_rhs": (C.c_int, [P(Pairs), P(Boundary), P(DeltaSPHParams), V, V, V, V, V, V, V, V]),
+    "dsph_volume": (C.c_int, [I64, V, V, V, V]),
+    "dsph_deltasph_normalization_v": (C.c_int, [P(Pairs), P(Boundary), V, V, V, V, V, V, V]),
     "dsph_boundary_ghosts": (C.c_int, [V, V, V, I64, F32, V, V]),
     "dsph_mdbc": (C.c_int, [P(Pairs), V, V, V, F32, F32, I32, V, V, V]),
     "dsph_rk_update": (C.c_int, [I64, V, V, V, V, V, V, V, V, V, F32, V]),

@@ -59,7 +59,7 @@ struct Layout {
   size_t Bpos, Bvel, Bmass, Brho, Buid;
   size_t Spos[3], Svel[3], Srho[3];
   size_t ku[4], kr[4];
-  size_t grad, avg, sq_part, sq_dom, status;
+  size_t grad, vol, avg, sq_part, sq_dom, status;
   size_t total;
   int64_t npad, sq_cap;
 };
@@ -103,6 +103,7 @@ Layout make_layout(int64_t n, int ndom, int64_t ncell, int width) {
     L.kr[s] = take(4 * n);
   }
   L.grad = take(8 * n);
+  L.vol = take(4 * n);
   L.avg = take(8 * n);
   L.sq_part = take(8 * L.sq_cap);
   L.sq_dom = take(4 * L.sq_cap);
@@ -499,14 +500,19 @@ int stage_rhs(const Ctx& c, const dsph_deltasph_params* prm, const float2* wpos,
   const PairsDev P = batch_pairs(c, wpos);
   const float* mass = c.p<float>(L.Bmass);
   float2* grad = c.p<float2>(L.grad);
+  float* vol = c.p<float>(L.vol);
+  int s;
   if (prm->diffusion) {
-    int s = launch_normalization_per_particle(P, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), mass, rho, grad, c.block,
-                                              c.st);
+    // V_j = m_j / rho_j once per particle for the normalization pass
+    s = launch_volume(c.n, mass, rho, vol, c.st);
+    if (s != DSPH_OK) return s;
+    s = launch_normalization_per_particle(P, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), mass, rho, vol, grad, c.block,
+                                          c.st);
     if (s != DSPH_OK) return s;
   }
   RhsArgs A;
-  int s = make_rhs_args(prm, reinterpret_cast<const float*>(vel), mass, rho, reinterpret_cast<const float*>(grad),
-                        reinterpret_cast<float*>(ku), kr, nullptr, &A);
+  s = make_rhs_args(prm, reinterpret_cast<const float*>(vel), mass, rho, reinterpret_cast<const float*>(grad),
+                    reinterpret_cast<float*>(ku), kr, nullptr, &A);
   if (s != DSPH_OK) return s;
   return launch_rhs_per_particle(P, A, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), c.block, c.st);
 }
@@ -790,7 +796,7 @@ extern "C" int dsph_batch_replay_pass(const dsph_batch* batch, const dsph_deltas
   const float* rho = c.p<float>(L.Srho[2]);
   if (pass == 1)
     return launch_normalization_per_particle(P, c.p<Dom>(L.doms), c.p<int32_t>(L.dom_of), c.p<float>(L.Bmass), rho,
-                                             c.p<float2>(L.grad), c.block, c.st);
+                                             c.p<float>(L.vol), c.p<float2>(L.grad), c.block, c.st);
   RhsArgs A;
   s = make_rhs_args(prm, c.p<float>(L.Svel[2]), c.p<float>(L.Bmass), rho, c.p<float>(L.grad), c.p<float>(L.ku[3]),
                     c.p<float>(L.kr[3]), nullptr, &A);

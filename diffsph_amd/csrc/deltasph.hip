@@ -93,10 +93,11 @@ struct DomainPerParticle {
 // sum_b V_b (rho_i - rho_b) gradW_ib (densityGradient, deltaPlus.py:262-263).
 // One pass over the row: L_i does not depend on j, so the renormalized
 // gradient is L_i applied once to s_i = sum_j V_j (rho_i - rho_j) gradW_ij.
-template <class DF>
+template <class DF, bool kVol>
 __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, BoundaryDev B, DF domain,
                                                                const float* __restrict__ mass,
                                                                const float* __restrict__ rho,
+                                                               const float* __restrict__ vol,
                                                                float4* __restrict__ L_out,
                                                                float2* __restrict__ ev_out,
                                                                float2* __restrict__ grad_rho) {
@@ -109,17 +110,19 @@ __global__ void __launch_bounds__(kBlock) k_dsph_normalization(PairsDev P, Bound
   const RowRef rr = row_ref(P, i);
   const float rho_i = rho[i];
   float m00 = 0.f, m01 = 0.f, m10 = 0.f, m11 = 0.f, sx = 0.f, sy = 0.f;
+  // with kVol the record carries V_j (= m_j / rho_j, the same quotient
+  // formed once per particle) in place of m_j
   auto ld = [&](int j) {
     NbrXMR d;
     d.x = P.rpos[j];
-    d.m = mass[j];
+    d.m = kVol ? vol[j] : mass[j];
     d.r = rho[j];
     return d;
   };
   scan_row<4, true>(P.nbr, rr, ld, [&](int j, const NbrXMR& d) {
     const float h = pick_h(P.mode, hi, h_of(P.rh, j, P.h0));
     const Geom g = pair_geom(xi, d.x, h, ks(h), dm);
-    const float Vj = d.m / d.r;
+    const float Vj = kVol ? d.m : d.m / d.r;
     const float s = -(g.q * g.h);  // -(distances * supports)
     const float qx = s * g.ex, qy = s * g.ey;
     const float kx = Vj * g.gx, ky = Vj * g.gy;
@@ -653,9 +656,26 @@ static int check_boundary(const dsph_boundary* b, const dsph_pairs* fluid) {
   return DSPH_OK;
 }
 
-extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_boundary* boundary, const float* mass,
-                                           const float* rho, float* L, float* eigvals, float* grad_rho,
-                                           void* stream) {
+__global__ void __launch_bounds__(kBlock) k_volume(int n, const float* __restrict__ mass,
+                                                   const float* __restrict__ rho, float* __restrict__ vol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) vol[i] = mass[i] / rho[i];
+}
+
+int dsph::launch_volume(int n, const float* mass, const float* rho, float* vol, hipStream_t st) {
+  if (n <= 0) return DSPH_OK;
+  k_volume<<<grid_for(n), kBlock, 0, st>>>(n, mass, rho, vol);
+  return launch_status();
+}
+
+extern "C" int dsph_volume(int64_t n, const float* mass, const float* rho, float* vol, void* stream) {
+  if (n < 0 || n >= INT32_MAX || (n > 0 && (!mass || !rho || !vol))) return DSPH_ERR_ARG;
+  return launch_volume((int)n, mass, rho, vol, as_stream(stream));
+}
+
+extern "C" int dsph_deltasph_normalization_v(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                             const float* mass, const float* rho, const float* vol, float* L,
+                                             float* eigvals, float* grad_rho, void* stream) {
   int s = check_same_set(pairs);
   if (s != DSPH_OK) return s;
   s = check_boundary(boundary, pairs);
@@ -664,10 +684,22 @@ extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_b
   if (!mass || !rho || !grad_rho) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   BoundaryDev B = make_boundary(boundary);
-  k_dsph_normalization<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, B, OneDomain{P.dom}, mass, rho, reinterpret_cast<float4*>(L), reinterpret_cast<float2*>(eigvals),
-      reinterpret_cast<float2*>(grad_rho));
+  float4* Lo = reinterpret_cast<float4*>(L);
+  float2* eo = reinterpret_cast<float2*>(eigvals);
+  float2* go = reinterpret_cast<float2*>(grad_rho);
+  if (vol)
+    k_dsph_normalization<OneDomain, true><<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+        P, B, OneDomain{P.dom}, mass, rho, vol, Lo, eo, go);
+  else
+    k_dsph_normalization<OneDomain, false><<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+        P, B, OneDomain{P.dom}, mass, rho, nullptr, Lo, eo, go);
   return launch_status();
+}
+
+extern "C" int dsph_deltasph_normalization(const dsph_pairs* pairs, const dsph_boundary* boundary, const float* mass,
+                                           const float* rho, float* L, float* eigvals, float* grad_rho,
+                                           void* stream) {
+  return dsph_deltasph_normalization_v(pairs, boundary, mass, rho, nullptr, L, eigvals, grad_rho, stream);
 }
 
 int dsph::make_rhs_args(const dsph_deltasph_params* prm, const float* vel, const float* mass, const float* rho,
@@ -702,13 +734,17 @@ int dsph::make_rhs_args(const dsph_deltasph_params* prm, const float* vel, const
 }
 
 int dsph::launch_normalization_per_particle(const PairsDev& P, const Dom* doms, const int32_t* dom_of,
-                                            const float* mass, const float* rho, float2* grad_rho, int block,
-                                            hipStream_t st) {
+                                            const float* mass, const float* rho, const float* vol, float2* grad_rho,
+                                            int block, hipStream_t st) {
   if (block != 64 && block != 128 && block != 256) return DSPH_ERR_ARG;
   if (P.nq == 0) return DSPH_OK;
-  k_dsph_normalization<<<grid_for(P.nq, block), block, 0, st>>>(P, make_boundary(nullptr),
-                                                                DomainPerParticle{doms, dom_of}, mass, rho,
-                                                                nullptr, nullptr, grad_rho);
+  const DomainPerParticle dp{doms, dom_of};
+  if (vol)
+    k_dsph_normalization<DomainPerParticle, true><<<grid_for(P.nq, block), block, 0, st>>>(
+        P, make_boundary(nullptr), dp, mass, rho, vol, nullptr, nullptr, grad_rho);
+  else
+    k_dsph_normalization<DomainPerParticle, false><<<grid_for(P.nq, block), block, 0, st>>>(
+        P, make_boundary(nullptr), dp, mass, rho, nullptr, nullptr, nullptr, grad_rho);
   return launch_status();
 }
 

@@ -34,8 +34,11 @@ int make_rhs_args(const dsph_deltasph_params* prm, const float* vel, const float
 // particle set whose box is doms[dom_of[i]] (P.dom is ignored), launched
 // with `block` threads per workgroup (64, 128 or 256).
 int launch_normalization_per_particle(const PairsDev& P, const Dom* doms, const int32_t* dom_of,
-                                      const float* mass, const float* rho, float2* grad_rho, int block,
-                                      hipStream_t st);
+                                      const float* mass, const float* rho, const float* vol, float2* grad_rho,
+                                      int block, hipStream_t st);
+// vol[i] = mass[i] / rho[i]: the particle volume the normalization pass
+// reads as V_j instead of forming the same quotient once per pair.
+int launch_volume(int n, const float* mass, const float* rho, float* vol, hipStream_t st);
 int launch_rhs_per_particle(const PairsDev& P, const RhsArgs& A, const Dom* doms, const int32_t* dom_of,
                             int block, hipStream_t st);
 

@@ -291,6 +291,7 @@ class DeltaSPHSolver:
         self.on_sorted = None       # optional callback(solver) after the per-step sort
         n = self.n
         self._grad = torch.empty(n, 2, **f)
+        self._vol = torch.empty(n, **f)
         self._k = [(torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f)) for _ in range(4)]
         self._stage = (torch.empty(n, 2, **f), torch.empty(n, 2, **f), torch.empty(n, **f))
         self._widths = {}           # list kind -> row width of the one-pass list
@@ -366,8 +367,11 @@ class DeltaSPHSolver:
         bp = C.byref(bnd) if bnd is not None else None
         L_ = lib()
         if self.cfg.diffusion or L is not None or eigvals is not None:
-            check(L_.dsph_deltasph_normalization(C.byref(pr), bp, ptr(self.mass), ptr(rho), ptr(L), ptr(eigvals),
-                                                 ptr(self._grad), st), "deltasph_normalization")
+            # V_j = m_j / rho_j once per particle for the normalization pass
+            check(L_.dsph_volume(self.n, ptr(self.mass), ptr(rho), ptr(self._vol), st), "volume")
+            check(L_.dsph_deltasph_normalization_v(C.byref(pr), bp, ptr(self.mass), ptr(rho), ptr(self._vol),
+                                                   ptr(L), ptr(eigvals), ptr(self._grad), st),
+                  "deltasph_normalization")
         check(L_.dsph_deltasph_rhs(C.byref(pr), bp, C.byref(self.prm), ptr(vel), ptr(self.mass), ptr(rho),
                                    ptr(self._grad), ptr(dudt), ptr(drhodt), ptr(terms), st), "deltasph_rhs")
         self._keep = (nl, wpos, keep)

@@ -693,6 +693,21 @@ int dsph_deltasph_rhs(const dsph_pairs* pairs, const dsph_boundary* boundary,
                       const float* mass, const float* rho, const float* grad_rho,
                       float* dudt, float* drhodt, float* terms, void* stream);
 
+/* vol[i] = mass[i] / rho[i] [n]: the fluid volume V_j = m_j / rho_j of the
+ * pair terms above (deltaPlus.py forms it per pair from the neighborDict),
+ * computed once per particle. */
+int dsph_volume(int64_t n, const float* mass, const float* rho, float* vol, void* stream);
+
+/* dsph_deltasph_normalization reading the fluid V_j from `vol` (from
+ * dsph_volume at the same mass and rho; NULL: formed per pair) in place of
+ * m_j, instead of dividing m_j by rho_j for every pair: the same quotient,
+ * so the results are bit-identical, and the pass is bound by VALU issue,
+ * where the division is ~10 instructions (a further gathered field costs
+ * the right-hand side pass more than its division, so it keeps it). */
+int dsph_deltasph_normalization_v(const dsph_pairs* pairs, const dsph_boundary* boundary,
+                                  const float* mass, const float* rho, const float* vol,
+                                  float* L, float* eigvals, float* grad_rho, void* stream);
+
 /* Ghost particles of the wall (buildBoundaryGhostParticles,
  * v2/modules/mDBC.py:9-17): x_g = x_b + (d + max(d, -h)) n, d the signed
  * distance (negative inside the wall), n the normal pointing into it. */
