@@ -105,6 +105,9 @@ def parse():
     ap.add_argument("--neighbor-width", type=int, default=67,
                     help="row width of the one-pass neighbor list (0: count pass + two-pass list)")
     ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--tile-cap", type=int, default=None,
+                    help="dfsph: neighbor tiles of at most this many particles per workgroup in the pressure sweeps "
+                         "(0: the sweeps gather from global memory; default: the solver's)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -161,7 +164,8 @@ def main():
         dd = d if seed == args.seed else sampling.dfsph_random_flow(args.nside, seed=seed, jitter_width=0.1)
         return DFSPHSolver(T(dd["pos"]), T(dd["vel"]), T(dd["mass"]), T(dd["area"]),
                            DFSPHConfig(dt=dt, support=d["h"], warm_start=warm, neighbor_width=args.neighbor_width,
-                                       threshold_den=threshold_den))
+                                       threshold_den=threshold_den,
+                                       **({} if args.tile_cap is None else {"tile_cap": args.tile_cap})))
 
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
     t0 = time.perf_counter()
@@ -315,10 +319,14 @@ def main():
         f"sweep B (dsph_pressure_update_rec): {t_b * 1e6:.1f} us, model {model_b / 1e6:.1f} MB -> "
         f"{model_b / t_b / 1e9:.0f} GB/s (layout {bytes_b / 1e6:.1f} MB -> {bytes_b / t_b / 1e9:.0f} GB/s); "
         f"pairs {P} ({P / n:.1f}/particle)")
+    tiles = solver.tile_stats()
+    if tiles:
+        log(f"neighbor tiles: {tiles}")
+    sfx = "_tiled" if tiles else ""
     if t_b >= t_a:
-        kname, tb, mbyt, byt = "k_dfsph_sweep_b", t_b, model_b, bytes_b
+        kname, tb, mbyt, byt = "k_dfsph_sweep_b" + sfx, t_b, model_b, bytes_b
     else:
-        kname, tb, mbyt, byt = "k_dfsph_sweep_a", t_a, model_a, bytes_a
+        kname, tb, mbyt, byt = "k_dfsph_sweep_a" + sfx, t_a, model_a, bytes_a
     ach = mbyt / tb / 1e9
     # HBM bytes per launch from the committed PMC profile of the same kernel
     # (FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy): its ratio to the
@@ -333,8 +341,9 @@ def main():
         if prof_bytes and prof_alg:
             ratio = prof_bytes / prof_alg
             traffic = int(round(ratio * byt))
-    other = "k_dfsph_sweep_a" if kname == "k_dfsph_sweep_b" else "k_dfsph_sweep_b"
-    t_o, m_o = (t_a, model_a) if kname == "k_dfsph_sweep_b" else (t_b, model_b)
+    b_is_max = t_b >= t_a
+    other = ("k_dfsph_sweep_a" if b_is_max else "k_dfsph_sweep_b") + sfx
+    t_o, m_o = (t_a, model_a) if b_is_max else (t_b, model_b)
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "kernel_us": round(tb * 1e6, 2), "algorithmic_bytes": int(mbyt),
@@ -343,6 +352,7 @@ def main():
                 "layout_frac": round(byt / tb / 1e9 / HBM_PEAK_GBS, 4),
                 "traffic_over_layout": None if ratio is None else round(ratio, 4),
                 "neighbor_index": "int16 delta" if idx16 else "int32",
+                "neighbor_tiles": tiles or None,
                 "other_sweep": {"kernel": other, "achieved": round(m_o / t_o / 1e9, 1),
                                 "kernel_us": round(t_o * 1e6, 2)}}
 

@@ -18,14 +18,17 @@ c_f32p = C.c_void_p  # device pointers are passed as void*
 
 
 class Domain(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [("min", C.c_float * 2), ("max", C.c_float * 2), ("periodic", C.c_int32 * 2)]
 
 
 class Grid(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [("ncell", C.c_int32 * 2), ("hcell", C.c_float), ("pad_", C.c_int32)]
 
 
 class Pairs(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("nq", C.c_int64),
         ("nr", C.c_int64),
@@ -42,10 +45,16 @@ class Pairs(C.Structure):
         ("slice_off", C.c_void_p),
         ("nbr16", C.c_void_p),
         ("nbr16_overflow", C.c_void_p),
+        ("tile_slot", C.c_void_p),
+        ("tile_runs", C.c_void_p),
+        ("tile_n", C.c_void_p),
+        ("tile_cap", C.c_int32),
+        ("tile_pad_", C.c_int32),
     ]
 
 
 class PairArrays(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("num_rows", C.c_int64),
         ("num_ref", C.c_int64),
@@ -62,6 +71,7 @@ class PairArrays(C.Structure):
 
 
 class DfsphParams(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("dt", C.c_float),
         ("rho0", C.c_float),
@@ -77,6 +87,7 @@ class DfsphParams(C.Structure):
 
 
 class SolveStats(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("iterations", C.c_int32),
         ("converged", C.c_int32),
@@ -86,6 +97,7 @@ class SolveStats(C.Structure):
 
 
 class ViscParams(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("alpha", C.c_float),
         ("cs", C.c_float),
@@ -97,6 +109,7 @@ class ViscParams(C.Structure):
 
 
 class DeltaSPHParams(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("rho0", C.c_float),
         ("cs", C.c_float),
@@ -115,6 +128,7 @@ class DeltaSPHParams(C.Structure):
 
 
 class Boundary(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("pairs", C.POINTER(Pairs)),
         ("mass", C.c_void_p),
@@ -128,6 +142,7 @@ class Boundary(C.Structure):
 
 
 class RK4Stages(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("dxdt", C.c_void_p * 4),
         ("dudt", C.c_void_p * 4),
@@ -136,6 +151,7 @@ class RK4Stages(C.Structure):
 
 
 class BicgstabStats(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("iterations", C.c_int32),
         ("status", C.c_int32),
@@ -145,6 +161,7 @@ class BicgstabStats(C.Structure):
 
 
 class TimestepParams(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("support", C.c_float),
         ("kernel_scale", C.c_float),
@@ -160,6 +177,7 @@ class TimestepParams(C.Structure):
 
 
 class Batch(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("n", C.c_int64),
         ("ncell", C.c_int64),
@@ -175,6 +193,7 @@ class Batch(C.Structure):
 
 
 class ShiftParams(C.Structure):
+    __slots__ = ()  # a misspelt field name raises instead of being ignored
     _fields_ = [
         ("dx", C.c_double),
         ("support", C.c_double),
@@ -215,6 +234,7 @@ SIGNATURES = {
     "dsph_sell_to_csr": (C.c_int, [P(Pairs), V, V]),
     "dsph_wrap_positions": (C.c_int, [P(Domain), V, I64, V, V]),
     "dsph_neighbor_compact16": (C.c_int, [P(Pairs), I32, V, I64, V, V]),
+    "dsph_neighbor_tiles": (C.c_int, [P(Pairs), P(Grid), V, I32, V, I64, V, V, V]),
     "dsph_scan_workspace": (SZ, [I64]),
     "dsph_scan_exclusive": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
     "dsph_gather_f32": (C.c_int, [V, V, I64, I32, V, V]),

@@ -209,6 +209,10 @@ struct PairsDev {
   const int32_t* slice;
   const int16_t* nbr16;  // optional 16-bit deltas (pressure sweeps)
   const int32_t* ovf;    // nonzero: nbr16 invalid
+  const uint16_t* tslot; // optional neighbor tiles (pressure sweeps)
+  const int32_t* truns;
+  const int32_t* tn;
+  int tcap;
 };
 
 inline PairsDev make_pairs(const dsph_pairs& p) {
@@ -228,6 +232,10 @@ inline PairsDev make_pairs(const dsph_pairs& p) {
   o.slice = p.slice_off;
   o.nbr16 = p.nbr16;
   o.ovf = p.nbr16_overflow;
+  o.tslot = p.tile_slot;
+  o.truns = p.tile_runs;
+  o.tn = p.tile_n;
+  o.tcap = p.tile_cap;
   return o;
 }
 

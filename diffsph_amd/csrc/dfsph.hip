@@ -572,6 +572,22 @@ __device__ __forceinline__ float sweep_b_kv(const PairsDev& P, float2 xi, const 
   return sum;
 }
 
+// The pressure update of row i from its neighbor sum; returns the row's
+// term of the error metric.
+__device__ __forceinline__ double sweep_b_finish(const SweepBArgs& B, int i, float sum, float2 xr) {
+  const float K = -B.dtf * sum;
+  const float s = B.src[i];
+  const float res = K - s;
+  float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
+  if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
+  B.p[i] = pn;
+  if (B.xp) B.xp[i] = make_float4(xr.x, xr.y, pn, 0.f);
+  if (B.residual) B.residual[i] = res;
+  if (B.isph) return (double)fabsf(res);
+  const float rr0 = res / B.rho0;
+  return (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
+}
+
 // One row of sweep B; returns the row's term of the error metric.
 __device__ __forceinline__ double sweep_b_row(const PairsDev& P, const SweepBArgs& B, int i) {
   float2 ai, xr;
@@ -607,17 +623,7 @@ __device__ __forceinline__ double sweep_b_row(const PairsDev& P, const SweepBArg
       sum += (aj.x - ai.x) * c.x + (aj.y - ai.y) * c.y;
     }
   }
-  const float K = -B.dtf * sum;
-  const float s = B.src[i];
-  const float res = K - s;
-  float pn = B.p[i] + B.omega * (s - K) / B.alpha[i];
-  if (B.clamp) pn = pn < 0.0f ? 0.0f : pn;  // torch.max(p, 0)
-  B.p[i] = pn;
-  if (B.xp) B.xp[i] = make_float4(xr.x, xr.y, pn, 0.f);
-  if (B.residual) B.residual[i] = res;
-  if (B.isph) return (double)fabsf(res);
-  const float rr0 = res / B.rho0;
-  return (double)(rr0 < B.neg_thr ? B.neg_thr : rr0);
+  return sweep_b_finish(B, i, sum, xr);
 }
 
 // Sweep B (dfsph.py:39-50): K_i = -dtf sum_j (a_j - a_i) . V_j gradW_ij,
@@ -633,6 +639,409 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_sweep_b(PairsDev P, SweepBArgs
   if (i < P.nq) err = sweep_b_row(P, B, i);
   double bs = block_sum_d(err, sh);
   if (threadIdx.x == 0) B.partials[lb] = bs;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbor tiles.  The 256 rows of a workgroup are consecutive particles in
+// cell order; the search finds their neighbors in the 3 x 3 cells around
+// each row's cell, which in cell order are a few runs of consecutive
+// particles (three for a workgroup within one row of cells, ~900 particles
+// at 16 per cell).  The sweeps load those runs' 16-byte records into LDS
+// once per workgroup (contiguous loads) and read them there through a
+// 16-bit slot per list entry, instead of one global gather per entry.
+// Arithmetic and summation order are those of the gathering sweeps, so the
+// results are bit-identical.
+static_assert(kBlock == DSPH_TILE_ROWS, "one tile per workgroup of sweep rows");
+// The tiled sweeps keep 8 waves per SIMD (their kernel arguments and two
+// row paths would take 100+ SGPRs, i.e. 7 waves; the excess goes to VGPR
+// lanes): 78.7 -> 72.7 us (A) and 82.1 -> 75.1 us (B) at 1M particles.
+#define DSPH_TILED_ATTR __attribute__((amdgpu_waves_per_eu(8)))
+constexpr int kTileMaxRuns = DSPH_TILE_MAX_RUNS;
+constexpr int kTileRunStride = 2 * kTileMaxRuns + 1;  // [R, start_0, len_0, ...]
+constexpr int kTileMaxSegs = 4;  // rows of cells one workgroup may span (9 ranges each)
+
+struct IdxSlot {
+  const uint16_t* a;
+  __device__ __forceinline__ int operator()(int k) const { return (int)__builtin_nontemporal_load(a + k); }
+};
+
+// Wave-wide helpers of the tile build (64 lanes, all active).
+__device__ __forceinline__ int wave_incl_max(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d, 64);
+    if (lane >= d) v = v > y ? v : y;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
+  }
+  return v;
+}
+
+// The cell holding sorted particle i, the last c with cell_start[c] <= i,
+// by a 64-way search of the wave (3 rounds for 62k cells).
+__device__ __forceinline__ int wave_cell_of(const int32_t* __restrict__ cs, int ncell, int i) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = ncell - 1;  // cs[lo] <= i; the answer is in [lo, hi]
+  while (hi > lo) {
+    const int step = (hi - lo + 64) / 64;
+    const int c = lo + lane * step;
+    const bool ok = c <= hi && cs[c] <= i;
+    const unsigned long long m = __ballot(ok);
+    lo += (63 - __clzll(m)) * step;
+    hi = lo + step - 1 < hi ? lo + step - 1 : hi;
+  }
+  return lo;
+}
+
+// Tile runs, one wave per workgroup of rows: the cells of its rows span
+// rows y0..y1 of the grid (segments); each segment's 3 x 3 stencil is three
+// rows of cells by one interval (plus the pieces a periodic seam wraps),
+// each a range of consecutive particles.  Lane l < 36 forms range l, the
+// wave sorts them by start and merges overlapping or touching ones into
+// disjoint runs.  tile_n = the runs' particle count, or -1 when the rows
+// span more than kTileMaxSegs rows of cells or the runs number more than
+// kTileMaxRuns.
+__global__ void __launch_bounds__(kBlock) k_tile_runs(int nq, int ncx, int ncy, int px, int py,
+                                                      const int32_t* __restrict__ cs, int ntiles,
+                                                      int32_t* __restrict__ tile_runs, int32_t* __restrict__ tile_n) {
+  __shared__ int srt_s[kBlock], srt_e[kBlock];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int b = blockIdx.x * (kBlock / 64) + wv;
+  if (b >= ntiles) return;  // uniform over the wave; no block barrier follows
+  int* ss = srt_s + wv * 64;
+  int* se = srt_e + wv * 64;
+  const int ncell = ncx * ncy;
+  const int i0 = b * kBlock;
+  const int i1 = (i0 + kBlock < nq ? i0 + kBlock : nq) - 1;
+  const int c0 = wave_cell_of(cs, ncell, i0), c1 = wave_cell_of(cs, ncell, i1);
+  const int y0 = c0 / ncx, y1 = c1 / ncx;
+  if (y1 - y0 + 1 > kTileMaxSegs) {
+    if (lane == 0) tile_n[b] = -1;
+    return;
+  }
+  constexpr int kEmpty = 0x7fffffff;
+  int rs = kEmpty, re = kEmpty;
+  if (lane < kTileMaxSegs * 9) {
+    const int y = y0 + lane / 9, dy = (lane / 3) % 3 - 1, part = lane % 3;
+    int r = y + dy;
+    bool ok = y <= y1;
+    if (r < 0 || r >= ncy) {
+      ok = ok && py;
+      r += r < 0 ? ncy : -ncy;
+    }
+    const int a = y == y0 ? c0 - y0 * ncx : 0;
+    const int e = y == y1 ? c1 - y1 * ncx : ncx - 1;
+    int lo = a - 1, hi = e + 1, xa = 0, xb = -1;
+    if (hi - lo + 1 >= ncx) {
+      if (part == 0) xa = 0, xb = ncx - 1;
+    } else if (part == 0) {
+      xa = lo < 0 ? 0 : lo;
+      xb = hi >= ncx ? ncx - 1 : hi;
+    } else if (part == 1) {
+      if (lo < 0 && px) xa = ncx + lo, xb = ncx - 1;
+    } else {
+      if (hi >= ncx && px) xa = 0, xb = hi - ncx;
+    }
+    if (ok && xb >= xa) {
+      const int s0 = cs[r * ncx + xa], s1 = cs[r * ncx + xb + 1];
+      if (s1 > s0) rs = s0, re = s1;
+    }
+  }
+  // rank sort by (start, lane)
+  int rank = 0;
+  for (int u = 0; u < 64; ++u) {
+    const int su = __shfl(rs, u, 64);
+    rank += (su < rs || (su == rs && u < lane)) ? 1 : 0;
+  }
+  ss[rank] = rs;
+  se[rank] = re;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  rs = ss[lane];
+  re = se[lane];
+  const bool valid = rs != kEmpty;
+  // merge: a range opens a run when it starts past every earlier end
+  const int pm_in = wave_incl_max(valid ? re : -1);
+  int pm_ex = __shfl_up(pm_in, 1, 64);
+  if (lane == 0) pm_ex = -1;
+  const bool opens = valid && rs > pm_ex;
+  const int run = wave_incl_sum(opens ? 1 : 0) - 1;
+  const int R = __shfl(run + 1, 63, 64);
+  const bool next_opens = __shfl_down(opens ? 1 : 0, 1, 64) != 0 || lane == 63;
+  const bool next_valid = lane < 63 ? __shfl_down(valid ? 1 : 0, 1, 64) != 0 : false;
+  const bool closes = valid && (next_opens || !next_valid);
+  // run r: start from its opening lane, end (the prefix max) from its closing lane
+  __builtin_amdgcn_wave_barrier();
+  if (opens && run < kTileMaxRuns) ss[run] = rs;
+  if (closes && run < kTileMaxRuns) se[run] = pm_in;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (R > kTileMaxRuns) {
+    if (lane == 0) tile_n[b] = -1;
+    return;
+  }
+  const int len = lane < R ? se[lane] - ss[lane] : 0;
+  const int tot = wave_incl_sum(len);
+  int32_t* tr = tile_runs + (size_t)b * kTileRunStride;
+  if (lane == 0) tr[0] = R;
+  if (lane < R) {
+    tr[1 + 2 * lane] = ss[lane];
+    tr[2 + 2 * lane] = len;
+  }
+  if (lane == 63) tile_n[b] = tot;
+}
+
+// Slots of one row's entries, kSlotUnroll at a time: their index loads in
+// flight together, then every run (a uniform loop, its bounds read once per
+// group) claims the entries it holds.  False when a neighbor lies in no
+// run.
+template <int U>
+__device__ __forceinline__ bool tile_claim(const int* jj, int* sl, const int* run_s, const int* run_e,
+                                           const int* run_d, int R) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) sl[u] = -1;
+  for (int r = 0; r < R; ++r) {
+    const int s0 = run_s[r], e0 = run_e[r], d0 = run_d[r];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (jj[u] >= s0 && jj[u] < e0) sl[u] = jj[u] + d0;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int u = 0; u < U; ++u) ok = ok && sl[u] >= 0;
+  return ok;
+}
+
+template <class Idx>
+__device__ __forceinline__ bool tile_slots_row(const RowRef& rr, const Idx& idx, const int* run_s, const int* run_e,
+                                               const int* run_d, int R, uint16_t* __restrict__ slot) {
+  constexpr int kSlotUnroll = 8;
+  bool ok = true;
+  int c = 0, k = rr.base;
+  for (; c + kSlotUnroll <= rr.cnt; c += kSlotUnroll, k += kSlotUnroll * rr.step) {
+    int jj[kSlotUnroll], sl[kSlotUnroll];
+#pragma unroll
+    for (int u = 0; u < kSlotUnroll; ++u) jj[u] = idx(k + u * rr.step);
+    ok = tile_claim<kSlotUnroll>(jj, sl, run_s, run_e, run_d, R) && ok;
+#pragma unroll
+    for (int u = 0; u < kSlotUnroll; ++u) slot[k + u * rr.step] = (uint16_t)sl[u];
+  }
+  for (; c < rr.cnt; ++c, k += rr.step) {
+    int j = idx(k), sl;
+    ok = tile_claim<1>(&j, &sl, run_s, run_e, run_d, R) && ok;
+    slot[k] = (uint16_t)sl;
+  }
+  return ok;
+}
+
+// Entry slots of tile b: the position of each entry's neighbor in the runs
+// laid end to end.  The run of the previous entry is tried first (a row's
+// entries come a run at a time).  An entry outside every run marks the
+// tile unusable (tile_n = -1).
+__global__ void __launch_bounds__(kBlock) k_tile_slots(PairsDev P, int cap, uint16_t* __restrict__ slot,
+                                                       int64_t slot_cap, const int32_t* __restrict__ tile_runs,
+                                                       int32_t* __restrict__ tile_n) {
+  __shared__ int run_s[kTileMaxRuns], run_e[kTileMaxRuns], run_d[kTileMaxRuns];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int S = tile_n[b];
+  if (S < 0 || S > cap) return;  // uniform
+  const int32_t* tr = tile_runs + (size_t)b * kTileRunStride;
+  const int R = tr[0];
+  if (t < 64) {  // wave 0: run u in lane u, offsets by a prefix sum
+    const int s0 = t < R ? tr[1 + 2 * t] : 0, len = t < R ? tr[2 + 2 * t] : 0;
+    const int o = wave_incl_sum(len) - len;
+    if (t < R) {
+      run_s[t] = s0;
+      run_e[t] = s0 + len;
+      run_d[t] = o - s0;
+    }
+  }
+  __syncthreads();
+  const int i = b * kBlock + t;
+  if (i >= P.nq) return;
+  const RowRef rr = row_ref(P, i);
+  if (rr.cnt > 0 && (rr.base < 0 || (int64_t)rr.base + (int64_t)(rr.cnt - 1) * rr.step >= slot_cap)) {
+    tile_n[b] = -1;
+    return;
+  }
+  const bool bad = use_idx16(P) ? !tile_slots_row(rr, Idx16{P.nbr16, i, P.nr}, run_s, run_e, run_d, R, slot)
+                                 : !tile_slots_row(rr, Idx32{P.nbr}, run_s, run_e, run_d, R, slot);
+  if (bad) tile_n[b] = -1;
+}
+
+// Loads tile lb's records into LDS, run after run; false (uniform) when the
+// workgroup has no usable tile.
+__device__ __forceinline__ bool load_tile(const PairsDev& P, int lb, const float4* __restrict__ rec, float4* tile) {
+  const int S = P.tn[lb];
+  if (S < 0 || S > P.tcap) return false;
+  const int32_t* tr = P.truns + (size_t)lb * kTileRunStride;
+  const int R = tr[0];
+  int o = 0;
+  for (int u = 0; u < R; ++u) {
+    const int s0 = tr[1 + 2 * u], len = tr[2 + 2 * u];
+    for (int s = threadIdx.x; s < len; s += kBlock) tile[o + s] = rec[s0 + s];
+    o += len;
+  }
+  __syncthreads();
+  return true;
+}
+
+// Sweep A with the (x, y, p) records of the workgroup's neighbors in LDS.
+__global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_dfsph_sweep_a_tiled(PairsDev P, SweepAArgs A,
+                                                                                const SolveCtrl* ctrl) {
+  extern __shared__ float4 tile[];
+  if (ctrl && ctrl->done) return;
+  const int lb = logical_block();
+  const int i = lb * kBlock + threadIdx.x;
+  if (!load_tile(P, lb, A.xp, tile)) {
+    if (i < P.nq) sweep_a_row(P, A, i);
+    return;
+  }
+  if (i >= P.nq) return;
+  const float4 own = A.xp[i];
+  const float sj = A.diff ? -1.0f : 1.0f;
+  float ax = 0.f, ay = 0.f;
+  sweep_a_kv(P, P.qpos[i], row_ref(P, i), IdxSlot{P.tslot}, A.kv, GatherXP4{tile}, own.z, sj, ax, ay);
+  const float r = A.rho[i];
+  const float2 a = make_float2(-ax / r, -ay / r);
+  if (A.xa)
+    A.xa[i] = make_float4(own.x, own.y, a.x, a.y);
+  else
+    A.acc[i] = a;
+}
+
+// Sweep B with the (x, y, a) records of the workgroup's neighbors in LDS.
+__global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_dfsph_sweep_b_tiled(PairsDev P, SweepBArgs B) {
+  extern __shared__ float4 tile[];
+  __shared__ double sh[kBlock / 64];
+  if (B.ctrl && B.ctrl->done) return;
+  const int lb = logical_block();
+  const int i = lb * kBlock + threadIdx.x;
+  double err = 0.0;
+  if (load_tile(P, lb, B.xa, tile)) {
+    if (i < P.nq) {
+      const float4 own = B.xa[i];
+      const float2 ai = make_float2(own.z, own.w);
+      const float sum = sweep_b_kv(P, P.qpos[i], row_ref(P, i), IdxSlot{P.tslot}, B.kv, GatherXA4{tile}, ai);
+      err = sweep_b_finish(B, i, sum, make_float2(own.x, own.y));
+    }
+  } else if (i < P.nq) {
+    err = sweep_b_row(P, B, i);
+  }
+  double bs = block_sum_d(err, sh);
+  if (threadIdx.x == 0) B.partials[lb] = bs;
+}
+
+// Divergence source (as k_dfsph_source) with the (x, y, v*) records of the
+// workgroup's neighbors in LDS.
+__global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_dfsph_source_tiled(
+    PairsDev P, const float* __restrict__ kv, const float2* __restrict__ vel, const float2* __restrict__ adv,
+    const float* __restrict__ vol, const float* __restrict__ areas, float dt, int density, float* __restrict__ src,
+    const float4* __restrict__ rec) {
+  extern __shared__ float4 tile[];
+  const int b = blockIdx.x;
+  const int i = b * kBlock + threadIdx.x;
+  const bool tiled = load_tile(P, b, rec, tile);
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const float2 a_i = adv ? adv[i] : make_float2(0.f, 0.f);
+  const float2 vi = make_float2(vel[i].x + dt * a_i.x, vel[i].y + dt * a_i.y);
+  const RowRef rr = row_ref(P, i);
+  float dv;
+  if (tiled)
+    dv = source_kv(P, xi, rr, IdxSlot{P.tslot}, kv, GatherXV4{tile}, vi);
+  else if (use_idx16(P))
+    dv = source_kv(P, xi, rr, Idx16{P.nbr16, i, P.nr}, kv, GatherXV4{rec}, vi);
+  else
+    dv = source_kv(P, xi, rr, Idx32{P.nbr}, kv, GatherXV4{rec}, vi);
+  src[i] = density ? (1.0f - areas[i] / vol[i]) - dt * dv : dv;
+}
+
+// The warm-start feature sums (as k_ws_features with W_ij streamed and the
+// (V, p_div_prev, s, p_den_prev) records): four entries' loads in flight,
+// then their sums in entry order.
+template <class Idx>
+__device__ __forceinline__ void ws_feature_sums(const RowRef& rr, const Idx& idx, const float4* rec,
+                                                const float* __restrict__ pair_w, float& s1, float& s3, float& s5) {
+  int n = 0, k = rr.base;
+  for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+    int j[4];
+    float w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = idx(k + u * rr.step);
+      w[u] = __builtin_nontemporal_load(pair_w + k + u * rr.step);
+    }
+    float4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = rec[j[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float vw = r[u].x * w[u];
+      s1 += vw * r[u].y;
+      s3 += vw * r[u].z;
+      s5 += vw * r[u].w;
+    }
+  }
+  for (; n < rr.cnt; ++n, k += rr.step) {
+    const float4 r = rec[idx(k)];
+    const float vw = r.x * __builtin_nontemporal_load(pair_w + k);
+    s1 += vw * r.y;
+    s3 += vw * r.z;
+    s5 += vw * r.w;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_ws_features_tiled(
+    PairsDev P, const float* __restrict__ rho, const float* __restrict__ pdp, const float* __restrict__ pnp,
+    const float* __restrict__ src, const float* __restrict__ alpha, float rho0, const float* __restrict__ pair_w,
+    const float* __restrict__ mean, const float* __restrict__ istd, float* __restrict__ X,
+    const float4* __restrict__ rec) {
+  extern __shared__ float4 tile[];
+  const int b = blockIdx.x;
+  const int i = b * kBlock + threadIdx.x;
+  const bool tiled = load_tile(P, b, rec, tile);
+  if (i >= P.nq) return;
+  float s1 = 0.f, s3 = 0.f, s5 = 0.f;
+  const RowRef rr = row_ref(P, i);
+  if (tiled)
+    ws_feature_sums(rr, IdxSlot{P.tslot}, tile, pair_w, s1, s3, s5);
+  else if (use_idx16(P))
+    ws_feature_sums(rr, Idx16{P.nbr16, i, P.nr}, rec, pair_w, s1, s3, s5);
+  else
+    ws_feature_sums(rr, Idx32{P.nbr}, rec, pair_w, s1, s3, s5);
+  float f[8] = {pdp[i], s1, src[i], s3, pnp[i], s5, rho[i] / rho0 - 1.0f, alpha[i]};
+  float* o = X + (int64_t)i * 16;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = (f[c] - mean[c]) * istd[c];
+#pragma unroll
+  for (int c = 8; c < 16; ++c) o[c] = 0.0f;
+}
+
+// Sweep launches: over neighbor tiles when the list carries them and the
+// sweep gathers records.
+static void launch_sweep_a(const PairsDev& P, const SweepAArgs& A, const SolveCtrl* ctrl, int nb, hipStream_t st) {
+  if (P.tslot && A.kv && A.xp)
+    k_dfsph_sweep_a_tiled<<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, A, ctrl);
+  else
+    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, A.kv, A.vol, A.rho, A.p, A.acc, ctrl, A.diff, A.xp, A.xa);
+}
+
+static void launch_sweep_b(const PairsDev& P, const SweepBArgs& B, int nb, hipStream_t st) {
+  if (P.tslot && B.kv && B.xa)
+    k_dfsph_sweep_b_tiled<<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, B);
+  else
+    k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
 }
 
 // One block: fixed-order sum of the sweep-B partials, the reference's loop
@@ -941,6 +1350,29 @@ extern "C" int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit, i
   return launch_status();
 }
 
+extern "C" int dsph_neighbor_tiles(const dsph_pairs* pairs, const dsph_grid* grid, const int32_t* cell_start,
+                                   int32_t cap, uint16_t* slot, int64_t slot_capacity, int32_t* tile_runs,
+                                   int32_t* tile_n, void* stream) {
+  if (!pairs || !grid) return DSPH_ERR_ARG;
+  dsph_pairs q = *pairs;
+  q.tile_slot = nullptr;  // the tiles being built are not read
+  int s = check_pairs(&q);
+  if (s != DSPH_OK) return s;
+  if (q.nq != q.nr || cap < 1 || cap > DSPH_TILE_MAX_CAP || slot_capacity < 0) return DSPH_ERR_ARG;
+  if (grid->ncell[0] < 1 || grid->ncell[1] < 1 || (int64_t)grid->ncell[0] * grid->ncell[1] >= INT32_MAX)
+    return DSPH_ERR_ARG;
+  if (q.nq == 0) return DSPH_OK;
+  if (!slot || !tile_runs || !tile_n || !cell_start) return DSPH_ERR_ARG;
+  hipStream_t st = as_stream(stream);
+  const int nt = grid_for(q.nq);
+  k_tile_runs<<<(nt + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(
+      (int)q.nq, grid->ncell[0], grid->ncell[1], q.domain.periodic[0] != 0, q.domain.periodic[1] != 0, cell_start,
+      nt, tile_runs, tile_n);
+  DSPH_LAUNCH_CHECK();
+  k_tile_slots<<<nt, kBlock, 0, st>>>(make_pairs(q), cap, slot, slot_capacity, tile_runs, tile_n);
+  return launch_status();
+}
+
 extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, const float* mass, const float* areas,
                                   const float* vol, const float* rho, float dt, float rho0,
                                   const dsph_visc_params* visc, float* alpha_div, float* alpha_den, float* div_src,
@@ -1023,9 +1455,14 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
                                         reinterpret_cast<const float2*>(adv), prm->dt, (int)n, xv);
       DSPH_LAUNCH_CHECK();
     }
-    k_dfsph_source<<<nb, kBlock, 0, st>>>(P, kv, reinterpret_cast<const float2*>(vel),
-                                          reinterpret_cast<const float2*>(adv), vol, areas, prm->dt,
-                                          prm->source_density, src, xv);
+    if (P.tslot && kv && xv)
+      k_dfsph_source_tiled<<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(
+          P, kv, reinterpret_cast<const float2*>(vel), reinterpret_cast<const float2*>(adv), vol, areas, prm->dt,
+          prm->source_density, src, xv);
+    else
+      k_dfsph_source<<<nb, kBlock, 0, st>>>(P, kv, reinterpret_cast<const float2*>(vel),
+                                            reinterpret_cast<const float2*>(adv), vol, areas, prm->dt,
+                                            prm->source_density, src, xv);
   } else {
     DSPH_CHECK_HIP(hipMemcpyAsync(src, src0, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
   }
@@ -1048,9 +1485,9 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     int k = chunk;
     if (launched + k > prm->max_iters) k = prm->max_iters - launched;
     for (int t = 0; t < k; ++t) {
-      k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl,
-                                             B.isph, xp, xa);
-      k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+      launch_sweep_a(P, SweepAArgs{kv, vol, rho, p, reinterpret_cast<float2*>(accel), B.isph, xp, xa}, ctrl, nb,
+                     st);
+      launch_sweep_b(P, B, nb, st);
       k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
     }
     DSPH_LAUNCH_CHECK();
@@ -1064,8 +1501,8 @@ extern "C" int dsph_dfsph_solve(const dsph_pairs* pairs, const dsph_dfsph_params
     DSPH_CHECK_HIP(hipStreamSynchronize(st));
   }
   // Final pressure acceleration (dfsph.py:105).
-  k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, B.isph, xp,
-                                         nullptr);
+  launch_sweep_a(P, SweepAArgs{kv, vol, rho, p, reinterpret_cast<float2*>(accel), B.isph, xp, nullptr}, nullptr, nb,
+                 st);
   DSPH_LAUNCH_CHECK();
   if (stats) {
     stats->iterations = host.iter;
@@ -1084,8 +1521,8 @@ extern "C" int dsph_pressure_accel(const dsph_pairs* pairs, const float* kgrad, 
   if (pairs->nq == 0) return DSPH_OK;
   if (!vol || !rho || !p || !accel) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, kgrad, vol, rho, p, reinterpret_cast<float2*>(accel), nullptr, 0, nullptr, nullptr);
+  launch_sweep_a(P, SweepAArgs{kgrad, vol, rho, p, reinterpret_cast<float2*>(accel), 0, nullptr, nullptr}, nullptr,
+                 grid_for(P.nq), as_stream(stream));
   return launch_status();
 }
 
@@ -1100,7 +1537,7 @@ extern "C" int dsph_pressure_update(const dsph_pairs* pairs, const dsph_dfsph_pa
   if (!src || !alpha || !accel || !p || !block_partials || (!kgrad && !vol)) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
   SweepBArgs B = make_sweep_b(prm, kgrad, vol, src, alpha, p, residual, accel, block_partials, nullptr, n);
-  k_dfsph_sweep_b<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(P, B);
+  launch_sweep_b(P, B, grid_for(n), as_stream(stream));
   return launch_status();
 }
 
@@ -1124,9 +1561,10 @@ extern "C" int dsph_pressure_accel_rec(const dsph_pairs* pairs, const float* kgr
   if (pairs->nq == 0) return DSPH_OK;
   if (!kgrad || !rho || !xp || !xa) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_dfsph_sweep_a<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, kgrad, nullptr, rho, nullptr, nullptr, nullptr, 0, reinterpret_cast<const float4*>(xp),
-      reinterpret_cast<float4*>(xa));
+  launch_sweep_a(P,
+                 SweepAArgs{kgrad, nullptr, rho, nullptr, nullptr, 0, reinterpret_cast<const float4*>(xp),
+                            reinterpret_cast<float4*>(xa)},
+                 nullptr, grid_for(P.nq), as_stream(stream));
   return launch_status();
 }
 
@@ -1143,7 +1581,7 @@ extern "C" int dsph_pressure_update_rec(const dsph_pairs* pairs, const dsph_dfsp
   SweepBArgs B = make_sweep_b(prm, kgrad, nullptr, src, alpha, p, residual, nullptr, block_partials, nullptr, n);
   B.xa = reinterpret_cast<const float4*>(xa);
   B.xp = reinterpret_cast<float4*>(xp);
-  k_dfsph_sweep_b<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(P, B);
+  launch_sweep_b(P, B, grid_for(n), as_stream(stream));
   return launch_status();
 }
 
@@ -1174,9 +1612,8 @@ extern "C" int dsph_dfsph_iterate(const dsph_pairs* pairs, const dsph_dfsph_para
   B.xp = xp;
   if (xp) k_pack_xp<<<nb, kBlock, 0, st>>>(reinterpret_cast<const float2*>(P.rpos), p, (int)n, xp);
   for (int t = 0; t < iterations; ++t) {
-    k_dfsph_sweep_a<<<nb, kBlock, 0, st>>>(P, kv, vol, rho, p, reinterpret_cast<float2*>(accel), ctrl, B.isph, xp,
-                                           xa);
-    k_dfsph_sweep_b<<<nb, kBlock, 0, st>>>(P, B);
+    launch_sweep_a(P, SweepAArgs{kv, vol, rho, p, reinterpret_cast<float2*>(accel), B.isph, xp, xa}, ctrl, nb, st);
+    launch_sweep_b(P, B, nb, st);
     k_dfsph_reduce<<<1, kBlock, 0, st>>>(partials, nb, (int)n, ctrl);
   }
   DSPH_LAUNCH_CHECK();
@@ -1253,7 +1690,11 @@ extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const
     k_pack_features<<<grid_for(P.nq), kBlock, 0, st>>>(P.nq, vol, p_div_prev, div_src, p_den_prev, rec);
     DSPH_LAUNCH_CHECK();
   }
-  k_ws_features<<<grid_for(P.nq), kBlock, 0, st>>>(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0,
-                                                   pair_w, mean16, istd16, X, rec);
+  if (P.tslot && rec)
+    k_ws_features_tiled<<<grid_for(P.nq), kBlock, (size_t)P.tcap * sizeof(float4), st>>>(
+        P, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0, pair_w, mean16, istd16, X, rec);
+  else
+    k_ws_features<<<grid_for(P.nq), kBlock, 0, st>>>(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0,
+                                                     pair_w, mean16, istd16, X, rec);
   return launch_status();
 }

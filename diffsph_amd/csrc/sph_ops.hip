@@ -281,6 +281,9 @@ int check_pairs(const dsph_pairs* p) {
   if (!p || p->nq < 0 || p->nr < 0 || p->nq >= INT32_MAX || p->nr >= INT32_MAX) return DSPH_ERR_ARG;
   if (p->mode < 0 || p->mode > 2) return DSPH_ERR_ARG;
   if (p->nbr16 && !p->nbr16_overflow) return DSPH_ERR_ARG;
+  if (p->tile_slot && (!p->tile_runs || !p->tile_n || p->tile_cap < 1 || p->tile_cap > DSPH_TILE_MAX_CAP ||
+                       p->nq != p->nr))
+    return DSPH_ERR_ARG;
   if (p->nq > 0 && (!p->qpos || (!p->slice_off && !p->offsets))) return DSPH_ERR_ARG;
   if (p->nq > 0 && p->slice_off && !p->counts) return DSPH_ERR_ARG;
   if (p->nr > 0 && !p->rpos) return DSPH_ERR_ARG;

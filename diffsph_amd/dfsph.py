@@ -26,7 +26,7 @@ import torch
 
 from . import _lib
 from ._lib import DsphError, check, lib, ptr
-from .neighborhood import WORKSPACE, CellGrid, Delta16Buffers, NeighborList, search
+from .neighborhood import TILE_RUN_STRIDE, WORKSPACE, CellGrid, Delta16Buffers, NeighborList, search
 
 
 @dataclass
@@ -63,6 +63,14 @@ class DFSPHConfig:
     # Odd: a power-of-two slice stride (64 x 64 entries) costs the pressure
     # sweeps 4-6 % at 1M particles, an odd one about 2 %
     neighbor_width: int = 67
+    # neighbor tiles for the pressure sweeps: each workgroup of 256 rows
+    # stages its neighbors' records in LDS (at most tile_cap of them, 16 B
+    # each; a workgroup with more reads them from global memory) and the
+    # sweeps read them there; 0: the sweeps gather from global memory.
+    # Results are identical either way.  A workgroup of the 1M random flow
+    # stages ~910 particles (the 3 x 3 cells around its rows' cells, ~1020
+    # at most); 18 KB of LDS per workgroup keeps 8 waves per SIMD
+    tile_cap: int = 1152
     # one cooperative (persistent) launch per pressure solve: disabled, see
     # DFSPHSolver.__init__
     persistent: bool = False
@@ -129,6 +137,7 @@ class DFSPHSolver:
         # divergence iterations)
         self.hooks = {}
         self._d16 = Delta16Buffers()
+        self._tile_bufs = None
         self._nbr_width = max(int(cfg.neighbor_width), 0)
 
     # -- helpers ---------------------------------------------------------
@@ -170,7 +179,29 @@ class DFSPHSolver:
         if cfg.compact_index and not fused:
             buf, flag = self._d16.get(need, self.pos.device)
             self.nl.attach_compact16(buf, flag, cfg.compact_limit)
+        if cfg.tile_cap > 0:
+            nt = (self.n + 255) // 256
+            t = self._tile_bufs
+            if t is None or t[0].numel() < need:
+                dev = self.pos.device
+                t = self._tile_bufs = (torch.empty(need + need // 8, dtype=torch.int16, device=dev),
+                                       torch.empty(nt * TILE_RUN_STRIDE, dtype=torch.int32, device=dev),
+                                       torch.empty(nt, dtype=torch.int32, device=dev))
+            self.nl.attach_tiles(cfg.tile_cap, *t)
         return self.nl
+
+    def tile_stats(self) -> dict:
+        """Neighbor-tile occupancy of the current list (synchronises): the
+        share of workgroups whose neighbors fit their tile and the mean /
+        largest neighbor count per workgroup."""
+        if self.nl is None or self.nl.tiles is None:
+            return {}
+        cap = self.nl.tiles[3]
+        tn = self.nl.tiles[2][: (self.n + 255) // 256].cpu()
+        ok = (tn >= 0) & (tn <= cap)
+        v = tn[tn >= 0].float()
+        return {"cap": cap, "tiled": float(ok.float().mean()), "mean": float(v.mean()) if v.numel() else 0.0,
+                "max": int(tn.max())}
 
     def _visc_params(self):
         cfg = self.cfg

@@ -28,6 +28,8 @@ from . import _lib
 from ._lib import DsphError, check, lib, ptr
 
 MODES = {"scatter": 0, "gather": 1, "symmetric": 2}
+TILE_MAX_RUNS = 16                      # DSPH_TILE_MAX_RUNS
+TILE_RUN_STRIDE = 2 * TILE_MAX_RUNS + 1  # int32 per workgroup in tile_runs
 
 
 class Workspace:
@@ -137,6 +139,7 @@ class NeighborList:
     nbr16: torch.Tensor | None = None           # 16-bit modular deltas (pressure sweeps)
     nbr16_overflow: torch.Tensor | None = None  # int32 device flag
     width: int = 0          # row width of the one-pass fixed-width layout (0: two-pass)
+    tiles: tuple | None = None  # (slot uint16 as int16, tile_runs, tile_n, cap): pressure-sweep tiles
 
     def attach_compact16(self, buf16: torch.Tensor, flag: torch.Tensor, limit: int = 32767):
         """Write the 16-bit delta copy of the list into `buf16` (int16, at
@@ -147,6 +150,19 @@ class NeighborList:
         check(lib().dsph_neighbor_compact16(C.byref(pr), limit, ptr(buf16), buf16.numel(), ptr(flag),
                                             _lib.stream_handle(buf16.device)), "neighbor_compact16")
         self.nbr16, self.nbr16_overflow = buf16, flag
+
+    def attach_tiles(self, cap: int, slot: torch.Tensor, tile_runs: torch.Tensor, tile_n: torch.Tensor):
+        """Build the neighbor tiles of this (square) list for the pressure
+        sweeps and use them there: `slot` (int16 storage of uint16 slots, at
+        least `entries` long), `tile_runs` (int32, ceil(nq / 256) *
+        TILE_RUN_STRIDE) and `tile_n` (int32, ceil(nq / 256)).  Workgroups
+        whose neighbors do not fit `cap` read them from global memory."""
+        self.tiles = None
+        pr = self.pairs()
+        check(lib().dsph_neighbor_tiles(C.byref(pr), C.byref(self.grid.grid), ptr(self.grid.cell_start), int(cap),
+                                        ptr(slot), slot.numel(), ptr(tile_runs), ptr(tile_n),
+                                        _lib.stream_handle(slot.device)), "neighbor_tiles")
+        self.tiles = (slot, tile_runs, tile_n, int(cap))
 
     def ensure_offsets(self) -> torch.Tensor:
         """The CSR offsets (a list built with a deferred overflow check has
@@ -185,6 +201,9 @@ class NeighborList:
         p.slice_off = ptr(self.slice_off)
         p.nbr16 = ptr(self.nbr16)
         p.nbr16_overflow = ptr(self.nbr16_overflow)
+        if self.tiles is not None:
+            p.tile_slot, p.tile_runs, p.tile_n = ptr(self.tiles[0]), ptr(self.tiles[1]), ptr(self.tiles[2])
+            p.tile_cap = self.tiles[3]
         return p
 
     def csr_nbr(self) -> torch.Tensor:

@@ -92,7 +92,26 @@ typedef struct dsph_pairs {
    * a delta did not fit and the sweeps read nbr). */
   const int16_t* nbr16;
   const int32_t* nbr16_overflow;
+  /* Optional neighbor tiles for the DFSPH pressure sweeps (tile_slot NULL:
+   * not used; nq == nr), written by dsph_neighbor_tiles.  The neighbors of
+   * the rows of workgroup b (rows 256 b .. 256 b + 255) lie in R disjoint
+   * runs of consecutive particles, tile_runs[b * (2 DSPH_TILE_MAX_RUNS + 1)
+   * ...] = {R, start_0, length_0, start_1, ...}, tile_n[b] particles in
+   * all; entry k of such a row holds in tile_slot[k] the position of its
+   * neighbor in the runs laid end to end.  The sweeps stage a workgroup's
+   * runs of neighbor records in LDS once and read them there.  tile_n[b] < 0
+   * or > tile_cap: that workgroup's rows read their neighbors from global
+   * memory (the nbr16 / nbr entries). */
+  const uint16_t* tile_slot;
+  const int32_t* tile_runs;
+  const int32_t* tile_n;
+  int32_t tile_cap;         /* 1 .. DSPH_TILE_MAX_CAP */
+  int32_t tile_pad_;
 } dsph_pairs;
+
+#define DSPH_TILE_ROWS 256
+#define DSPH_TILE_MAX_CAP 4096
+#define DSPH_TILE_MAX_RUNS 16
 
 /* 16-bit neighbor deltas for the pressure sweeps (halves their index
  * stream): out[k] = (nbr[k] - i) reduced mod nr into [-nr/2, nr/2], for
@@ -107,6 +126,27 @@ typedef struct dsph_pairs {
 int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit,
                             int16_t* out, int64_t capacity,
                             int32_t* overflow, void* stream);
+
+/* Neighbor tiles of a square list (nq == nr) for the DFSPH pressure sweeps
+ * (the tile_* fields of dsph_pairs), from the cell grid the list was
+ * searched on (`grid`, `cell_start` of dsph_grid_build): for each
+ * workgroup b of 256 rows, the particles of the cells around its rows'
+ * cells (the 3 x 3 stencil of the search, periodic as the domain is) as
+ * sorted disjoint runs in tile_runs[b * (2 DSPH_TILE_MAX_RUNS + 1) ...],
+ * their count in tile_n[b], and for every entry k of those rows the
+ * position of its neighbor in the runs in slot[k] (`slot_capacity` >= the
+ * layout's length).  A workgroup whose runs hold more than `cap` particles,
+ * whose rows span more than 4 rows of cells, whose runs number more than
+ * DSPH_TILE_MAX_RUNS, or one of whose neighbors lies outside its runs gets
+ * tile_n[b] = count or -1 and is not tiled: the sweeps read its neighbors
+ * from global memory.  Neighbor indices are read from nbr16 when it is
+ * valid, else from nbr.  cap: 1 .. DSPH_TILE_MAX_CAP.  No counterpart in
+ * the reference: a layout of the neighbor list that the DFSPH sweeps
+ * (v2/modules/dfsph.py:35-50) read. */
+int dsph_neighbor_tiles(const dsph_pairs* pairs, const dsph_grid* grid,
+                        const int32_t* cell_start, int32_t cap, uint16_t* slot,
+                        int64_t slot_capacity, int32_t* tile_runs,
+                        int32_t* tile_n, void* stream);
 
 /* Sliced-ELL layout from row counts: slice_off[s] = 64 * sum over earlier
  * slices of their longest row; slice_off[nslices] = padded entry count,
