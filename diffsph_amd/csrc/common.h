@@ -239,6 +239,44 @@ inline PairsDev make_pairs(const dsph_pairs& p) {
   return o;
 }
 
+// Neighbor tiles (the tile_* fields of dsph_pairs): tile b holds the
+// particles of R disjoint runs, tile_runs[b * kTileRunStride ...] =
+// {R, start_0, length_0, ...}, laid end to end; entry k of a row of
+// workgroup b names its neighbor by its position in them, tile_slot[k].
+constexpr int kTileMaxRuns = DSPH_TILE_MAX_RUNS;
+constexpr int kTileRunStride = 2 * kTileMaxRuns + 1;
+
+struct IdxSlot {
+  const uint16_t* a;
+  __device__ __forceinline__ int operator()(int k) const { return (int)__builtin_nontemporal_load(a + k); }
+};
+
+// Stages tile lb in LDS, run after run: put(o, j) for its particles j laid
+// end to end (o = 0, 1, ...; contiguous loads within a run).  False
+// (uniform over the workgroup, before any barrier) when the workgroup has
+// no usable tile.
+template <class Put>
+__device__ __forceinline__ bool stage_tile(const PairsDev& P, int lb, const Put& put) {
+  const int S = P.tn[lb];
+  if (S < 0 || S > P.tcap) return false;
+  const int32_t* tr = P.truns + (size_t)lb * kTileRunStride;
+  const int R = tr[0];
+  int o = 0;
+  for (int u = 0; u < R; ++u) {
+    const int s0 = tr[1 + 2 * u], len = tr[2 + 2 * u];
+    for (int s = threadIdx.x; s < len; s += blockDim.x) put(o + s, s0 + s);
+    o += len;
+  }
+  __syncthreads();
+  return true;
+}
+
+// tile[o] = rec(j).
+template <class T, class Rec>
+__device__ __forceinline__ bool load_tile_by(const PairsDev& P, int lb, const Rec& rec, T* tile) {
+  return stage_tile(P, lb, [&](int o, int j) { tile[o] = rec(j); });
+}
+
 // The kernel constants of a pair list (evaluated once per thread).
 __device__ __forceinline__ KSel ksel(const PairsDev& P) {
   KSel s;

@@ -93,34 +93,28 @@ struct PrepArgs {
   float visc_factor;   // 2(d+2) nu rho0 for the viscid scheme
 };
 
-__global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
-                                                          float* __restrict__ alpha_den, float* __restrict__ div_src,
-                                                          float2* __restrict__ visc_acc, float* __restrict__ kv_out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nq) return;
+// Neighbor data of one entry of the prepare pass; c2 = V_j^2 / (area_j rho0)
+// of the alpha term.
+struct PrepNb {
+  float2 x, v;
+  float h, vol, c2;
+};
+
+// One row of the prepare pass: neighbors by index idx(k) and data src(j),
+// four entries' loads ahead of their sums (the sums run in entry order).
+template <class Idx, class Src>
+__device__ __forceinline__ void prepare_row(const PairsDev& P, const PrepArgs& A, int i, const Idx& idx,
+                                            const Src& src, float* __restrict__ alpha_div,
+                                            float* __restrict__ alpha_den, float* __restrict__ div_src,
+                                            float2* __restrict__ visc_acc, float* __restrict__ kv_out) {
   const float2 xi = P.qpos[i];
   const float hi = h_of(P.qh, i, P.h0);
   const KSel ks = ksel(P);
   const float2 vi = A.vel[i];
   float k1x = 0.f, k1y = 0.f, k2 = 0.f, dv = 0.f, vsx = 0.f, vsy = 0.f;
   const RowRef rr = row_ref(P, i);
-  // neighbor data of one entry, gathered four entries ahead (the sums run
-  // in entry order)
-  struct Nb {
-    float2 x, v;
-    float h, vol, area;
-  };
-  auto load = [&](int j) {
-    Nb d;
-    d.x = P.rpos[j];
-    d.v = A.vel[j];
-    d.h = h_of(P.rh, j, P.h0);
-    d.vol = A.vol[j];
-    d.area = A.areas[j];
-    return d;
-  };
   int n = 0, k = rr.base;
-  auto body = [&](int k, const Nb& d) {
+  auto body = [&](int k, const PrepNb& d) {
     const float h = pick_h(P.mode, hi, d.h);
     Geom g = pair_geom(xi, d.x, h, ks(h), P.dom);
     const float Vj = d.vol;
@@ -133,9 +127,8 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
     // alpha terms (dfsph.py:13-20)
     k1x += kx;
     k1y += ky;
-    const float c2 = (Vj * Vj) / (d.area * A.rho0);
     const float gx = g.dx * t, gy = g.dy * t;
-    k2 += c2 * (gx * gx + gy * gy);
+    k2 += d.c2 * (gx * gx + gy * gy);
     // divergence 'difference' of v (sphOps.py:92-101)
     dv += (vj.x - vi.x) * kx + (vj.y - vi.y) * ky;
     if (A.visc_on) {
@@ -152,15 +145,15 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
   };
   for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
     int j[4];
-    Nb d[4];
+    PrepNb d[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) j[u] = P.nbr[k + u * rr.step];
+    for (int u = 0; u < 4; ++u) j[u] = idx(k + u * rr.step);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) d[u] = load(j[u]);
+    for (int u = 0; u < 4; ++u) d[u] = src(j[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) body(k + u * rr.step, d[u]);
   }
-  for (; n < rr.cnt; ++n, k += rr.step) body(k, load(P.nbr[k]));
+  for (; n < rr.cnt; ++n, k += rr.step) body(k, src(idx(k)));
   const float Vi = A.vol[i];
   const float mass_i = A.areas[i] * A.rho0;
   const float k1sq = k1x * k1x + k1y * k1y;
@@ -181,6 +174,60 @@ __global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A
       s = A.visc_factor / A.rho[i];
     visc_acc[i] = make_float2(s * vsx, s * vsy);
   }
+}
+
+// The neighbor data of particle j from the global arrays.
+__device__ __forceinline__ PrepNb prep_nb(const PairsDev& P, const PrepArgs& A, int j) {
+  PrepNb d;
+  d.x = P.rpos[j];
+  d.v = A.vel[j];
+  d.h = h_of(P.rh, j, P.h0);
+  d.vol = A.vol[j];
+  d.c2 = (d.vol * d.vol) / (A.areas[j] * A.rho0);
+  return d;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dfsph_prepare(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
+                                                          float* __restrict__ alpha_den, float* __restrict__ div_src,
+                                                          float2* __restrict__ visc_acc, float* __restrict__ kv_out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nq) return;
+  prepare_row(
+      P, A, i, [&](int k) { return P.nbr[k]; }, [&](int j) { return prep_nb(P, A, j); }, alpha_div, alpha_den,
+      div_src, visc_acc, kv_out);
+}
+
+// The prepare pass over neighbor tiles (uniform support): (x, y, v) and
+// (V, c2) of the workgroup's neighbors staged in LDS, c2 evaluated once per
+// staged particle instead of once per entry (the same expression).
+__global__ void __launch_bounds__(kBlock) k_dfsph_prepare_tiled(PairsDev P, PrepArgs A, float* __restrict__ alpha_div,
+                                                                float* __restrict__ alpha_den,
+                                                                float* __restrict__ div_src,
+                                                                float2* __restrict__ visc_acc,
+                                                                float* __restrict__ kv_out) {
+  extern __shared__ float4 tile_xv[];
+  float2* tile_vc = reinterpret_cast<float2*>(tile_xv + P.tcap);
+  const int b = blockIdx.x;
+  const int i = b * kBlock + threadIdx.x;
+  const bool tiled = stage_tile(P, b, [&](int o, int j) {
+    const PrepNb d = prep_nb(P, A, j);
+    tile_xv[o] = make_float4(d.x.x, d.x.y, d.v.x, d.v.y);
+    tile_vc[o] = make_float2(d.vol, d.c2);
+  });
+  if (i >= P.nq) return;
+  if (tiled)
+    prepare_row(
+        P, A, i, IdxSlot{P.tslot},
+        [&](int s) {
+          const float4 a = tile_xv[s];
+          const float2 c = tile_vc[s];
+          return PrepNb{make_float2(a.x, a.y), make_float2(a.z, a.w), P.h0, c.x, c.y};
+        },
+        alpha_div, alpha_den, div_src, visc_acc, kv_out);
+  else
+    prepare_row(
+        P, A, i, [&](int k) { return P.nbr[k]; }, [&](int j) { return prep_nb(P, A, j); }, alpha_div, alpha_den,
+        div_src, visc_acc, kv_out);
 }
 
 // Source of a solve with advection adv (may be NULL = 0): div(v*) with
@@ -656,14 +703,8 @@ static_assert(kBlock == DSPH_TILE_ROWS, "one tile per workgroup of sweep rows");
 // row paths would take 100+ SGPRs, i.e. 7 waves; the excess goes to VGPR
 // lanes): 78.7 -> 72.7 us (A) and 82.1 -> 75.1 us (B) at 1M particles.
 #define DSPH_TILED_ATTR __attribute__((amdgpu_waves_per_eu(8)))
-constexpr int kTileMaxRuns = DSPH_TILE_MAX_RUNS;
-constexpr int kTileRunStride = 2 * kTileMaxRuns + 1;  // [R, start_0, len_0, ...]
 constexpr int kTileMaxSegs = 4;  // rows of cells one workgroup may span (9 ranges each)
 
-struct IdxSlot {
-  const uint16_t* a;
-  __device__ __forceinline__ int operator()(int k) const { return (int)__builtin_nontemporal_load(a + k); }
-};
 
 // Wave-wide helpers of the tile build (64 lanes, all active).
 __device__ __forceinline__ int wave_incl_max(int v) {
@@ -879,21 +920,9 @@ __global__ void __launch_bounds__(kBlock) k_tile_slots(PairsDev P, int cap, uint
   if (bad) tile_n[b] = -1;
 }
 
-// Loads tile lb's records into LDS, run after run; false (uniform) when the
-// workgroup has no usable tile.
+// Loads tile lb's 16-byte records into LDS.
 __device__ __forceinline__ bool load_tile(const PairsDev& P, int lb, const float4* __restrict__ rec, float4* tile) {
-  const int S = P.tn[lb];
-  if (S < 0 || S > P.tcap) return false;
-  const int32_t* tr = P.truns + (size_t)lb * kTileRunStride;
-  const int R = tr[0];
-  int o = 0;
-  for (int u = 0; u < R; ++u) {
-    const int s0 = tr[1 + 2 * u], len = tr[2 + 2 * u];
-    for (int s = threadIdx.x; s < len; s += kBlock) tile[o + s] = rec[s0 + s];
-    o += len;
-  }
-  __syncthreads();
-  return true;
+  return load_tile_by(P, lb, [&](int j) { return rec[j]; }, tile);
 }
 
 // Sweep A with the (x, y, p) records of the workgroup's neighbors in LDS.
@@ -1401,8 +1430,13 @@ extern "C" int dsph_dfsph_prepare(const dsph_pairs* pairs, const float* vel, con
   A.visc_eps = visc ? visc->eps : 0.f;
   A.pi_switch = visc ? visc->pi_switch : 0;
   A.visc_factor = visc ? (float)(2.0 * (2 + 2) * (double)visc->alpha * (double)visc->rho0) : 0.f;
-  k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
-      P, A, alpha_div, alpha_den, div_src, reinterpret_cast<float2*>(visc_acc), kgrad);
+  if (P.tslot && !P.qh && !P.rh)
+    k_dfsph_prepare_tiled<<<grid_for(P.nq), kBlock, (size_t)P.tcap * (sizeof(float4) + sizeof(float2)),
+                            as_stream(stream)>>>(P, A, alpha_div, alpha_den, div_src,
+                                                 reinterpret_cast<float2*>(visc_acc), kgrad);
+  else
+    k_dfsph_prepare<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(
+        P, A, alpha_div, alpha_den, div_src, reinterpret_cast<float2*>(visc_acc), kgrad);
   return launch_status();
 }
 

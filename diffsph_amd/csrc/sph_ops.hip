@@ -51,6 +51,76 @@ __global__ void __launch_bounds__(kBlock) k_density(PairsDev P, const float* __r
   if (vol) vol[i] = qmass[i] / acc;
 }
 
+// Density over neighbor tiles (uniform support): the workgroup's neighbors'
+// (x, y, m) staged in LDS; arithmetic and order as k_density.
+__global__ void __launch_bounds__(kBlock) k_density_tiled(PairsDev P, const float* __restrict__ rmass,
+                                                          const float* __restrict__ qmass, float* __restrict__ rho,
+                                                          float* __restrict__ vol, float* __restrict__ w_out) {
+  extern __shared__ float4 tile[];
+  const int b = blockIdx.x;
+  const int i = b * kBlock + threadIdx.x;
+  const bool tiled = load_tile_by(
+      P, b,
+      [&](int j) {
+        const float2 x = P.rpos[j];
+        return make_float4(x.x, x.y, rmass[j], 0.f);
+      },
+      tile);
+  if (i >= P.nq) return;
+  const float2 xi = P.qpos[i];
+  const KConst kk = ksel(P)(P.h0);
+  float acc = 0.0f;
+  const RowRef rr = row_ref(P, i);
+  auto body = [&](int k, float4 d) {
+    const float dx = mod_offset(xi.x - d.x, P.dom, 0), dy = mod_offset(xi.y - d.y, P.dom, 1);
+    const float q = sqrt_rn(dist2_exact(dx, dy)) * kk.ih;
+    const float w = wendland_w(q, kk);
+    if (w_out) w_out[k] = w;
+    acc += d.z * w;
+  };
+  auto rec = [&](int j) {
+    const float2 x = P.rpos[j];
+    return make_float4(x.x, x.y, rmass[j], 0.f);
+  };
+  int n = 0, k = rr.base;
+  if (tiled) {
+    const IdxSlot idx{P.tslot};
+    for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+      int sl[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) sl[u] = idx(k + u * rr.step);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) body(k + u * rr.step, tile[sl[u]]);
+    }
+    for (; n < rr.cnt; ++n, k += rr.step) body(k, tile[idx(k)]);
+  } else {
+    for (; n + 4 <= rr.cnt; n += 4, k += 4 * rr.step) {
+      int j[4];
+      float4 d[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) j[u] = P.nbr[k + u * rr.step];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) d[u] = rec(j[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) body(k + u * rr.step, d[u]);
+    }
+    for (; n < rr.cnt; ++n, k += rr.step) body(k, rec(P.nbr[k]));
+  }
+  rho[i] = acc;
+  if (vol) vol[i] = qmass[i] / acc;
+}
+
+// The density launch: over neighbor tiles when the list carries them and
+// the support is uniform.
+static void launch_density(const PairsDev& P, const float* rmass, const float* qmass, float* rho, float* vol,
+                           float* w_out, hipStream_t st) {
+  if (P.tslot && !P.qh && !P.rh)
+    k_density_tiled<<<grid_for(P.nq), kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, rmass, qmass, rho, vol,
+                                                                                      w_out);
+  else
+    k_density<<<grid_for(P.nq), kBlock, 0, st>>>(P, rmass, qmass, rho, vol, w_out);
+}
+
 // ---------------------------------------------------------------------------
 // Generic operation (v2/sphOps.py:6-149).
 // ---------------------------------------------------------------------------
@@ -318,7 +388,7 @@ extern "C" int dsph_density(const dsph_pairs* pairs, const float* rmass, const f
   if (pairs->nq == 0) return DSPH_OK;
   if (!rmass || !rho || (vol && !qmass)) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_density<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, rmass, qmass, rho, vol, nullptr);
+  launch_density(P, rmass, qmass, rho, vol, nullptr, as_stream(stream));
   return launch_status();
 }
 
@@ -329,7 +399,7 @@ extern "C" int dsph_density_w(const dsph_pairs* pairs, const float* rmass, const
   if (pairs->nq == 0) return DSPH_OK;
   if (!rmass || !rho || (vol && !qmass) || !pair_w) return DSPH_ERR_ARG;
   PairsDev P = make_pairs(*pairs);
-  k_density<<<grid_for(P.nq), kBlock, 0, as_stream(stream)>>>(P, rmass, qmass, rho, vol, pair_w);
+  launch_density(P, rmass, qmass, rho, vol, pair_w, as_stream(stream));
   return launch_status();
 }
 
