@@ -251,6 +251,47 @@ struct IdxSlot {
   __device__ __forceinline__ int operator()(int k) const { return (int)__builtin_nontemporal_load(a + k); }
 };
 
+// Wave-wide inclusive scans (64 lanes, all active).
+__device__ __forceinline__ int wave_incl_max(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d, 64);
+    if (lane >= d) v = v > y ? v : y;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
+  }
+  return v;
+}
+
+// The run table of tile b into LDS (arrays of kTileMaxRuns), loaded by wave
+// 0: run u holds particles [run_s[u], run_e[u]), staged at positions
+// j + run_d[u].  Returns R; the caller synchronises the workgroup before
+// reading the table.  Only for a tile whose tile_n is valid.
+__device__ __forceinline__ int load_run_table(const int32_t* __restrict__ tile_runs, int b, int* run_s, int* run_e,
+                                              int* run_d) {
+  const int32_t* tr = tile_runs + (size_t)b * kTileRunStride;
+  const int R = tr[0];
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const int s0 = t < R ? tr[1 + 2 * t] : 0, len = t < R ? tr[2 + 2 * t] : 0;
+    const int o = wave_incl_sum(len) - len;
+    if (t < R) {
+      run_s[t] = s0;
+      run_e[t] = s0 + len;
+      run_d[t] = o - s0;
+    }
+  }
+  return R;
+}
+
 // Stages tile lb in LDS, run after run: put(o, j) for its particles j laid
 // end to end (o = 0, 1, ...; contiguous loads within a run).  False
 // (uniform over the workgroup, before any barrier) when the workgroup has

@@ -706,26 +706,6 @@ static_assert(kBlock == DSPH_TILE_ROWS, "one tile per workgroup of sweep rows");
 constexpr int kTileMaxSegs = 4;  // rows of cells one workgroup may span (9 ranges each)
 
 
-// Wave-wide helpers of the tile build (64 lanes, all active).
-__device__ __forceinline__ int wave_incl_max(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(v, d, 64);
-    if (lane >= d) v = v > y ? v : y;
-  }
-  return v;
-}
-__device__ __forceinline__ int wave_incl_sum(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int y = __shfl_up(v, d, 64);
-    if (lane >= d) v += y;
-  }
-  return v;
-}
-
 // The cell holding sorted particle i, the last c with cell_start[c] <= i,
 // by a 64-way search of the wave (3 rounds for 62k cells).
 __device__ __forceinline__ int wave_cell_of(const int32_t* __restrict__ cs, int ncell, int i) {
@@ -896,17 +876,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_slots(PairsDev P, int cap, uint
   const int b = blockIdx.x, t = threadIdx.x;
   const int S = tile_n[b];
   if (S < 0 || S > cap) return;  // uniform
-  const int32_t* tr = tile_runs + (size_t)b * kTileRunStride;
-  const int R = tr[0];
-  if (t < 64) {  // wave 0: run u in lane u, offsets by a prefix sum
-    const int s0 = t < R ? tr[1 + 2 * t] : 0, len = t < R ? tr[2 + 2 * t] : 0;
-    const int o = wave_incl_sum(len) - len;
-    if (t < R) {
-      run_s[t] = s0;
-      run_e[t] = s0 + len;
-      run_d[t] = o - s0;
-    }
-  }
+  const int R = load_run_table(tile_runs, b, run_s, run_e, run_d);
   __syncthreads();
   const int i = b * kBlock + t;
   if (i >= P.nq) return;
