@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
   // offsets idx - i (a per-thread column) and written column by column after
   // the scan: one contiguous line per store of the wave instead of scattered
   // words; an offset beyond 16 bits is written at once (slot marked)
-  __shared__ int16_t stage[kBFillStage][kBlock];
+  __shared__ int16_t stage[kBFillStage + 1][kBlock];  // + the sink row of the unconditional writes
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int total = 0;
   if (i < n) {
@@ -254,16 +254,16 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
                     [&](int, const BCand& c) {
                       const float dx = mod_offset(qi.x - c.x.x, d, 0);
                       const float dy = mod_offset(qi.y - c.x.y, d, 1);
-                      if (within_support(dist2_exact(dx, dy), h)) {
-                        const int di = c.idx - i;
-                        if (total < kBFillStage && total < width && di > -32768 && di <= 32767) {
-                          stage[total][threadIdx.x] = (int16_t)di;
-                        } else if (total < width) {
-                          if (total < kBFillStage) stage[total][threadIdx.x] = (int16_t)-32768;
-                          nbr[base + (int64_t)total * 64] = c.idx;
-                        }
-                        ++total;
-                      }
+                      // every candidate writes the staging slot of the next
+                      // entry (the sink row past the staged ones); a kept
+                      // candidate advances `total`, so its value stays
+                      const bool kp = within_support(dist2_exact(dx, dy), h);
+                      const int di = c.idx - i;
+                      const bool fits = di > -32768 && di <= 32767;
+                      const bool st = total < kBFillStage && total < width;
+                      stage[st ? total : kBFillStage][threadIdx.x] = fits ? (int16_t)di : (int16_t)-32768;
+                      if (kp && !(st && fits) && total < width) nbr[base + (int64_t)total * 64] = c.idx;
+                      total += kp ? 1 : 0;
                     });
     };
     for (int oy = -1; oy <= 1; ++oy) {

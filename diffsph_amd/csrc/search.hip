@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
                                                           int16_t* __restrict__ d16, int32_t* __restrict__ ovf,
                                                           int fixed_w, int32_t* __restrict__ counts,
                                                           int32_t* __restrict__ row_ovf) {
-  __shared__ int16_t stage[kFillStage > 0 ? kFillStage : 1][kBlock];
+  // one extra row: the sink of the unconditional staging writes below
+  __shared__ int16_t stage[kFillStage + 1][kBlock];
   bool bad = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
@@ -230,17 +231,23 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   for_candidates(a, qi, [&](int b, int e) {
     scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
                   [&](int j, const Cand& r) {
-                    if (keep_pair(qi, hi, r.x, r.h, a)) {
+                    // Every candidate writes the staging slot of the next
+                    // entry (row kFillStage, a sink, once the row is past
+                    // the staged entries); a kept candidate then advances
+                    // n, so its value stays and the next candidates write
+                    // the following slot.  Only an entry past the staged
+                    // ones, or whose delta needs more than 16 bits, takes
+                    // the branch to global memory.
+                    const bool kp = keep_pair(qi, hi, r.x, r.h, a);
+                    const int dj = j - i;
+                    const bool fits = dj > -32768 && dj <= 32767;
+                    const bool st = n < staged;
+                    stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)dj : kStagedOut;
+                    if (kp && !(st && fits)) {
                       const int64_t o = base + n * step;
-                      const int dj = j - i;
-                      if (n < staged && dj > -32768 && dj <= 32767) {
-                        stage[n][threadIdx.x] = (int16_t)dj;
-                      } else {
-                        if (n < staged) stage[n][threadIdx.x] = kStagedOut;
-                        if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
-                      }
-                      ++n;
+                      if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
                     }
+                    n += kp ? 1 : 0;
                   });
   });
   for (int64_t t = 0; t < staged; ++t) {
