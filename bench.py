@@ -245,13 +245,22 @@ def main():
         tight = {"threshold_den": 1e-4, "plain": tp, "learned": tl,
                  "speedup_vs_plain": round(tl["value"] / tp["value"], 3)}
 
-    if args.copy_calibration:
-        src = torch.empty(64 << 20, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    # achievable stream bandwidth on this box: a 256 MiB device copy (read +
+    # write), timed with HIP events; with --copy-calibration it is also the
+    # reference kernel of the PMC byte counters
+    src = torch.empty(64 << 20, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    c1.record()
+    c1.synchronize()
+    copy_gbs = 2 * src.numel() * 4 / (c0.elapsed_time(c1) / 10 * 1e-3) / 1e9
+    del src, dst
+    log(f"device copy 256 MiB: {copy_gbs:.0f} GB/s (read + write)")
 
     # ---- roofline of the pressure sweeps (HIP events on the solver's stream)
     L = _lib.lib()
@@ -351,8 +360,11 @@ def main():
                 "layout_bytes": int(byt), "layout_achieved": round(byt / tb / 1e9, 1),
                 "layout_frac": round(byt / tb / 1e9 / HBM_PEAK_GBS, 4),
                 "traffic_over_layout": None if ratio is None else round(ratio, 4),
-                "neighbor_index": "int16 delta" if idx16 else "int32",
+                "neighbor_index": "uint16 tile slot" if tiles else ("int16 delta" if idx16 else "int32"),
                 "neighbor_tiles": tiles or None,
+                "copy_gbs": round(copy_gbs, 1),
+                "layout_frac_of_copy": round(byt / tb / 1e9 / copy_gbs, 4),
+                "traffic_frac_of_copy": None if traffic is None else round(traffic / tb / 1e9 / copy_gbs, 4),
                 "other_sweep": {"kernel": other, "achieved": round(m_o / t_o / 1e9, 1),
                                 "kernel_us": round(t_o * 1e6, 2)}}
 
