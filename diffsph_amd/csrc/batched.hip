@@ -249,22 +249,32 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
     const int64_t base = (int64_t)(i >> 6) * 64 * width + (i & 63);
     const int ncx = g.nc[0], ncy = g.nc[1];
     const int cx = bcell(qi.x, d, 0, g), cy = bcell(qi.y, d, 1, g);
-    auto visit = [&](int jb, int je) {
-      scan_range<4>(jb, je, [&](int j) { return BCand{spos[j], si ? si[j] : j}; },
-                    [&](int, const BCand& c) {
-                      const float dx = mod_offset(qi.x - c.x.x, d, 0);
-                      const float dy = mod_offset(qi.y - c.x.y, d, 1);
-                      // every candidate writes the staging slot of the next
-                      // entry (the sink row past the staged ones); a kept
-                      // candidate advances `total`, so its value stays
-                      const bool kp = within_support(dist2_exact(dx, dy), h);
-                      const int di = c.idx - i;
-                      const bool fits = di > -32768 && di <= 32767;
-                      const bool st = total < kBFillStage && total < width;
-                      stage[st ? total : kBFillStage][threadIdx.x] = fits ? (int16_t)di : (int16_t)-32768;
-                      if (kp && !(st && fits) && total < width) nbr[base + (int64_t)total * 64] = c.idx;
-                      total += kp ? 1 : 0;
-                    });
+    // every candidate writes the staging slot of the next entry (the sink
+    // row past the staged ones); a kept candidate advances `total`, so its
+    // value stays
+    auto take = [&](bool kp, int idx) {
+      const int di = idx - i;
+      const bool fits = di > -32768 && di <= 32767;
+      const bool st = total < kBFillStage && total < width;
+      stage[st ? total : kBFillStage][threadIdx.x] = fits ? (int16_t)di : (int16_t)-32768;
+      if (kp && !(st && fits) && total < width) nbr[base + (int64_t)total * 64] = idx;
+      total += kp ? 1 : 0;
+    };
+    auto load = [&](int j) { return BCand{spos[j], si ? si[j] : j}; };
+    // off a periodic seam (a periodic axis of 7+ cells, no wrapped row or
+    // column) the plain difference is the min-image offset
+    const bool small = (d.periodic[0] && g.nc[0] < 7) || (d.periodic[1] && g.nc[1] < 7);
+    auto visit = [&](int jb, int je, bool seam) {
+      if (seam)
+        scan_range<4>(jb, je, load, [&](int, const BCand& c) {
+          const float dx = mod_offset(qi.x - c.x.x, d, 0);
+          const float dy = mod_offset(qi.y - c.x.y, d, 1);
+          take(within_support(dist2_exact(dx, dy), h), c.idx);
+        });
+      else
+        scan_range<4>(jb, je, load, [&](int, const BCand& c) {
+          take(within_support(dist2_exact(qi.x - c.x.x, qi.y - c.x.y), h), c.idx);
+        });
     };
     for (int oy = -1; oy <= 1; ++oy) {
       int yy = cy + oy;
@@ -275,15 +285,16 @@ __global__ void __launch_bounds__(kBlock) k_bfill(int n, const int32_t* __restri
       }
       const int row = g.cell_off + yy * ncx;
       const bool wraps = d.periodic[0] && (cx == 0 || cx == ncx - 1);
+      const bool seam = small || wraps || yy != cy + oy;
       if (!wraps) {
         const int x0 = cx - 1 < 0 ? 0 : cx - 1;
         const int x1 = cx + 1 > ncx - 1 ? ncx - 1 : cx + 1;
-        visit(cs[row + x0], cs[row + x1 + 1]);
+        visit(cs[row + x0], cs[row + x1 + 1], seam);
       } else {
         for (int ox = -1; ox <= 1; ++ox) {
           int xx = cx + ox;
           xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
-          visit(cs[row + xx], cs[row + xx + 1]);
+          visit(cs[row + xx], cs[row + xx + 1], seam);
         }
       }
     }

@@ -118,11 +118,29 @@ __device__ __forceinline__ bool keep_pair(float2 qi, float hi, float2 rj, float 
   return within_support(dist2_exact(dx, dy), pick_h(a.mode, hi, hj));
 }
 
-// Visit the candidate slices of query i in offset order (dy=-1,0,1; dx=-1,0,1).
+// keep_pair for a candidate range that does not cross a periodic seam: the
+// plain difference is the min-image offset there (mod_offset returns it
+// unchanged), so it is not evaluated.
+__device__ __forceinline__ bool keep_pair_near(float2 qi, float hi, float2 rj, float hj, const SearchArgs& a) {
+  return within_support(dist2_exact(qi.x - rj.x, qi.y - rj.y), pick_h(a.mode, hi, hj));
+}
+
+// A periodic axis of fewer than 7 cells: adjacent cells (the last one up to
+// two cells wide) may hold pairs farther apart than half the domain, so
+// every range takes mod_offset.  From 7 cells on, two adjacent cells span
+// less than L - 5 L / 8 < L / 2, so only ranges across the seam can.
+__device__ __forceinline__ bool small_periodic_grid(const SearchArgs& a) {
+  return (a.d.periodic[0] && a.g.nc[0] < 7) || (a.d.periodic[1] && a.g.nc[1] < 7);
+}
+
+// Visit the candidate slices of query i in offset order (dy=-1,0,1;
+// dx=-1,0,1): f(b, e, seam), seam true when the slice's pairs may cross a
+// periodic seam (a wrapped row or column of cells, or a small periodic grid).
 template <class F>
 __device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, F&& f) {
   const int ncx = a.g.nc[0], ncy = a.g.nc[1];
   int cx = cell_coord(qi.x, a.d, 0, a.g), cy = cell_coord(qi.y, a.d, 1, a.g);
+  const bool small = small_periodic_grid(a);
   for (int oy = -1; oy <= 1; ++oy) {
     int yy = cy + oy;
     if (a.d.periodic[1]) {
@@ -132,15 +150,16 @@ __device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, F
     }
     const int row = yy * ncx;
     bool wraps = a.d.periodic[0] && (cx == 0 || cx == ncx - 1);
+    const bool seam = small || wraps || yy != cy + oy;
     if (!wraps) {
       int x0 = cx - 1 < 0 ? 0 : cx - 1;
       int x1 = cx + 1 > ncx - 1 ? ncx - 1 : cx + 1;
-      f(a.cs[row + x0], a.cs[row + x1 + 1]);
+      f(a.cs[row + x0], a.cs[row + x1 + 1], seam);
     } else {
       for (int ox = -1; ox <= 1; ++ox) {
         int xx = cx + ox;
         xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
-        f(a.cs[row + xx], a.cs[row + xx + 1]);
+        f(a.cs[row + xx], a.cs[row + xx + 1], seam);
       }
     }
   }
@@ -153,9 +172,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
   if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
   float hi = h_of(a.qh, i, a.h0);
   int c = 0;
-  for_candidates(a, qi, [&](int b, int e) {
-    scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
-                  [&](int, const Cand& r) { c += keep_pair(qi, hi, r.x, r.h, a) ? 1 : 0; });
+  auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
+  for_candidates(a, qi, [&](int b, int e, bool seam) {
+    if (seam)
+      scan_range<4>(b, e, load, [&](int, const Cand& r) { c += keep_pair(qi, hi, r.x, r.h, a) ? 1 : 0; });
+    else
+      scan_range<4>(b, e, load, [&](int, const Cand& r) { c += keep_pair_near(qi, hi, r.x, r.h, a) ? 1 : 0; });
   });
   counts[i] = c;
 }
@@ -228,27 +250,28 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   }
   const int64_t staged = (slice || fixed_w > 0) ? (kFillStage < len ? kFillStage : len) : 0;
   int64_t n = 0;
-  for_candidates(a, qi, [&](int b, int e) {
-    scan_range<4>(b, e, [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; },
-                  [&](int j, const Cand& r) {
-                    // Every candidate writes the staging slot of the next
-                    // entry (row kFillStage, a sink, once the row is past
-                    // the staged entries); a kept candidate then advances
-                    // n, so its value stays and the next candidates write
-                    // the following slot.  Only an entry past the staged
-                    // ones, or whose delta needs more than 16 bits, takes
-                    // the branch to global memory.
-                    const bool kp = keep_pair(qi, hi, r.x, r.h, a);
-                    const int dj = j - i;
-                    const bool fits = dj > -32768 && dj <= 32767;
-                    const bool st = n < staged;
-                    stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)dj : kStagedOut;
-                    if (kp && !(st && fits)) {
-                      const int64_t o = base + n * step;
-                      if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
-                    }
-                    n += kp ? 1 : 0;
-                  });
+  // Every candidate writes the staging slot of the next entry (row
+  // kFillStage, a sink, once the row is past the staged entries); a kept
+  // candidate then advances n, so its value stays and the next candidates
+  // write the following slot.  Only an entry past the staged ones, or whose
+  // delta needs more than 16 bits, takes the branch to global memory.
+  auto take = [&](bool kp, int j) {
+    const int dj = j - i;
+    const bool fits = dj > -32768 && dj <= 32767;
+    const bool st = n < staged;
+    stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)dj : kStagedOut;
+    if (kp && !(st && fits)) {
+      const int64_t o = base + n * step;
+      if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
+    }
+    n += kp ? 1 : 0;
+  };
+  auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
+  for_candidates(a, qi, [&](int b, int e, bool seam) {
+    if (seam)
+      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair(qi, hi, r.x, r.h, a), j); });
+    else
+      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair_near(qi, hi, r.x, r.h, a), j); });
   });
   for (int64_t t = 0; t < staged; ++t) {
     const int64_t o = base + t * step;
