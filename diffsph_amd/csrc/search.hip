@@ -200,9 +200,13 @@ constexpr int16_t kStagedOut = -32768;  // slot already written to global memory
 
 // With `d16` the 16-bit modular delta (j - i) mod nref of every entry is
 // written beside it (rows = reference slots); a delta beyond int16 sets *ovf.
-__device__ __forceinline__ void put_entry(int32_t* __restrict__ nbr, int16_t* __restrict__ d16, int64_t o, int j,
-                                          int i, int nref, bool& bad) {
-  nbr[o] = j;
+// With `map` (and no d16) the entry is map[j]: the reference slot's index in
+// the caller's own order (its sort index), as dsph_remap_neighbors would
+// write it afterwards.
+__device__ __forceinline__ void put_entry(int32_t* __restrict__ nbr, int16_t* __restrict__ d16,
+                                          const int32_t* __restrict__ map, int64_t o, int j, int i, int nref,
+                                          bool& bad) {
+  nbr[o] = map ? map[j] : j;
   if (d16) {
     int d = j - i;
     const int half = nref / 2;
@@ -224,7 +228,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
                                                           int32_t* __restrict__ nbr, int64_t cap,
                                                           int16_t* __restrict__ d16, int32_t* __restrict__ ovf,
                                                           int fixed_w, int32_t* __restrict__ counts,
-                                                          int32_t* __restrict__ row_ovf) {
+                                                          int32_t* __restrict__ row_ovf,
+                                                          const int32_t* __restrict__ map) {
   // one extra row: the sink of the unconditional staging writes below
   __shared__ int16_t stage[kFillStage + 1][kBlock];
   bool bad = false;
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)dj : kStagedOut;
     if (kp && !(st && fits)) {
       const int64_t o = base + n * step;
-      if (n < len && o < cap) put_entry(nbr, d16, o, j, i, a.nref, bad);
+      if (n < len && o < cap) put_entry(nbr, d16, map, o, j, i, a.nref, bad);
     }
     n += kp ? 1 : 0;
   };
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     if (o >= cap) continue;
     if (t < n) {
       const int16_t v = stage[t][threadIdx.x];
-      if (v != kStagedOut) put_entry(nbr, d16, o, i + v, i, a.nref, bad);
+      if (v != kStagedOut) put_entry(nbr, d16, map, o, i + v, i, a.nref, bad);
     } else if (fixed_w == 0) {
       nbr[o] = -1;
     }
@@ -498,7 +503,7 @@ extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* gr
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
   k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity, nullptr,
-                                                                   nullptr, 0, nullptr, nullptr);
+                                                                   nullptr, 0, nullptr, nullptr, nullptr);
   return launch_status();
 }
 
@@ -522,23 +527,23 @@ extern "C" int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* 
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
   k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow, 0, nullptr,
-                                                    nullptr);
+                                                    nullptr, nullptr);
   return launch_status();
 }
 
-extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
-                                        const float* sorted_pos, const float* sorted_h, int64_t nref,
-                                        const float* qpos, const float* qh, int64_t nq, int32_t qpos_wrapped,
-                                        float h_uniform, int32_t mode, int32_t width, int32_t* counts,
-                                        int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16,
-                                        int32_t* delta_overflow, int32_t* row_overflow, void* stream) {
+namespace dsph {
+static int fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                      const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos, const float* qh,
+                      int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode, int32_t width, int32_t* counts,
+                      int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16, int32_t* delta_overflow,
+                      int32_t* row_overflow, const int32_t* map, void* stream) {
   SearchArgs a;
   int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
                       h_uniform, mode);
   if (s != DSPH_OK) return s;
   if (width < 1 || width > 4096 || !row_overflow || !slice_off || (nbr16 && !delta_overflow)) return DSPH_ERR_ARG;
   // 16-bit deltas are j - i modulo n: row i must be reference slot i
-  if (nbr16 && (nq != nref || !qpos_wrapped)) return DSPH_ERR_ARG;
+  if (nbr16 && (nq != nref || !qpos_wrapped || map)) return DSPH_ERR_ARG;
   const int64_t nslices = (nq + 63) / 64;
   if (nslices * 64 * (int64_t)width >= INT32_MAX) return DSPH_ERR_ARG;
   if (capacity < nslices * 64 * (int64_t)width) return DSPH_ERR_CAPACITY;
@@ -549,8 +554,31 @@ extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_gr
   if (nq == 0) return DSPH_OK;
   if (!counts || !nbr) return DSPH_ERR_ARG;
   k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, nullptr, nullptr, nbr, capacity, nbr16, delta_overflow, width,
-                                                    counts, row_overflow);
+                                                    counts, row_overflow, map);
   return launch_status();
+}
+}  // namespace dsph
+
+extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const int32_t* cell_start,
+                                        const float* sorted_pos, const float* sorted_h, int64_t nref,
+                                        const float* qpos, const float* qh, int64_t nq, int32_t qpos_wrapped,
+                                        float h_uniform, int32_t mode, int32_t width, int32_t* counts,
+                                        int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16,
+                                        int32_t* delta_overflow, int32_t* row_overflow, void* stream) {
+  return fill_fixed(domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped, h_uniform, mode,
+                    width, counts, slice_off, nbr, capacity, nbr16, delta_overflow, row_overflow, nullptr, stream);
+}
+
+extern "C" int dsph_neighbor_fill_fixed_map(const dsph_domain* domain, const dsph_grid* grid,
+                                            const int32_t* cell_start, const float* sorted_pos,
+                                            const float* sorted_h, int64_t nref, const float* qpos, const float* qh,
+                                            int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                            int32_t width, int32_t* counts, int32_t* slice_off, int32_t* nbr,
+                                            int64_t capacity, const int32_t* map, int32_t* row_overflow,
+                                            void* stream) {
+  if (nref > 0 && !map) return DSPH_ERR_ARG;
+  return fill_fixed(domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped, h_uniform, mode,
+                    width, counts, slice_off, nbr, capacity, nullptr, nullptr, row_overflow, map, stream);
 }
 
 extern "C" size_t dsph_sell_workspace(int64_t nq) {

@@ -303,17 +303,19 @@ class DeltaSPHSolver:
         return 0 if self._last_nl is None else self._last_nl.pair_count()
 
     # -- pieces ------------------------------------------------------------
-    def _search(self, kind: str, grid: CellGrid, q: torch.Tensor):
+    def _search(self, kind: str, grid: CellGrid, q: torch.Tensor, remap: bool = False):
         """Neighbor list of the wrapped queries `q` against `grid`: the
         one-pass fixed-width list while its rows fit, else the two-pass one
-        (and the next search of this kind is wider)."""
+        (and the next search of this kind is wider).  `remap`: entries are
+        the reference particles' indices in the grid's input order."""
         w = self._widths.get(kind, max(int(self.cfg.neighbor_width), 0))
         if self._deferred is not None:
             if w < 1:
                 raise DsphError("captured steps need one-pass neighbor lists (neighbor_width > 0)")
             return search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed", width=w,
-                          deferred=self._deferred)
-        nl = search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed" if w > 0 else "sell", width=w)
+                          deferred=self._deferred, remap=remap)
+        nl = search(grid, q, None, "scatter", qpos_wrapped=True, layout="fixed" if w > 0 else "sell", width=w,
+                    remap=remap)
         if w > 0 and nl.width == 0:
             self._widths[kind] = (int(nl.counts.max()) + 16) | 1
         return nl
@@ -325,9 +327,7 @@ class DeltaSPHSolver:
         cfg = self.cfg
         grid = CellGrid.build(pos, None, self.domain, cfg.support * cfg.cell_scale, h_uniform=cfg.support)
         wpos = wrap_positions(pos, self.domain)
-        nl = self._search("fluid", grid, wpos)
-        check(lib().dsph_remap_neighbors(ptr(nl.nbr), nl.nbr.numel(), ptr(grid.sort_index), self.n,
-                                         _lib.stream_handle(pos.device)), "remap_neighbors")
+        nl = self._search("fluid", grid, wpos, remap=True)
         pr = nl.pairs()
         pr.rpos = ptr(wpos)
         pr.rh = None
@@ -341,9 +341,7 @@ class DeltaSPHSolver:
         st = _lib.stream_handle(wpos.device)
         nl_b = self._search("wall", W.grid, wpos)
         pr_b = nl_b.pairs()
-        nl_g = self._search("ghost", grid, W.ghost)
-        check(lib().dsph_remap_neighbors(ptr(nl_g.nbr), nl_g.nbr.numel(), ptr(grid.sort_index), self.n, st),
-              "remap_neighbors")
+        nl_g = self._search("ghost", grid, W.ghost, remap=True)
         pr_g = nl_g.pairs()
         pr_g.rpos = ptr(wpos)
         pr_g.rh = None

@@ -235,7 +235,7 @@ class Delta16Buffers:
 
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
            qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None,
-           width: int = 67, deferred: torch.Tensor | None = None) -> NeighborList:
+           width: int = 67, deferred: torch.Tensor | None = None, remap: bool = False) -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
     Counts, scans (host reads of the pair and entry totals, as the
@@ -253,7 +253,12 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     the list is returned as is (rows longer than `width` truncated), with
     ``num_pairs`` None (read on demand by :meth:`NeighborList.pair_count`).
     The caller checks the flag later and redoes the work when it is set;
-    this is what lets a whole step be captured into a graph."""
+    this is what lets a whole step be captured into a graph.
+
+    ``remap``: the entries are the reference particles' indices in the
+    grid's input order (``grid.sort_index[j]``) instead of sorted slots, as
+    :func:`dsph_remap_neighbors` would make them ("fixed" layout: in the
+    same pass; other layouts: a pass after the fill)."""
     if delta16 is not None and not qpos_wrapped:
         raise DsphError("search: 16-bit neighbor deltas need the grid's own sorted positions as queries "
                         "(qpos_wrapped=True)")
@@ -269,8 +274,11 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
             C.c_float(grid.h_uniform), m)
     if deferred is not None and (layout != "fixed" or int(width) < 1):
         raise DsphError("search: deferred overflow checks need the fixed-width layout")
+    if remap and delta16 is not None:
+        raise DsphError("search: remapped entries have no 16-bit deltas")
     if layout == "fixed":
-        nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st, deferred)
+        nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st, deferred,
+                           grid.sort_index if remap else None)
         if nl is not None:
             return nl
         layout = "sell"
@@ -302,6 +310,9 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     else:
         check(L.dsph_neighbor_fill(*args, ptr(offsets), ptr(slice_off), entries, ptr(nbr), cap, st),
               "neighbor_fill")
+        if remap:
+            check(L.dsph_remap_neighbors(ptr(nbr), nbr.numel(), ptr(grid.sort_index), grid.n, st),
+                  "remap_neighbors")
     if qpos_wrapped:
         wq = qpos
     else:
@@ -310,7 +321,7 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
                         grid, m, slice_off, entries, buf16, flag16)
 
 
-def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st, deferred=None):
+def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st, deferred=None, remap=None):
     nq = qpos.shape[0]
     dev = qpos.device
     nsl = (nq + 63) // 64
@@ -321,8 +332,12 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
     buf16 = flag16 = None
     if delta16 is not None:
         buf16, flag16 = delta16.get(cap, dev)
-    check(L.dsph_neighbor_fill_fixed(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(buf16),
-                                     ptr(flag16), ptr(info), st), "neighbor_fill_fixed")
+    if remap is not None:
+        check(L.dsph_neighbor_fill_fixed_map(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(remap),
+                                             ptr(info), st), "neighbor_fill_fixed_map")
+    else:
+        check(L.dsph_neighbor_fill_fixed(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(buf16),
+                                         ptr(flag16), ptr(info), st), "neighbor_fill_fixed")
     if deferred is not None:
         # the pair kernels read the sliced rows (counts + slice offsets): the
         # CSR offsets are built only if asked for (NeighborList.ensure_offsets)

@@ -245,6 +245,19 @@ int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_grid* grid,
                              int32_t* delta_overflow, int32_t* row_overflow,
                              void* stream);
 
+/* dsph_neighbor_fill_fixed whose entries are map[j] for the reference slot
+ * j (map: the sort index of the reference grid, nref entries), i.e. the
+ * list dsph_remap_neighbors would make of it, written in the same pass.  No
+ * 16-bit deltas.  No counterpart in the reference. */
+int dsph_neighbor_fill_fixed_map(const dsph_domain* domain, const dsph_grid* grid,
+                                 const int32_t* cell_start, const float* sorted_pos,
+                                 const float* sorted_h, int64_t nref,
+                                 const float* qpos, const float* qh, int64_t nq,
+                                 int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                 int32_t width, int32_t* counts, int32_t* slice_off,
+                                 int32_t* nbr, int64_t capacity, const int32_t* map,
+                                 int32_t* row_overflow, void* stream);
+
 /* dsph_neighbor_fill that also writes the 16-bit modular deltas of
  * dsph_neighbor_compact16 (d = j - i reduced mod nref; |d| <= 32767, else
  * *overflow is set) in the same pass, for a list whose rows are the sorted
