@@ -65,6 +65,22 @@ __device__ __forceinline__ PGeo pgeo(float dx, float dy, float h) {
   return g;
 }
 
+// The geometry of the mirrored pair (d -> -d): every quantity of pgeo is
+// either even in d (r, e, q, rr, w1, w1p: the squares, sums and kernel
+// values are the same floats) or odd and formed as d times an even one
+// (dx, dy, ex, ey, gx, gy: IEEE negation commutes with the products), so
+// this equals pgeo(-dx, -dy, h) bit for bit without its sqrt and division.
+__device__ __forceinline__ PGeo mirrored(const PGeo& g) {
+  PGeo m = g;
+  m.dx = -g.dx;
+  m.dy = -g.dy;
+  m.ex = -g.ex;
+  m.ey = -g.ey;
+  m.gx = -g.gx;
+  m.gy = -g.gy;
+  return m;
+}
+
 // J_xhat^T v (J_xhat is symmetric): v/e - d (d.v) / (r e^2)
 __device__ __forceinline__ float2 jxhat(const PGeo& g, float vx, float vy) {
   const float dv = g.dx * vx + g.dy * vy;
@@ -317,7 +333,7 @@ __global__ void __launch_bounds__(kBlock) k_vjp_main(PairsDev P, VjpConst K, Vjp
       const float4 m = A.lM[j];
       cj = {a.x, a.y, A.lc[j], s.x, s.y, m.x, m.y, m.z, m.w};
     }
-    const PGeo g2 = pgeo(-dx, -dy, K.h);
+    const PGeo g2 = mirrored(g1);
     const PairOut o2 = pair_vjp(g2, K, rj, rk, uj, uk, mk, gj, gk, cj);
     lxx -= o2.gdx;
     lxy -= o2.gdy;
