@@ -379,7 +379,7 @@ extern "C" const char* dsph_status_string(int s) {
   }
 }
 
-extern "C" int dsph_abi_version(void) { return 3; }
+extern "C" int dsph_abi_version(void) { return 4; }
 
 extern "C" int dsph_density(const dsph_pairs* pairs, const float* rmass, const float* qmass, float* rho, float* vol,
                             void* stream) {

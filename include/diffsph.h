@@ -164,6 +164,7 @@ int dsph_sell_layout(const int32_t* counts, int64_t nq, int32_t* slice_off,
 int dsph_sell_to_csr(const dsph_pairs* pairs, int32_t* out_nbr, void* stream);
 
 const char* dsph_status_string(int status);
+/* 4: dsph_pairs carries the neighbor-tile fields. */
 int dsph_abi_version(void);
 
 /* ---------------------------------------------------------------------- */
