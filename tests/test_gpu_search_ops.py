@@ -119,6 +119,26 @@ def test_search_edge_cases():
         CellGrid.build(pos, None, dom, 0.9, h_uniform=0.9)
 
 
+@pytest.mark.parametrize("layout,width", [("fixed", 67), ("fixed", 8), ("sell", 0)])
+def test_remapped_entries_equal_sort_index_of_slots(layout, width):
+    """search(remap=True) writes grid.sort_index[j] for every entry j of the
+    plain list: in the fixed-width fill itself, in a pass after the two-pass
+    fill, and through the fallback when a row outgrows the width (8)."""
+    dev = torch.device("cuda:0")
+    d = sampling.dfsph_random_flow(96, seed=5, jitter_width=0.1)
+    dom = _lib.make_domain((-1, -1), (1, 1), (True, True))
+    g = CellGrid.build(torch.from_numpy(d["pos"]).to(dev), None, dom, d["h"], h_uniform=d["h"])
+    # queries: the particles in their input order (not the grid's slots)
+    q = torch.from_numpy(d["pos"]).to(dev)
+    a = search(g, q, None, "scatter", layout=layout, width=width)
+    b = search(g, q, None, "scatter", layout=layout, width=width, remap=True)
+    print(f"layout {layout} width {width}: {a.width} / {b.width}, {a.pair_count()} pairs", flush=True)
+    assert torch.equal(a.counts, b.counts) and a.width == b.width
+    si = g.sort_index.cpu().numpy()
+    ja, jb = a.csr_nbr().cpu().numpy(), b.csr_nbr().cpu().numpy()
+    np.testing.assert_array_equal(si[ja], jb)
+
+
 def _sorted_setup(orc, nside, seed, jit=0.1):
     d = sampling.dfsph_random_flow(nside, seed=seed, jitter_width=jit)
     dom_o = orc.make_domain((-1, -1), (1, 1), (True, True))
