@@ -105,6 +105,8 @@ def parse():
     ap.add_argument("--neighbor-width", type=int, default=67,
                     help="row width of the one-pass neighbor list (0: count pass + two-pass list)")
     ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--rings", type=int, default=2, choices=[1, 2],
+                    help="dfsph: warm-start features from one or two neighbor rings")
     ap.add_argument("--tile-cap", type=int, default=None,
                     help="dfsph: neighbor tiles of at most this many particles per workgroup in the pressure sweeps "
                          "(0: the sweeps gather from global memory; default: the solver's)")
@@ -170,7 +172,7 @@ def main():
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
     t0 = time.perf_counter()
     net = train_warm_start(lambda r: make(args.seed + 100 + r), rounds=args.train_rounds, steps=args.train_steps,
-                           sample=0.1, train_steps=400, log=log)
+                           sample=0.1, train_steps=400, log=log, rings=args.rings)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     log(f"warm-start network trained in {t_train:.1f} s")
@@ -433,7 +435,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"dfsph_random_flow_{args.nside}x{args.nside}", "particles": n,
-                   "solver": "DFSPH, learned warm start (MLP 16-64-64-64-1, bf16 MFMA) with plain-Jacobi judge",
+                   "solver": "DFSPH, learned warm start (MLP 16-64-64-64-1, bf16 MFMA; "
+                             f"{8 + 3 * (args.rings - 1)} features from {args.rings} neighbor ring(s)) "
+                             "with plain-Jacobi judge",
                    "dt": dt, "pairs": P, "neighbor_width": args.neighbor_width, "learned": rnd(learned), "plain": rnd(plain),
                    "p_prev_warm_start": rnd(prev),
                    "speedup_vs_plain": round(learned["value"] / plain["value"], 3),

@@ -1027,6 +1027,50 @@ __global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_ws_features_tiled(
   for (int c = 8; c < 16; ++c) o[c] = 0.0f;
 }
 
+// Second-ring warm-start features.  k_pack_ring packs (V, X[,1], X[,3],
+// X[,5]) of the raw first-ring features; k_ws_ring sums them over the row
+// exactly as the feature pass does (ws_feature_sums, W_ij streamed), writes
+// the sums to columns 8-10 and normalises the row's columns in place
+// ((f - mean16[c]) * istd16[c] for c < nfeat, 0 beyond): a row writes only
+// its own X row and reads its neighbors through the packed records.
+__global__ void __launch_bounds__(kBlock) k_pack_ring(int n, const float* __restrict__ vol,
+                                                      const float* __restrict__ X, float4* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float* x = X + (int64_t)i * 16;
+    rec[i] = make_float4(vol[i], x[1], x[3], x[5]);
+  }
+}
+
+template <bool kTiled>
+__global__ void __launch_bounds__(kBlock) k_ws_ring(PairsDev P, const float* __restrict__ pair_w,
+                                                    const float4* __restrict__ rec, const float* __restrict__ mean,
+                                                    const float* __restrict__ istd, int nfeat, float* __restrict__ X) {
+  extern __shared__ float4 tile[];
+  const int b = blockIdx.x;
+  const int i = b * kBlock + threadIdx.x;
+  bool tiled = false;
+  if (kTiled) tiled = load_tile(P, b, rec, tile);
+  if (i >= P.nq) return;
+  float s1 = 0.f, s3 = 0.f, s5 = 0.f;
+  const RowRef rr = row_ref(P, i);
+  if (tiled)
+    ws_feature_sums(rr, IdxSlot{P.tslot}, tile, pair_w, s1, s3, s5);
+  else if (use_idx16(P))
+    ws_feature_sums(rr, Idx16{P.nbr16, i, P.nr}, rec, pair_w, s1, s3, s5);
+  else
+    ws_feature_sums(rr, Idx32{P.nbr}, rec, pair_w, s1, s3, s5);
+  float* o = X + (int64_t)i * 16;
+  float f[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) f[c] = o[c];
+  f[8] = s1;
+  f[9] = s3;
+  f[10] = s5;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) o[c] = c < nfeat ? (f[c] - mean[c]) * istd[c] : 0.0f;
+}
+
 // Sweep launches: over neighbor tiles when the list carries them and the
 // sweep gathers records.
 static void launch_sweep_a(const PairsDev& P, const SweepAArgs& A, const SolveCtrl* ctrl, int nb, hipStream_t st) {
@@ -1675,6 +1719,27 @@ __global__ void __launch_bounds__(kBlock) k_pack_features(int n, const float* __
                                                           const float* __restrict__ pnp, float4* __restrict__ rec) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) rec[i] = make_float4(vol[i], pdp[i], src[i], pnp[i]);
+}
+
+extern "C" int dsph_ws_features_ring(const dsph_pairs* pairs, const float* vol, const float* pair_w, float* X,
+                                     int32_t nfeat, const float* mean16, const float* istd16, float* records,
+                                     void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr || nfeat < 11 || nfeat > 16) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vol || !pair_w || !X || !mean16 || !istd16 || !records) return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  hipStream_t st = as_stream(stream);
+  float4* rec = reinterpret_cast<float4*>(records);
+  const int nb = grid_for(P.nq);
+  k_pack_ring<<<nb, kBlock, 0, st>>>(P.nq, vol, X, rec);
+  DSPH_LAUNCH_CHECK();
+  if (P.tslot)
+    k_ws_ring<true><<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, pair_w, rec, mean16, istd16, nfeat, X);
+  else
+    k_ws_ring<false><<<nb, kBlock, 0, st>>>(P, pair_w, rec, mean16, istd16, nfeat, X);
+  return launch_status();
 }
 
 extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p_div_prev,

@@ -643,6 +643,16 @@ int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
                      const float* mean16, const float* istd16, float* X,
                      float* records, void* stream);
 
+/* Second-ring features over the raw features X of dsph_ws_features (mean 0,
+ * istd 1): columns 8, 9, 10 receive sum_j V_j W_ij X[j][c] for c = 1, 3, 5
+ * (pair_w required; records: scratch [n][4] floats), in the summation order
+ * of dsph_ws_features; then every row is normalised in place, (f -
+ * mean16[c]) * istd16[c] for c < nfeat (11 .. 16) and 0 beyond. */
+int dsph_ws_features_ring(const dsph_pairs* pairs, const float* vol,
+                          const float* pair_w, float* X, int32_t nfeat,
+                          const float* mean16, const float* istd16,
+                          float* records, void* stream);
+
 /* MLP 16 -> 64 -> 64 -> 64 -> 1 (ReLU), bf16 weights and layer inputs on
  * MFMA with fp32 accumulation.  Parameters are a flat fp32 array of
  * dsph_mlp_num_params() values: W1[16][64] b1[64] W2[64][64] b2[64]

@@ -227,3 +227,33 @@ def test_features_match_oracle_interpolation():
         np.testing.assert_array_equal(got["X"][:, col], ref.reshape(-1), err_msg=f"feature column {col}")
     np.testing.assert_array_equal(got["X"][:, 0], got["pdiv"])
     np.testing.assert_array_equal(got["X"][:, 2], got["src"])
+
+
+@pytest.mark.parametrize("tile_cap", [0, 4096])
+def test_second_ring_features_match_two_first_ring_passes(tile_cap):
+    """dsph_ws_features_ring (second-ring sums of the first ring's sums in
+    columns 8-10, normalisation in place) equals the first-ring feature pass
+    applied to the first ring's sum columns followed by the normalisation,
+    bit for bit, with and without neighbor tiles."""
+    d = sampling.dfsph_random_flow(128, seed=3, jitter_width=0.1)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+    sv = DFSPHSolver(T(d["pos"]), T(d["vel"]), T(d["mass"]), T(d["area"]),
+                     DFSPHConfig(dt=3.0 * d["dx"] / 10.0, support=d["h"], tile_cap=tile_cap))
+    sv.keep_pair_w = True
+    sv.step()
+    sv.step()
+    net = WarmStartNet("cuda:0", rings=2)
+    raw = net.features(sv, normalized=False)
+    net.fit_normalization(raw, torch.randn(raw.shape[0], device=dev))
+    fused = net.features(sv)
+    z, o = torch.zeros(16, device=dev), torch.ones(16, device=dev)
+    first = net._feature_pass(sv, sv.p_div, sv.div_src, sv.p_den, z, o)
+    second = net._feature_pass(sv, first[:, 1].contiguous(), first[:, 3].contiguous(), first[:, 5].contiguous(), z, o)
+    comp = first.clone()
+    comp[:, 8:11] = second[:, [1, 3, 5]]
+    comp = (comp - net.mean) * net.istd
+    comp[:, 11:] = 0.0
+    np.testing.assert_array_equal(fused.cpu().numpy(), comp.cpu().numpy())
+    np.testing.assert_array_equal(raw[:, 8:11].cpu().numpy(), second[:, [1, 3, 5]].cpu().numpy())
+    assert float(fused[:, 8:11].abs().max()) > 0
