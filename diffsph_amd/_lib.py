@@ -259,7 +259,7 @@ SIGNATURES = {
     "dsph_density_error": (C.c_int, [V, I64, F32, P(C.c_double), V, SZ, V]),
     "dsph_integrate_semi_implicit": (C.c_int, [V, V, V, V, V, V, V, I64, F32, V, V]),
     "dsph_ws_features": (C.c_int, [P(Pairs), V, V, V, V, V, V, F32, V, V, V, V, V, V]),
-    "dsph_ws_features_ring": (C.c_int, [P(Pairs), V, V, V, I32, V, V, V, V]),
+    "dsph_ws_features_ring": (C.c_int, [P(Pairs), V, V, V, V, V, V, F32, V, V, V, I32, V, V, V, V]),
     "dsph_density_w": (C.c_int, [P(Pairs), V, V, V, V, V, V]),
     "dsph_mlp_num_params": (C.c_int, []),
     "dsph_mlp_num_inputs": (C.c_int, []),

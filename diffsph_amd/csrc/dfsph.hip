@@ -1005,7 +1005,7 @@ __global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_ws_features_tiled(
     PairsDev P, const float* __restrict__ rho, const float* __restrict__ pdp, const float* __restrict__ pnp,
     const float* __restrict__ src, const float* __restrict__ alpha, float rho0, const float* __restrict__ pair_w,
     const float* __restrict__ mean, const float* __restrict__ istd, float* __restrict__ X,
-    const float4* __restrict__ rec) {
+    const float4* __restrict__ rec, float4* __restrict__ ring) {
   extern __shared__ float4 tile[];
   const int b = blockIdx.x;
   const int i = b * kBlock + threadIdx.x;
@@ -1025,50 +1025,35 @@ __global__ void __launch_bounds__(kBlock) DSPH_TILED_ATTR k_ws_features_tiled(
   for (int c = 0; c < 8; ++c) o[c] = (f[c] - mean[c]) * istd[c];
 #pragma unroll
   for (int c = 8; c < 16; ++c) o[c] = 0.0f;
+  if (ring) ring[i] = make_float4(rec[i].x, s1, s3, s5);
 }
 
-// Second-ring warm-start features.  k_pack_ring packs (V, X[,1], X[,3],
-// X[,5]) of the raw first-ring features; k_ws_ring sums them over the row
-// exactly as the feature pass does (ws_feature_sums, W_ij streamed), writes
-// the sums to columns 8-10 and normalises the row's columns in place
-// ((f - mean16[c]) * istd16[c] for c < nfeat, 0 beyond): a row writes only
-// its own X row and reads its neighbors through the packed records.
-__global__ void __launch_bounds__(kBlock) k_pack_ring(int n, const float* __restrict__ vol,
-                                                      const float* __restrict__ X, float4* __restrict__ rec) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const float* x = X + (int64_t)i * 16;
-    rec[i] = make_float4(vol[i], x[1], x[3], x[5]);
-  }
-}
-
+// Second-ring warm-start features: the first-ring pass wrote (V, raw sums
+// of p_div_prev, s, p_den_prev) per particle into `ring`; k_ws_ring sums
+// those over the row exactly as the feature pass sums its records
+// (ws_feature_sums, W_ij streamed) and writes them, normalised, to columns
+// 8-10 of the row ((f - mean16[c]) * istd16[c] for c < nfeat, else 0).
 template <bool kTiled>
 __global__ void __launch_bounds__(kBlock) k_ws_ring(PairsDev P, const float* __restrict__ pair_w,
-                                                    const float4* __restrict__ rec, const float* __restrict__ mean,
+                                                    const float4* __restrict__ ring, const float* __restrict__ mean,
                                                     const float* __restrict__ istd, int nfeat, float* __restrict__ X) {
   extern __shared__ float4 tile[];
   const int b = blockIdx.x;
   const int i = b * kBlock + threadIdx.x;
   bool tiled = false;
-  if (kTiled) tiled = load_tile(P, b, rec, tile);
+  if (kTiled) tiled = load_tile(P, b, ring, tile);
   if (i >= P.nq) return;
-  float s1 = 0.f, s3 = 0.f, s5 = 0.f;
+  float t[3] = {0.f, 0.f, 0.f};
   const RowRef rr = row_ref(P, i);
   if (tiled)
-    ws_feature_sums(rr, IdxSlot{P.tslot}, tile, pair_w, s1, s3, s5);
+    ws_feature_sums(rr, IdxSlot{P.tslot}, tile, pair_w, t[0], t[1], t[2]);
   else if (use_idx16(P))
-    ws_feature_sums(rr, Idx16{P.nbr16, i, P.nr}, rec, pair_w, s1, s3, s5);
+    ws_feature_sums(rr, Idx16{P.nbr16, i, P.nr}, ring, pair_w, t[0], t[1], t[2]);
   else
-    ws_feature_sums(rr, Idx32{P.nbr}, rec, pair_w, s1, s3, s5);
+    ws_feature_sums(rr, Idx32{P.nbr}, ring, pair_w, t[0], t[1], t[2]);
   float* o = X + (int64_t)i * 16;
-  float f[16];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) f[c] = o[c];
-  f[8] = s1;
-  f[9] = s3;
-  f[10] = s5;
-#pragma unroll
-  for (int c = 0; c < 16; ++c) o[c] = c < nfeat ? (f[c] - mean[c]) * istd[c] : 0.0f;
+  for (int c = 8; c < 11; ++c) o[c] = c < nfeat ? (t[c - 8] - mean[c]) * istd[c] : 0.0f;
 }
 
 // Sweep launches: over neighbor tiles when the list carries them and the
@@ -1124,9 +1109,10 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
                                                         const float* __restrict__ pair_w,
                                                         const float* __restrict__ mean,
                                                         const float* __restrict__ istd, float* __restrict__ X,
-                                                        const float4* __restrict__ rec) {
+                                                        const float4* __restrict__ rec, float4* __restrict__ ring) {
   // rec (with pair_w): (V, p_div_prev, s, p_den_prev) per particle, so that
-  // an entry costs one gather instead of four
+  // an entry costs one gather instead of four; ring (may be NULL): receives
+  // (V, raw sums of p_div_prev, s, p_den_prev), the second ring's records
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.nq) return;
   const float2 xi = P.qpos[i];
@@ -1203,6 +1189,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_features(PairsDev P, const float*
   for (int c = 0; c < 8; ++c) o[c] = (f[c] - mean[c]) * istd[c];
 #pragma unroll
   for (int c = 8; c < 16; ++c) o[c] = 0.0f;
+  if (ring) ring[i] = make_float4(vol[i], s1, s3, s5);
 }
 
 // ---------------------------------------------------------------------------
@@ -1721,26 +1708,20 @@ __global__ void __launch_bounds__(kBlock) k_pack_features(int n, const float* __
   if (i < n) rec[i] = make_float4(vol[i], pdp[i], src[i], pnp[i]);
 }
 
-extern "C" int dsph_ws_features_ring(const dsph_pairs* pairs, const float* vol, const float* pair_w, float* X,
-                                     int32_t nfeat, const float* mean16, const float* istd16, float* records,
-                                     void* stream) {
-  int s = check_pairs(pairs);
-  if (s != DSPH_OK) return s;
-  if (pairs->nq != pairs->nr || nfeat < 11 || nfeat > 16) return DSPH_ERR_ARG;
-  if (pairs->nq == 0) return DSPH_OK;
-  if (!vol || !pair_w || !X || !mean16 || !istd16 || !records) return DSPH_ERR_ARG;
-  PairsDev P = make_pairs(*pairs);
-  hipStream_t st = as_stream(stream);
-  float4* rec = reinterpret_cast<float4*>(records);
-  const int nb = grid_for(P.nq);
-  k_pack_ring<<<nb, kBlock, 0, st>>>(P.nq, vol, X, rec);
-  DSPH_LAUNCH_CHECK();
-  if (P.tslot)
-    k_ws_ring<true><<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, pair_w, rec, mean16, istd16, nfeat, X);
+namespace dsph {
+static int launch_ws_features(const PairsDev& P, const float* vol, const float* rho, const float* pdp,
+                              const float* pnp, const float* src, const float* alpha, float rho0, const float* pair_w,
+                              const float* mean16, const float* istd16, float* X, const float4* rec, float4* ring,
+                              hipStream_t st) {
+  if (P.tslot && rec)
+    k_ws_features_tiled<<<grid_for(P.nq), kBlock, (size_t)P.tcap * sizeof(float4), st>>>(
+        P, rho, pdp, pnp, src, alpha, rho0, pair_w, mean16, istd16, X, rec, ring);
   else
-    k_ws_ring<false><<<nb, kBlock, 0, st>>>(P, pair_w, rec, mean16, istd16, nfeat, X);
+    k_ws_features<<<grid_for(P.nq), kBlock, 0, st>>>(P, vol, rho, pdp, pnp, src, alpha, rho0, pair_w, mean16,
+                                                     istd16, X, rec, ring);
   return launch_status();
 }
+}  // namespace dsph
 
 extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const float* rho, const float* p_div_prev,
                                 const float* p_den_prev, const float* div_src, const float* alpha_div, float rho0,
@@ -1759,11 +1740,35 @@ extern "C" int dsph_ws_features(const dsph_pairs* pairs, const float* vol, const
     k_pack_features<<<grid_for(P.nq), kBlock, 0, st>>>(P.nq, vol, p_div_prev, div_src, p_den_prev, rec);
     DSPH_LAUNCH_CHECK();
   }
-  if (P.tslot && rec)
-    k_ws_features_tiled<<<grid_for(P.nq), kBlock, (size_t)P.tcap * sizeof(float4), st>>>(
-        P, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0, pair_w, mean16, istd16, X, rec);
+  return launch_ws_features(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0, pair_w, mean16, istd16,
+                            X, rec, nullptr, st);
+}
+
+extern "C" int dsph_ws_features_ring(const dsph_pairs* pairs, const float* vol, const float* rho,
+                                     const float* p_div_prev, const float* p_den_prev, const float* div_src,
+                                     const float* alpha_div, float rho0, const float* pair_w, const float* mean16,
+                                     const float* istd16, int32_t nfeat, float* X, float* records,
+                                     float* ring_records, void* stream) {
+  int s = check_pairs(pairs);
+  if (s != DSPH_OK) return s;
+  if (pairs->nq != pairs->nr || nfeat < 11 || nfeat > 16) return DSPH_ERR_ARG;
+  if (pairs->nq == 0) return DSPH_OK;
+  if (!vol || !rho || !p_div_prev || !p_den_prev || !div_src || !alpha_div || !mean16 || !istd16 || !X ||
+      !pair_w || !records || !ring_records)
+    return DSPH_ERR_ARG;
+  PairsDev P = make_pairs(*pairs);
+  hipStream_t st = as_stream(stream);
+  float4* rec = reinterpret_cast<float4*>(records);
+  float4* ring = reinterpret_cast<float4*>(ring_records);
+  k_pack_features<<<grid_for(P.nq), kBlock, 0, st>>>(P.nq, vol, p_div_prev, div_src, p_den_prev, rec);
+  DSPH_LAUNCH_CHECK();
+  s = launch_ws_features(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0, pair_w, mean16, istd16, X,
+                         rec, ring, st);
+  if (s != DSPH_OK) return s;
+  const int nb = grid_for(P.nq);
+  if (P.tslot)
+    k_ws_ring<true><<<nb, kBlock, (size_t)P.tcap * sizeof(float4), st>>>(P, pair_w, ring, mean16, istd16, nfeat, X);
   else
-    k_ws_features<<<grid_for(P.nq), kBlock, 0, st>>>(P, vol, rho, p_div_prev, p_den_prev, div_src, alpha_div, rho0,
-                                                     pair_w, mean16, istd16, X, rec);
+    k_ws_ring<false><<<nb, kBlock, 0, st>>>(P, pair_w, ring, mean16, istd16, nfeat, X);
   return launch_status();
 }

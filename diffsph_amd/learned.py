@@ -42,15 +42,15 @@ def _pair_w(solver):
     return solver.wpair if getattr(solver, "keep_pair_w", False) else None
 
 
-def _records(solver):
+def _records(solver, name: str = "_feat_rec"):
     """Grow-only [n][4] scratch for the packed neighbor fields of the feature
-    pass (used with the kept W_ij)."""
+    pass (used with the kept W_ij), kept on the solver under `name`."""
     if _pair_w(solver) is None:
         return None
-    r = getattr(solver, "_feat_rec", None)
+    r = getattr(solver, name, None)
     if r is None or r.shape[0] < solver.n:
         r = torch.empty(solver.n, 4, dtype=torch.float32, device=solver.pos.device)
-        solver._feat_rec = r
+        setattr(solver, name, r)
     return r
 
 
@@ -145,16 +145,20 @@ class WarmStartNet:
             istd = self.istd if normalized else torch.ones(NIN, device=self.device)
             return self._feature_pass(solver, pdp, solver.div_src, pnp, mean, istd)
         raw0, raw1 = torch.zeros(NIN, device=self.device), torch.ones(NIN, device=self.device)
-        X = self._feature_pass(solver, pdp, solver.div_src, pnp, raw0, raw1)
         mean = self.mean if normalized else raw0
         istd = self.istd if normalized else raw1
         pw, rec = _pair_w(solver), _records(solver)
         if pw is not None and rec is not None:
-            check(lib().dsph_ws_features_ring(C.byref(solver.nl.pairs()), ptr(solver.vol), ptr(pw), ptr(X),
-                                              self.nfeat, ptr(mean), ptr(istd), ptr(rec),
+            X = torch.empty(n, NIN, device=self.device)
+            ring = _records(solver, "_ring_rec")
+            check(lib().dsph_ws_features_ring(C.byref(solver.nl.pairs()), ptr(solver.vol), ptr(solver.rho),
+                                              ptr(pdp), ptr(pnp), ptr(solver.div_src), ptr(solver.alpha_div),
+                                              C.c_float(solver.cfg.rho0), ptr(pw), ptr(mean), ptr(istd),
+                                              self.nfeat, ptr(X), ptr(rec), ptr(ring),
                                               _lib.stream_handle(self.device)), "ws_features_ring")
             return X
         # without the kept W_ij: the first-ring pass over the first ring's sums
+        X = self._feature_pass(solver, pdp, solver.div_src, pnp, raw0, raw1)
         X2 = self._feature_pass(solver, X[:, 1].contiguous(), X[:, 3].contiguous(), X[:, 5].contiguous(), raw0, raw1)
         X[:, NFEAT:NFEAT + 3] = X2[:, [1, 3, 5]]
         X = (X - mean) * istd

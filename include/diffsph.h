@@ -643,15 +643,20 @@ int dsph_ws_features(const dsph_pairs* pairs, const float* vol,
                      const float* mean16, const float* istd16, float* X,
                      float* records, void* stream);
 
-/* Second-ring features over the raw features X of dsph_ws_features (mean 0,
- * istd 1): columns 8, 9, 10 receive sum_j V_j W_ij X[j][c] for c = 1, 3, 5
- * (pair_w required; records: scratch [n][4] floats), in the summation order
- * of dsph_ws_features; then every row is normalised in place, (f -
- * mean16[c]) * istd16[c] for c < nfeat (11 .. 16) and 0 beyond. */
+/* dsph_ws_features plus three second-ring features: columns 8, 9, 10
+ * receive sum_j V_j W_ij t_j for the first ring's raw sums t = (column 1,
+ * 3, 5 before normalisation), in the summation order of dsph_ws_features,
+ * normalised like the others ((f - mean16[c]) * istd16[c] for c < nfeat,
+ * 11 .. 16; 0 beyond).  pair_w is required; records and ring_records are
+ * scratch [n][4] floats each (ring_records receives (V, raw sums) of the
+ * first pass, the records of the second). */
 int dsph_ws_features_ring(const dsph_pairs* pairs, const float* vol,
-                          const float* pair_w, float* X, int32_t nfeat,
-                          const float* mean16, const float* istd16,
-                          float* records, void* stream);
+                          const float* rho, const float* p_div_prev,
+                          const float* p_den_prev, const float* div_src,
+                          const float* alpha_div, float rho0, const float* pair_w,
+                          const float* mean16, const float* istd16, int32_t nfeat,
+                          float* X, float* records, float* ring_records,
+                          void* stream);
 
 /* MLP 16 -> 64 -> 64 -> 64 -> 1 (ReLU), bf16 weights and layer inputs on
  * MFMA with fp32 accumulation.  Parameters are a flat fp32 array of
