@@ -97,7 +97,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--train-rounds", type=int, default=8)
+    ap.add_argument("--train-rounds", type=int, default=16)
     ap.add_argument("--train-steps", type=int, default=48, help="trajectory steps per training round")
     ap.add_argument("--no-ablation", action="store_true", help="skip the p_prev (warm start 1.0) run")
     ap.add_argument("--tight", action="store_true",
@@ -105,6 +105,8 @@ def parse():
     ap.add_argument("--neighbor-width", type=int, default=67,
                     help="row width of the one-pass neighbor list (0: count pass + two-pass list)")
     ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--train-opt-steps", type=int, default=6400, help="optimizer steps per training round")
+    ap.add_argument("--train-sample", type=float, default=0.1, help="share of particles recorded per training step")
     ap.add_argument("--rings", type=int, default=2, choices=[1, 2],
                     help="dfsph: warm-start features from one or two neighbor rings")
     ap.add_argument("--tile-cap", type=int, default=None,
@@ -172,7 +174,7 @@ def main():
     log(f"workload: {n} particles, h={d['h']:.6g}, dt={dt:.6g}")
     t0 = time.perf_counter()
     net = train_warm_start(lambda r: make(args.seed + 100 + r), rounds=args.train_rounds, steps=args.train_steps,
-                           sample=0.1, train_steps=400, log=log, rings=args.rings)
+                           sample=args.train_sample, train_steps=args.train_opt_steps, log=log, rings=args.rings)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     log(f"warm-start network trained in {t_train:.1f} s")
