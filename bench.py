@@ -322,9 +322,10 @@ def main():
     # The bytes this layout moves (same counting rules): sweep A (x,y,p) record 16 + own x 8 + rho 4 +
     # count 4 + (x,y,a) record (w) 16 = 48 B/particle; sweep B (x,y,a) 16 + own x 8 + p 4 (r) + 4 (w) +
     # s 4 + alpha 4 + count 4 + residual 4 (w) + (x,y,p) (w) 16 = 64 B/particle; per entry the index
-    # (2 as a 16-bit delta, else 4) and the stored coefficient s_ij 4.
+    # (2 as a tile slot or a 16-bit delta, else 4) and the stored coefficient s_ij 4.
     idx16 = solver.nl.nbr16 is not None and int(solver.nl.nbr16_overflow.item()) == 0
-    per_pair = 6 if idx16 else 8
+    tiled = solver.tile_stats()["tiled"] if solver.nl.tiles is not None else 0.0
+    per_pair = 4 + 2 * tiled + (2 if idx16 else 4) * (1.0 - tiled)
     bytes_a = 48 * n + per_pair * P
     bytes_b = 64 * n + per_pair * P
     log(f"sweep A (dsph_pressure_accel_rec): {t_a * 1e6:.1f} us, model {model_a / 1e6:.1f} MB -> "

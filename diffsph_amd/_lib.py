@@ -231,6 +231,8 @@ SIGNATURES = {
                                            V, I64, V, V, V, V]),
     "dsph_neighbor_fill_fixed_map": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, I32, V,
                                                V, V, I64, V, V, V]),
+    "dsph_neighbor_fill_fixed_tiles": (C.c_int, [P(Domain), P(Grid), V, V, V, I64, V, V, I64, I32, F32, I32, I32,
+                                                 V, V, V, I64, V, V, I32, V, I64, V, V, V, V]),
     "dsph_sell_workspace": (SZ, [I64]),
     "dsph_sell_layout": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
     "dsph_sell_to_csr": (C.c_int, [P(Pairs), V, V]),

@@ -1380,6 +1380,21 @@ extern "C" int dsph_neighbor_compact16(const dsph_pairs* pairs, int32_t limit, i
   return launch_status();
 }
 
+namespace dsph {
+int launch_tile_runs(int64_t nq, const dsph_grid* grid, const dsph_domain* domain, const int32_t* cell_start,
+                     int32_t* tile_runs, int32_t* tile_n, hipStream_t st) {
+  if (grid->ncell[0] < 1 || grid->ncell[1] < 1 || (int64_t)grid->ncell[0] * grid->ncell[1] >= INT32_MAX)
+    return DSPH_ERR_ARG;
+  if (nq == 0) return DSPH_OK;
+  const int nt = grid_for(nq);
+  k_tile_runs<<<(nt + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(
+      (int)nq, grid->ncell[0], grid->ncell[1], domain->periodic[0] != 0, domain->periodic[1] != 0, cell_start, nt,
+      tile_runs, tile_n);
+  DSPH_LAUNCH_CHECK();
+  return DSPH_OK;
+}
+}  // namespace dsph
+
 extern "C" int dsph_neighbor_tiles(const dsph_pairs* pairs, const dsph_grid* grid, const int32_t* cell_start,
                                    int32_t cap, uint16_t* slot, int64_t slot_capacity, int32_t* tile_runs,
                                    int32_t* tile_n, void* stream) {
@@ -1395,10 +1410,8 @@ extern "C" int dsph_neighbor_tiles(const dsph_pairs* pairs, const dsph_grid* gri
   if (!slot || !tile_runs || !tile_n || !cell_start) return DSPH_ERR_ARG;
   hipStream_t st = as_stream(stream);
   const int nt = grid_for(q.nq);
-  k_tile_runs<<<(nt + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(
-      (int)q.nq, grid->ncell[0], grid->ncell[1], q.domain.periodic[0] != 0, q.domain.periodic[1] != 0, cell_start,
-      nt, tile_runs, tile_n);
-  DSPH_LAUNCH_CHECK();
+  s = launch_tile_runs(q.nq, grid, &q.domain, cell_start, tile_runs, tile_n, st);
+  if (s != DSPH_OK) return s;
   k_tile_slots<<<nt, kBlock, 0, st>>>(make_pairs(q), cap, slot, slot_capacity, tile_runs, tile_n);
   return launch_status();
 }

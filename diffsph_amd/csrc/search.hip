@@ -13,6 +13,9 @@ namespace dsph {
 
 int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, void* ws, size_t ws_bytes, hipStream_t st);
 size_t scan_ws(int64_t n);
+// Tile run tables of a square list over the grid (k_tile_runs).
+int launch_tile_runs(int64_t nq, const dsph_grid* grid, const dsph_domain* domain, const int32_t* cell_start,
+                     int32_t* tile_runs, int32_t* tile_n, hipStream_t st);
 int scan_read_total(const void* ws, int64_t* total_host, hipStream_t st);
 
 struct GridDev {
@@ -220,19 +223,119 @@ __device__ __forceinline__ void put_entry(int32_t* __restrict__ nbr, int16_t* __
   }
 }
 
+// Tile slots written by the fill (fixed-width layout, rows = reference
+// slots): workgroup b of the fill is tile b of the list; each entry also
+// gets its neighbor's slot, j + run_d[u] for the run u holding j.  The tile
+// runs come first (they depend on the cells only).
+struct FillTiles {
+  int32_t* runs;        // run tables, kTileRunStride ints per tile
+  int32_t* n;           // particles per tile (set to -1 here when unusable)
+  uint16_t* slot;       // per entry, beside nbr
+  int cap;
+};
+
+// Largest tile whose staged (run << 12 | slot) values stay below the
+// sentinel 0xffff.
+constexpr int kFillTileMax = 4095;
+constexpr int16_t kStagedNoRun = -1;  // slot mode: entry already written to global memory
+
+// Row i of the fill.  kSlot: staged values are (u << 12) | slot, u the run
+// of the candidate range in the tile's run table (rt_*), and the slots are
+// written beside the entries; otherwise they are j - i.
+template <bool kSlot>
+__device__ __forceinline__ int64_t fill_row(const SearchArgs& a, int i, float2 qi, float hi, int64_t base,
+                                            int64_t step, int64_t len, int64_t staged, int fixed_w,
+                                            int16_t (*stage)[kBlock], int32_t* __restrict__ nbr, int64_t cap,
+                                            int16_t* __restrict__ d16, const int32_t* __restrict__ map,
+                                            const FillTiles& T, const int* rt_s, const int* rt_e, const int* rt_d,
+                                            int R, bool& bad, bool& tbad) {
+  constexpr int16_t kOut = kSlot ? kStagedNoRun : kStagedOut;
+  int64_t n = 0;
+  // Every candidate writes the staging slot of the next entry (row
+  // kFillStage, a sink, once the row is past the staged entries); a kept
+  // candidate then advances n, so its value stays and the next candidates
+  // write the following slot.  Only an entry past the staged ones, or whose
+  // value does not fit the staging (a delta beyond 16 bits; a range in no
+  // run), takes the branch to global memory.
+  auto take = [&](bool kp, int j, int sb, bool range_ok) {
+    const int v = j + sb;
+    const bool fits = kSlot ? range_ok : (v > -32768 && v <= 32767);
+    const bool st = n < staged;
+    stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)v : kOut;
+    if (kp && !(st && fits)) {
+      const int64_t o = base + n * step;
+      if (n < len && o < cap) {
+        put_entry(nbr, d16, map, o, j, i, a.nref, bad);
+        if (kSlot && fits) T.slot[o] = (uint16_t)(v & 0xfff);
+      }
+    }
+    n += kp ? 1 : 0;
+  };
+  auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
+  for_candidates(a, qi, [&](int b, int e, bool seam) {
+    // staged value of candidate j: j + sb
+    int sb = -i;
+    bool ok = true;
+    if (kSlot && e > b) {
+      int u = -1;
+      for (int r = 0; r < R; ++r) u = (b >= rt_s[r] && e <= rt_e[r]) ? r : u;
+      ok = u >= 0;
+      tbad = tbad || !ok;
+      sb = ok ? (u << 12) + rt_d[u] : 0;
+    }
+    if (seam)
+      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair(qi, hi, r.x, r.h, a), j, sb, ok); });
+    else
+      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair_near(qi, hi, r.x, r.h, a), j, sb, ok); });
+  });
+  for (int64_t t = 0; t < staged; ++t) {
+    const int64_t o = base + t * step;
+    if (o >= cap) continue;
+    if (t < n) {
+      const int16_t v = stage[t][threadIdx.x];
+      if (v == kOut) continue;
+      if (kSlot) {
+        const int w = (uint16_t)v, s = w & 0xfff;
+        put_entry(nbr, d16, map, o, s - rt_d[w >> 12], i, a.nref, bad);
+        T.slot[o] = (uint16_t)s;
+      } else {
+        put_entry(nbr, d16, map, o, i + v, i, a.nref, bad);
+      }
+    } else if (fixed_w == 0) {
+      nbr[o] = -1;
+    }
+  }
+  return n;
+}
+
 // fixed_w > 0: the one-pass fixed-width layout (slice s at s * 64 * fixed_w,
 // no count pass): counts[i] = min(entries, fixed_w), *row_ovf set when a row
-// has more entries than fixed_w (the list is then incomplete).
+// has more entries than fixed_w (the list is then incomplete).  kTiles: the
+// fixed-width layout with tile slots (T); a tile that does not fit T.cap
+// (or kFillTileMax) is filled without them, and one whose rows name a
+// candidate range outside its runs gets tile_n = -1.
+template <bool kTiles>
 __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ slice,
                                                           int32_t* __restrict__ nbr, int64_t cap,
                                                           int16_t* __restrict__ d16, int32_t* __restrict__ ovf,
                                                           int fixed_w, int32_t* __restrict__ counts,
                                                           int32_t* __restrict__ row_ovf,
-                                                          const int32_t* __restrict__ map) {
-  // one extra row: the sink of the unconditional staging writes below
+                                                          const int32_t* __restrict__ map, FillTiles T) {
+  // one extra row: the sink of the unconditional staging writes
   __shared__ int16_t stage[kFillStage + 1][kBlock];
-  bool bad = false;
+  __shared__ int rt_s[kTileMaxRuns], rt_e[kTileMaxRuns], rt_d[kTileMaxRuns];
+  bool fit = false;
+  int R = 0;
+  if (kTiles) {
+    // every thread reads tile_n before any may set it to -1 (barrier)
+    const int S = T.n[blockIdx.x];
+    fit = S >= 0 && S <= T.cap && S <= kFillTileMax;
+    if (fit) R = load_run_table(T.runs, blockIdx.x, rt_s, rt_e, rt_d);
+    __syncthreads();
+    if (S > kFillTileMax && S <= T.cap && threadIdx.x == 0) T.n[blockIdx.x] = -1;
+  }
+  bool bad = false, tbad = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   float2 qi = a.qpos[i];
@@ -254,40 +357,11 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     len = (int64_t)off[i + 1] - off[i];
   }
   const int64_t staged = (slice || fixed_w > 0) ? (kFillStage < len ? kFillStage : len) : 0;
-  int64_t n = 0;
-  // Every candidate writes the staging slot of the next entry (row
-  // kFillStage, a sink, once the row is past the staged entries); a kept
-  // candidate then advances n, so its value stays and the next candidates
-  // write the following slot.  Only an entry past the staged ones, or whose
-  // delta needs more than 16 bits, takes the branch to global memory.
-  auto take = [&](bool kp, int j) {
-    const int dj = j - i;
-    const bool fits = dj > -32768 && dj <= 32767;
-    const bool st = n < staged;
-    stage[st ? n : kFillStage][threadIdx.x] = fits ? (int16_t)dj : kStagedOut;
-    if (kp && !(st && fits)) {
-      const int64_t o = base + n * step;
-      if (n < len && o < cap) put_entry(nbr, d16, map, o, j, i, a.nref, bad);
-    }
-    n += kp ? 1 : 0;
-  };
-  auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
-  for_candidates(a, qi, [&](int b, int e, bool seam) {
-    if (seam)
-      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair(qi, hi, r.x, r.h, a), j); });
-    else
-      scan_range<4>(b, e, load, [&](int j, const Cand& r) { take(keep_pair_near(qi, hi, r.x, r.h, a), j); });
-  });
-  for (int64_t t = 0; t < staged; ++t) {
-    const int64_t o = base + t * step;
-    if (o >= cap) continue;
-    if (t < n) {
-      const int16_t v = stage[t][threadIdx.x];
-      if (v != kStagedOut) put_entry(nbr, d16, map, o, i + v, i, a.nref, bad);
-    } else if (fixed_w == 0) {
-      nbr[o] = -1;
-    }
-  }
+  const int64_t n = (kTiles && fit) ? fill_row<true>(a, i, qi, hi, base, step, len, staged, fixed_w, stage, nbr,
+                                                     cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad)
+                                    : fill_row<false>(a, i, qi, hi, base, step, len, staged, fixed_w, stage, nbr,
+                                                      cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad);
+  if (kTiles && tbad) T.n[blockIdx.x] = -1;
   if (fixed_w > 0) {
     // entries past the row's count are never read: no padding
     counts[i] = n < len ? (int32_t)n : (int32_t)len;
@@ -502,8 +576,8 @@ extern "C" int dsph_neighbor_fill(const dsph_domain* domain, const dsph_grid* gr
   if (nq == 0) return DSPH_OK;
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(a, offsets, slice_off, nbr, capacity, nullptr,
-                                                                   nullptr, 0, nullptr, nullptr, nullptr);
+  k_neighbor_fill<false><<<grid_for(nq), kBlock, 0, as_stream(stream)>>>(
+      a, offsets, slice_off, nbr, capacity, nullptr, nullptr, 0, nullptr, nullptr, nullptr, FillTiles{});
   return launch_status();
 }
 
@@ -526,8 +600,8 @@ extern "C" int dsph_neighbor_fill16(const dsph_domain* domain, const dsph_grid* 
   if (nq == 0) return DSPH_OK;
   if ((!offsets && !slice_off) || (capacity > 0 && !nbr) || capacity < 0 || layout_total < 0) return DSPH_ERR_ARG;
   if (layout_total > capacity) return DSPH_ERR_CAPACITY;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow, 0, nullptr,
-                                                    nullptr, nullptr);
+  k_neighbor_fill<false><<<grid_for(nq), kBlock, 0, st>>>(a, offsets, slice_off, nbr, capacity, nbr16, overflow, 0,
+                                                           nullptr, nullptr, nullptr, FillTiles{});
   return launch_status();
 }
 
@@ -536,7 +610,7 @@ static int fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const in
                       const float* sorted_pos, const float* sorted_h, int64_t nref, const float* qpos, const float* qh,
                       int64_t nq, int32_t qpos_wrapped, float h_uniform, int32_t mode, int32_t width, int32_t* counts,
                       int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16, int32_t* delta_overflow,
-                      int32_t* row_overflow, const int32_t* map, void* stream) {
+                      int32_t* row_overflow, const int32_t* map, const FillTiles* tiles, void* stream) {
   SearchArgs a;
   int s = make_search(a, domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped,
                       h_uniform, mode);
@@ -553,8 +627,15 @@ static int fill_fixed(const dsph_domain* domain, const dsph_grid* grid, const in
   DSPH_LAUNCH_CHECK();
   if (nq == 0) return DSPH_OK;
   if (!counts || !nbr) return DSPH_ERR_ARG;
-  k_neighbor_fill<<<grid_for(nq), kBlock, 0, st>>>(a, nullptr, nullptr, nbr, capacity, nbr16, delta_overflow, width,
-                                                    counts, row_overflow, map);
+  if (tiles) {
+    int s2 = launch_tile_runs(nq, grid, domain, cell_start, tiles->runs, tiles->n, st);
+    if (s2 != DSPH_OK) return s2;
+    k_neighbor_fill<true><<<grid_for(nq), kBlock, 0, st>>>(a, nullptr, nullptr, nbr, capacity, nbr16, delta_overflow,
+                                                          width, counts, row_overflow, map, *tiles);
+  } else {
+    k_neighbor_fill<false><<<grid_for(nq), kBlock, 0, st>>>(a, nullptr, nullptr, nbr, capacity, nbr16, delta_overflow,
+                                                           width, counts, row_overflow, map, FillTiles{});
+  }
   return launch_status();
 }
 }  // namespace dsph
@@ -566,7 +647,8 @@ extern "C" int dsph_neighbor_fill_fixed(const dsph_domain* domain, const dsph_gr
                                         int32_t* slice_off, int32_t* nbr, int64_t capacity, int16_t* nbr16,
                                         int32_t* delta_overflow, int32_t* row_overflow, void* stream) {
   return fill_fixed(domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped, h_uniform, mode,
-                    width, counts, slice_off, nbr, capacity, nbr16, delta_overflow, row_overflow, nullptr, stream);
+                    width, counts, slice_off, nbr, capacity, nbr16, delta_overflow, row_overflow, nullptr, nullptr,
+                    stream);
 }
 
 extern "C" int dsph_neighbor_fill_fixed_map(const dsph_domain* domain, const dsph_grid* grid,
@@ -578,7 +660,27 @@ extern "C" int dsph_neighbor_fill_fixed_map(const dsph_domain* domain, const dsp
                                             void* stream) {
   if (nref > 0 && !map) return DSPH_ERR_ARG;
   return fill_fixed(domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped, h_uniform, mode,
-                    width, counts, slice_off, nbr, capacity, nullptr, nullptr, row_overflow, map, stream);
+                    width, counts, slice_off, nbr, capacity, nullptr, nullptr, row_overflow, map, nullptr, stream);
+}
+
+extern "C" int dsph_neighbor_fill_fixed_tiles(const dsph_domain* domain, const dsph_grid* grid,
+                                              const int32_t* cell_start, const float* sorted_pos,
+                                              const float* sorted_h, int64_t nref, const float* qpos,
+                                              const float* qh, int64_t nq, int32_t qpos_wrapped, float h_uniform,
+                                              int32_t mode, int32_t width, int32_t* counts, int32_t* slice_off,
+                                              int32_t* nbr, int64_t capacity, int16_t* nbr16,
+                                              int32_t* delta_overflow, int32_t tile_cap, uint16_t* slot,
+                                              int64_t slot_capacity, int32_t* tile_runs, int32_t* tile_n,
+                                              int32_t* row_overflow, void* stream) {
+  // the tiles of a list whose rows are the reference slots
+  if (nq != nref || !qpos_wrapped) return DSPH_ERR_ARG;
+  if (tile_cap < 1 || tile_cap > DSPH_TILE_MAX_CAP || slot_capacity < 0) return DSPH_ERR_ARG;
+  if (nq > 0 && (!slot || !tile_runs || !tile_n)) return DSPH_ERR_ARG;
+  if (width >= 1 && slot_capacity < ((nq + 63) / 64) * 64 * (int64_t)width) return DSPH_ERR_CAPACITY;
+  FillTiles t{tile_runs, tile_n, slot, tile_cap};
+  return fill_fixed(domain, grid, cell_start, sorted_pos, sorted_h, nref, qpos, qh, nq, qpos_wrapped, h_uniform, mode,
+                    width, counts, slice_off, nbr, capacity, nbr16, delta_overflow, row_overflow, nullptr, &t,
+                    stream);
 }
 
 extern "C" size_t dsph_sell_workspace(int64_t nq) {

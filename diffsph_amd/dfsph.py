@@ -86,6 +86,19 @@ class StepStats:
     num_pairs: int = 0
 
 
+@dataclass
+class JudgedGuess:
+    """An initial divergence pressure under review ('div_init' may return
+    one): the solve runs `check_after` iterations from `p`; when it has not
+    converged by then, ``verdict(solver, error)`` (error: the solver's
+    error metric of the last iteration, NaN when the solve stopped on one)
+    returns None to continue the same solve, or a pressure to solve from
+    afresh instead."""
+    p: torch.Tensor
+    check_after: int
+    verdict: object
+
+
 class DFSPHSolver:
     """Owns the particle state (cell-ordered, on the GPU) and steps it."""
 
@@ -167,8 +180,15 @@ class DFSPHSolver:
         self.grid = grid
         fused = cfg.compact_index and cfg.compact_limit >= 32767
         w = self._nbr_width
+        tiles = None
+        if cfg.tile_cap > 0 and w > 0:
+            # the fixed-width list's tiles, built by its fill; the 16-bit
+            # deltas are left out there (only a workgroup without a usable
+            # tile would read them, and it reads the int32 entries instead)
+            tiles = (cfg.tile_cap, *self._tile_buffers(((self.n + 63) // 64) * 64 * w))
         self.nl = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True,
-                         delta16=self._d16 if fused else None, layout="fixed" if w > 0 else "sell", width=w)
+                         delta16=self._d16 if fused and tiles is None else None, layout="fixed" if w > 0 else "sell",
+                         width=w, tiles=tiles)
         if w > 0 and self.nl.width == 0:
             # a row outgrew the width: this step used the two-pass list
             self._nbr_width = (int(self.nl.counts.max()) + 16) | 1
@@ -179,16 +199,20 @@ class DFSPHSolver:
         if cfg.compact_index and not fused:
             buf, flag = self._d16.get(need, self.pos.device)
             self.nl.attach_compact16(buf, flag, cfg.compact_limit)
-        if cfg.tile_cap > 0:
-            nt = (self.n + 255) // 256
-            t = self._tile_bufs
-            if t is None or t[0].numel() < need:
-                dev = self.pos.device
-                t = self._tile_bufs = (torch.empty(need + need // 8, dtype=torch.int16, device=dev),
-                                       torch.empty(nt * TILE_RUN_STRIDE, dtype=torch.int32, device=dev),
-                                       torch.empty(nt, dtype=torch.int32, device=dev))
-            self.nl.attach_tiles(cfg.tile_cap, *t)
+        if cfg.tile_cap > 0 and self.nl.tiles is None:
+            self.nl.attach_tiles(cfg.tile_cap, *self._tile_buffers(need))
         return self.nl
+
+    def _tile_buffers(self, need: int):
+        """(slot, tile_runs, tile_n) with room for `need` entries."""
+        nt = (self.n + 255) // 256
+        t = self._tile_bufs
+        if t is None or t[0].numel() < need:
+            dev = self.pos.device
+            t = self._tile_bufs = (torch.empty(need + need // 8, dtype=torch.int16, device=dev),
+                                   torch.empty(nt * TILE_RUN_STRIDE, dtype=torch.int32, device=dev),
+                                   torch.empty(nt, dtype=torch.int32, device=dev))
+        return t
 
     def tile_stats(self) -> dict:
         """Neighbor-tile occupancy of the current list (synchronises): the
@@ -279,6 +303,29 @@ class DFSPHSolver:
         nat = ((p - p_last) * self.alpha_div).abs().mean() / (cfg.omega_div * cfg.rho0)
         return float(err.value), float(res.abs().mean()) / cfg.rho0, float(nat)
 
+    def probe(self, p0: torch.Tensor, iterations: int):
+        """(error, p): `iterations` Jacobi iterations of the divergence solve
+        from p0 on a copy (the solver state is unchanged), p the pressure
+        after them and error the solver's error metric of the last one (one
+        host synchronisation)."""
+        cfg = self.cfg
+        prm = _lib.DfsphParams()
+        prm.dt, prm.rho0, prm.omega = cfg.dt, cfg.rho0, cfg.omega_div
+        prm.error_threshold = cfg.threshold_div / cfg.rho0
+        prm.min_iters, prm.max_iters = iterations, iterations
+        prm.clamp_pressure = 1 if cfg.clamp_pressure else 0
+        prm.source_density = 0
+        L = lib()
+        ws = WORKSPACE.get(L.dsph_dfsph_workspace(self.n), p0.device)
+        p = p0.clone()
+        acc = torch.empty(self.n, 2, device=p.device)
+        err = C.c_double(0.0)
+        check(L.dsph_dfsph_iterate(C.byref(self.nl.pairs()), C.byref(prm), ptr(self.kgrad), ptr(self.rho),
+                                   ptr(self.vol), ptr(self.alpha_div), ptr(self.div_src), ptr(p), ptr(acc), None,
+                                   iterations, C.byref(err), ptr(ws), ws.numel(), _lib.stream_handle(p.device)),
+              "dfsph_iterate")
+        return float(err.value), p
+
     def first_iteration_error(self, p0: torch.Tensor) -> float:
         """Solver error metric after one divergence iteration from p0."""
         return self.probe_guess(p0)[0]
@@ -332,12 +379,37 @@ class DFSPHSolver:
         if vp is None:
             self.visc.zero_()
         # divergence solve (advection = 0: v* = v)
+        judged = None
         if "div_init" in self.hooks:
-            p_div = self.hooks["div_init"](self).contiguous()
+            p_div = self.hooks["div_init"](self)
+            if isinstance(p_div, JudgedGuess):
+                judged, p_div = p_div, p_div.p
+            p_div = p_div.contiguous()
         else:
             p_div = (cfg.warm_start * self.p_div) if self.p_div is not None else torch.zeros_like(self.rho)
         split = self.hooks.get("div_split")
-        if split is None:
+        if judged is not None and split is None and 0 < judged.check_after < cfg.max_iters:
+            # the guess's first iterations are those of the solve itself (two
+            # phases, as below): kept, the solve goes on; rejected, it starts
+            # again from the verdict's pressure
+            k1 = judged.check_after
+            s = self._solve(pairs, False, p_div, self.a_div, None, max_iters=k1, check_every=k1)
+            iters = s.iterations
+            if not s.converged:
+                # k1 iterations without convergence, or fewer when the error
+                # is not a number (the stopping rule ends the solve then)
+                restart = judged.verdict(self, s.final_error)
+                if restart is None:
+                    prev = self._last_iters.get(False, 0)
+                    self._last_iters[False] = max(prev - k1, 0)
+                    s = self._solve(pairs, False, p_div, self.a_div, None, min_iters=max(cfg.min_iters - k1, 1),
+                                    max_iters=cfg.max_iters - k1)
+                    iters += s.iterations
+                else:
+                    p_div = restart.contiguous()
+                    s = self._solve(pairs, False, p_div, self.a_div, None)
+                    iters = s.iterations
+        elif split is None:
             s = self._solve(pairs, False, p_div, self.a_div, None)
             iters = s.iterations
         else:

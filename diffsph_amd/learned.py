@@ -32,6 +32,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr
+from .dfsph import JudgedGuess
 from .neighborhood import WORKSPACE
 
 NFEAT = 8       # used feature columns
@@ -234,11 +235,13 @@ class WarmStartNet:
         """Use the network as the divergence-solve warm start of `solver`.
 
         Plain Jacobi is the fallback and the judge.  On judged steps
-        `probe_iters` Jacobi iterations are run (on copies) from the
-        network's guess and from the solver's own warm start
-        (warm_start * p_prev); the network's guess is kept unless its error
-        metric after those iterations exceeds `tolerance` times the plain
-        one (floored at the convergence threshold) or is not finite.  The
+        the solve's first `probe_iters` Jacobi iterations run from the
+        network's guess, and as many from the solver's own warm start
+        (warm_start * p_prev) on a copy; the network's guess is kept (the
+        solve goes on) unless its error metric after those iterations
+        exceeds `tolerance` times the plain one (floored at the convergence
+        threshold) or is not finite (the solve starts again from the plain
+        guess).  A guess that converges within them is kept.  The
         judge is deliberately lenient, a guard against a broken network
         rather than a per-step optimiser: the network's gain is over the
         trajectory (it keeps the low-frequency pressure that 0.75 p_prev
@@ -258,14 +261,28 @@ class WarmStartNet:
             plain = sv.cfg.warm_start * sv.p_div
             p_net = self.predict_pressure(sv)
             judged = state["force"] or (judge_every > 0 and state["k"] % judge_every == 0)
+            state["k"] += 1
             if judged:
-                e_net = sv.probe_guess(p_net, probe_iters)[0]
-                e_plain = sv.probe_guess(plain, probe_iters)[0]
-                state["use_net"] = math.isfinite(e_net) and e_net <= tolerance * max(e_plain, thr)
-                self.judge.err_net, self.judge.err_plain = e_net, e_plain
                 self.judge.judged += 1
                 state["force"] = False
-            state["k"] += 1
+                state["use_net"] = True
+                self.judge.err_net, self.judge.err_plain = 0.0, float("nan")
+
+                def verdict(sv, e_net):
+                    # the network's error is the solve's own after probe_iters
+                    # iterations; the plain guess's is probed on a copy
+                    e_plain = sv.probe(plain, probe_iters)[0]
+                    ok = math.isfinite(e_net) and e_net <= tolerance * max(e_plain, thr)
+                    self.judge.err_net, self.judge.err_plain = e_net, e_plain
+                    state["use_net"] = ok
+                    if ok:
+                        return None
+                    self.judge.used_net -= 1
+                    self.judge.used_plain += 1
+                    return plain
+
+                self.judge.used_net += 1
+                return JudgedGuess(p_net, probe_iters, verdict)
             if state["use_net"]:
                 self.judge.used_net += 1
                 return p_net

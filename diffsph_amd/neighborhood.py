@@ -235,7 +235,8 @@ class Delta16Buffers:
 
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
            qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None,
-           width: int = 67, deferred: torch.Tensor | None = None, remap: bool = False) -> NeighborList:
+           width: int = 67, deferred: torch.Tensor | None = None, remap: bool = False,
+           tiles: tuple | None = None) -> NeighborList:
     """Neighbor list of the query set `qpos` against the sorted set `grid`.
 
     Counts, scans (host reads of the pair and entry totals, as the
@@ -258,7 +259,13 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
     ``remap``: the entries are the reference particles' indices in the
     grid's input order (``grid.sort_index[j]``) instead of sorted slots, as
     :func:`dsph_remap_neighbors` would make them ("fixed" layout: in the
-    same pass; other layouts: a pass after the fill)."""
+    same pass; other layouts: a pass after the fill).
+
+    ``tiles`` = (cap, slot, tile_runs, tile_n), buffers as
+    :meth:`NeighborList.attach_tiles` takes them ("fixed" layout, square
+    list): the list's neighbor tiles are built with the fill (the run
+    tables first, each entry's slot in the fill itself) and attached.  A
+    list that falls back to "sell" comes back without tiles."""
     if delta16 is not None and not qpos_wrapped:
         raise DsphError("search: 16-bit neighbor deltas need the grid's own sorted positions as queries "
                         "(qpos_wrapped=True)")
@@ -276,9 +283,11 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
         raise DsphError("search: deferred overflow checks need the fixed-width layout")
     if remap and delta16 is not None:
         raise DsphError("search: remapped entries have no 16-bit deltas")
+    if tiles is not None and (remap or not qpos_wrapped or layout != "fixed"):
+        raise DsphError("search: tiles are built for a square fixed-width list (qpos_wrapped, no remap)")
     if layout == "fixed":
         nl = _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, int(width), st, deferred,
-                           grid.sort_index if remap else None)
+                           grid.sort_index if remap else None, tiles)
         if nl is not None:
             return nl
         layout = "sell"
@@ -321,7 +330,8 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
                         grid, m, slice_off, entries, buf16, flag16)
 
 
-def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st, deferred=None, remap=None):
+def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, width, st, deferred=None, remap=None,
+                  tiles=None):
     nq = qpos.shape[0]
     dev = qpos.device
     nsl = (nq + 63) // 64
@@ -332,7 +342,12 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
     buf16 = flag16 = None
     if delta16 is not None:
         buf16, flag16 = delta16.get(cap, dev)
-    if remap is not None:
+    if tiles is not None:
+        tcap, slot, truns, tn = tiles
+        check(L.dsph_neighbor_fill_fixed_tiles(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(buf16),
+                                               ptr(flag16), int(tcap), ptr(slot), slot.numel(), ptr(truns), ptr(tn),
+                                               ptr(info), st), "neighbor_fill_fixed_tiles")
+    elif remap is not None:
         check(L.dsph_neighbor_fill_fixed_map(*args, width, ptr(counts), ptr(slice_off), ptr(nbr), cap, ptr(remap),
                                              ptr(info), st), "neighbor_fill_fixed_map")
     else:
@@ -352,7 +367,10 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
         if ovf:
             return None
     wq = qpos if qpos_wrapped else wrap_positions(qpos, grid.domain)
-    return NeighborList(counts, offsets, nbr, npairs, wq, qh, grid, m, slice_off, cap, buf16, flag16, width)
+    nl = NeighborList(counts, offsets, nbr, npairs, wq, qh, grid, m, slice_off, cap, buf16, flag16, width)
+    if tiles is not None:
+        nl.tiles = (tiles[1], tiles[2], tiles[3], int(tiles[0]))
+    return nl
 
 
 def wrap_positions(pos: torch.Tensor, domain: _lib.Domain) -> torch.Tensor:

@@ -259,6 +259,27 @@ int dsph_neighbor_fill_fixed_map(const dsph_domain* domain, const dsph_grid* gri
                                  int32_t* nbr, int64_t capacity, const int32_t* map,
                                  int32_t* row_overflow, void* stream);
 
+/* dsph_neighbor_fill_fixed that also builds the list's neighbor tiles (the
+ * tile_* fields of dsph_pairs; the list must be square: nq == nref and
+ * qpos_wrapped): the run tables tile_runs / tile_n first (from the cells
+ * alone), then, in the fill itself, each entry's 16-bit tile slot beside
+ * it (slot: slot_capacity >= ceil(nq / 64) * 64 * width entries).  The
+ * same tables and slots as dsph_neighbor_tiles(cap = tile_cap) makes of
+ * the filled list, without its pass over the entries; a tile of more than
+ * 4095 particles is left unusable (tile_n = -1).  nbr16 (optional) as in
+ * dsph_neighbor_fill_fixed.  No counterpart in the reference. */
+int dsph_neighbor_fill_fixed_tiles(const dsph_domain* domain, const dsph_grid* grid,
+                                   const int32_t* cell_start, const float* sorted_pos,
+                                   const float* sorted_h, int64_t nref,
+                                   const float* qpos, const float* qh, int64_t nq,
+                                   int32_t qpos_wrapped, float h_uniform, int32_t mode,
+                                   int32_t width, int32_t* counts, int32_t* slice_off,
+                                   int32_t* nbr, int64_t capacity, int16_t* nbr16,
+                                   int32_t* delta_overflow, int32_t tile_cap,
+                                   uint16_t* slot, int64_t slot_capacity,
+                                   int32_t* tile_runs, int32_t* tile_n,
+                                   int32_t* row_overflow, void* stream);
+
 /* dsph_neighbor_fill that also writes the 16-bit modular deltas of
  * dsph_neighbor_compact16 (d = j - i reduced mod nref; |d| <= 32767, else
  * *overflow is set) in the same pass, for a list whose rows are the sorted

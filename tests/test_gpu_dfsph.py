@@ -152,9 +152,10 @@ def test_dfsph_solver_limits_and_nan_guard():
 def test_compact16_deltas_decode_and_overflow():
     """The 16-bit modular deltas written by the fill pass decode to the
     int32 list entry for entry and equal the separate compaction pass; a
-    limit below the largest delta raises the device flag."""
+    limit below the largest delta raises the device flag (a list without
+    neighbor tiles: with them the fill writes tile slots instead)."""
     d, dt = _setup(64, 2)
-    s = _solver(d, dt)
+    s = _solver(d, dt, tile_cap=0)
     s.neighbor_step()
     nl = s.nl
     assert nl.nbr16 is not None and int(nl.nbr16_overflow.item()) == 0

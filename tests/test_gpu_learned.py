@@ -257,3 +257,40 @@ def test_second_ring_features_match_two_first_ring_passes(tile_cap):
     np.testing.assert_array_equal(fused.cpu().numpy(), comp.cpu().numpy())
     np.testing.assert_array_equal(raw[:, 8:11].cpu().numpy(), second[:, [1, 3, 5]].cpu().numpy())
     assert float(fused[:, 8:11].abs().max()) > 0
+
+
+@pytest.mark.parametrize("restart", [False, True])
+def test_judged_guess_is_the_solve_it_chooses(restart):
+    """A divergence guess under review (div_init returning a JudgedGuess):
+    kept, the step is the solve from that guess; rejected, the solve from
+    the verdict's pressure -- fields bit for bit, iteration counts equal.
+    The verdict sees the error metric a probe of the guess reports."""
+    from diffsph_amd.dfsph import JudgedGuess
+    d = sampling.dfsph_random_flow(96, seed=6, jitter_width=0.1)
+    dt = 3.0 * d["dx"] / 10.0
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+    mk = lambda: DFSPHSolver(T(d["pos"]), T(d["vel"]), T(d["mass"]), T(d["area"]),  # noqa: E731
+                             DFSPHConfig(dt=dt, support=d["h"]))
+    a, b = mk(), mk()
+    for s in (a, b):
+        s.step()
+        s.step()
+    guess = lambda sv: 0.25 * sv.p_div  # noqa: E731
+    other = lambda sv: 0.9 * sv.p_div  # noqa: E731
+    seen = []
+
+    def verdict(sv, err):
+        seen.append((err, sv.probe(guess(sv), 3)[0]))
+        return other(sv) if restart else None
+
+    a.hooks["div_init"] = other if restart else guess
+    b.hooks["div_init"] = lambda sv: JudgedGuess(guess(sv), 3, verdict)
+    print(f"case restart={restart}", flush=True)
+    for k in range(2):
+        sa, sb = a.step(), b.step()
+        print(f"step {k}: div iters {sa.div_iters} / {sb.div_iters}; errors {seen[-1]}", flush=True)
+        assert len(seen) == k + 1 and seen[-1][0] == seen[-1][1]
+        assert (sa.div_iters, sa.den_iters) == (sb.div_iters, sb.den_iters)
+        for name in ("pos", "vel", "rho", "p_div", "p_den", "a_div"):
+            assert torch.equal(getattr(a, name), getattr(b, name)), name

@@ -135,3 +135,82 @@ def test_tiled_feature_pass_bit_identical(nside, seed):
     xb = net.features(b, normalized=False).cpu().numpy()
     np.testing.assert_array_equal(xa, xb)
     assert np.abs(xa[:, 1]).max() > 0  # the neighbor sums are not empty
+
+
+def _fused_vs_two_pass(pos, domain, h, cap, d16, width=67):
+    """The fixed-width list with the tiles built in its fill vs the same
+    list filled first and tiled by dsph_neighbor_tiles afterwards."""
+    from diffsph_amd.neighborhood import CellGrid, Delta16Buffers, search
+    dev = pos.device
+    grid = CellGrid.build(pos, None, domain, h, h_uniform=h)
+    n = pos.shape[0]
+    nt = (n + 255) // 256
+    ent = ((n + 63) // 64) * 64 * width
+    mk = lambda: (torch.full((ent,), -7, dtype=torch.int16, device=dev),  # noqa: E731
+                  torch.full((nt * TILE_RUN_STRIDE,), -7, dtype=torch.int32, device=dev),
+                  torch.full((nt,), -7, dtype=torch.int32, device=dev))
+    ta, tb = mk(), mk()
+    da, db = (Delta16Buffers(), Delta16Buffers()) if d16 else (None, None)
+    a = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True, layout="fixed", width=width,
+               delta16=da, tiles=(cap, *ta))
+    b = search(grid, grid.sorted_pos, None, "scatter", qpos_wrapped=True, layout="fixed", width=width, delta16=db)
+    assert a.width == width and b.width == width and a.tiles is not None
+    b.attach_tiles(cap, *tb)
+    return a, b, ta, tb
+
+
+def _entries(nl):
+    """Flat positions of every row's entries in the sliced layout."""
+    counts = nl.counts.cpu().numpy().astype(np.int64)
+    so = nl.slice_off.cpu().numpy().astype(np.int64)
+    i = np.repeat(np.arange(counts.size), counts)
+    c = np.arange(i.size) - np.repeat(np.cumsum(counts) - counts, counts)
+    return i, so[i // 64] + 64 * c + i % 64
+
+
+# 96^2: workgroups span two rows of cells and wrap the seam; 272^2; a
+# 5 x 5 periodic grid (every candidate range takes the seam path); cap 0:
+# the median tile size, so that some tiles do not fit and get no slots;
+# cap 64: none fit
+@pytest.mark.parametrize("nside,seed,cap,d16", [(96, 0, 4096, True), (272, 3, 1152, False), (272, 3, 0, True),
+                                                 (24, 5, 4096, False), (96, 2, 64, False)])
+def test_fill_built_tiles_equal_two_pass_tiles(nside, seed, cap, d16):
+    from diffsph_amd import _lib
+    d, _ = _setup(nside, seed)
+    print(f"case nside={nside} seed={seed} cap={cap} d16={d16}", flush=True)
+    dev = torch.device("cuda:0")
+    pos = torch.from_numpy(d["pos"]).to(dev)
+    h = float(d["h"])
+    if nside == 24:
+        h = 0.34  # a 5 x 5 cell grid over [-1, 1]^2
+    domain = _lib.make_domain((-1.0, -1.0), (1.0, 1.0), (True, True))
+    median = cap == 0
+    if median:
+        cap = int(np.median(_fused_vs_two_pass(pos, domain, h, 4096, False)[2][2].cpu().numpy()))
+        print(f"median tile {cap}", flush=True)
+    a, b, ta, tb = _fused_vs_two_pass(pos, domain, h, cap, d16)
+    np.testing.assert_array_equal(a.counts.cpu().numpy(), b.counts.cpu().numpy())
+    i, k = _entries(a)
+    np.testing.assert_array_equal(a.nbr.cpu().numpy()[k], b.nbr.cpu().numpy()[k])
+    if d16:
+        np.testing.assert_array_equal(a.nbr16.cpu().numpy()[k], b.nbr16.cpu().numpy()[k])
+        assert int(a.nbr16_overflow.item()) == int(b.nbr16_overflow.item())
+    tna, tnb = ta[2].cpu().numpy(), tb[2].cpu().numpy()
+    ra = ta[1].cpu().numpy().reshape(-1, TILE_RUN_STRIDE)
+    rb = tb[1].cpu().numpy().reshape(-1, TILE_RUN_STRIDE)
+    np.testing.assert_array_equal(tna, tnb)
+    for t in range(tna.size):
+        if tna[t] >= 0:
+            R = ra[t, 0]
+            np.testing.assert_array_equal(ra[t, :1 + 2 * R], rb[t, :1 + 2 * R])
+    usable = (tna >= 0) & (tna <= cap)
+    if cap == 64:
+        assert not usable.any()
+    if median:
+        assert 0 < usable.mean() < 1
+    if cap == 4096:
+        assert usable.all()
+    sel = usable[i // 256]
+    sa = ta[0].cpu().numpy().view(np.uint16)[k[sel]]
+    sb = tb[0].cpu().numpy().view(np.uint16)[k[sel]]
+    np.testing.assert_array_equal(sa, sb)
