@@ -80,6 +80,7 @@ class WarmStartNet:
         self.rings = rings
         self.nfeat = NFEAT + 3 * (rings - 1)
         self.device = torch.device(device)
+        self._raw = None  # (zeros, ones) [NIN]: the identity normalisation
         self.nparam = L.dsph_mlp_num_params()
         assert L.dsph_mlp_num_inputs() == NIN
         g = torch.Generator().manual_seed(seed)
@@ -138,14 +139,15 @@ class WarmStartNet:
 
     def features(self, solver, normalized: bool = True) -> torch.Tensor:
         n = solver.n
-        zeros = torch.zeros(n, device=self.device)
-        pdp = solver.p_div if solver.p_div is not None else zeros
-        pnp = solver.p_den if solver.p_den is not None else zeros
+        pdp = solver.p_div if solver.p_div is not None else torch.zeros(n, device=self.device)
+        pnp = solver.p_den if solver.p_den is not None else torch.zeros(n, device=self.device)
+        if self._raw is None:
+            self._raw = (torch.zeros(NIN, device=self.device), torch.ones(NIN, device=self.device))
+        raw0, raw1 = self._raw
         if self.rings == 1:
-            mean = self.mean if normalized else torch.zeros(NIN, device=self.device)
-            istd = self.istd if normalized else torch.ones(NIN, device=self.device)
+            mean = self.mean if normalized else raw0
+            istd = self.istd if normalized else raw1
             return self._feature_pass(solver, pdp, solver.div_src, pnp, mean, istd)
-        raw0, raw1 = torch.zeros(NIN, device=self.device), torch.ones(NIN, device=self.device)
         mean = self.mean if normalized else raw0
         istd = self.istd if normalized else raw1
         pw, rec = _pair_w(solver), _records(solver)
@@ -169,7 +171,7 @@ class WarmStartNet:
     def predict_pressure(self, solver) -> torch.Tensor:
         X = self.features(solver)
         y = self.forward(X)
-        p = solver.p_div + (y * self.t_std + self.t_mean)
+        p = solver.p_div + y.mul_(self.t_std).add_(self.t_mean)
         return p.clamp_min_(0.0) if solver.cfg.clamp_pressure else p
 
     # -- training -----------------------------------------------------------
@@ -258,10 +260,16 @@ class WarmStartNet:
         def div_init(sv):
             if sv.p_div is None:
                 return torch.zeros_like(sv.rho)
-            plain = sv.cfg.warm_start * sv.p_div
-            p_net = self.predict_pressure(sv)
             judged = state["force"] or (judge_every > 0 and state["k"] % judge_every == 0)
             state["k"] += 1
+
+            def plain():
+                return sv.cfg.warm_start * sv.p_div
+
+            if not judged and not state["use_net"]:
+                self.judge.used_plain += 1
+                return plain()
+            p_net = self.predict_pressure(sv)
             if judged:
                 self.judge.judged += 1
                 state["force"] = False
@@ -271,7 +279,8 @@ class WarmStartNet:
                 def verdict(sv, e_net):
                     # the network's error is the solve's own after probe_iters
                     # iterations; the plain guess's is probed on a copy
-                    e_plain = sv.probe(plain, probe_iters)[0]
+                    q = plain()
+                    e_plain = sv.probe(q, probe_iters)[0]
                     ok = math.isfinite(e_net) and e_net <= tolerance * max(e_plain, thr)
                     self.judge.err_net, self.judge.err_plain = e_net, e_plain
                     state["use_net"] = ok
@@ -279,15 +288,12 @@ class WarmStartNet:
                         return None
                     self.judge.used_net -= 1
                     self.judge.used_plain += 1
-                    return plain
+                    return q
 
                 self.judge.used_net += 1
                 return JudgedGuess(p_net, probe_iters, verdict)
-            if state["use_net"]:
-                self.judge.used_net += 1
-                return p_net
-            self.judge.used_plain += 1
-            return plain
+            self.judge.used_net += 1
+            return p_net
 
         def after_div(sv):
             it = sv._last_iters.get(False, 0)
