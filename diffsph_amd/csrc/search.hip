@@ -56,12 +56,37 @@ __global__ void __launch_bounds__(kBlock) k_cell_scatter(int n, const int32_t* _
   sort_index[cell_start[keys[i]] + slot[i]] = i;
 }
 
-// Insertion sort of each cell's slice (cells hold ~16 particles).
+// Each cell's slice sorted by original index (cells hold ~16 particles),
+// one thread per cell.  A slice of up to kSortLds entries is loaded into
+// the thread's LDS column with its loads in flight together, and each entry
+// goes to its rank (the count of smaller entries: the indices are
+// distinct); a longer one is insertion-sorted in place.
+constexpr int kSortLds = 32;
 __global__ void __launch_bounds__(kBlock) k_cell_sort(int ncell, const int32_t* __restrict__ cell_start,
                                                       int32_t* __restrict__ sort_index) {
+  __shared__ int32_t col[kSortLds][kBlock];
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   int b = cell_start[c], e = cell_start[c + 1];
+  const int m = e - b;
+  const int t = threadIdx.x;
+  if (m <= kSortLds) {
+    for (int a0 = 0; a0 < m; a0 += 8) {
+      int v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = a0 + u < m ? sort_index[b + a0 + u] : 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (a0 + u < m) col[a0 + u][t] = v[u];
+    }
+    for (int a = 0; a < m; ++a) {
+      const int v = col[a][t];
+      int r = 0;
+      for (int u = 0; u < m; ++u) r += col[u][t] < v ? 1 : 0;
+      sort_index[b + r] = v;
+    }
+    return;
+  }
   for (int a = b + 1; a < e; ++a) {
     int v = sort_index[a];
     int k = a - 1;

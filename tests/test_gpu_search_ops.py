@@ -372,3 +372,26 @@ def test_search_1m_matches_oracle_bitexact(orc):
         e = idx[valid]
         assert np.array_equal((i_ + d16[e]) % n, full[e])
         assert np.abs(d16[e]).max() > 1000
+
+
+# cells of 16 (LDS rank sort), 32 +- 4 (both sort paths in one grid) and 64
+# particles (insertion sort only)
+@pytest.mark.parametrize("nside,hscale", [(96, 1.0), (96, 1.4), (96, 2.0)])
+def test_grid_cell_order_matches_oracle(orc, nside, hscale):
+    """The per-cell sort (entries ranked in LDS up to 32 per cell, longer
+    cells insertion-sorted in place) gives the oracle's sort index: within
+    each cell, increasing original index."""
+    d, per = _case(nside, 9, 0.5)
+    h = float(d["h"]) * hscale
+    print(f"case nside={nside} h={h}", flush=True)
+    dom_o = orc.make_domain((-1, -1), (1, 1), per)
+    g_o = orc.grid_build(dom_o, d["pos"], h)
+    g = CellGrid.build(torch.from_numpy(d["pos"]).to("cuda:0"), None, _lib.make_domain((-1, -1), (1, 1), per), h,
+                       h_uniform=h)
+    cs = g.cell_start.cpu().numpy()
+    np.testing.assert_array_equal(cs, g_o.cell_start)
+    m = np.diff(cs)
+    print(f"cell sizes: max {m.max()}, cells over 32: {(m > 32).sum()}", flush=True)
+    if hscale == 1.4:
+        assert (m > 32).any() and (m <= 32).any()
+    np.testing.assert_array_equal(g.sort_index.cpu().numpy(), g_o.sort_index)
