@@ -242,6 +242,7 @@ SIGNATURES = {
     "dsph_scan_workspace": (SZ, [I64]),
     "dsph_scan_exclusive": (C.c_int, [V, I64, V, P(I64), V, SZ, V]),
     "dsph_gather_f32": (C.c_int, [V, V, I64, I32, V, V]),
+    "dsph_gather_set": (C.c_int, [V, I64, I32, V, V, V, V]),
     "dsph_scatter_f32": (C.c_int, [V, V, I64, I32, V, V]),
     "dsph_pairs_export": (C.c_int, [P(Pairs), V, V, V, V, V, V, V, V, V, V, V, V, SZ, V]),
     "dsph_pairs_export_workspace": (SZ, [I64]),

@@ -283,6 +283,32 @@ __global__ void __launch_bounds__(kBlock) k_gather(const float* __restrict__ src
   int c = (int)(t - p * w);
   dst[t] = src[(int64_t)idx[p] * w + c];
 }
+// Several per-particle arrays permuted by one index in one pass:
+// dst_f[p] = src_f[idx[p]], words[f] 4-byte words per element (1, 2 or 4).
+constexpr int kGatherFields = 8;
+struct GatherSet {
+  const uint32_t* src[kGatherFields];
+  uint32_t* dst[kGatherFields];
+  int words[kGatherFields];
+  int nf;
+};
+
+__global__ void __launch_bounds__(kBlock) k_gather_set(GatherSet g, const int32_t* __restrict__ idx, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t j = idx[p];
+  for (int f = 0; f < g.nf; ++f) {
+    const int w = g.words[f];
+    if (w == 1) {
+      g.dst[f][p] = g.src[f][j];
+    } else if (w == 2) {
+      reinterpret_cast<uint2*>(g.dst[f])[p] = reinterpret_cast<const uint2*>(g.src[f])[j];
+    } else {
+      reinterpret_cast<uint4*>(g.dst[f])[p] = reinterpret_cast<const uint4*>(g.src[f])[j];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_scatter(const float* __restrict__ src, const int32_t* __restrict__ idx,
                                                     int64_t n, int w, float* __restrict__ dst) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -472,6 +498,27 @@ extern "C" int dsph_gather_f32(const float* src, const int32_t* index, int64_t n
   if (n == 0) return DSPH_OK;
   if (!src || !index || !dst) return DSPH_ERR_ARG;
   k_gather<<<grid_for(n * width), kBlock, 0, as_stream(stream)>>>(src, index, n, width, dst);
+  return launch_status();
+}
+
+extern "C" int dsph_gather_set(const int32_t* index, int64_t n, int32_t nfields, const void* const* src,
+                               void* const* dst, const int32_t* words, void* stream) {
+  if (n < 0 || n >= INT32_MAX || nfields < 0 || nfields > kGatherFields) return DSPH_ERR_ARG;
+  if (n == 0 || nfields == 0) return DSPH_OK;
+  if (!index || !src || !dst || !words) return DSPH_ERR_ARG;
+  GatherSet g{};
+  g.nf = nfields;
+  for (int f = 0; f < nfields; ++f) {
+    const int w = words[f];
+    const size_t align = (size_t)w * 4;
+    if ((w != 1 && w != 2 && w != 4) || !src[f] || !dst[f]) return DSPH_ERR_ARG;
+    if (reinterpret_cast<uintptr_t>(src[f]) % align || reinterpret_cast<uintptr_t>(dst[f]) % align)
+      return DSPH_ERR_ARG;
+    g.src[f] = static_cast<const uint32_t*>(src[f]);
+    g.dst[f] = static_cast<uint32_t*>(dst[f]);
+    g.words[f] = w;
+  }
+  k_gather_set<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(g, index, (int)n);
   return launch_status();
 }
 

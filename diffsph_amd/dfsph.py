@@ -166,17 +166,27 @@ class DFSPHSolver:
             out = t[idx.long()]
         return out
 
+    def _permute_state(self, idx):
+        """The per-particle state into the new cell order, one pass over all
+        arrays (dsph_gather_set)."""
+        names = [k for k in ("pos", "vel", "mass", "areas", "uid", "p_div", "p_den") if getattr(self, k) is not None]
+        src = [getattr(self, k) for k in names]
+        dst = [torch.empty_like(t) for t in src]
+        for t in src:
+            if not t.is_contiguous() or t.shape[0] != self.n:
+                raise DsphError("state arrays must be contiguous with one row per particle")
+        words = [t.element_size() * (t.numel() // self.n) // 4 for t in src]
+        k = len(src)
+        check(lib().dsph_gather_set(ptr(idx), self.n, k, (C.c_void_p * k)(*[t.data_ptr() for t in src]),
+                                    (C.c_void_p * k)(*[t.data_ptr() for t in dst]), (C.c_int32 * k)(*words),
+                                    _lib.stream_handle(idx.device)), "gather_set")
+        for name, t in zip(names, dst):
+            setattr(self, name, t)
+
     def neighbor_step(self):
         cfg = self.cfg
         grid = CellGrid.build(self.pos, None, self.domain, cfg.support, h_uniform=cfg.support)
-        si = grid.sort_index
-        self.pos = self._permute(self.pos, si)
-        self.vel = self._permute(self.vel, si)
-        self.mass = self._permute(self.mass, si)
-        self.areas = self._permute(self.areas, si)
-        self.uid = self._permute(self.uid, si)
-        self.p_div = self._permute(self.p_div, si)
-        self.p_den = self._permute(self.p_den, si)
+        self._permute_state(grid.sort_index)
         self.grid = grid
         fused = cfg.compact_index and cfg.compact_limit >= 32767
         w = self._nbr_width

@@ -318,6 +318,13 @@ int dsph_scan_exclusive(const int32_t* in, int64_t n, int32_t* out,
  * order: dst[p] = src[index[p]] for `width` floats per particle. */
 int dsph_gather_f32(const float* src, const int32_t* index, int64_t n,
                     int32_t width, float* dst, void* stream);
+/* Up to 8 per-particle arrays permuted by one index in one pass:
+ * dst[f][p] = src[f][index[p]], words[f] 4-byte words per element (1, 2 or
+ * 4; the arrays aligned to the element size).  src, dst and words are host
+ * arrays of nfields entries.  No counterpart in the reference. */
+int dsph_gather_set(const int32_t* index, int64_t n, int32_t nfields,
+                    const void* const* src, void* const* dst,
+                    const int32_t* words, void* stream);
 /* dst[index[p]] = src[p] */
 int dsph_scatter_f32(const float* src, const int32_t* index, int64_t n,
                      int32_t width, float* dst, void* stream);

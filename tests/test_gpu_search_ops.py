@@ -395,3 +395,31 @@ def test_grid_cell_order_matches_oracle(orc, nside, hscale):
     if hscale == 1.4:
         assert (m > 32).any() and (m <= 32).any()
     np.testing.assert_array_equal(g.sort_index.cpu().numpy(), g_o.sort_index)
+
+
+def test_gather_set_permutes_every_array():
+    """dsph_gather_set: arrays of 1, 2 and 4 words per element (float32,
+    float32 pairs, int64, float32 quads) permuted by one index in one pass
+    equal torch's indexing; misaligned or oversized requests are refused."""
+    import ctypes as C
+    from diffsph_amd._lib import DsphError, check, lib, ptr, stream_handle
+    dev = torch.device("cuda:0")
+    n = 100003
+    g = torch.Generator().manual_seed(4)
+    idx = torch.randperm(n, generator=g).to(torch.int32).to(dev)
+    src = [torch.randn(n, generator=g).to(dev), torch.randn(n, 2, generator=g).to(dev),
+           torch.randint(-2**40, 2**40, (n,), generator=g, dtype=torch.int64).to(dev),
+           torch.randn(n, 4, generator=g).to(dev)]
+    dst = [torch.empty_like(t) for t in src]
+    words = [1, 2, 2, 4]
+    k = len(src)
+    mk = lambda ts: (C.c_void_p * k)(*[t.data_ptr() for t in ts])  # noqa: E731
+    check(lib().dsph_gather_set(ptr(idx), n, k, mk(src), mk(dst), (C.c_int32 * k)(*words), stream_handle(dev)),
+          "gather_set")
+    for s, d in zip(src, dst):
+        assert torch.equal(d, s[idx.long()])
+    bad = [src[1][:, 1:]]  # 4-byte aligned, 2-word elements
+    with pytest.raises(DsphError):
+        check(lib().dsph_gather_set(ptr(idx), n - 1, 1, (C.c_void_p * 1)(bad[0].data_ptr()),
+                                    (C.c_void_p * 1)(dst[1].data_ptr()), (C.c_int32 * 1)(2), stream_handle(dev)),
+              "gather_set")
