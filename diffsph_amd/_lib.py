@@ -269,6 +269,7 @@ SIGNATURES = {
     "dsph_mlp_packed_bytes": (SZ, []),
     "dsph_mlp_pack": (C.c_int, [V, V, V]),
     "dsph_mlp_forward": (C.c_int, [V, I64, V, V, V, V]),
+    "dsph_mlp_forward_affine": (C.c_int, [V, I64, V, V, V, F32, F32, I32, V, V]),
     "dsph_mlp_train_workspace": (SZ, [I64]),
     "dsph_mlp_loss_grad": (C.c_int, [V, V, I64, F32, V, V, V, V, V, SZ, V]),
     "dsph_adam_step": (C.c_int, [V, V, V, V, I64, F32, F32, F32, F32, F32, I32, V]),

@@ -265,9 +265,23 @@ __device__ __forceinline__ float4v infer_tile(const InferW& W, float4 xv) {
 // Grid-stride over 16-sample wave tiles, two tiles per pass so that their
 // independent MFMA chains overlap, the next pass's inputs in flight (no
 // block-wide synchronisation).
+// Output epilogue (base != NULL): Y = base + (y * scale + shift), negative
+// values set to 0 when clamp (NaN kept).
+struct FwdOut {
+  const float* base;
+  float scale, shift;
+  int clamp;
+  __device__ __forceinline__ float operator()(float y, int m) const {
+    if (!base) return y;
+    const float v = base[m] + (y * scale + shift);
+    return (clamp && v < 0.0f) ? 0.0f : v;
+  }
+};
+
 __global__ void __launch_bounds__(kBlock) k_mlp_forward(const float* __restrict__ X, int n,
                                                         const float* __restrict__ prm,
-                                                        const short4v* __restrict__ pk, float* __restrict__ Y) {
+                                                        const short4v* __restrict__ pk, float* __restrict__ Y,
+                                                        FwdOut out) {
   const int l = threadIdx.x & 63, c = l & 15, q = l >> 4;
   InferW W;
 #pragma unroll
@@ -318,8 +332,8 @@ __global__ void __launch_bounds__(kBlock) k_mlp_forward(const float* __restrict_
     const float4v y0 = infer_tile(W, x0);
     const float4v y1 = infer_tile(W, x1);
     const int m0 = t * 16 + c, m1 = t1 * 16 + c;
-    if (q == 0 && m0 < n) Y[m0] = y0[0];
-    if (q == 0 && t1 < ntiles && m1 < n) Y[m1] = y1[0];
+    if (q == 0 && m0 < n) Y[m0] = out(y0[0], m0);
+    if (q == 0 && t1 < ntiles && m1 < n) Y[m1] = out(y1[0], m1);
   }
 }
 
@@ -602,8 +616,20 @@ extern "C" int dsph_mlp_forward(const float* X, int64_t n, const float* params, 
   if (n == 0) return DSPH_OK;
   if (!X || !params || !packed || !Y) return DSPH_ERR_ARG;
   if (reinterpret_cast<uintptr_t>(X) % 16 != 0) return DSPH_ERR_ARG;  // rows are read as float4
-  k_mlp_forward<<<mlp_forward_grid(n), kBlock, 0, as_stream(stream)>>>(X, (int)n, params,
-                                                                       static_cast<const short4v*>(packed), Y);
+  k_mlp_forward<<<mlp_forward_grid(n), kBlock, 0, as_stream(stream)>>>(
+      X, (int)n, params, static_cast<const short4v*>(packed), Y, FwdOut{nullptr, 1.0f, 0.0f, 0});
+  return launch_status();
+}
+
+extern "C" int dsph_mlp_forward_affine(const float* X, int64_t n, const float* params, const void* packed,
+                                       const float* base, float scale, float shift, int32_t clamp, float* Y,
+                                       void* stream) {
+  if (n < 0 || n >= INT32_MAX) return DSPH_ERR_ARG;
+  if (n == 0) return DSPH_OK;
+  if (!X || !params || !packed || !Y || !base) return DSPH_ERR_ARG;
+  if (reinterpret_cast<uintptr_t>(X) % 16 != 0) return DSPH_ERR_ARG;
+  k_mlp_forward<<<mlp_forward_grid(n), kBlock, 0, as_stream(stream)>>>(
+      X, (int)n, params, static_cast<const short4v*>(packed), Y, FwdOut{base, scale, shift, clamp ? 1 : 0});
   return launch_status();
 }
 

@@ -170,9 +170,13 @@ class WarmStartNet:
 
     def predict_pressure(self, solver) -> torch.Tensor:
         X = self.features(solver)
-        y = self.forward(X)
-        p = solver.p_div + y.mul_(self.t_std).add_(self.t_mean)
-        return p.clamp_min_(0.0) if solver.cfg.clamp_pressure else p
+        # p_prev + (y * t_std + t_mean), clamped at 0: in the forward pass's store
+        p = torch.empty_like(solver.p_div)
+        check(lib().dsph_mlp_forward_affine(ptr(X), X.shape[0], ptr(self.params), ptr(self.packed), ptr(solver.p_div),
+                                            float(self.t_std), float(self.t_mean),
+                                            1 if solver.cfg.clamp_pressure else 0, ptr(p),
+                                            _lib.stream_handle(X.device)), "mlp_forward_affine")
+        return p
 
     # -- training -----------------------------------------------------------
     def fit_normalization(self, X_raw: torch.Tensor, T_raw: torch.Tensor):

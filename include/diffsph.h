@@ -699,6 +699,13 @@ size_t dsph_mlp_packed_bytes(void);
 int dsph_mlp_pack(const float* params, void* packed, void* stream);
 int dsph_mlp_forward(const float* X, int64_t n, const float* params,
                      const void* packed, float* Y, void* stream);
+/* dsph_mlp_forward with its output mapped in the same pass:
+ * Y = base + (y * scale + shift) (fp32, in that order), then negative
+ * values set to 0 when clamp (NaN stays NaN); the warm-start pressure
+ * p_prev + de-normalised network output.  No counterpart in the reference. */
+int dsph_mlp_forward_affine(const float* X, int64_t n, const float* params,
+                            const void* packed, const float* base, float scale,
+                            float shift, int32_t clamp, float* Y, void* stream);
 /* loss = sum_m w (y_m - target_m)^2 and its gradient w.r.t. every
  * parameter (grad: dsph_mlp_num_params() floats, deterministic fixed-order
  * reduction).  loss may be NULL. */

@@ -294,3 +294,29 @@ def test_judged_guess_is_the_solve_it_chooses(restart):
         assert (sa.div_iters, sa.den_iters) == (sb.div_iters, sb.den_iters)
         for name in ("pos", "vel", "rho", "p_div", "p_den", "a_div"):
             assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
+@pytest.mark.parametrize("clamp", [True, False])
+def test_forward_affine_equals_forward_then_map(clamp):
+    """dsph_mlp_forward_affine (the warm-start pressure written by the
+    forward pass) equals the forward pass followed by base + (y * scale +
+    shift) and the clamp at 0, bit for bit; NaN stays NaN."""
+    from diffsph_amd._lib import check, lib, ptr, stream_handle
+    net = _net(5)
+    rng = np.random.default_rng(11)
+    n = 70001
+    X = torch.from_numpy(rng.standard_normal((n, 16)).astype(np.float32)).cuda()
+    base = torch.from_numpy(rng.standard_normal(n).astype(np.float32)).cuda()
+    base[7] = float("nan")
+    scale, shift = 0.37, -0.21
+    y = net.forward(X)
+    ref = base + (y * scale + shift)
+    if clamp:
+        ref = torch.where(ref < 0, torch.zeros_like(ref), ref)
+    out = torch.empty(n, device="cuda:0")
+    check(lib().dsph_mlp_forward_affine(ptr(X), n, ptr(net.params), ptr(net.packed), ptr(base), scale, shift,
+                                        1 if clamp else 0, ptr(out), stream_handle(X.device)), "forward_affine")
+    assert torch.isnan(out[7]) and torch.isnan(ref[7])
+    assert torch.equal(torch.nan_to_num(out, nan=7.0), torch.nan_to_num(ref, nan=7.0))
+    if clamp:
+        assert float(out.nan_to_num(0.0).min()) >= 0.0
