@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(int ncell, const int32_t* 
   int b = cell_start[c], e = cell_start[c + 1];
   const int m = e - b;
   const int t = threadIdx.x;
+  if (m > DSPH_CELL_MAX) return;  // a diverged state: the search reports it
   if (m <= kSortLds) {
     for (int a0 = 0; a0 < m; a0 += 8) {
       int v[8];
@@ -164,11 +165,19 @@ __device__ __forceinline__ bool small_periodic_grid(const SearchArgs& a) {
 // Visit the candidate slices of query i in offset order (dy=-1,0,1;
 // dx=-1,0,1): f(b, e, seam), seam true when the slice's pairs may cross a
 // periodic seam (a wrapped row or column of cells, or a small periodic grid).
+// A slice longer than three cells of DSPH_CELL_MAX is skipped and sets
+// `over`, as does a query cell of more than DSPH_CELL_MAX particles (a
+// diverged state: the work stays bounded, the caller reports it).
+constexpr int kRangeMax = 3 * DSPH_CELL_MAX;
 template <class F>
-__device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, F&& f) {
+__device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, bool& over, F&& f) {
   const int ncx = a.g.nc[0], ncy = a.g.nc[1];
   int cx = cell_coord(qi.x, a.d, 0, a.g), cy = cell_coord(qi.y, a.d, 1, a.g);
   const bool small = small_periodic_grid(a);
+  {
+    const int c = cy * ncx + cx;
+    if (a.cs[c + 1] - a.cs[c] > DSPH_CELL_MAX) over = true;  // the query's own cell
+  }
   for (int oy = -1; oy <= 1; ++oy) {
     int yy = cy + oy;
     if (a.d.periodic[1]) {
@@ -179,15 +188,21 @@ __device__ __forceinline__ void for_candidates(const SearchArgs& a, float2 qi, F
     const int row = yy * ncx;
     bool wraps = a.d.periodic[0] && (cx == 0 || cx == ncx - 1);
     const bool seam = small || wraps || yy != cy + oy;
+    auto visit = [&](int b, int e) {
+      if (e - b > kRangeMax)
+        over = true;
+      else
+        f(b, e, seam);
+    };
     if (!wraps) {
       int x0 = cx - 1 < 0 ? 0 : cx - 1;
       int x1 = cx + 1 > ncx - 1 ? ncx - 1 : cx + 1;
-      f(a.cs[row + x0], a.cs[row + x1 + 1], seam);
+      visit(a.cs[row + x0], a.cs[row + x1 + 1]);
     } else {
       for (int ox = -1; ox <= 1; ++ox) {
         int xx = cx + ox;
         xx = xx < 0 ? xx + ncx : (xx >= ncx ? xx - ncx : xx);
-        f(a.cs[row + xx], a.cs[row + xx + 1], seam);
+        visit(a.cs[row + xx], a.cs[row + xx + 1]);
       }
     }
   }
@@ -200,8 +215,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_count(SearchArgs a, int32_t
   if (!a.wrapped) qi = make_float2(wrap_coord(qi.x, a.d, 0), wrap_coord(qi.y, a.d, 1));
   float hi = h_of(a.qh, i, a.h0);
   int c = 0;
+  bool over = false;  // counted without the skipped ranges; the host checks the cells
   auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
-  for_candidates(a, qi, [&](int b, int e, bool seam) {
+  for_candidates(a, qi, over, [&](int b, int e, bool seam) {
     if (seam)
       scan_range<4>(b, e, load, [&](int, const Cand& r) { c += keep_pair(qi, hi, r.x, r.h, a) ? 1 : 0; });
     else
@@ -273,7 +289,7 @@ __device__ __forceinline__ int64_t fill_row(const SearchArgs& a, int i, float2 q
                                             int16_t (*stage)[kBlock], int32_t* __restrict__ nbr, int64_t cap,
                                             int16_t* __restrict__ d16, const int32_t* __restrict__ map,
                                             const FillTiles& T, const int* rt_s, const int* rt_e, const int* rt_d,
-                                            int R, bool& bad, bool& tbad) {
+                                            int R, bool& bad, bool& tbad, bool& over) {
   constexpr int16_t kOut = kSlot ? kStagedNoRun : kStagedOut;
   int64_t n = 0;
   // Every candidate writes the staging slot of the next entry (row
@@ -297,7 +313,7 @@ __device__ __forceinline__ int64_t fill_row(const SearchArgs& a, int i, float2 q
     n += kp ? 1 : 0;
   };
   auto load = [&](int j) { return Cand{a.rpos[j], h_of(a.rh, j, a.h0)}; };
-  for_candidates(a, qi, [&](int b, int e, bool seam) {
+  for_candidates(a, qi, over, [&](int b, int e, bool seam) {
     // staged value of candidate j: j + sb
     int sb = -i;
     bool ok = true;
@@ -360,7 +376,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
     __syncthreads();
     if (S > kFillTileMax && S <= T.cap && threadIdx.x == 0) T.n[blockIdx.x] = -1;
   }
-  bool bad = false, tbad = false;
+  bool bad = false, tbad = false, over = false;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
   float2 qi = a.qpos[i];
@@ -383,14 +399,15 @@ __global__ void __launch_bounds__(kBlock) k_neighbor_fill(SearchArgs a, const in
   }
   const int64_t staged = (slice || fixed_w > 0) ? (kFillStage < len ? kFillStage : len) : 0;
   const int64_t n = (kTiles && fit) ? fill_row<true>(a, i, qi, hi, base, step, len, staged, fixed_w, stage, nbr,
-                                                     cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad)
+                                                     cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad, over)
                                     : fill_row<false>(a, i, qi, hi, base, step, len, staged, fixed_w, stage, nbr,
-                                                      cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad);
+                                                      cap, d16, map, T, rt_s, rt_e, rt_d, R, bad, tbad, over);
   if (kTiles && tbad) T.n[blockIdx.x] = -1;
   if (fixed_w > 0) {
     // entries past the row's count are never read: no padding
     counts[i] = n < len ? (int32_t)n : (int32_t)len;
-    if (n > len) *row_ovf = 1;
+    if (n > len) atomicMax(row_ovf, 1);
+    if (over) atomicMax(row_ovf, 2);
   } else {
     for (int64_t t = n > staged ? n : staged; t < len; ++t) {
       const int64_t o = base + t * step;

@@ -233,6 +233,11 @@ class Delta16Buffers:
         return self.buf, self.flag
 
 
+CELL_MAX = 4096  # DSPH_CELL_MAX
+_OVERFULL = (f"search: a grid cell holds more than {CELL_MAX} particles (a diverged or collapsed state; NaN "
+             "positions fall into cell 0): the list would be incomplete")
+
+
 def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: str = "scatter",
            qpos_wrapped: bool = False, layout: str = "sell", delta16: Delta16Buffers | None = None,
            width: int = 67, deferred: torch.Tensor | None = None, remap: bool = False,
@@ -291,6 +296,9 @@ def search(grid: CellGrid, qpos: torch.Tensor, qh: torch.Tensor | None, mode: st
         if nl is not None:
             return nl
         layout = "sell"
+    cs = grid.cell_start
+    if grid.n > 0 and int((cs[1:] - cs[:-1]).max()) > CELL_MAX:
+        raise DsphError(_OVERFULL)
     check(L.dsph_neighbor_count(*args, ptr(counts), st), "neighbor_count")
     offsets = torch.empty(nq + 1, dtype=torch.int32, device=dev)
     ws = WORKSPACE.get(L.dsph_scan_workspace(nq), dev)
@@ -364,6 +372,8 @@ def _search_fixed(L, args, grid, qpos, qh, m, qpos_wrapped, counts, delta16, wid
         check(L.dsph_scan_exclusive(ptr(counts), nq, ptr(offsets), None, ptr(ws), ws.numel(), st), "scan")
         info[1:2].copy_(offsets[nq:nq + 1])
         ovf, npairs = (int(v) for v in info.cpu())
+        if ovf >= 2:
+            raise DsphError(_OVERFULL)
         if ovf:
             return None
     wq = qpos if qpos_wrapped else wrap_positions(qpos, grid.domain)

@@ -113,6 +113,13 @@ typedef struct dsph_pairs {
 #define DSPH_TILE_MAX_CAP 4096
 #define DSPH_TILE_MAX_RUNS 16
 
+/* Particles per grid cell beyond which the state is taken as diverged or
+ * collapsed (NaN positions land in cell 0): such a cell is left unsorted
+ * and the neighbor search skips candidate ranges longer than 3 cells of
+ * this size, so that every kernel's work stays bounded.  The fixed-width
+ * fills report it with *row_overflow = 2 (1: a row longer than width). */
+#define DSPH_CELL_MAX 4096
+
 /* 16-bit neighbor deltas for the pressure sweeps (halves their index
  * stream): out[k] = (nbr[k] - i) reduced mod nr into [-nr/2, nr/2], for
  * every entry of every row in the list's own layout (padding entries are

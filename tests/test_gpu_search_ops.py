@@ -423,3 +423,31 @@ def test_gather_set_permutes_every_array():
         check(lib().dsph_gather_set(ptr(idx), n - 1, 1, (C.c_void_p * 1)(bad[0].data_ptr()),
                                     (C.c_void_p * 1)(dst[1].data_ptr()), (C.c_int32 * 1)(2), stream_handle(dev)),
               "gather_set")
+
+
+@pytest.mark.parametrize("kind", ["stacked", "nan"])
+def test_overfull_cell_raises(kind):
+    """A diverged or collapsed state (5000 particles at one point, or 5000
+    NaN positions, which fall into cell 0) overfills a cell: the grid build
+    and the search stay bounded and the search raises, in the one-pass and
+    the two-pass layouts; a DFSPH step on such a state raises too."""
+    from diffsph_amd._lib import DsphError
+    from diffsph_amd.dfsph import DFSPHConfig, DFSPHSolver
+    d, per = _case(96, 10, 0.1)
+    extra = np.full((5000, 2), np.nan if kind == "nan" else 0.1, np.float32)
+    pos = np.concatenate([d["pos"], extra]).astype(np.float32)
+    print(f"case {kind}: {pos.shape[0]} particles", flush=True)
+    dev = torch.device("cuda:0")
+    dom = _lib.make_domain((-1, -1), (1, 1), per)
+    g = CellGrid.build(torch.from_numpy(pos).to(dev), None, dom, d["h"], h_uniform=d["h"])
+    assert int(torch.diff(g.cell_start).max()) >= 5000
+    for layout in ("fixed", "sell"):
+        with pytest.raises(DsphError, match="more than 4096"):
+            search(g, g.sorted_pos, None, "scatter", qpos_wrapped=True, layout=layout)
+    n = pos.shape[0]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)  # noqa: E731
+    vel = np.concatenate([d["vel"], np.zeros((5000, 2), np.float32)])
+    with pytest.raises(DsphError, match="more than 4096"):
+        s = DFSPHSolver(T(pos), T(vel), T(np.full(n, d["mass"][0])), T(np.full(n, d["area"][0])),
+                        DFSPHConfig(dt=3.0 * d["dx"] / 10.0, support=d["h"]))
+        s.step()
