@@ -154,18 +154,6 @@ class DFSPHSolver:
         self._nbr_width = max(int(cfg.neighbor_width), 0)
 
     # -- helpers ---------------------------------------------------------
-    def _permute(self, t, idx):
-        if t is None:
-            return None
-        w = 1 if t.dim() == 1 else t.shape[1]
-        out = torch.empty_like(t)
-        if t.dtype == torch.float32:
-            check(lib().dsph_gather_f32(ptr(t), ptr(idx), self.n, w, ptr(out), _lib.stream_handle(t.device)),
-                  "gather")
-        else:
-            out = t[idx.long()]
-        return out
-
     def _permute_state(self, idx):
         """The per-particle state into the new cell order, one pass over all
         arrays (dsph_gather_set)."""
